@@ -1,0 +1,115 @@
+/*
+ * mg_sph.h — C-ABI of the MI355X (gfx950) Morphogenesis SPH step library (libmg_sph.so).
+ *
+ * Plain pointers and sizes only; no device pointers cross this boundary. A handle owns
+ * all device memory and one HIP stream. Buffers are exchanged in the reference layout
+ * (src/fluid.h:138-206): mg_upload / mg_download take host memory laid out exactly like
+ * the reference's host mirrors m_Fluid.mcpu[buf_id]. Every function returns MG_OK (0) or
+ * a negative MG_ERR_* code; mg_last_error() describes the last failure. Step functions
+ * are asynchronous on the handle's stream; mg_download, mg_get_counts and mg_sync wait.
+ * A handle is not thread-safe.
+ *
+ * Each declaration names the reference interface it replaces.
+ */
+#ifndef MG_SPH_H
+#define MG_SPH_H
+
+#include "mg_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mg_handle mg_handle;
+
+/* Allocation for max_points particle slots with B bonds per particle.
+ * Replaces FluidSystem::AllocateParticles / CreateFluidBuffers
+ * (src/fluid_system.cpp:771-845, src/host_CL.cpp:1521-1606). */
+int mg_create(mg_handle** out, int max_points, int bonds_per_particle);
+/* Replaces FluidSystem::Exit (src/fluid_system.cpp:640-650) without calling exit(). */
+int mg_destroy(mg_handle* h);
+/* Text of the last error on this thread ("" if none). */
+const char* mg_last_error(void);
+
+/* Grid set-up. Replaces FluidSetupCL(num, gsrch, res, size, delta, gmin, gmax, total, chk)
+ * (src/host_CL.cpp:39-174) and allocates the grid buffers (AllocateGrid,
+ * src/fluid_system.cpp:872-928). */
+int mg_fluid_setup(mg_handle* h, int num, int gsrch, const int32_t res[3], const float size[3],
+                   const float delta[3], const float gmin[3], const float gmax[3], int total,
+                   int chk);
+/* Same, from a prepared MgGrid. */
+int mg_set_grid(mg_handle* h, const MgGrid* g);
+/* Parameters. Replaces FluidParamCL(ss, sr, pr, mass, rest, bmin, bmax, estiff, istiff,
+ * visc, surface_t, damp, fmin, fmax, ffreq, gslope, gx, gy, gz, al, vl, a_f, a_p)
+ * (src/fluid_system.cpp:521-610). */
+int mg_fluid_param(mg_handle* h, float ss, float sr, float pr, float mass, float rest,
+                   const float bmin[3], const float bmax[3], float estiff, float istiff,
+                   float visc, float surface_t, float damp, float fmin, float fmax, float ffreq,
+                   float gslope, float gx, float gy, float gz, float al, float vl, float a_f,
+                   float a_p);
+/* Same, from a prepared MgParams (including the diff_scale / secrete_scale additions). */
+int mg_set_params(mg_handle* h, const MgParams* p);
+/* Replaces UpdateGenome (src/fluid_system.cpp:494-502); MgGenome has FGenome's layout. */
+int mg_set_genome(mg_handle* h, const MgGenome* g);
+/* Replaces setFreeze (src/fluid_system.cpp:2020-2035). */
+int mg_set_freeze(mg_handle* h, int freeze);
+
+/* Host -> device copy of one buffer in the reference layout; bytes must equal
+ * mg_buffer_bytes(buf_id, N, C, B). Replaces TransferToCL (src/host_CL.cpp:1608-1634). */
+int mg_upload(mg_handle* h, int buf_id, const void* src, size_t bytes);
+/* Device -> host copy of one buffer in the reference layout. Replaces TransferFromCL
+ * (src/host_CL.cpp:1636-1668). Waits for the stream. */
+int mg_download(mg_handle* h, int buf_id, void* dst, size_t bytes);
+/* mNumPoints and mActivePoints as the last sort left them (src/host_CL.cpp:1072-1093). */
+int mg_get_counts(mg_handle* h, uint32_t* num_points, uint32_t* active_points);
+/* Wait for all work queued on the handle. */
+int mg_sync(mg_handle* h);
+
+/* ---- phases (src/fluid_system.h:619-651) ---- */
+/* InsertParticlesCL(gcell, gndx, gcnt) (src/host_CL.cpp:228-426); non-null pointers receive
+ * FGCELL, FGNDX (N entries each) and FBIN_COUNT (C entries), as the reference's optional
+ * readback does. */
+int mg_insert_particles(mg_handle* h, uint32_t* gcell, uint32_t* gndx, uint32_t* gcnt);
+/* PrefixSumCellsCL(zero_offsets) (src/host_CL.cpp:429-863; src/host_CUDA.cpp:307-400). */
+int mg_prefix_sum_cells(mg_handle* h, int zero_offsets);
+/* CountingSortFullCL(ppos) (src/host_CL.cpp:1064-1413; src/host_CUDA.cpp:498-580),
+ * including the dense gene lists. */
+int mg_counting_sort_full(mg_handle* h);
+/* ComputePressureCL (src/host_CL.cpp:1918-1934). */
+int mg_compute_pressure(mg_handle* h);
+/* ComputeForceCL (src/host_CL.cpp:1950-1963). */
+int mg_compute_force(mg_handle* h);
+/* TransferPosVelVeval + AdvanceCL(time, dt, ss) + TransferPosVelVevalFromTemp
+ * (src/host_CL.cpp:199-226, 2182-2216). */
+int mg_advance(mg_handle* h, float time, float dt, float ss);
+/* ComputeDiffusionCL (src/host_CL.cpp:1936-1948). */
+int mg_compute_diffusion(mg_handle* h);
+/* ComputeGenesCL (src/host_CL.cpp:1965-2019). */
+int mg_compute_genes(mg_handle* h);
+/* ZeroVelCL (src/host_CL.cpp:2175-2180). */
+int mg_zero_vel(mg_handle* h);
+
+/* ---- composites (src/fluid_system.cpp:1919-2014) ---- */
+int mg_physical_sort(mg_handle* h);                          /* Run2PhysicalSort      */
+int mg_inner_physical_loop(mg_handle* h, float time, float dt); /* Run2InnerPhysicalLoop */
+int mg_gene_action(mg_handle* h);                            /* Run2GeneAction        */
+/* nsteps x [Run2InnerPhysicalLoop, Run2GeneAction if flags & MG_STEP_GENES,
+ * Run2PhysicalSort] — the hot loop of Run2Simulation (src/fluid_system.cpp:2671-2684)
+ * with steps_per_InnerPhysicalLoop = 1. Internal time advances by dt per step. */
+int mg_step(mg_handle* h, int nsteps, float dt, unsigned flags);
+
+/* ---- measurement (no reference counterpart; for bench.py and profiling) ----
+ * Like mg_step, with a HIP event pair around every phase launch on the handle's stream.
+ * ms_out[phase] accumulates milliseconds and calls_out[phase] the launches, for phase ids
+ * MG_PH_*; both arrays need MG_PH_COUNT entries (either may be null). */
+enum {
+    MG_PH_PRESSURE = 0, MG_PH_FORCE, MG_PH_ADVANCE, MG_PH_DIFFUSION, MG_PH_GENES, MG_PH_INSERT,
+    MG_PH_SCAN, MG_PH_PLACE, MG_PH_SORT, MG_PH_LISTS, MG_PH_FILL, MG_PH_COUNT
+};
+int mg_step_timed(mg_handle* h, int nsteps, float dt, unsigned flags, double* ms_out,
+                  int* calls_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,302 @@
+/*
+ * mg_types.h — shared plain-C data model for the Morphogenesis SPH step.
+ *
+ * Used by the HIP library (morphogenesis_amd/csrc), the CPU oracle (oracle/) and
+ * the C++ FluidSystem facade. Everything here is host/device neutral POD.
+ *
+ * Buffer ids and per-particle strides follow the reference data model
+ * (src/fluid.h:138-206, src/fluid.h:72-75 for the bond record, src/fluid.h:90-103
+ * for NUM_TF/NUM_GENES). The bond count per particle B (the reference's
+ * compile-time BONDS_PER_PARTICLE) is a run-time parameter here.
+ */
+#ifndef MG_TYPES_H
+#define MG_TYPES_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include <math.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- buffer ids: same numbering as the reference FBufs slots (src/fluid.h:138-206) ---- */
+enum {
+    MG_FPOS = 0,          /* float4[N]  (cl_float3 is 16 B)                        */
+    MG_FVEL = 1,          /* float4[N]                                             */
+    MG_FVEVAL = 2,        /* float4[N]                                             */
+    MG_FFORCE = 3,        /* float4[N]                                             */
+    MG_FPRESS = 4,        /* float[N]                                              */
+    MG_FDENSITY = 5,      /* float[N]  (holds 1/density, src/fluid_system_cuda.cu:699) */
+    MG_FAGE = 6,          /* uint[N]                                               */
+    MG_FCOLOR = 7,        /* uint[N]                                               */
+    MG_FGCELL = 8,        /* uint[N]                                               */
+    MG_FGNDX = 9,         /* uint[N]                                               */
+    MG_FELASTIDX = 14,    /* uint[N][B][9] bond records                            */
+    MG_FNERVEIDX = 15,    /* uint[N]                                               */
+    MG_FCONC = 16,        /* float[N][16]  (AoS, as the kernels index it)          */
+    MG_FEPIGEN = 17,      /* uint[16][N]   (SoA, stride = maxPoints)               */
+    MG_FBIN = 18,         /* uint[N]                                               */
+    MG_FBIN_COUNT = 19,   /* uint[C]                                               */
+    MG_FBIN_OFFSET = 20,  /* uint[C]                                               */
+    MG_FPARTICLEIDX = 29, /* uint[N][B][2] incoming bonds (particle, slot)         */
+    MG_FPARTICLE_ID = 30, /* uint[N]                                               */
+    MG_FMASS_RADIUS = 31, /* uint[N]                                               */
+    MG_FBIN_COUNT_ACTIVE_GENES = 32,  /* uint[16][C]                               */
+    MG_FBIN_OFFSET_ACTIVE_GENES = 33, /* uint[16][C]                               */
+    MG_FDENSE_LIST_LENGTHS = 34,      /* uint[16]                                  */
+    MG_FDENSE_LISTS = 35,             /* uint[16][N], entries past length = MG_NONE */
+    MG_MAX_BUF = 44
+};
+
+#define MG_NUM_TF 16
+#define MG_NUM_GENES 16
+#define MG_DATA_PER_BOND 9
+#define MG_MAX_ADJ 64
+
+/* bond record slots (src/fluid.h:154-155) */
+enum {
+    MG_B_INDEX = 0,     /* partner's current (sorted) index, uint            */
+    MG_B_LIMIT = 1,     /* elastic limit, float                              */
+    MG_B_REST = 2,      /* rest length, float                                */
+    MG_B_MODULUS = 3,   /* modulus, float                                    */
+    MG_B_DAMP = 4,      /* damping coefficient, float                        */
+    MG_B_ID = 5,        /* partner particle ID, uint                         */
+    MG_B_STRAIN2 = 6,   /* strain^2 integrator, float on the force path      */
+    MG_B_STRAIN = 7,    /* strain integrator, float                          */
+    MG_B_CHANGE = 8     /* change-type bits, uint                            */
+};
+
+#define MG_GRID_UNDEF 0xFFFFFFFFu /* src/fluid_system_cuda.cuh:43 */
+/* Empty-slot sentinel. The reference's CUDA path uses 65535 (src/host_CL.cpp:14), which
+ * collides with real indices once N > 65535; the OpenCL path uses 0xFFFFFFFF
+ * (src/kernels/kernelTest.cl:584). This build uses 0xFFFFFFFF throughout. */
+#define MG_NONE 0xFFFFFFFFu
+#define MG_EPSILON 0.00001f /* src/fluid_system_cuda.cuh:41 */
+#define MG_INT_MAX 2147483647
+
+/* ---- status codes ---- */
+enum {
+    MG_OK = 0,
+    MG_ERR_ARG = -1,
+    MG_ERR_HIP = -2,
+    MG_ERR_STATE = -3,
+    MG_ERR_UNSUPPORTED = -4,
+    MG_ERR_SIZE = -5,
+    MG_ERR_ALLOC = -6
+};
+
+/* ---- step flags for mg_step ---- */
+#define MG_STEP_GENES 1u   /* run Run2GeneAction (diffusion + genes) each step  */
+#define MG_STEP_GRAPH 2u   /* replay the step as a captured hipGraph             */
+
+/*
+ * Genome: same field order and sizes as the reference FGenome (src/fluid.h:398-422),
+ * so a FGenome can be passed by pointer.
+ */
+typedef struct MgGenome {
+    uint32_t mutability[MG_NUM_GENES];
+    uint32_t delay[MG_NUM_GENES];
+    uint32_t sensitivity[MG_NUM_GENES][MG_NUM_GENES];
+    uint32_t tf_diffusability[MG_NUM_TF];
+    uint32_t tf_breakdown_rate[MG_NUM_TF];
+    int32_t secrete[MG_NUM_GENES][2 * MG_NUM_TF + 1];
+    int32_t activate[MG_NUM_GENES][2 * MG_NUM_GENES + 1];
+    float tanh_param[3][8];
+    float param[3][12];
+} MgGenome;
+
+/*
+ * Simulation parameters used by the step: the FParams fields the step kernels read
+ * (src/fluid.h:278-346), with the derived constants of FluidParamCL
+ * (src/fluid_system.cpp:521-610). Two fields are additions of this build and default
+ * to 1.0, which reproduces the reference formulas exactly:
+ *   diff_scale    multiplies the (uint) TF diffusability in diffusion;
+ *   secrete_scale multiplies the (int) secretion rate in gene action.
+ */
+typedef struct MgParams {
+    float psimscale, psmoothradius, pradius, r2, pmass, prest_dens;
+    float pextstiff, pintstiff, pvisc, psurface_t, pdamp;
+    float pforce_min, pforce_max, pforce_freq, pground_slope;
+    float pgravity[3];
+    float pboundmin[3];
+    float pboundmax[3];
+    float AL, AL2, VL, VL2;
+    float pdist, poly6kern, spikykern, lapkern, gausskern, wendlandC2kern;
+    float H, d2, rd2, vterm;
+    float actuation_factor, actuation_period;
+    float diff_scale, secrete_scale;
+    int32_t freeze;
+    uint32_t frame;
+} MgParams;
+
+/* Uniform grid (src/host_CL.cpp:39-106 FluidSetupCL; src/fluid_system.cpp:2073-2131 SetupGrid). */
+typedef struct MgGrid {
+    int32_t res[3];
+    int32_t scan_max[3];
+    float size[3];
+    float delta[3];
+    float gmin[3];
+    float gmax[3];
+    int32_t total;
+    int32_t srch;
+    int32_t adj_cnt;
+    int32_t chk;
+    int32_t adj[MG_MAX_ADJ];
+} MgGrid;
+
+/*
+ * Host-side derivation of MgParams from the FluidParamCL argument list
+ * (src/fluid_system.cpp:521-610). The expressions and their float/double typing follow
+ * the reference so the constants are the ones the reference would upload.
+ */
+static inline void mg_params_from_fluid_param(
+    MgParams* p, float ss, float sr, float pr, float mass, float rest, const float bmin[3],
+    const float bmax[3], float estiff, float istiff, float visc, float surface_tension,
+    float damp, float fmin, float fmax, float ffreq, float gslope, float gx, float gy,
+    float gz, float al, float vl, float a_f, float a_p)
+{
+    int k;
+    p->psimscale = ss;
+    p->psmoothradius = sr;
+    p->pradius = pr;
+    p->r2 = sr * sr;
+    p->pmass = mass;
+    p->prest_dens = rest;
+    for (k = 0; k < 3; k++) { p->pboundmin[k] = bmin[k]; p->pboundmax[k] = bmax[k]; }
+    p->pextstiff = estiff;
+    p->pintstiff = istiff;
+    p->pvisc = visc;
+    p->psurface_t = surface_tension;
+    p->pdamp = damp;
+    p->pforce_min = fmin;
+    p->pforce_max = fmax;
+    p->pforce_freq = ffreq;
+    p->pground_slope = gslope;
+    p->pgravity[0] = gx; p->pgravity[1] = gy; p->pgravity[2] = gz;
+    p->AL = al;
+    p->AL2 = al * al;
+    p->VL = vl;
+    p->VL2 = vl * vl;
+    p->pdist = powf(p->pmass / p->prest_dens, 1 / 3.0f);
+    p->poly6kern = 315.0f / (64.0f * 3.141592f * powf(sr, 9.0f));
+    p->wendlandC2kern = 21 / (2 * 3.141592f);
+    p->spikykern = -45.0f / (3.141592f * powf(sr, 6.0f));
+    p->lapkern = 45.0f / (3.141592f * powf(sr, 6.0f));
+    p->gausskern = 1.0f / powf(3.141592f * 2.0f * sr * sr, 3.0f / 2.0f);
+    p->H = p->psmoothradius / p->psimscale;
+    p->d2 = p->psimscale * p->psimscale;
+    p->rd2 = p->r2 / p->d2;
+    p->vterm = p->lapkern * p->pvisc;
+    p->actuation_factor = a_f;
+    p->actuation_period = a_p;
+    p->diff_scale = 1.0f;
+    p->secrete_scale = 1.0f;
+    p->freeze = 0;
+    p->frame = 0;
+}
+
+/*
+ * Host-side grid set-up mirroring FluidSetupCL (src/host_CL.cpp:39-106): scan window
+ * gridScanMax = res - gridSrch, and the gridAdj offsets (y, z, x nesting).
+ */
+static inline int mg_grid_from_setup(MgGrid* g, int gsrch, const int32_t res[3],
+                                     const float size[3], const float delta[3],
+                                     const float gmin[3], const float gmax[3], int total,
+                                     int chk)
+{
+    int k, x, y, z, cell = 0;
+    if (gsrch < 1 || gsrch * gsrch * gsrch > MG_MAX_ADJ) return MG_ERR_ARG;
+    for (k = 0; k < 3; k++) {
+        g->res[k] = res[k];
+        g->size[k] = size[k];
+        g->delta[k] = delta[k];
+        g->gmin[k] = gmin[k];
+        g->gmax[k] = gmax[k];
+        g->scan_max[k] = res[k] - gsrch;
+    }
+    g->total = total;
+    g->srch = gsrch;
+    g->adj_cnt = gsrch * gsrch * gsrch;
+    g->chk = chk;
+    for (k = 0; k < MG_MAX_ADJ; k++) g->adj[k] = 0;
+    for (y = 0; y < gsrch; y++)
+        for (z = 0; z < gsrch; z++)
+            for (x = 0; x < gsrch; x++) g->adj[cell++] = (y * res[2] + z) * res[0] + x;
+    return MG_OK;
+}
+
+/*
+ * Host-side SetupGrid (src/fluid_system.cpp:2073-2131) followed by FluidSetupCL: computes
+ * res = ceil(extent / cell), size = res*cell, delta = res/size, gridSrch =
+ * floor(2*sr/ss/cell)+1 (min 2) in float as the reference host does.
+ */
+static inline int mg_grid_setup(MgGrid* g, const float vmin[3], const float vmax[3],
+                                float sim_scale, float cell_size, float smooth_radius)
+{
+    int32_t res[3];
+    float size[3], delta[3];
+    float world_cellsize = cell_size / sim_scale;
+    int k, gsrch;
+    for (k = 0; k < 3; k++) {
+        size[k] = vmax[k] - vmin[k];
+        res[k] = (int32_t)ceilf(size[k] / world_cellsize);
+        size[k] = res[k] * cell_size / sim_scale;
+        delta[k] = (float)res[k] / size[k];
+    }
+    gsrch = (int)(floorf(2.0f * (smooth_radius / sim_scale) / world_cellsize) + 1.0f);
+    if (gsrch < 2) gsrch = 2;
+    return mg_grid_from_setup(g, gsrch, res, size, delta, vmin, vmax,
+                              (int)(res[0] * res[1] * res[2]), 0);
+}
+
+/* Bytes of one particle's record in buffer `buf` (0 for grid-sized / unsupported ids). */
+static inline size_t mg_bytes_per_particle(int buf, int B)
+{
+    switch (buf) {
+    case MG_FPOS: case MG_FVEL: case MG_FVEVAL: case MG_FFORCE: return 16;
+    case MG_FPRESS: case MG_FDENSITY: case MG_FAGE: case MG_FCOLOR: case MG_FGCELL:
+    case MG_FGNDX: case MG_FNERVEIDX: case MG_FBIN: case MG_FPARTICLE_ID:
+    case MG_FMASS_RADIUS: return 4;
+    case MG_FELASTIDX: return (size_t)B * MG_DATA_PER_BOND * 4;
+    case MG_FPARTICLEIDX: return (size_t)B * 2 * 4;
+    case MG_FCONC: return MG_NUM_TF * 4;
+    case MG_FEPIGEN: return MG_NUM_GENES * 4;
+    default: return 0;
+    }
+}
+
+/* Whether `buf` is a buffer id the step exchanges (its size may be 0, e.g. bonds at B = 0). */
+static inline int mg_buffer_known(int buf)
+{
+    switch (buf) {
+    case MG_FPOS: case MG_FVEL: case MG_FVEVAL: case MG_FFORCE: case MG_FPRESS: case MG_FDENSITY:
+    case MG_FAGE: case MG_FCOLOR: case MG_FGCELL: case MG_FGNDX: case MG_FELASTIDX:
+    case MG_FNERVEIDX: case MG_FCONC: case MG_FEPIGEN: case MG_FBIN: case MG_FBIN_COUNT:
+    case MG_FBIN_OFFSET: case MG_FPARTICLEIDX: case MG_FPARTICLE_ID: case MG_FMASS_RADIUS:
+    case MG_FBIN_COUNT_ACTIVE_GENES: case MG_FBIN_OFFSET_ACTIVE_GENES: case MG_FDENSE_LIST_LENGTHS:
+    case MG_FDENSE_LISTS: return 1;
+    default: return 0;
+    }
+}
+
+/* Total bytes of buffer `buf` for N particle slots, C cells, B bonds (0 = unsupported). */
+static inline size_t mg_buffer_bytes(int buf, int N, int C, int B)
+{
+    size_t pp = mg_bytes_per_particle(buf, B);
+    if (pp) return pp * (size_t)N;
+    switch (buf) {
+    case MG_FELASTIDX: case MG_FPARTICLEIDX: return 0; /* B == 0 */
+    case MG_FBIN_COUNT: case MG_FBIN_OFFSET: return (size_t)C * 4;
+    case MG_FBIN_COUNT_ACTIVE_GENES: case MG_FBIN_OFFSET_ACTIVE_GENES:
+        return (size_t)MG_NUM_GENES * C * 4;
+    case MG_FDENSE_LIST_LENGTHS: return MG_NUM_GENES * 4;
+    case MG_FDENSE_LISTS: return (size_t)MG_NUM_GENES * N * 4;
+    default: return 0;
+    }
+}
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MG_TYPES_H */

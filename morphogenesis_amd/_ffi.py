@@ -1,0 +1,252 @@
+"""ctypes mirror of include/mg_types.h and loader for libmg_sph.so.
+
+The structures below must match include/mg_types.h field for field; a size check
+against the C library (mg_sizeof_*) guards against drift.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(HERE)
+LIB_DIR = os.path.join(HERE, "lib")
+LIB_PATH = os.path.join(LIB_DIR, "libmg_sph.so")
+
+# buffer ids (src/fluid.h:138-206)
+FPOS, FVEL, FVEVAL, FFORCE, FPRESS, FDENSITY, FAGE, FCOLOR, FGCELL, FGNDX = range(10)
+FELASTIDX, FNERVEIDX, FCONC, FEPIGEN, FBIN, FBIN_COUNT, FBIN_OFFSET = 14, 15, 16, 17, 18, 19, 20
+FPARTICLEIDX, FPARTICLE_ID, FMASS_RADIUS = 29, 30, 31
+FBIN_COUNT_ACTIVE_GENES, FBIN_OFFSET_ACTIVE_GENES, FDENSE_LIST_LENGTHS, FDENSE_LISTS = 32, 33, 34, 35
+
+PARTICLE_BUFFERS = [FPOS, FVEL, FVEVAL, FFORCE, FPRESS, FDENSITY, FAGE, FCOLOR, FGCELL, FGNDX,
+                    FELASTIDX, FNERVEIDX, FCONC, FEPIGEN, FBIN, FPARTICLEIDX, FPARTICLE_ID,
+                    FMASS_RADIUS]
+GRID_BUFFERS = [FBIN_COUNT, FBIN_OFFSET, FBIN_COUNT_ACTIVE_GENES, FBIN_OFFSET_ACTIVE_GENES,
+                FDENSE_LIST_LENGTHS, FDENSE_LISTS]
+BUFFER_NAMES = {
+    FPOS: "FPOS", FVEL: "FVEL", FVEVAL: "FVEVAL", FFORCE: "FFORCE", FPRESS: "FPRESS",
+    FDENSITY: "FDENSITY", FAGE: "FAGE", FCOLOR: "FCOLOR", FGCELL: "FGCELL", FGNDX: "FGNDX",
+    FELASTIDX: "FELASTIDX", FNERVEIDX: "FNERVEIDX", FCONC: "FCONC", FEPIGEN: "FEPIGEN",
+    FBIN: "FBIN", FBIN_COUNT: "FBIN_COUNT", FBIN_OFFSET: "FBIN_OFFSET",
+    FPARTICLEIDX: "FPARTICLEIDX", FPARTICLE_ID: "FPARTICLE_ID", FMASS_RADIUS: "FMASS_RADIUS",
+    FBIN_COUNT_ACTIVE_GENES: "FBIN_COUNT_ACTIVE_GENES",
+    FBIN_OFFSET_ACTIVE_GENES: "FBIN_OFFSET_ACTIVE_GENES",
+    FDENSE_LIST_LENGTHS: "FDENSE_LIST_LENGTHS", FDENSE_LISTS: "FDENSE_LISTS",
+}
+
+NUM_TF = 16
+NUM_GENES = 16
+DATA_PER_BOND = 9
+NONE = 0xFFFFFFFF
+GRID_UNDEF = 0xFFFFFFFF
+STEP_GENES = 1
+STEP_GRAPH = 2
+
+MG_OK = 0
+ERRORS = {-1: "MG_ERR_ARG", -2: "MG_ERR_HIP", -3: "MG_ERR_STATE", -4: "MG_ERR_UNSUPPORTED",
+          -5: "MG_ERR_SIZE", -6: "MG_ERR_ALLOC"}
+
+
+class MgGenome(C.Structure):
+    _fields_ = [
+        ("mutability", C.c_uint32 * NUM_GENES),
+        ("delay", C.c_uint32 * NUM_GENES),
+        ("sensitivity", (C.c_uint32 * NUM_GENES) * NUM_GENES),
+        ("tf_diffusability", C.c_uint32 * NUM_TF),
+        ("tf_breakdown_rate", C.c_uint32 * NUM_TF),
+        ("secrete", (C.c_int32 * (2 * NUM_TF + 1)) * NUM_GENES),
+        ("activate", (C.c_int32 * (2 * NUM_GENES + 1)) * NUM_GENES),
+        ("tanh_param", (C.c_float * 8) * 3),
+        ("param", (C.c_float * 12) * 3),
+    ]
+
+
+class MgParams(C.Structure):
+    _fields_ = [(n, C.c_float) for n in (
+        "psimscale", "psmoothradius", "pradius", "r2", "pmass", "prest_dens",
+        "pextstiff", "pintstiff", "pvisc", "psurface_t", "pdamp",
+        "pforce_min", "pforce_max", "pforce_freq", "pground_slope")] + [
+        ("pgravity", C.c_float * 3), ("pboundmin", C.c_float * 3), ("pboundmax", C.c_float * 3),
+    ] + [(n, C.c_float) for n in (
+        "AL", "AL2", "VL", "VL2", "pdist", "poly6kern", "spikykern", "lapkern", "gausskern",
+        "wendlandC2kern", "H", "d2", "rd2", "vterm", "actuation_factor", "actuation_period",
+        "diff_scale", "secrete_scale")] + [("freeze", C.c_int32), ("frame", C.c_uint32)]
+
+
+class MgGrid(C.Structure):
+    _fields_ = [
+        ("res", C.c_int32 * 3), ("scan_max", C.c_int32 * 3),
+        ("size", C.c_float * 3), ("delta", C.c_float * 3),
+        ("gmin", C.c_float * 3), ("gmax", C.c_float * 3),
+        ("total", C.c_int32), ("srch", C.c_int32), ("adj_cnt", C.c_int32), ("chk", C.c_int32),
+        ("adj", C.c_int32 * 64),
+    ]
+
+
+def f32(x) -> float:
+    """Round a Python float to the nearest float32 (host float arithmetic of the reference)."""
+    return float(np.float32(x))
+
+
+def params_from_fluid_param(ss, sr, pr, mass, rest, bmin, bmax, estiff, istiff, visc, surface_t,
+                            damp, fmin, fmax, ffreq, gslope, gx, gy, gz, al, vl, a_f=0.0,
+                            a_p=1.0) -> MgParams:
+    """FluidParamCL (src/fluid_system.cpp:521-610) in float32, as mg_params_from_fluid_param."""
+    F = np.float32
+    p = MgParams()
+    sr_, ss_ = F(sr), F(ss)
+    p.psimscale = ss_
+    p.psmoothradius = sr_
+    p.pradius = F(pr)
+    p.r2 = F(sr_ * sr_)
+    p.pmass = F(mass)
+    p.prest_dens = F(rest)
+    for k in range(3):
+        p.pboundmin[k] = F(bmin[k])
+        p.pboundmax[k] = F(bmax[k])
+    p.pextstiff, p.pintstiff, p.pvisc = F(estiff), F(istiff), F(visc)
+    p.psurface_t, p.pdamp = F(surface_t), F(damp)
+    p.pforce_min, p.pforce_max, p.pforce_freq, p.pground_slope = F(fmin), F(fmax), F(ffreq), F(gslope)
+    p.pgravity[0], p.pgravity[1], p.pgravity[2] = F(gx), F(gy), F(gz)
+    p.AL, p.VL = F(al), F(vl)
+    p.AL2 = F(F(al) * F(al))
+    p.VL2 = F(F(vl) * F(vl))
+    pi = F(3.141592)
+    p.pdist = F(np.power(F(p.pmass) / F(p.prest_dens), F(1) / F(3)))
+    p.poly6kern = F(F(315.0) / (F(64.0) * pi * np.power(sr_, F(9.0))))
+    p.wendlandC2kern = F(F(21) / (F(2) * pi))
+    p.spikykern = F(F(-45.0) / (pi * np.power(sr_, F(6.0))))
+    p.lapkern = F(F(45.0) / (pi * np.power(sr_, F(6.0))))
+    p.gausskern = F(F(1.0) / np.power(pi * F(2.0) * sr_ * sr_, F(1.5)))
+    p.H = F(sr_ / ss_)
+    p.d2 = F(ss_ * ss_)
+    p.rd2 = F(F(p.r2) / F(p.d2))
+    p.vterm = F(F(p.lapkern) * F(p.pvisc))
+    p.actuation_factor, p.actuation_period = F(a_f), F(a_p)
+    p.diff_scale = 1.0
+    p.secrete_scale = 1.0
+    p.freeze = 0
+    p.frame = 0
+    return p
+
+
+def grid_setup(vmin, vmax, sim_scale, cell_size, smooth_radius) -> MgGrid:
+    """SetupGrid (src/fluid_system.cpp:2073-2131) + FluidSetupCL (src/host_CL.cpp:39-106),
+    float32 host arithmetic, as mg_grid_setup in include/mg_types.h."""
+    F = np.float32
+    g = MgGrid()
+    world_cellsize = F(F(cell_size) / F(sim_scale))
+    res = []
+    for k in range(3):
+        size = F(F(vmax[k]) - F(vmin[k]))
+        r = int(np.ceil(F(size / world_cellsize)))
+        size = F(F(F(r) * F(cell_size)) / F(sim_scale))
+        g.res[k] = r
+        g.size[k] = size
+        g.delta[k] = F(F(r) / size)
+        g.gmin[k] = F(vmin[k])
+        g.gmax[k] = F(vmax[k])
+        res.append(r)
+    gsrch = int(np.floor(F(F(F(2.0) * F(F(smooth_radius) / F(sim_scale))) / world_cellsize)) + F(1.0))
+    gsrch = max(gsrch, 2)
+    for k in range(3):
+        g.scan_max[k] = res[k] - gsrch
+    g.total = res[0] * res[1] * res[2]
+    g.srch = gsrch
+    g.adj_cnt = gsrch ** 3
+    g.chk = 0
+    cell = 0
+    for y in range(gsrch):
+        for z in range(gsrch):
+            for x in range(gsrch):
+                g.adj[cell] = (y * res[2] + z) * res[0] + x
+                cell += 1
+    return g
+
+
+def buffer_shape(buf: int, n: int, c: int, b: int):
+    """numpy (shape, dtype) of a buffer in the reference layout."""
+    u, f = np.uint32, np.float32
+    table = {
+        FPOS: ((n, 4), f), FVEL: ((n, 4), f), FVEVAL: ((n, 4), f), FFORCE: ((n, 4), f),
+        FPRESS: ((n,), f), FDENSITY: ((n,), f), FAGE: ((n,), u), FCOLOR: ((n,), u),
+        FGCELL: ((n,), u), FGNDX: ((n,), u), FNERVEIDX: ((n,), u), FBIN: ((n,), u),
+        FPARTICLE_ID: ((n,), u), FMASS_RADIUS: ((n,), u),
+        FELASTIDX: ((n, b, DATA_PER_BOND), u), FPARTICLEIDX: ((n, b, 2), u),
+        FCONC: ((n, NUM_TF), f), FEPIGEN: ((NUM_GENES, n), u),
+        FBIN_COUNT: ((c,), u), FBIN_OFFSET: ((c,), u),
+        FBIN_COUNT_ACTIVE_GENES: ((NUM_GENES, c), u), FBIN_OFFSET_ACTIVE_GENES: ((NUM_GENES, c), u),
+        FDENSE_LIST_LENGTHS: ((NUM_GENES,), u), FDENSE_LISTS: ((NUM_GENES, n), u),
+    }
+    return table[buf]
+
+
+class MgError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load_library(path: str | None = None):
+    """Load libmg_sph.so (the HIP product path). Raises if it is missing: there is no
+    CPU fallback."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise MgError(f"{p} is missing; run __graft_entry__.build() (hipcc, gfx950) first")
+    lib = C.CDLL(p)
+    declare(lib, "mg")
+    lib.mg_last_error.restype = C.c_char_p
+    lib.mg_last_error.argtypes = []
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def declare(lib, prefix: str):
+    """Declare the shared entry-point signatures for the mg_ (HIP) or mgo_ (oracle) library."""
+    h = C.c_void_p
+    P = C.POINTER
+    sig = {
+        "destroy": [h],
+        "set_params": [h, P(MgParams)],
+        "set_genome": [h, P(MgGenome)],
+        "set_freeze": [h, C.c_int],
+        "upload": [h, C.c_int, C.c_void_p, C.c_size_t],
+        "download": [h, C.c_int, C.c_void_p, C.c_size_t],
+        "get_counts": [h, P(C.c_uint32), P(C.c_uint32)],
+        "prefix_sum_cells": [h] + ([C.c_int] if prefix == "mg" else []),
+        "counting_sort_full": [h],
+        "compute_pressure": [h],
+        "compute_force": [h],
+        "advance": [h, C.c_float, C.c_float, C.c_float],
+        "compute_diffusion": [h],
+        "compute_genes": [h],
+        "zero_vel": [h],
+        "physical_sort": [h],
+        "inner_physical_loop": [h, C.c_float, C.c_float],
+        "gene_action": [h],
+        "step": [h, C.c_int, C.c_float, C.c_uint],
+        "set_grid": [h, P(MgGrid)],
+    }
+    if prefix == "mg":
+        sig["create"] = [P(h), C.c_int, C.c_int]
+        sig["insert_particles"] = [h, P(C.c_uint32), P(C.c_uint32), P(C.c_uint32)]
+        sig["sync"] = [h]
+        sig["fluid_setup"] = [h, C.c_int, C.c_int, P(C.c_int32), P(C.c_float), P(C.c_float),
+                              P(C.c_float), P(C.c_float), C.c_int, C.c_int]
+    else:
+        sig["create"] = [P(h), C.c_int, C.c_int, C.c_int]
+        sig["insert_particles"] = [h]
+        sig["set_threads"] = [h, C.c_int]
+    for name, args in sig.items():
+        fn = getattr(lib, f"{prefix}_{name}")
+        fn.argtypes = args
+        fn.restype = C.c_int
+    return lib

@@ -1,0 +1,1027 @@
+/*
+ * mg_capi.hip — host side of libmg_sph.so: the C-ABI of include/mg_sph.h.
+ *
+ * A handle owns every device buffer and one HIP stream (default priority). Buffers cross
+ * the boundary only in the reference layout; the conversion kernels below translate to the
+ * internal HBM layout described in mg_device.h. Nothing here falls back to the CPU: a
+ * missing GPU or a failed HIP call is reported as MG_ERR_HIP.
+ */
+#include "mg_kernels.hip"
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/mg_sph.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+void set_err(const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+}
+
+#define HIPCHK(x)                                                                             \
+    do {                                                                                      \
+        hipError_t e_ = (x);                                                                  \
+        if (e_ != hipSuccess) {                                                               \
+            set_err("%s failed: %s (%s:%d)", #x, hipGetErrorString(e_), __FILE__, __LINE__);  \
+            return MG_ERR_HIP;                                                                \
+        }                                                                                     \
+    } while (0)
+
+#define CHK(x)                                                                                \
+    do {                                                                                      \
+        int r_ = (x);                                                                         \
+        if (r_ != MG_OK) return r_;                                                           \
+    } while (0)
+
+inline unsigned nblk(size_t n, unsigned bs = 256) { return (unsigned)((n + bs - 1) / bs); }
+
+/* ------------------------------------------------ layout conversion kernels */
+__global__ void k_set_xyz(float4* dst, const float4* src, int n)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 s = src[i], d = dst[i];
+    dst[i] = make_float4(s.x, s.y, s.z, d.w);
+}
+__global__ void k_get_xyz(float4* dst, const float4* src, int n, int zero)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 s = src[i];
+    dst[i] = zero ? make_float4(0.f, 0.f, 0.f, 0.f) : make_float4(s.x, s.y, s.z, 0.0f);
+}
+__global__ void k_set_w(float4* dst, const float* src, int n)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    ((float*)&dst[i])[3] = src[i];
+}
+__global__ void k_get_w(float* dst, const float4* src, int n)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    dst[i] = src[i].w;
+}
+__global__ void k_iota(uint32_t* p, int n)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = (uint32_t)i;
+}
+/* home-indexed u32 field -> sorted order; slots >= tail get `reset` unless keep_stale */
+__global__ void k_gather_home(uint32_t* dst, const uint32_t* src, const uint32_t* HOME,
+                              const uint32_t* ctr, int n, int keep_stale, uint32_t reset)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    dst[s] = (keep_stale || (uint32_t)s < ctr[MG_C_TAIL]) ? src[HOME[s]] : reset;
+}
+__global__ void k_gather_epigen(uint32_t* dst, const uint32_t* src, const uint32_t* HOME,
+                                const uint32_t* ctr, int n)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    int g = blockIdx.y;
+    if (s >= n) return;
+    dst[(size_t)g * n + s] = ((uint32_t)s < ctr[MG_C_TAIL]) ? src[(size_t)g * n + HOME[s]] : 0u;
+}
+/* home-indexed bond storage -> bond storage indexed by sorted slot (normalisation) */
+__global__ void k_norm_bonds(uint32_t* nPartner, float4* nPar, float2* nInt, uint2* nCold,
+                             uint2* nPidx, const uint32_t* Partner, const float4* Par,
+                             const float2* Int, const uint2* Cold, const uint2* Pidx,
+                             const uint32_t* HOME, const uint32_t* SLOT, const uint32_t* ctr,
+                             int n)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    int a = blockIdx.y;
+    if (s >= n) return;
+    size_t o = (size_t)a * n + s;
+    if ((uint32_t)s < ctr[MG_C_TAIL]) {
+        size_t hi = (size_t)a * n + HOME[s];
+        uint32_t jh = Partner[hi];
+        nPartner[o] = (jh == MG_NONE) ? MG_NONE : SLOT[jh];
+        nPar[o] = Par[hi];
+        nInt[o] = Int[hi];
+        nCold[o] = Cold[hi];
+        uint2 pi = Pidx[hi];
+        nPidx[o] = make_uint2(pi.x == MG_NONE ? MG_NONE : SLOT[pi.x], pi.y);
+    } else {
+        float nf = __uint_as_float(MG_NONE);
+        nPartner[o] = MG_NONE;
+        nPar[o] = make_float4(nf, nf, nf, nf);
+        nInt[o] = make_float2(nf, nf);
+        nCold[o] = make_uint2(MG_NONE, MG_NONE);
+        nPidx[o] = make_uint2(MG_NONE, MG_NONE);
+    }
+}
+/* reference bond records [N][B][9] -> internal (after normalisation HOME is identity) */
+__global__ void k_upload_bonds(uint32_t* Partner, float4* Par, float2* Int, uint2* Cold,
+                               const uint32_t* rec, int n, int B)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    int a = blockIdx.y;
+    if (s >= n) return;
+    const uint32_t* w = rec + ((size_t)s * B + a) * MG_DATA_PER_BOND;
+    size_t o = (size_t)a * n + s;
+    Partner[o] = (w[0] < (uint32_t)n) ? w[0] : MG_NONE;
+    Par[o] = make_float4(__uint_as_float(w[1]), __uint_as_float(w[2]), __uint_as_float(w[3]),
+                         __uint_as_float(w[4]));
+    Int[o] = make_float2(__uint_as_float(w[6]), __uint_as_float(w[7]));
+    Cold[o] = make_uint2(w[5], w[8]);
+}
+__global__ void k_download_bonds(uint32_t* rec, const uint32_t* Partner, const float4* Par,
+                                 const float2* Int, const uint2* Cold, const uint32_t* HOME,
+                                 const uint32_t* SLOT, const uint32_t* ctr, int n, int B)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    int a = blockIdx.y;
+    if (s >= n) return;
+    uint32_t* w = rec + ((size_t)s * B + a) * MG_DATA_PER_BOND;
+    if ((uint32_t)s >= ctr[MG_C_TAIL]) {
+        for (int k = 0; k < MG_DATA_PER_BOND; k++) w[k] = MG_NONE;
+        return;
+    }
+    size_t hi = (size_t)a * n + HOME[s];
+    uint32_t jh = Partner[hi];
+    float4 bp = Par[hi];
+    float2 bi = Int[hi];
+    uint2 bc = Cold[hi];
+    w[0] = (jh == MG_NONE) ? MG_NONE : SLOT[jh];
+    w[1] = __float_as_uint(bp.x);
+    w[2] = __float_as_uint(bp.y);
+    w[3] = __float_as_uint(bp.z);
+    w[4] = __float_as_uint(bp.w);
+    w[5] = bc.x;
+    w[6] = __float_as_uint(bi.x);
+    w[7] = __float_as_uint(bi.y);
+    w[8] = bc.y;
+}
+__global__ void k_upload_pidx(uint2* Pidx, const uint32_t* rec, int n, int B)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    int a = blockIdx.y;
+    if (s >= n) return;
+    const uint32_t* w = rec + ((size_t)s * B + a) * 2;
+    Pidx[(size_t)a * n + s] = make_uint2(w[0] < (uint32_t)n ? w[0] : MG_NONE, w[1]);
+}
+__global__ void k_download_pidx(uint32_t* rec, const uint2* Pidx, const uint32_t* HOME,
+                                const uint32_t* SLOT, const uint32_t* ctr, int n, int B)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    int a = blockIdx.y;
+    if (s >= n) return;
+    uint32_t* w = rec + ((size_t)s * B + a) * 2;
+    if ((uint32_t)s >= ctr[MG_C_TAIL]) {
+        w[0] = MG_NONE;
+        w[1] = MG_NONE;
+        return;
+    }
+    uint2 pi = Pidx[(size_t)a * n + HOME[s]];
+    w[0] = pi.x == MG_NONE ? MG_NONE : SLOT[pi.x];
+    w[1] = pi.y;
+}
+__global__ void k_download_fbin(uint32_t* dst, const uint32_t* FBIN, const uint32_t* ctr, int n)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    dst[s] = ((uint32_t)s < ctr[MG_C_HWM]) ? (uint32_t)s : FBIN[s];
+}
+
+} // namespace
+
+/* ================================================================== handle */
+struct mg_handle {
+    int N = 0, B = 0, C = 0;
+    MgParams p;
+    MgGrid g;
+    MgGenome gen;
+    bool have_params = false, have_grid = false;
+    hipStream_t st = nullptr;
+    /* sorted ping-pong halves: index `cur` is current for P/V/VEL/HOME/GCELL/GNDX/GMASK,
+     * `ccur` for CONC (diffusion flips it on its own) */
+    float4 *P[2] = {}, *V[2] = {}, *VEL[2] = {};
+    float* CONC[2] = {};
+    uint32_t *HOME[2] = {}, *GCELL[2] = {}, *GNDX[2] = {}, *GMASK[2] = {};
+    int cur = 0, ccur = 0;
+    float4* F = nullptr;
+    bool force_zero = true;
+    /* home-indexed */
+    uint32_t *SLOT = nullptr, *EPIGEN = nullptr, *BPARTNER = nullptr;
+    float4* BPAR = nullptr;
+    float2* BINT = nullptr;
+    uint2 *BCOLD = nullptr, *PIDX = nullptr;
+    uint32_t *AGE = nullptr, *COLOR = nullptr, *PID = nullptr, *MRAD = nullptr, *NERVE = nullptr;
+    uint32_t* FBIN = nullptr;
+    uint32_t *INC_OFF = nullptr, *INC = nullptr;
+    size_t inc_cap = 0;
+    /* grid */
+    uint32_t *CNT = nullptr, *OFF = nullptr, *GCNT = nullptr, *GOFF = nullptr, *LENS = nullptr;
+    uint32_t *LISTS = nullptr, *PERM = nullptr;
+    uint32_t* ctr = nullptr;
+    void* staging = nullptr;
+    size_t staging_bytes = 0;
+    bool normalized = true;
+    double time = 0.0;
+    std::vector<void*> allocs;
+    /* phase timing (mg_step_timed) */
+    bool timing = false;
+    struct Mark {
+        hipEvent_t a, b;
+        int ph;
+    };
+    std::vector<Mark> marks;
+    std::vector<hipEvent_t> pool;
+    size_t pool_used = 0;
+};
+
+namespace {
+
+template <typename T> int dalloc(mg_handle* h, T** p, size_t count)
+{
+    void* q = nullptr;
+    size_t bytes = std::max<size_t>(count * sizeof(T), 256);
+    HIPCHK(hipMalloc(&q, bytes));
+    h->allocs.push_back(q);
+    *p = (T*)q;
+    return MG_OK;
+}
+
+void dfree(mg_handle* h, void* p)
+{
+    if (!p) return;
+    auto it = std::find(h->allocs.begin(), h->allocs.end(), p);
+    if (it != h->allocs.end()) h->allocs.erase(it);
+    (void)hipFree(p);
+}
+
+hipEvent_t pool_event(mg_handle* h)
+{
+    if (h->pool_used == h->pool.size()) {
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        h->pool.push_back(e);
+    }
+    return h->pool[h->pool_used++];
+}
+
+/* Brackets the launches of one phase with an event pair when the handle is timing. */
+struct PhaseScope {
+    mg_handle* h;
+    int ph;
+    hipEvent_t a = nullptr;
+    PhaseScope(mg_handle* h_, int ph_) : h(h_), ph(ph_)
+    {
+        if (h->timing && (a = pool_event(h))) (void)hipEventRecord(a, h->st);
+    }
+    ~PhaseScope()
+    {
+        if (!a) return;
+        hipEvent_t b = pool_event(h);
+        if (!b) return;
+        (void)hipEventRecord(b, h->st);
+        h->marks.push_back({a, b, ph});
+    }
+};
+
+int fill_u32(mg_handle* h, void* p, uint32_t v, size_t n)
+{
+    if (n == 0) return MG_OK;
+    unsigned blocks = (unsigned)std::min<size_t>(nblk(n), 4096);
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocks), dim3(256), 0, h->st, (uint32_t*)p, v, n);
+    HIPCHK(hipGetLastError());
+    return MG_OK;
+}
+
+int staging(mg_handle* h, size_t bytes)
+{
+    if (bytes <= h->staging_bytes) return MG_OK;
+    dfree(h, h->staging);
+    h->staging = nullptr;
+    h->staging_bytes = 0;
+    CHK(dalloc(h, (uint8_t**)&h->staging, bytes));
+    h->staging_bytes = bytes;
+    return MG_OK;
+}
+
+DS make_ds(mg_handle* h)
+{
+    DS d;
+    const int c = h->cur, o = 1 - h->cur;
+    d.P = h->P[c]; d.V = h->V[c]; d.VEL = h->VEL[c]; d.F = h->F;
+    d.CONC = h->CONC[h->ccur];
+    d.HOME = h->HOME[c]; d.GCELL = h->GCELL[c]; d.GNDX = h->GNDX[c]; d.GMASK = h->GMASK[c];
+    d.P2 = h->P[o]; d.V2 = h->V[o]; d.VEL2 = h->VEL[o];
+    d.CONC2 = h->CONC[1 - h->ccur];
+    d.HOME2 = h->HOME[o]; d.GCELL2 = h->GCELL[o]; d.GNDX2 = h->GNDX[o]; d.GMASK2 = h->GMASK[o];
+    d.SLOT = h->SLOT; d.EPIGEN = h->EPIGEN; d.BPARTNER = h->BPARTNER;
+    d.BPAR = h->BPAR; d.BINT = h->BINT;
+    d.INC_OFF = h->INC_OFF; d.INC = h->INC;
+    d.CNT = h->CNT; d.OFF = h->OFF; d.GCNT = h->GCNT; d.GOFF = h->GOFF;
+    d.PERM = h->PERM; d.LISTS = h->LISTS; d.LENS = h->LENS;
+    d.ctr = h->ctr;
+    d.N = h->N; d.B = h->B; d.C = h->C;
+    return d;
+}
+
+int ready(mg_handle* h)
+{
+    if (!h) { set_err("null handle"); return MG_ERR_ARG; }
+    if (!h->have_grid || !h->have_params) {
+        set_err("grid and parameters must be set before stepping");
+        return MG_ERR_STATE;
+    }
+    return MG_OK;
+}
+
+/* Rebuild the incoming-bond CSR from BPARTNER (home space): entries of particle j are the
+ * (owner home, slot) pairs whose partner is j, ascending. */
+int rebuild_inc(mg_handle* h)
+{
+    const int N = h->N, B = h->B;
+    std::vector<uint32_t> partner((size_t)N * B);
+    if (B > 0) {
+        HIPCHK(hipMemcpyAsync(partner.data(), h->BPARTNER, partner.size() * 4, hipMemcpyDeviceToHost, h->st));
+        HIPCHK(hipStreamSynchronize(h->st));
+    }
+    std::vector<uint32_t> off((size_t)N + 1, 0);
+    for (int a = 0; a < B; a++)
+        for (int k = 0; k < N; k++) {
+            uint32_t j = partner[(size_t)a * N + k];
+            if (j < (uint32_t)N) off[j + 1]++;
+        }
+    for (int j = 0; j < N; j++) off[j + 1] += off[j];
+    std::vector<uint32_t> ent(std::max<uint32_t>(off[N], 1));
+    std::vector<uint32_t> fillp(off.begin(), off.end() - 1);
+    for (int k = 0; k < N; k++)
+        for (int a = 0; a < B; a++) {
+            uint32_t j = partner[(size_t)a * N + k];
+            if (j < (uint32_t)N) ent[fillp[j]++] = (uint32_t)k * (uint32_t)B + (uint32_t)a;
+        }
+    if (ent.size() > h->inc_cap) {
+        dfree(h, h->INC);
+        h->INC = nullptr;
+        CHK(dalloc(h, &h->INC, ent.size()));
+        h->inc_cap = ent.size();
+    }
+    HIPCHK(hipMemcpyAsync(h->INC_OFF, off.data(), off.size() * 4, hipMemcpyHostToDevice, h->st));
+    HIPCHK(hipMemcpyAsync(h->INC, ent.data(), ent.size() * 4, hipMemcpyHostToDevice, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    return MG_OK;
+}
+
+/* Make the home index equal to the sorted slot again (HOME = SLOT = identity), materialising
+ * the sort's reset tail, so uploads can address home-indexed storage by slot. */
+int normalize(mg_handle* h)
+{
+    if (h->normalized) return MG_OK;
+    const int N = h->N, B = h->B;
+    const uint32_t* HOME = h->HOME[h->cur];
+    auto regather = [&](uint32_t** field, int keep_stale, uint32_t reset) -> int {
+        uint32_t* nf = nullptr;
+        CHK(dalloc(h, &nf, N));
+        hipLaunchKernelGGL(k_gather_home, dim3(nblk(N)), dim3(256), 0, h->st, nf, *field, HOME,
+                           h->ctr, N, keep_stale, reset);
+        HIPCHK(hipGetLastError());
+        dfree(h, *field);
+        *field = nf;
+        return MG_OK;
+    };
+    CHK(regather(&h->AGE, 1, 0));
+    CHK(regather(&h->COLOR, 1, 0));
+    CHK(regather(&h->MRAD, 1, 0));
+    CHK(regather(&h->NERVE, 1, 0));
+    CHK(regather(&h->PID, 0, MG_NONE));
+    {
+        uint32_t* ne = nullptr;
+        CHK(dalloc(h, &ne, (size_t)MG_NUM_GENES * N));
+        hipLaunchKernelGGL(k_gather_epigen, dim3(nblk(N), MG_NUM_GENES), dim3(256), 0, h->st, ne,
+                           h->EPIGEN, HOME, h->ctr, N);
+        HIPCHK(hipGetLastError());
+        dfree(h, h->EPIGEN);
+        h->EPIGEN = ne;
+    }
+    if (B > 0) {
+        uint32_t* nP = nullptr;
+        float4* nPar = nullptr;
+        float2* nInt = nullptr;
+        uint2 *nCold = nullptr, *nPidx = nullptr;
+        CHK(dalloc(h, &nP, (size_t)B * N));
+        CHK(dalloc(h, &nPar, (size_t)B * N));
+        CHK(dalloc(h, &nInt, (size_t)B * N));
+        CHK(dalloc(h, &nCold, (size_t)B * N));
+        CHK(dalloc(h, &nPidx, (size_t)B * N));
+        hipLaunchKernelGGL(k_norm_bonds, dim3(nblk(N), B), dim3(256), 0, h->st, nP, nPar, nInt,
+                           nCold, nPidx, h->BPARTNER, h->BPAR, h->BINT, h->BCOLD, h->PIDX, HOME,
+                           h->SLOT, h->ctr, N);
+        HIPCHK(hipGetLastError());
+        dfree(h, h->BPARTNER); dfree(h, h->BPAR); dfree(h, h->BINT); dfree(h, h->BCOLD); dfree(h, h->PIDX);
+        h->BPARTNER = nP; h->BPAR = nPar; h->BINT = nInt; h->BCOLD = nCold; h->PIDX = nPidx;
+    }
+    hipLaunchKernelGGL(k_iota, dim3(nblk(N)), dim3(256), 0, h->st, h->HOME[h->cur], N);
+    hipLaunchKernelGGL(k_iota, dim3(nblk(N)), dim3(256), 0, h->st, h->SLOT, N);
+    HIPCHK(hipGetLastError());
+    uint32_t tail = (uint32_t)N;
+    HIPCHK(hipMemcpyAsync(h->ctr + MG_C_TAIL, &tail, 4, hipMemcpyHostToDevice, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    h->normalized = true;
+    return rebuild_inc(h);
+}
+
+/* --------------------------------------------------------------- phases */
+int do_insert(mg_handle* h, int gene_counts)
+{
+    DS d = make_ds(h);
+    {
+        PhaseScope ps(h, MG_PH_FILL);
+        CHK(fill_u32(h, h->CNT, 0, h->C));
+        CHK(fill_u32(h, h->OFF, 0, h->C));
+        if (gene_counts) {
+            CHK(fill_u32(h, h->GCNT, 0, (size_t)MG_NUM_GENES * h->C));
+            CHK(fill_u32(h, h->GOFF, 0, (size_t)MG_NUM_GENES * h->C));
+        }
+    }
+    PhaseScope ps(h, MG_PH_INSERT);
+    hipLaunchKernelGGL(k_insert, dim3(nblk(h->N)), dim3(256), 0, h->st, d, h->g, gene_counts);
+    HIPCHK(hipGetLastError());
+    return MG_OK;
+}
+
+int grid_scan_max_index(const MgGrid& g)
+{
+    return (g.scan_max[1] * g.res[2] + g.scan_max[2]) * g.res[0] + g.scan_max[0];
+}
+
+int do_scan(mg_handle* h, int genes)
+{
+    PhaseScope ps(h, MG_PH_SCAN);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, h->st, h->CNT, h->OFF, h->C, h->ctr, 1,
+                       grid_scan_max_index(h->g), (uint32_t*)nullptr);
+    HIPCHK(hipGetLastError());
+    if (genes) {
+        hipLaunchKernelGGL(k_scan, dim3(MG_NUM_GENES), dim3(1024), 0, h->st, h->GCNT, h->GOFF,
+                           h->C, h->ctr, 0, 0, h->LENS);
+        HIPCHK(hipGetLastError());
+    }
+    return MG_OK;
+}
+
+int do_sort(mg_handle* h, int lists)
+{
+    DS d = make_ds(h);
+    {
+        PhaseScope ps(h, MG_PH_PLACE);
+        hipLaunchKernelGGL(k_place, dim3(nblk(h->N)), dim3(256), 0, h->st, d);
+        hipLaunchKernelGGL(k_cellsort, dim3(nblk(h->C)), dim3(256), 0, h->st, d);
+    }
+    {
+        PhaseScope ps(h, MG_PH_SORT);
+        const float vmax = std::max(std::max(h->g.gmax[0], h->g.gmax[1]), h->g.gmax[2]);
+        hipLaunchKernelGGL(k_sort_gather, dim3(nblk(h->N)), dim3(256), 0, h->st, d, vmax);
+        hipLaunchKernelGGL(k_sort_finish, dim3(1), dim3(64), 0, h->st, h->ctr);
+    }
+    HIPCHK(hipGetLastError());
+    h->cur = 1 - h->cur;
+    h->ccur = 1 - h->ccur;
+    h->force_zero = true;
+    h->normalized = false;
+    if (lists) {
+        PhaseScope ps(h, MG_PH_LISTS);
+        DS d2 = make_ds(h);
+        CHK(fill_u32(h, h->LISTS, MG_NONE, (size_t)MG_NUM_GENES * h->N));
+        hipLaunchKernelGGL(k_dense_lists, dim3(nblk(h->C)), dim3(256), 0, h->st, d2);
+        HIPCHK(hipGetLastError());
+    }
+    return MG_OK;
+}
+
+int ensure_force(mg_handle* h)
+{
+    if (h->force_zero) {
+        PhaseScope ps(h, MG_PH_FILL);
+        CHK(fill_u32(h, h->F, 0, (size_t)h->N * 4));
+        h->force_zero = false;
+    }
+    return MG_OK;
+}
+
+int do_pressure(mg_handle* h)
+{
+    PhaseScope ps(h, MG_PH_PRESSURE);
+    hipLaunchKernelGGL(k_pressure, dim3(nblk(h->N)), dim3(256), 0, h->st, make_ds(h), h->p, h->g);
+    HIPCHK(hipGetLastError());
+    return MG_OK;
+}
+
+int do_force(mg_handle* h)
+{
+    CHK(ensure_force(h));
+    PhaseScope ps(h, MG_PH_FORCE);
+    hipLaunchKernelGGL(k_force, dim3(nblk(h->N)), dim3(256), 0, h->st, make_ds(h), h->p, h->g, 1);
+    HIPCHK(hipGetLastError());
+    return MG_OK;
+}
+
+int do_advance(mg_handle* h, float dt)
+{
+    CHK(ensure_force(h));
+    PhaseScope ps(h, MG_PH_ADVANCE);
+    hipLaunchKernelGGL(k_advance, dim3(nblk(h->N)), dim3(256), 0, h->st, make_ds(h), h->p, dt);
+    HIPCHK(hipGetLastError());
+    return MG_OK;
+}
+
+int do_diffusion(mg_handle* h)
+{
+    PhaseScope ps(h, MG_PH_DIFFUSION);
+    hipLaunchKernelGGL(k_diffusion, dim3(nblk(h->N)), dim3(256), 0, h->st, make_ds(h), h->p, h->g, h->gen);
+    HIPCHK(hipGetLastError());
+    h->ccur = 1 - h->ccur;
+    return MG_OK;
+}
+
+int do_genes(mg_handle* h)
+{
+    PhaseScope ps(h, MG_PH_GENES);
+    hipLaunchKernelGGL(k_genes, dim3(nblk(h->N)), dim3(256), 0, h->st, make_ds(h), h->p, h->gen);
+    HIPCHK(hipGetLastError());
+    return MG_OK;
+}
+
+} // namespace
+
+/* ================================================================== C-ABI */
+extern "C" {
+
+const char* mg_last_error(void) { return g_err.c_str(); }
+
+int mg_create(mg_handle** out, int max_points, int bonds_per_particle)
+{
+    if (!out || max_points <= 0 || bonds_per_particle < 0 || bonds_per_particle > 64) {
+        set_err("mg_create: bad arguments (max_points=%d, B=%d)", max_points, bonds_per_particle);
+        return MG_ERR_ARG;
+    }
+    if ((size_t)max_points * (size_t)(bonds_per_particle > 0 ? bonds_per_particle : 1) >= 0xFFFFFFFFull) {
+        set_err("mg_create: max_points * B must fit in 32 bits");
+        return MG_ERR_ARG;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        set_err("mg_create: no HIP device");
+        return MG_ERR_HIP;
+    }
+    mg_handle* h = new mg_handle;
+    h->N = max_points;
+    h->B = bonds_per_particle;
+    std::memset(&h->p, 0, sizeof(h->p));
+    std::memset(&h->g, 0, sizeof(h->g));
+    std::memset(&h->gen, 0, sizeof(h->gen));
+    auto fail = [&](int r) { mg_destroy(h); return r; };
+    if (hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking) != hipSuccess) {
+        set_err("mg_create: hipStreamCreate failed");
+        delete h;
+        return MG_ERR_HIP;
+    }
+    const size_t N = (size_t)max_points, B = (size_t)bonds_per_particle;
+    int r = MG_OK;
+    for (int k = 0; k < 2 && r == MG_OK; k++) {
+        r = dalloc(h, &h->P[k], N);
+        if (r == MG_OK) r = dalloc(h, &h->V[k], N);
+        if (r == MG_OK) r = dalloc(h, &h->VEL[k], N);
+        if (r == MG_OK) r = dalloc(h, &h->CONC[k], N * MG_NUM_TF);
+        if (r == MG_OK) r = dalloc(h, &h->HOME[k], N);
+        if (r == MG_OK) r = dalloc(h, &h->GCELL[k], N);
+        if (r == MG_OK) r = dalloc(h, &h->GNDX[k], N);
+        if (r == MG_OK) r = dalloc(h, &h->GMASK[k], N);
+    }
+    if (r == MG_OK) r = dalloc(h, &h->F, N);
+    if (r == MG_OK) r = dalloc(h, &h->SLOT, N);
+    if (r == MG_OK) r = dalloc(h, &h->EPIGEN, N * MG_NUM_GENES);
+    if (r == MG_OK) r = dalloc(h, &h->BPARTNER, std::max<size_t>(B * N, 1));
+    if (r == MG_OK) r = dalloc(h, &h->BPAR, std::max<size_t>(B * N, 1));
+    if (r == MG_OK) r = dalloc(h, &h->BINT, std::max<size_t>(B * N, 1));
+    if (r == MG_OK) r = dalloc(h, &h->BCOLD, std::max<size_t>(B * N, 1));
+    if (r == MG_OK) r = dalloc(h, &h->PIDX, std::max<size_t>(B * N, 1));
+    if (r == MG_OK) r = dalloc(h, &h->AGE, N);
+    if (r == MG_OK) r = dalloc(h, &h->COLOR, N);
+    if (r == MG_OK) r = dalloc(h, &h->PID, N);
+    if (r == MG_OK) r = dalloc(h, &h->MRAD, N);
+    if (r == MG_OK) r = dalloc(h, &h->NERVE, N);
+    if (r == MG_OK) r = dalloc(h, &h->FBIN, N);
+    if (r == MG_OK) r = dalloc(h, &h->INC_OFF, N + 1);
+    if (r == MG_OK) r = dalloc(h, &h->PERM, N);
+    if (r == MG_OK) r = dalloc(h, &h->LISTS, N * MG_NUM_GENES);
+    if (r == MG_OK) r = dalloc(h, &h->LENS, MG_NUM_GENES);
+    if (r == MG_OK) r = dalloc(h, &h->ctr, MG_C_COUNT);
+    if (r != MG_OK) return fail(r);
+    /* initial state: what the reference's host allocation leaves (zeroed buffers, sentinel
+     * bonds), every slot its own home, no sort yet */
+    const int n = max_points;
+    for (int k = 0; k < 2; k++) {
+        fill_u32(h, h->P[k], 0, N * 4);
+        fill_u32(h, h->V[k], 0, N * 4);
+        fill_u32(h, h->VEL[k], 0, N * 4);
+        fill_u32(h, h->CONC[k], 0, N * MG_NUM_TF);
+        fill_u32(h, h->GCELL[k], MG_GRID_UNDEF, N);
+        fill_u32(h, h->GNDX[k], 0, N);
+        fill_u32(h, h->GMASK[k], 0, N);
+    }
+    fill_u32(h, h->F, 0, N * 4);
+    fill_u32(h, h->EPIGEN, 0, N * MG_NUM_GENES);
+    fill_u32(h, h->BPARTNER, MG_NONE, B * N);
+    fill_u32(h, h->BPAR, MG_NONE, B * N * 4);
+    fill_u32(h, h->BINT, MG_NONE, B * N * 2);
+    fill_u32(h, h->BCOLD, MG_NONE, B * N * 2);
+    fill_u32(h, h->PIDX, MG_NONE, B * N * 2);
+    fill_u32(h, h->AGE, 0, N);
+    fill_u32(h, h->COLOR, 0, N);
+    fill_u32(h, h->PID, MG_NONE, N);
+    fill_u32(h, h->MRAD, 0, N);
+    fill_u32(h, h->NERVE, 0, N);
+    fill_u32(h, h->FBIN, 0, N);
+    fill_u32(h, h->INC_OFF, 0, N + 1);
+    fill_u32(h, h->LISTS, MG_NONE, N * MG_NUM_GENES);
+    fill_u32(h, h->LENS, 0, MG_NUM_GENES);
+    hipLaunchKernelGGL(k_iota, dim3(nblk(n)), dim3(256), 0, h->st, h->HOME[0], n);
+    hipLaunchKernelGGL(k_iota, dim3(nblk(n)), dim3(256), 0, h->st, h->SLOT, n);
+    uint32_t c0[MG_C_COUNT] = {0};
+    c0[MG_C_NUM_POINTS] = (uint32_t)n;
+    c0[MG_C_ACTIVE] = (uint32_t)n;
+    c0[MG_C_TAIL] = (uint32_t)n;
+    if (hipMemcpyAsync(h->ctr, c0, sizeof(c0), hipMemcpyHostToDevice, h->st) != hipSuccess ||
+        hipStreamSynchronize(h->st) != hipSuccess) {
+        set_err("mg_create: initialisation failed");
+        return fail(MG_ERR_HIP);
+    }
+    *out = h;
+    return MG_OK;
+}
+
+int mg_destroy(mg_handle* h)
+{
+    if (!h) return MG_ERR_ARG;
+    if (h->st) (void)hipStreamSynchronize(h->st);
+    for (void* q : h->allocs) (void)hipFree(q);
+    h->allocs.clear();
+    for (hipEvent_t e : h->pool) (void)hipEventDestroy(e);
+    h->pool.clear();
+    if (h->st) (void)hipStreamDestroy(h->st);
+    delete h;
+    return MG_OK;
+}
+
+int mg_set_grid(mg_handle* h, const MgGrid* g)
+{
+    if (!h || !g || g->total <= 0 || g->adj_cnt <= 0 || g->adj_cnt > MG_MAX_ADJ) {
+        set_err("mg_set_grid: bad grid");
+        return MG_ERR_ARG;
+    }
+    for (int k = 0; k < 3; k++)
+        if (g->scan_max[k] < 1 || g->scan_max[k] + g->srch > g->res[k]) {
+            set_err("mg_set_grid: scan window outside the grid");
+            return MG_ERR_ARG;
+        }
+    if (g->res[0] * g->res[1] * g->res[2] != g->total) {
+        set_err("mg_set_grid: total != res.x*res.y*res.z");
+        return MG_ERR_ARG;
+    }
+    if (h->have_grid && g->total != h->C) {
+        dfree(h, h->CNT); dfree(h, h->OFF); dfree(h, h->GCNT); dfree(h, h->GOFF);
+        h->have_grid = false;
+    }
+    if (!h->have_grid) {
+        h->C = g->total;
+        CHK(dalloc(h, &h->CNT, h->C));
+        CHK(dalloc(h, &h->OFF, h->C));
+        CHK(dalloc(h, &h->GCNT, (size_t)MG_NUM_GENES * h->C));
+        CHK(dalloc(h, &h->GOFF, (size_t)MG_NUM_GENES * h->C));
+        CHK(fill_u32(h, h->CNT, 0, h->C));
+        CHK(fill_u32(h, h->OFF, 0, h->C));
+        CHK(fill_u32(h, h->GCNT, 0, (size_t)MG_NUM_GENES * h->C));
+        CHK(fill_u32(h, h->GOFF, 0, (size_t)MG_NUM_GENES * h->C));
+    }
+    h->g = *g;
+    h->have_grid = true;
+    return MG_OK;
+}
+
+int mg_fluid_setup(mg_handle* h, int num, int gsrch, const int32_t res[3], const float size[3],
+                   const float delta[3], const float gmin[3], const float gmax[3], int total, int chk)
+{
+    if (!h || !res || !size || !delta || !gmin || !gmax) { set_err("mg_fluid_setup: null argument"); return MG_ERR_ARG; }
+    if (num != h->N) { set_err("mg_fluid_setup: num (%d) != max_points (%d)", num, h->N); return MG_ERR_ARG; }
+    MgGrid g;
+    if (mg_grid_from_setup(&g, gsrch, res, size, delta, gmin, gmax, total, chk) != MG_OK) {
+        set_err("mg_fluid_setup: bad gsrch");
+        return MG_ERR_ARG;
+    }
+    return mg_set_grid(h, &g);
+}
+
+int mg_set_params(mg_handle* h, const MgParams* p)
+{
+    if (!h || !p) { set_err("mg_set_params: null argument"); return MG_ERR_ARG; }
+    h->p = *p;
+    h->have_params = true;
+    return MG_OK;
+}
+
+int mg_fluid_param(mg_handle* h, float ss, float sr, float pr, float mass, float rest,
+                   const float bmin[3], const float bmax[3], float estiff, float istiff, float visc,
+                   float surface_t, float damp, float fmin, float fmax, float ffreq, float gslope,
+                   float gx, float gy, float gz, float al, float vl, float a_f, float a_p)
+{
+    if (!h || !bmin || !bmax) { set_err("mg_fluid_param: null argument"); return MG_ERR_ARG; }
+    MgParams p;
+    mg_params_from_fluid_param(&p, ss, sr, pr, mass, rest, bmin, bmax, estiff, istiff, visc,
+                               surface_t, damp, fmin, fmax, ffreq, gslope, gx, gy, gz, al, vl, a_f, a_p);
+    if (h->have_params) {
+        p.freeze = h->p.freeze;
+        p.diff_scale = h->p.diff_scale;
+        p.secrete_scale = h->p.secrete_scale;
+    }
+    return mg_set_params(h, &p);
+}
+
+int mg_set_genome(mg_handle* h, const MgGenome* g)
+{
+    if (!h || !g) { set_err("mg_set_genome: null argument"); return MG_ERR_ARG; }
+    h->gen = *g;
+    return MG_OK;
+}
+
+int mg_set_freeze(mg_handle* h, int freeze)
+{
+    if (!h) { set_err("mg_set_freeze: null handle"); return MG_ERR_ARG; }
+    h->p.freeze = freeze ? 1 : 0;
+    return MG_OK;
+}
+
+int mg_upload(mg_handle* h, int buf_id, const void* src, size_t bytes)
+{
+    if (!h || (!src && bytes)) { set_err("mg_upload: null argument"); return MG_ERR_ARG; }
+    const int N = h->N, B = h->B;
+    if (!h->have_grid && mg_bytes_per_particle(buf_id, B) == 0) {
+        set_err("mg_upload: grid buffer %d before mg_set_grid", buf_id);
+        return MG_ERR_STATE;
+    }
+    if (!mg_buffer_known(buf_id)) { set_err("mg_upload: unsupported buffer id %d", buf_id); return MG_ERR_UNSUPPORTED; }
+    const size_t need = mg_buffer_bytes(buf_id, N, h->C, B);
+    if (bytes != need) { set_err("mg_upload: buffer %d needs %zu bytes, got %zu", buf_id, need, bytes); return MG_ERR_SIZE; }
+    if (need == 0) return MG_OK;
+    CHK(normalize(h));
+    CHK(staging(h, bytes));
+    HIPCHK(hipMemcpyAsync(h->staging, src, bytes, hipMemcpyHostToDevice, h->st));
+    const int c = h->cur;
+    const unsigned nb = nblk(N);
+    void* stg = h->staging;
+    switch (buf_id) {
+    case MG_FPOS: hipLaunchKernelGGL(k_set_xyz, dim3(nb), dim3(256), 0, h->st, h->P[c], (const float4*)stg, N); break;
+    case MG_FVEL: hipLaunchKernelGGL(k_set_xyz, dim3(nb), dim3(256), 0, h->st, h->VEL[c], (const float4*)stg, N); break;
+    case MG_FVEVAL: hipLaunchKernelGGL(k_set_xyz, dim3(nb), dim3(256), 0, h->st, h->V[c], (const float4*)stg, N); break;
+    case MG_FFORCE:
+        hipLaunchKernelGGL(k_set_xyz, dim3(nb), dim3(256), 0, h->st, h->F, (const float4*)stg, N);
+        h->force_zero = false;
+        break;
+    case MG_FPRESS: hipLaunchKernelGGL(k_set_w, dim3(nb), dim3(256), 0, h->st, h->P[c], (const float*)stg, N); break;
+    case MG_FDENSITY: hipLaunchKernelGGL(k_set_w, dim3(nb), dim3(256), 0, h->st, h->V[c], (const float*)stg, N); break;
+    case MG_FAGE: HIPCHK(hipMemcpyAsync(h->AGE, stg, bytes, hipMemcpyDeviceToDevice, h->st)); break;
+    case MG_FCOLOR: HIPCHK(hipMemcpyAsync(h->COLOR, stg, bytes, hipMemcpyDeviceToDevice, h->st)); break;
+    case MG_FGCELL: HIPCHK(hipMemcpyAsync(h->GCELL[c], stg, bytes, hipMemcpyDeviceToDevice, h->st)); break;
+    case MG_FGNDX: HIPCHK(hipMemcpyAsync(h->GNDX[c], stg, bytes, hipMemcpyDeviceToDevice, h->st)); break;
+    case MG_FNERVEIDX: HIPCHK(hipMemcpyAsync(h->NERVE, stg, bytes, hipMemcpyDeviceToDevice, h->st)); break;
+    case MG_FPARTICLE_ID: HIPCHK(hipMemcpyAsync(h->PID, stg, bytes, hipMemcpyDeviceToDevice, h->st)); break;
+    case MG_FMASS_RADIUS: HIPCHK(hipMemcpyAsync(h->MRAD, stg, bytes, hipMemcpyDeviceToDevice, h->st)); break;
+    case MG_FBIN: {
+        HIPCHK(hipMemcpyAsync(h->FBIN, stg, bytes, hipMemcpyDeviceToDevice, h->st));
+        uint32_t z = 0;
+        HIPCHK(hipMemcpyAsync(h->ctr + MG_C_HWM, &z, 4, hipMemcpyHostToDevice, h->st));
+        HIPCHK(hipStreamSynchronize(h->st));
+        break;
+    }
+    case MG_FCONC: HIPCHK(hipMemcpyAsync(h->CONC[h->ccur], stg, bytes, hipMemcpyDeviceToDevice, h->st)); break;
+    case MG_FEPIGEN: HIPCHK(hipMemcpyAsync(h->EPIGEN, stg, bytes, hipMemcpyDeviceToDevice, h->st)); break;
+    case MG_FELASTIDX:
+        if (B > 0) {
+            hipLaunchKernelGGL(k_upload_bonds, dim3(nb, B), dim3(256), 0, h->st, h->BPARTNER, h->BPAR,
+                               h->BINT, h->BCOLD, (const uint32_t*)stg, N, B);
+            HIPCHK(hipGetLastError());
+            CHK(rebuild_inc(h));
+        }
+        break;
+    case MG_FPARTICLEIDX:
+        if (B > 0)
+            hipLaunchKernelGGL(k_upload_pidx, dim3(nb, B), dim3(256), 0, h->st, h->PIDX, (const uint32_t*)stg, N, B);
+        break;
+    case MG_FBIN_COUNT: HIPCHK(hipMemcpyAsync(h->CNT, stg, bytes, hipMemcpyDeviceToDevice, h->st)); break;
+    case MG_FBIN_OFFSET: HIPCHK(hipMemcpyAsync(h->OFF, stg, bytes, hipMemcpyDeviceToDevice, h->st)); break;
+    case MG_FBIN_COUNT_ACTIVE_GENES: HIPCHK(hipMemcpyAsync(h->GCNT, stg, bytes, hipMemcpyDeviceToDevice, h->st)); break;
+    case MG_FBIN_OFFSET_ACTIVE_GENES: HIPCHK(hipMemcpyAsync(h->GOFF, stg, bytes, hipMemcpyDeviceToDevice, h->st)); break;
+    case MG_FDENSE_LIST_LENGTHS: HIPCHK(hipMemcpyAsync(h->LENS, stg, bytes, hipMemcpyDeviceToDevice, h->st)); break;
+    case MG_FDENSE_LISTS: HIPCHK(hipMemcpyAsync(h->LISTS, stg, bytes, hipMemcpyDeviceToDevice, h->st)); break;
+    default: set_err("mg_upload: unsupported buffer id %d", buf_id); return MG_ERR_UNSUPPORTED;
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(h->st));
+    return MG_OK;
+}
+
+int mg_download(mg_handle* h, int buf_id, void* dst, size_t bytes)
+{
+    if (!h || (!dst && bytes)) { set_err("mg_download: null argument"); return MG_ERR_ARG; }
+    const int N = h->N, B = h->B;
+    if (!h->have_grid && mg_bytes_per_particle(buf_id, B) == 0) {
+        set_err("mg_download: grid buffer %d before mg_set_grid", buf_id);
+        return MG_ERR_STATE;
+    }
+    if (!mg_buffer_known(buf_id)) { set_err("mg_download: unsupported buffer id %d", buf_id); return MG_ERR_UNSUPPORTED; }
+    const size_t need = mg_buffer_bytes(buf_id, N, h->C, B);
+    if (bytes != need) { set_err("mg_download: buffer %d needs %zu bytes, got %zu", buf_id, need, bytes); return MG_ERR_SIZE; }
+    if (need == 0) return MG_OK;
+    CHK(staging(h, bytes));
+    const int c = h->cur;
+    const unsigned nb = nblk(N);
+    void* stg = h->staging;
+    const uint32_t* HOME = h->HOME[c];
+    const void* direct = nullptr;
+    switch (buf_id) {
+    case MG_FPOS: hipLaunchKernelGGL(k_get_xyz, dim3(nb), dim3(256), 0, h->st, (float4*)stg, h->P[c], N, 0); break;
+    case MG_FVEL: hipLaunchKernelGGL(k_get_xyz, dim3(nb), dim3(256), 0, h->st, (float4*)stg, h->VEL[c], N, 0); break;
+    case MG_FVEVAL: hipLaunchKernelGGL(k_get_xyz, dim3(nb), dim3(256), 0, h->st, (float4*)stg, h->V[c], N, 0); break;
+    case MG_FFORCE: hipLaunchKernelGGL(k_get_xyz, dim3(nb), dim3(256), 0, h->st, (float4*)stg, h->F, N, h->force_zero ? 1 : 0); break;
+    case MG_FPRESS: hipLaunchKernelGGL(k_get_w, dim3(nb), dim3(256), 0, h->st, (float*)stg, h->P[c], N); break;
+    case MG_FDENSITY: hipLaunchKernelGGL(k_get_w, dim3(nb), dim3(256), 0, h->st, (float*)stg, h->V[c], N); break;
+    case MG_FAGE: hipLaunchKernelGGL(k_gather_home, dim3(nb), dim3(256), 0, h->st, (uint32_t*)stg, h->AGE, HOME, h->ctr, N, 1, 0u); break;
+    case MG_FCOLOR: hipLaunchKernelGGL(k_gather_home, dim3(nb), dim3(256), 0, h->st, (uint32_t*)stg, h->COLOR, HOME, h->ctr, N, 1, 0u); break;
+    case MG_FNERVEIDX: hipLaunchKernelGGL(k_gather_home, dim3(nb), dim3(256), 0, h->st, (uint32_t*)stg, h->NERVE, HOME, h->ctr, N, 1, 0u); break;
+    case MG_FMASS_RADIUS: hipLaunchKernelGGL(k_gather_home, dim3(nb), dim3(256), 0, h->st, (uint32_t*)stg, h->MRAD, HOME, h->ctr, N, 1, 0u); break;
+    case MG_FPARTICLE_ID: hipLaunchKernelGGL(k_gather_home, dim3(nb), dim3(256), 0, h->st, (uint32_t*)stg, h->PID, HOME, h->ctr, N, 0, MG_NONE); break;
+    case MG_FGCELL: direct = h->GCELL[c]; break;
+    case MG_FGNDX: direct = h->GNDX[c]; break;
+    case MG_FBIN: hipLaunchKernelGGL(k_download_fbin, dim3(nb), dim3(256), 0, h->st, (uint32_t*)stg, h->FBIN, h->ctr, N); break;
+    case MG_FCONC: direct = h->CONC[h->ccur]; break;
+    case MG_FEPIGEN:
+        hipLaunchKernelGGL(k_gather_epigen, dim3(nb, MG_NUM_GENES), dim3(256), 0, h->st, (uint32_t*)stg, h->EPIGEN, HOME, h->ctr, N);
+        break;
+    case MG_FELASTIDX:
+        if (B > 0)
+            hipLaunchKernelGGL(k_download_bonds, dim3(nb, B), dim3(256), 0, h->st, (uint32_t*)stg, h->BPARTNER,
+                               h->BPAR, h->BINT, h->BCOLD, HOME, h->SLOT, h->ctr, N, B);
+        break;
+    case MG_FPARTICLEIDX:
+        if (B > 0)
+            hipLaunchKernelGGL(k_download_pidx, dim3(nb, B), dim3(256), 0, h->st, (uint32_t*)stg, h->PIDX, HOME,
+                               h->SLOT, h->ctr, N, B);
+        break;
+    case MG_FBIN_COUNT: direct = h->CNT; break;
+    case MG_FBIN_OFFSET: direct = h->OFF; break;
+    case MG_FBIN_COUNT_ACTIVE_GENES: direct = h->GCNT; break;
+    case MG_FBIN_OFFSET_ACTIVE_GENES: direct = h->GOFF; break;
+    case MG_FDENSE_LIST_LENGTHS: direct = h->LENS; break;
+    case MG_FDENSE_LISTS: direct = h->LISTS; break;
+    default: set_err("mg_download: unsupported buffer id %d", buf_id); return MG_ERR_UNSUPPORTED;
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(dst, direct ? direct : stg, bytes, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    return MG_OK;
+}
+
+int mg_get_counts(mg_handle* h, uint32_t* num_points, uint32_t* active_points)
+{
+    if (!h) { set_err("mg_get_counts: null handle"); return MG_ERR_ARG; }
+    uint32_t c[2];
+    HIPCHK(hipMemcpyAsync(c, h->ctr, 8, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    if (num_points) *num_points = c[0];
+    if (active_points) *active_points = c[1];
+    return MG_OK;
+}
+
+int mg_sync(mg_handle* h)
+{
+    if (!h) { set_err("mg_sync: null handle"); return MG_ERR_ARG; }
+    HIPCHK(hipStreamSynchronize(h->st));
+    return MG_OK;
+}
+
+int mg_insert_particles(mg_handle* h, uint32_t* gcell, uint32_t* gndx, uint32_t* gcnt)
+{
+    CHK(ready(h));
+    CHK(do_insert(h, 1));
+    if (gcell) HIPCHK(hipMemcpyAsync(gcell, h->GCELL[h->cur], (size_t)h->N * 4, hipMemcpyDeviceToHost, h->st));
+    if (gndx) HIPCHK(hipMemcpyAsync(gndx, h->GNDX[h->cur], (size_t)h->N * 4, hipMemcpyDeviceToHost, h->st));
+    if (gcnt) HIPCHK(hipMemcpyAsync(gcnt, h->CNT, (size_t)h->C * 4, hipMemcpyDeviceToHost, h->st));
+    if (gcell || gndx || gcnt) HIPCHK(hipStreamSynchronize(h->st));
+    return MG_OK;
+}
+
+int mg_prefix_sum_cells(mg_handle* h, int zero_offsets)
+{
+    CHK(ready(h));
+    if (zero_offsets != 1) {
+        set_err("mg_prefix_sum_cells: only zero_offsets=1 (exclusive offsets) is used by the step");
+        return MG_ERR_UNSUPPORTED;
+    }
+    return do_scan(h, 1);
+}
+
+int mg_counting_sort_full(mg_handle* h)
+{
+    CHK(ready(h));
+    return do_sort(h, 1);
+}
+
+int mg_compute_pressure(mg_handle* h) { CHK(ready(h)); return do_pressure(h); }
+int mg_compute_force(mg_handle* h) { CHK(ready(h)); return do_force(h); }
+int mg_advance(mg_handle* h, float time, float dt, float ss)
+{
+    (void)time; (void)ss;
+    CHK(ready(h));
+    return do_advance(h, dt);
+}
+int mg_compute_diffusion(mg_handle* h) { CHK(ready(h)); return do_diffusion(h); }
+int mg_compute_genes(mg_handle* h) { CHK(ready(h)); return do_genes(h); }
+int mg_zero_vel(mg_handle* h)
+{
+    CHK(ready(h));
+    hipLaunchKernelGGL(k_zero_vel, dim3(nblk(h->N)), dim3(256), 0, h->st, make_ds(h));
+    HIPCHK(hipGetLastError());
+    return MG_OK;
+}
+
+int mg_physical_sort(mg_handle* h)
+{
+    CHK(ready(h));
+    CHK(do_insert(h, 1));
+    CHK(do_scan(h, 1));
+    return do_sort(h, 1);
+}
+
+int mg_inner_physical_loop(mg_handle* h, float time, float dt)
+{
+    (void)time;
+    CHK(ready(h));
+    CHK(do_pressure(h));
+    CHK(do_force(h));
+    return do_advance(h, dt);
+}
+
+int mg_gene_action(mg_handle* h)
+{
+    CHK(ready(h));
+    CHK(do_diffusion(h));
+    return do_genes(h);
+}
+
+int mg_step(mg_handle* h, int nsteps, float dt, unsigned flags)
+{
+    CHK(ready(h));
+    if (nsteps < 0) { set_err("mg_step: nsteps < 0"); return MG_ERR_ARG; }
+    for (int s = 0; s < nsteps; s++) {
+        CHK(do_pressure(h));
+        CHK(do_force(h));
+        CHK(do_advance(h, dt));
+        h->time += dt;
+        if (flags & MG_STEP_GENES) {
+            CHK(do_diffusion(h));
+            CHK(do_genes(h));
+        }
+        CHK(do_insert(h, 1));
+        CHK(do_scan(h, 1));
+        CHK(do_sort(h, 1));
+    }
+    return MG_OK;
+}
+
+int mg_step_timed(mg_handle* h, int nsteps, float dt, unsigned flags, double* ms_out, int* calls_out)
+{
+    CHK(ready(h));
+    h->timing = true;
+    h->marks.clear();
+    h->pool_used = 0;
+    int r = mg_step(h, nsteps, dt, flags & ~MG_STEP_GRAPH);
+    h->timing = false;
+    if (r != MG_OK) return r;
+    HIPCHK(hipStreamSynchronize(h->st));
+    for (const auto& m : h->marks) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, m.a, m.b));
+        if (ms_out) ms_out[m.ph] += ms;
+        if (calls_out) calls_out[m.ph] += 1;
+    }
+    h->marks.clear();
+    return MG_OK;
+}
+
+} // extern "C"

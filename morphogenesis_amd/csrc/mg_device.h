@@ -1,0 +1,130 @@
+/*
+ * mg_device.h — device-side state and arithmetic shared by the step kernels.
+ *
+ * HBM layout (internal; the reference layout exists only at mg_upload/mg_download):
+ *   sorted, ping-ponged per step (index = current sorted slot s):
+ *     P[s]    float4 (pos.xyz, pressure)        V[s]   float4 (veval.xyz, 1/density)
+ *     VEL[s]  float4 (vel.xyz, 0)               CONC[s][16] float
+ *     HOME[s] creation slot of the occupant     GCELL[s], GNDX[s], GMASK[s] (gene bits)
+ *   home-indexed (index = creation slot h, never moved by the sort):
+ *     SLOT[h] current sorted slot or MG_NONE    EPIGEN[g][h]
+ *     BPARTNER[a][h] partner's home (slot-major) BPAR[a][h] float4 (limit, rest, modulus, damping)
+ *     BINT[a][h] float2 (strain^2, strain)      BCOLD[a][h] uint2 (partner ID, change bits)
+ *     PIDX[a][h] uint2 (incoming owner's home, owner slot), AGE/COLOR/PID/MRAD/NERVE[h]
+ *     INC_OFF[h]..INC_OFF[h+1]: incoming bonds (owner_home*B + slot), ascending.
+ */
+#ifndef MG_DEVICE_H
+#define MG_DEVICE_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <cfloat>
+
+#include "../../include/mg_types.h"
+
+/* device counters */
+enum {
+    MG_C_NUM_POINTS = 0,  /* mNumPoints of the last sort                              */
+    MG_C_ACTIVE = 1,      /* mActivePoints of the last sort                           */
+    MG_C_TAIL = 2,        /* slots >= this hold the sort's reset tail                 */
+    MG_C_HWM = 3,         /* FBIN[s] == s for s below this                            */
+    MG_C_LISTS = 4,       /* 1 once a sort has built the gene membership              */
+    MG_C_SCAN_NUM = 5,    /* FBIN_OFFSET[C-1] of the last scan (committed by the sort) */
+    MG_C_SCAN_ACTIVE = 6, /* FBIN_OFFSET[grid_ScanMax] of the last scan               */
+    MG_C_COUNT = 8
+};
+
+struct DS {
+    /* sorted, current */
+    float4 *P, *V, *VEL, *F;
+    float* CONC;
+    uint32_t *HOME, *GCELL, *GNDX, *GMASK;
+    /* sorted, other half of the ping-pong */
+    float4 *P2, *V2, *VEL2;
+    float* CONC2;
+    uint32_t *HOME2, *GCELL2, *GNDX2, *GMASK2;
+    /* home-indexed */
+    uint32_t *SLOT, *EPIGEN, *BPARTNER;
+    float4* BPAR;
+    float2* BINT;
+    uint32_t *INC_OFF, *INC;
+    /* grid */
+    uint32_t *CNT, *OFF, *GCNT, *GOFF, *PERM, *LISTS, *LENS;
+    uint32_t* ctr;
+    int N, B, C;
+};
+
+/* ---------------------------------------------------------------- arithmetic
+ * The expression trees below follow the reference kernels operation by operation; the
+ * oracle (oracle/mg_oracle.cpp) evaluates the same trees. Compiled with
+ * -ffp-contract=off and correctly rounded division/sqrt, so FP32 results are the
+ * IEEE results of these trees. */
+
+struct f3 {
+    float x, y, z;
+};
+__device__ __forceinline__ f3 mk3(float x, float y, float z) { return f3{x, y, z}; }
+__device__ __forceinline__ f3 xyz(float4 a) { return f3{a.x, a.y, a.z}; }
+__device__ __forceinline__ f3 operator+(f3 a, f3 b) { return mk3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ f3 operator-(f3 a, f3 b) { return mk3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ f3 operator*(f3 a, float s) { return mk3(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ f3 operator*(float s, f3 a) { return mk3(a.x * s, a.y * s, a.z * s); }
+/* src/cutil_math.h:389-393: float3 / float multiplies by the reciprocal */
+__device__ __forceinline__ f3 divs(f3 a, float s)
+{
+    float inv = 1.0f / s;
+    return a * inv;
+}
+__device__ __forceinline__ float dot3(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ bool has_nan3(f3 a) { return a.x != a.x || a.y != a.y || a.z != a.z; }
+
+/* One elastic bond seen from its owner o towards partner p
+ * (src/fluid_system_cuda.cu:3263-3314). Returns the acceleration term eterm and the
+ * strain; `apply` is false when eterm has a NaN component (the reference skips it). */
+__device__ __forceinline__ f3 bond_term(f3 opos, f3 oveval, f3 ppos, f3 pveval, float4 bp,
+                                        float pmass, float* strain_out, float* abs_dist_out)
+{
+    const float elastic_limit = bp.x, restlength = bp.y, modulus = bp.z, damping_coeff = bp.w;
+    f3 dist = opos - ppos;
+    float dsq = dist.x * dist.x + dist.y * dist.y + dist.z * dist.z;
+    float abs_dist = sqrtf(dsq) + FLT_MIN;
+    f3 rel_vel = pveval - oveval;
+    float spring_strain = fmaxf(0.0f, (abs_dist - restlength) / restlength);
+    float fac = (float)(abs_dist < elastic_limit ? 1 : 0);
+    f3 e1 = divs(dist, abs_dist) * spring_strain * modulus;
+    f3 e2 = damping_coeff * rel_vel;
+    f3 eterm = divs(fac * (e1 - e2), pmass);
+    *strain_out = spring_strain;
+    *abs_dist_out = abs_dist;
+    return eterm;
+}
+
+/* Fluid pair term for one neighbour within dsq < 1 (src/fluid_system_cuda.cu:3203-3221). */
+__device__ __forceinline__ f3 fluid_term(f3 dist, float r, float sr, float ipress, float jpress,
+                                         float idens, float jdens, f3 ivev, f3 jvev,
+                                         float surface_t)
+{
+    float t = sr - r;
+    float kern = t * t * t;
+    float press = 100.0f * (ipress + jpress);
+    float s = idens * jdens * 100.0f;
+    f3 pterm = (s * divs(dist, r)) * (press * kern - surface_t * (t * t));
+    f3 delta_v = jvev - ivev;
+    f3 vterm = (100000.0f * delta_v) * kern;
+    return pterm + vterm;
+}
+
+/* XCD-aware block index: the dispatcher places consecutive workgroups on different XCDs
+ * (round robin over 8); remap so each XCD walks one contiguous range of particles and
+ * neighbour data stays in that XCD's L2. */
+__device__ __forceinline__ uint32_t xcd_block()
+{
+    const uint32_t nb = gridDim.x, b = blockIdx.x;
+    const uint32_t per = nb / 8u, rem = nb % 8u;
+    const uint32_t x = b % 8u, k = b / 8u;
+    /* XCD x owns per (+1 if x < rem) consecutive logical blocks */
+    return x * per + (x < rem ? x : rem) + k;
+}
+
+#endif

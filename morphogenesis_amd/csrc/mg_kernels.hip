@@ -1,0 +1,553 @@
+/*
+ * mg_kernels.hip — gfx950 kernels of the Morphogenesis per-timestep SPH step.
+ *
+ * One thread per particle (or per cell) with wave64-aware access patterns; every kernel
+ * reads the live particle counts from device counters, so a step needs no host sync.
+ * The arithmetic follows the reference kernels in src/fluid_system_cuda.cu (cited per
+ * kernel) with the deterministic order described in mg_device.h.
+ */
+#include "mg_device.h"
+
+#include <cfloat>
+#include <climits>
+
+#define LAUNCH_BOUNDS_256 __launch_bounds__(256)
+
+/* ------------------------------------------------------------------ fills */
+__global__ LAUNCH_BOUNDS_256 void k_fill_u32(uint32_t* p, uint32_t v, size_t n)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = v;
+}
+
+/* -------------------------------------------------------------- insert (a4)
+ * insertParticles, src/fluid_system_cuda.cu:53-100. Lanes of one wavefront that fall in the
+ * same cell (the input is cell-sorted from the previous step, so they are contiguous)
+ * are counted with one atomicAdd per run: the run head adds the run length and
+ * broadcasts the base. FGNDX receives a slot in [0, count) exactly as the reference's
+ * atomicAdd does; k_cellsort later replaces it with the stable rank. */
+__global__ LAUNCH_BOUNDS_256 void k_insert(DS d, MgGrid g, int gene_counts)
+{
+    const uint32_t s = xcd_block() * blockDim.x + threadIdx.x;
+    const bool valid = s < (uint32_t)d.N;
+    const uint32_t lane = __lane_id();
+    uint32_t key = 0xFFFFFFFFu;
+    if (valid) {
+        float4 p = d.P[s];
+        float gx = (p.x - g.gmin[0]) * g.delta[0];
+        float gy = (p.y - g.gmin[1]) * g.delta[1];
+        float gz = (p.z - g.gmin[2]) * g.delta[2];
+        int cx = __float2int_rz(gx), cy = __float2int_rz(gy), cz = __float2int_rz(gz);
+        if (cx >= 1 && cx <= g.scan_max[0] && cy >= 1 && cy <= g.scan_max[1] && cz >= 1 &&
+            cz <= g.scan_max[2]) {
+            key = (uint32_t)((cy * g.res[2] + cz) * g.res[0] + cx);
+        } else {
+            d.GCELL[s] = MG_GRID_UNDEF;
+            if (s < d.ctr[MG_C_TAIL]) d.SLOT[d.HOME[s]] = MG_NONE; /* particle leaves the model */
+        }
+    }
+    /* run-length aggregation of equal keys across the wavefront */
+    uint32_t prev = __shfl_up(key, 1);
+    bool head = (lane == 0) || (prev != key);
+    uint64_t heads = __ballot(head);
+    uint64_t le = heads & ((2ull << lane) - 1ull);
+    uint32_t start = 63u - (uint32_t)__clzll((long long)le);
+    uint64_t gt = heads & ~((2ull << lane) - 1ull);
+    uint32_t next = gt ? (uint32_t)(__ffsll((long long)gt) - 1) : 64u;
+    uint32_t base = 0;
+    if (head && key != 0xFFFFFFFFu) base = atomicAdd(&d.CNT[key], next - start);
+    base = __shfl(base, (int)start);
+    if (key != 0xFFFFFFFFu) {
+        d.GCELL[s] = key;
+        d.GNDX[s] = base + (lane - start);
+        if (gene_counts) {
+            const uint32_t h = d.HOME[s];
+            for (int gene = 0; gene < MG_NUM_GENES; gene++)
+                if (d.EPIGEN[(size_t)gene * d.N + h] > 0u)
+                    atomicAdd(&d.GCNT[(size_t)gene * d.C + key], 1u);
+        }
+    }
+}
+
+/* ----------------------------------------------------------- scan (a5, a6)
+ * Exclusive prefix sum (prefixSum/prefixFixup with zero_offsets=1,
+ * src/fluid_system_cuda.cu:136-183, driven by src/host_CUDA.cpp:307-381). One 1024-thread
+ * block per segment (blockIdx.x); each thread owns a contiguous run of the segment.
+ * Segment 0 with `cell_counters` set also records OFF[C-1] and OFF[grid_ScanMax], which the
+ * sort commits as mNumPoints and mActivePoints (src/host_CUDA.cpp:501-511). `lens`, when given,
+ * receives count[C-1] + offset[C-1] per segment (tally_denselist_lengths,
+ * src/fluid_system_cuda.cu:185-194). */
+__global__ __launch_bounds__(1024) void k_scan(const uint32_t* in, uint32_t* out, int n,
+                                               uint32_t* ctr, int cell_counters, int gsm,
+                                               uint32_t* lens)
+{
+    __shared__ uint32_t wsum[16];
+    __shared__ uint32_t carry_s;
+    const int seg = blockIdx.x;
+    in += (size_t)seg * n;
+    out += (size_t)seg * n;
+    const int t = threadIdx.x;
+    const int per = (n + 1023) / 1024;
+    const int b0 = t * per;
+    const int b1 = min(n, b0 + per);
+    uint32_t local = 0;
+    for (int i = b0; i < b1; i++) local += in[i];
+    /* block exclusive scan of the per-thread sums */
+    const int lane = t & 63, w = t >> 6;
+    uint32_t x = local;
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    if (t == 0) {
+        uint32_t run = 0;
+        for (int k = 0; k < 16; k++) { uint32_t v = wsum[k]; wsum[k] = run; run += v; }
+        carry_s = run;
+    }
+    __syncthreads();
+    uint32_t run = wsum[w] + x - local;
+    for (int i = b0; i < b1; i++) { uint32_t v = in[i]; out[i] = run; run += v; }
+    __syncthreads();
+    if (t == 0) {
+        if (lens) lens[seg] = in[n - 1] + out[n - 1];
+        if (cell_counters) {
+            ctr[MG_C_SCAN_NUM] = out[n - 1];
+            ctr[MG_C_SCAN_ACTIVE] = out[gsm];
+        }
+    }
+    (void)carry_s;
+}
+
+/* ------------------------------------------------------ stable placement
+ * PERM[OFF[cell] + slot] = s, then each cell's run of PERM is sorted ascending so the
+ * intra-cell order is the pre-sort index order; FGNDX becomes that rank. */
+__global__ LAUNCH_BOUNDS_256 void k_place(DS d)
+{
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= (uint32_t)d.N) return;
+    const uint32_t c = d.GCELL[s];
+    if (c == MG_GRID_UNDEF) return;
+    d.PERM[d.OFF[c] + d.GNDX[s]] = s;
+}
+
+__global__ LAUNCH_BOUNDS_256 void k_cellsort(DS d)
+{
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= (uint32_t)d.C) return;
+    const uint32_t n = d.CNT[c];
+    if (n == 0) return;
+    uint32_t* run = d.PERM + d.OFF[c];
+    for (uint32_t i = 1; i < n; i++) {
+        uint32_t v = run[i];
+        uint32_t j = i;
+        while (j > 0 && run[j - 1] > v) { run[j] = run[j - 1]; j--; }
+        run[j] = v;
+    }
+    for (uint32_t r = 0; r < n; r++) d.GNDX[run[r]] = r;
+}
+
+/* ------------------------------------------------------- counting sort (a7)
+ * countingSortFull (src/fluid_system_cuda.cu:196-296) with the host resets of
+ * src/host_CL.cpp:1156-1206, written as a gather over destination slots: slot d < mNumPoints
+ * takes the particle PERM[d]; slots past mNumPoints get the reset values (FPOS filled with
+ * max(VOLMAX), FCONC zero) and keep the stale ones the reference leaves in place. Bond
+ * records and incoming lists do not move: they are addressed by home index, and the
+ * partner indices the reference rewrites here are resolved through SLOT. */
+__global__ LAUNCH_BOUNDS_256 void k_sort_gather(DS d, float vmax)
+{
+    const uint32_t dd = xcd_block() * blockDim.x + threadIdx.x;
+    if (dd >= (uint32_t)d.N) return;
+    const uint32_t nump = d.ctr[MG_C_SCAN_NUM];
+    const float4* csrc;
+    float4* cdst = (float4*)(d.CONC2 + (size_t)dd * MG_NUM_TF);
+    if (dd < nump) {
+        const uint32_t s = d.PERM[dd];
+        const uint32_t h = d.HOME[s];
+        d.P2[dd] = d.P[s];
+        d.V2[dd] = d.V[s];
+        d.VEL2[dd] = d.VEL[s];
+        csrc = (const float4*)(d.CONC + (size_t)s * MG_NUM_TF);
+        cdst[0] = csrc[0]; cdst[1] = csrc[1]; cdst[2] = csrc[2]; cdst[3] = csrc[3];
+        d.HOME2[dd] = h;
+        d.GCELL2[dd] = d.GCELL[s];
+        d.GNDX2[dd] = d.GNDX[s];
+        uint32_t m = 0;
+        for (int gene = 0; gene < MG_NUM_GENES; gene++)
+            if (d.EPIGEN[(size_t)gene * d.N + h] > 0u) m |= 1u << gene;
+        d.GMASK2[dd] = m;
+        d.SLOT[h] = dd;
+    } else {
+        float4 p = d.P[dd];
+        d.P2[dd] = make_float4(vmax, vmax, vmax, p.w);
+        d.V2[dd] = d.V[dd];
+        d.VEL2[dd] = d.VEL[dd];
+        const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+        cdst[0] = z; cdst[1] = z; cdst[2] = z; cdst[3] = z;
+        d.HOME2[dd] = d.HOME[dd];
+        d.GCELL2[dd] = d.GCELL[dd];
+        d.GNDX2[dd] = d.GNDX[dd];
+        d.GMASK2[dd] = 0u;
+    }
+}
+
+__global__ void k_sort_finish(uint32_t* ctr)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        const uint32_t nump = ctr[MG_C_SCAN_NUM];
+        ctr[MG_C_NUM_POINTS] = nump;
+        ctr[MG_C_ACTIVE] = ctr[MG_C_SCAN_ACTIVE];
+        ctr[MG_C_TAIL] = nump;
+        if (nump > ctr[MG_C_HWM]) ctr[MG_C_HWM] = nump;
+        ctr[MG_C_LISTS] = 1u;
+    }
+}
+
+/* countingSortDenseLists (src/fluid_system_cuda.cu:329-404): per bin, particles in order,
+ * genes with a nonzero epigenetic state append the particle to the gene's list. */
+__global__ LAUNCH_BOUNDS_256 void k_dense_lists(DS d)
+{
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= (uint32_t)d.C) return;
+    const uint32_t n = d.CNT[c];
+    if (n == 0) return;
+    const uint32_t base = d.OFF[c];
+    if (base + n > (uint32_t)d.N) return;
+    uint32_t ctrg[MG_NUM_GENES];
+    for (int g = 0; g < MG_NUM_GENES; g++) ctrg[g] = 0;
+    for (uint32_t q = base; q < base + n; q++) {
+        const uint32_t m = d.GMASK[q];
+        for (int g = 0; g < MG_NUM_GENES; g++)
+            if (m & (1u << g)) {
+                uint32_t at = d.GOFF[(size_t)g * d.C + c] + ctrg[g]++;
+                if (at < (uint32_t)d.N) d.LISTS[(size_t)g * d.N + at] = q;
+            }
+    }
+}
+
+/* ------------------------------------------------------------ pressure (a9)
+ * contributePressure/computePressure, src/fluid_system_cuda.cu:620-712. Per cell partial
+ * sums are added to the running total in gridAdj order. Writes FPRESS into P.w and
+ * FDENSITY (= 1/density) into V.w. */
+__global__ LAUNCH_BOUNDS_256 void k_pressure(DS d, MgParams p, MgGrid g)
+{
+    const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
+    if (i >= d.ctr[MG_C_ACTIVE]) return;
+    uint32_t gc = d.GCELL[i];
+    if (gc == MG_GRID_UNDEF) return;
+    gc -= (uint32_t)((1 * g.res[2] + 1) * g.res[0] + 1);
+    const f3 pi = xyz(d.P[i]);
+    const float r2 = p.r2, sr = p.psmoothradius;
+    float sum = 0.0f;
+    for (int c = 0; c < g.adj_cnt; c++) {
+        const uint32_t cell = gc + (uint32_t)g.adj[c];
+        const uint32_t n = d.CNT[cell];
+        float csum = 0.0f;
+        if (n != 0) {
+            const uint32_t o = d.OFF[cell];
+            for (uint32_t j = o; j < o + n; j++) {
+                f3 dist = pi - xyz(d.P[j]);
+                float dsq = dist.x * dist.x + dist.y * dist.y + dist.z * dist.z;
+                if (dsq < r2 && dsq > 0.0f) {
+                    float r = sqrtf(dsq);
+                    float q = r / sr;
+                    float b = 1.0f - q;
+                    b *= b;
+                    b *= b;
+                    csum += b * (4.0f * q + 1.0f);
+                }
+            }
+        }
+        sum += csum;
+    }
+    sum = sum * p.pmass * p.wendlandC2kern;
+    if (sum == 0.0f) sum = 1.0f;
+    ((float*)&d.P[i])[3] = (sum - p.prest_dens) * p.pintstiff;
+    ((float*)&d.V[i])[3] = 1.0f / sum;
+}
+
+/* --------------------------------------------------------------- force (a10)
+ * contributeForce/computeForce, src/fluid_system_cuda.cu:3167-3358. The owner subtracts its
+ * bond terms in slot order, adds pmass * (sum over cells of the per-cell fluid sums); the
+ * atomicAdd scatter of bond terms to partners is replaced by each particle gathering the
+ * terms of its incoming bonds (same expression, same operands, hence the same bits) in
+ * (owner home, slot) order after its own total:
+ *   FFORCE[i] = ((FFORCE[i] + own) + in_1) + in_2 ...
+ * `accumulate` = 0 starts from zero (FFORCE was reset by the sort). */
+__global__ LAUNCH_BOUNDS_256 void k_force(DS d, MgParams p, MgGrid g, int accumulate)
+{
+    const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
+    const uint32_t na = d.ctr[MG_C_ACTIVE];
+    if (i >= na) return;
+    uint32_t gc = d.GCELL[i];
+    if (gc == MG_GRID_UNDEF) return;
+    gc -= (uint32_t)((1 * g.res[2] + 1) * g.res[0] + 1);
+    const int N = d.N, B = d.B;
+    const uint32_t h = d.HOME[i];
+    const float4 P_i = d.P[i], V_i = d.V[i];
+    const f3 ipos = xyz(P_i), ivev = xyz(V_i);
+    const float ipress = P_i.w, idens = V_i.w;
+    const bool freeze = p.freeze != 0;
+
+    f3 force = mk3(0.0f, 0.0f, 0.0f);
+    for (int a = 0; a < B; a++) {
+        const size_t bi = (size_t)a * N + h;
+        const uint32_t jh = d.BPARTNER[bi];
+        if (jh == MG_NONE) continue;
+        const uint32_t j = d.SLOT[jh];
+        float4 bp = d.BPAR[bi];
+        if (j >= na || (double)bp.y < 0.000000001) continue;
+        float strain, abs_dist;
+        f3 eterm = bond_term(ipos, ivev, xyz(d.P[j]), xyz(d.V[j]), bp, p.pmass, &strain, &abs_dist);
+        float2 in = d.BINT[bi];
+        in.x = in.x + strain * strain;
+        in.y = in.y + strain;
+        d.BINT[bi] = in;
+        if (!has_nan3(eterm)) force = force - eterm;
+        if (abs_dist >= bp.x && !freeze) ((float*)&d.BPAR[bi])[1] = 0.0f;
+    }
+
+    const float sr = p.psmoothradius, st = p.psurface_t;
+    f3 fsum = mk3(0.0f, 0.0f, 0.0f);
+    for (int c = 0; c < g.adj_cnt; c++) {
+        const uint32_t cell = gc + (uint32_t)g.adj[c];
+        const uint32_t n = d.CNT[cell];
+        f3 cf = mk3(0.0f, 0.0f, 0.0f);
+        if (n != 0) {
+            const uint32_t o = d.OFF[cell];
+            for (uint32_t j = o; j < o + n; j++) {
+                const float4 Pj = d.P[j];
+                f3 dist = ipos - xyz(Pj);
+                float dsq = dist.x * dist.x + dist.y * dist.y + dist.z * dist.z;
+                float r = sqrtf(dsq);
+                if (dsq < 1.0f && dsq > 0.0f) {
+                    const float4 Vj = d.V[j];
+                    cf = cf + fluid_term(dist, r, sr, ipress, Pj.w, idens, Vj.w, ivev, xyz(Vj), st);
+                }
+            }
+        }
+        fsum = fsum + cf;
+    }
+    force = force + fsum * p.pmass;
+
+    f3 F = accumulate ? xyz(d.F[i]) : mk3(0.0f, 0.0f, 0.0f);
+    F = F + force;
+    /* incoming bonds */
+    const uint32_t e0 = d.INC_OFF[h], e1 = d.INC_OFF[h + 1];
+    for (uint32_t e = e0; e < e1; e++) {
+        const uint32_t packed = d.INC[e];
+        const uint32_t kh = packed / (uint32_t)B, b = packed - kh * (uint32_t)B;
+        const uint32_t k = d.SLOT[kh];
+        if (k >= na) continue;
+        const float4 bp = d.BPAR[(size_t)b * N + kh];
+        if ((double)bp.y < 0.000000001) continue;
+        float strain, abs_dist;
+        f3 eterm = bond_term(xyz(d.P[k]), xyz(d.V[k]), ipos, ivev, bp, p.pmass, &strain, &abs_dist);
+        if (!has_nan3(eterm)) F = F + eterm;
+    }
+    d.F[i] = make_float4(F.x, F.y, F.z, 0.0f);
+}
+
+/* ------------------------------------------------------------- advance (a11)
+ * advanceParticles, src/fluid_system_cuda.cu:3470-3614, in place. With the copies of
+ * src/host_CL.cpp:199-226 around it, a GRID_UNDEF particle ends unchanged. */
+__global__ LAUNCH_BOUNDS_256 void k_advance(DS d, MgParams p, float dt)
+{
+    const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
+    if (i >= d.ctr[MG_C_ACTIVE]) return;
+    if (d.GCELL[i] == MG_GRID_UNDEF) return;
+    const float4 P_i = d.P[i], V_i = d.V[i];
+    f3 pos = xyz(P_i), vev = xyz(V_i);
+    f3 accel = xyz(d.F[i]);
+    const float eps = MG_EPSILON, prad = p.pradius, slope = p.pground_slope;
+    const float estiff = p.pextstiff, damp = p.pdamp;
+    const f3 bmin = mk3(p.pboundmin[0], p.pboundmin[1], p.pboundmin[2]);
+    const f3 bmax = mk3(p.pboundmax[0], p.pboundmax[1], p.pboundmax[2]);
+    f3 norm;
+    float diff, adj, speed;
+    diff = prad - (pos.y - (bmin.y + (pos.x - bmin.x) * slope));
+    if (diff > eps) {
+        norm = mk3(-slope, (float)(1.0 - (double)slope), 0.0f);
+        adj = estiff * diff - damp * dot3(norm, vev);
+        norm = norm * adj; accel = accel + norm;
+    }
+    diff = prad - (bmax.y - pos.y);
+    if (diff > eps) {
+        norm = mk3(0.0f, -1.0f, 0.0f);
+        adj = estiff * diff - damp * dot3(norm, vev);
+        norm = norm * adj; accel = accel + norm;
+    }
+    diff = prad - (pos.x - bmin.x);
+    if (diff > eps) {
+        norm = mk3(1.0f, 0.0f, 0.0f);
+        adj = (p.pforce_min + 1.0f) * estiff * diff - damp * dot3(norm, vev);
+        norm = norm * adj; accel = accel + norm;
+    }
+    diff = prad - (bmax.x - pos.x);
+    if (diff > eps) {
+        norm = mk3(-1.0f, 0.0f, 0.0f);
+        adj = (p.pforce_max + 1.0f) * estiff * diff - damp * dot3(norm, vev);
+        norm = norm * adj; accel = accel + norm;
+    }
+    diff = prad - (pos.z - bmin.z);
+    if (diff > eps) {
+        norm = mk3(0.0f, 0.0f, 1.0f);
+        adj = estiff * diff - damp * dot3(norm, vev);
+        norm = norm * adj; accel = accel + norm;
+    }
+    diff = prad - (bmax.z - pos.z);
+    if (diff > eps) {
+        norm = mk3(0.0f, 0.0f, -1.0f);
+        adj = estiff * diff - damp * dot3(norm, vev);
+        norm = norm * adj; accel = accel + norm;
+    }
+    f3 dist = bmax - pos;
+    diff = 2.0f * prad - (dist.x + dist.y + dist.z);
+    if (diff > eps) {
+        norm = mk3(1.0f, 1.0f, 1.0f);
+        adj = estiff * diff - damp * dot3(norm, vev);
+        norm = norm * adj; accel = accel + norm;
+    }
+    accel = accel + mk3(p.pgravity[0], p.pgravity[1], p.pgravity[2]);
+    speed = accel.x * accel.x + accel.y * accel.y + accel.z * accel.z;
+    if (speed > p.AL2) accel = accel * (p.AL / sqrtf(speed));
+    f3 v = xyz(d.VEL[i]);
+    speed = v.x * v.x + v.y * v.y + v.z * v.z;
+    if (speed > p.VL2) {
+        speed = p.VL2;
+        v = v * (p.VL / sqrtf(speed));
+    }
+    f3 vnext = accel * dt + v;
+    f3 ve = (v + vnext) * 0.5f;
+    f3 np = pos + vnext * dt;
+    d.P[i] = make_float4(np.x, np.y, np.z, P_i.w);
+    d.V[i] = make_float4(ve.x, ve.y, ve.z, V_i.w);
+    d.VEL[i] = make_float4(vnext.x, vnext.y, vnext.z, 0.0f);
+}
+
+/* ----------------------------------------------------------- diffusion (a12)
+ * contributeDiffusion/computeDiffusion, src/fluid_system_cuda.cu:3026-3105, as a Jacobi
+ * update CONC -> CONC2 (every slot is written; slots that do not diffuse copy through). */
+__global__ LAUNCH_BOUNDS_256 void k_diffusion(DS d, MgParams p, MgGrid g, MgGenome gen)
+{
+    const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
+    if (i >= (uint32_t)d.N) return;
+    const float4* src = (const float4*)(d.CONC + (size_t)i * MG_NUM_TF);
+    float4* dst = (float4*)(d.CONC2 + (size_t)i * MG_NUM_TF);
+    float cur[MG_NUM_TF];
+    {
+        float4 a = src[0], b = src[1], c = src[2], e = src[3];
+        cur[0] = a.x; cur[1] = a.y; cur[2] = a.z; cur[3] = a.w;
+        cur[4] = b.x; cur[5] = b.y; cur[6] = b.z; cur[7] = b.w;
+        cur[8] = c.x; cur[9] = c.y; cur[10] = c.z; cur[11] = c.w;
+        cur[12] = e.x; cur[13] = e.y; cur[14] = e.z; cur[15] = e.w;
+    }
+    uint32_t gc = (i < d.ctr[MG_C_ACTIVE]) ? d.GCELL[i] : MG_GRID_UNDEF;
+    if (gc == MG_GRID_UNDEF) {
+        dst[0] = src[0]; dst[1] = src[1]; dst[2] = src[2]; dst[3] = src[3];
+        return;
+    }
+    gc -= (uint32_t)((1 * g.res[2] + 1) * g.res[0] + 1);
+    const float d2 = p.psimscale * p.psimscale;
+    const float r2 = p.r2 / d2;
+    float D[MG_NUM_TF], nw[MG_NUM_TF];
+    uint32_t dmask = 0;
+    for (int t = 0; t < MG_NUM_TF; t++) {
+        D[t] = (float)gen.tf_diffusability[t] * p.diff_scale;
+        nw[t] = 0.0f;
+        if (gen.tf_diffusability[t]) dmask |= 1u << t;
+    }
+    const f3 pi = xyz(d.P[i]);
+    for (int c = 0; c < g.adj_cnt; c++) {
+        const uint32_t cell = gc + (uint32_t)g.adj[c];
+        const uint32_t n = d.CNT[cell];
+        if (n == 0) continue;
+        const uint32_t o = d.OFF[cell];
+        for (uint32_t j = o; j < o + n; j++) {
+            f3 dist = pi - xyz(d.P[j]);
+            float dsq = dist.x * dist.x + dist.y * dist.y + dist.z * dist.z;
+            if (dsq < r2 && dsq > 0.0f) {
+                const float cc = (r2 - dsq) * d2;
+                const float* cj = d.CONC + (size_t)j * MG_NUM_TF;
+#pragma unroll
+                for (int t = 0; t < MG_NUM_TF; t++)
+                    if (dmask & (1u << t)) nw[t] += D[t] * cc * (cj[t] - cur[t]);
+            }
+        }
+    }
+    float out[MG_NUM_TF];
+#pragma unroll
+    for (int t = 0; t < MG_NUM_TF; t++) out[t] = (float)gen.tf_breakdown_rate[t] * (cur[t] + nw[t]);
+    dst[0] = make_float4(out[0], out[1], out[2], out[3]);
+    dst[1] = make_float4(out[4], out[5], out[6], out[7]);
+    dst[2] = make_float4(out[8], out[9], out[10], out[11]);
+    dst[3] = make_float4(out[12], out[13], out[14], out[15]);
+}
+
+/* --------------------------------------------------------------- genes (a13)
+ * computeGeneAction + tallyGeneAction (src/fluid_system_cuda.cu:714-772) fused per
+ * particle: genes in order 0..15 for the genes the particle's sort-time membership holds
+ * (the dense lists), secretion/activation deltas kept in registers, then the tally for
+ * members of the living list (gene 2) below mActivePoints. Fixed semantics are those of
+ * the oracle's genes() (float FCONC in AoS, one tally, bounded countdown). */
+__global__ LAUNCH_BOUNDS_256 void k_genes(DS d, MgParams p, MgGenome gen)
+{
+    const uint32_t s = xcd_block() * blockDim.x + threadIdx.x;
+    if (!d.ctr[MG_C_LISTS]) return;
+    if (s >= d.ctr[MG_C_NUM_POINTS]) return;
+    const uint32_t m = d.GMASK[s];
+    if (m == 0) return;
+    const uint32_t h = d.HOME[s];
+    const int N = d.N;
+    uint32_t epi[MG_NUM_GENES];
+#pragma unroll
+    for (int g = 0; g < MG_NUM_GENES; g++) epi[g] = d.EPIGEN[(size_t)g * N + h];
+    float conc[MG_NUM_TF], dconc[MG_NUM_TF];
+    uint32_t depi[MG_NUM_GENES];
+    float* cp = d.CONC + (size_t)s * MG_NUM_TF;
+#pragma unroll
+    for (int t = 0; t < MG_NUM_TF; t++) { conc[t] = cp[t]; dconc[t] = 0.0f; depi[t] = 0u; }
+    for (int gene = 0; gene < MG_NUM_GENES; gene++) {
+        if (!(m & (1u << gene))) continue;
+        const int delay = (int)epi[gene];
+        if (0 < delay && delay < MG_INT_MAX) {
+            epi[gene]--;
+            if (delay == 1 && gene + 1 < MG_NUM_GENES && epi[gene + 1]) epi[gene + 1] = gen.delay[gene + 1];
+        }
+        float activity = 0.0f;
+        for (int tf = 0; tf < MG_NUM_TF; tf++)
+            if (gen.sensitivity[gene][tf] != 0u) activity += (float)gen.sensitivity[gene][tf] * conc[tf];
+        const int numTF = gen.secrete[gene][2 * MG_NUM_TF];
+        for (int j = 0; j < numTF && j < MG_NUM_TF; j++) {
+            const int tf = gen.secrete[gene][j * 2];
+            const int rate = gen.secrete[gene][j * 2 + 1];
+            if (tf < 0 || tf >= MG_NUM_TF) continue;
+            dconc[tf] += ((float)rate * activity) * p.secrete_scale;
+        }
+        const int numL = gen.activate[gene][2 * MG_NUM_GENES];
+        for (int j = 0; j < numL && j < MG_NUM_GENES; j++) {
+            const int other = gen.activate[gene][j * 2];
+            const int threshold = gen.activate[gene][j * 2 + 1];
+            if (other < 0 || other >= MG_NUM_GENES) continue;
+            if ((float)threshold < activity) depi[other] += 1u;
+        }
+    }
+    if ((m & 4u) && s < d.ctr[MG_C_ACTIVE]) {
+        for (int t = 0; t < MG_NUM_TF; t++) cp[t] = conc[t] + dconc[t];
+        for (int g = 0; g < MG_NUM_GENES; g++) epi[g] += depi[g];
+    }
+#pragma unroll
+    for (int g = 0; g < MG_NUM_GENES; g++) d.EPIGEN[(size_t)g * N + h] = epi[g];
+}
+
+/* ZeroVelCL (src/host_CL.cpp:2175-2180) over the whole FVEL / FVEVAL buffers. */
+__global__ LAUNCH_BOUNDS_256 void k_zero_vel(DS d)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (uint32_t)d.N) return;
+    d.VEL[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    float4 v = d.V[i];
+    d.V[i] = make_float4(0.f, 0.f, 0.f, v.w);
+}

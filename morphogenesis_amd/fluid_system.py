@@ -1,0 +1,150 @@
+"""Python host mirror of the reference's FluidSystem step API over libmg_sph.so.
+
+Method names follow src/fluid_system.h:590-651 (Run2PhysicalSort, Run2InnerPhysicalLoop,
+Run2GeneAction, InsertParticlesCL, PrefixSumCellsCL, CountingSortFullCL, ComputePressureCL,
+ComputeForceCL, AdvanceCL, ComputeDiffusionCL, ComputeGenesCL, ZeroVelCL, TransferToCL,
+TransferFromCL, setFreeze). Errors raise MgError instead of calling exit()
+(src/fluid_system.cpp:447-452). The C++ facade in csrc/fluid_system.hpp offers the same
+interface to C++ callers.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _ffi as F
+
+
+class FluidSystem:
+    """One simulation instance on the GPU (a handle of libmg_sph.so)."""
+
+    prefix = "mg"
+
+    def __init__(self, max_points: int, bonds_per_particle: int, lib=None):
+        self.lib = lib or F.load_library()
+        self.n = int(max_points)
+        self.b = int(bonds_per_particle)
+        self.c = 0
+        self.h = C.c_void_p()
+        self._call("create", C.byref(self.h), self.n, self.b)
+        self.m_Time = 0.0
+
+    # ------------------------------------------------------------------ plumbing
+    def _call(self, name, *args):
+        r = getattr(self.lib, f"{self.prefix}_{name}")(*args)
+        if r != F.MG_OK:
+            raise F.MgError(f"{self.prefix}_{name} -> {F.ERRORS.get(r, r)}: {self.last_error()}")
+        return r
+
+    def last_error(self) -> str:
+        return self.lib.mg_last_error().decode(errors="replace")
+
+    def close(self):
+        if self.h:
+            self.lib.mg_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------------ set-up
+    def set_grid(self, grid: F.MgGrid):
+        self._call("set_grid", self.h, C.byref(grid))
+        self.c = int(grid.total)
+
+    def set_params(self, params: F.MgParams):
+        self._call("set_params", self.h, C.byref(params))
+
+    def set_genome(self, genome: F.MgGenome):
+        self._call("set_genome", self.h, C.byref(genome))
+
+    def setFreeze(self, freeze: bool):
+        self._call("set_freeze", self.h, int(bool(freeze)))
+
+    def upload(self, buf: int, arr: np.ndarray):
+        shape, dtype = F.buffer_shape(buf, self.n, self.c, self.b)
+        a = np.ascontiguousarray(arr, dtype=dtype).reshape(shape)
+        self._call("upload", self.h, int(buf), a.ctypes.data_as(C.c_void_p), a.nbytes)
+
+    def download(self, buf: int) -> np.ndarray:
+        shape, dtype = F.buffer_shape(buf, self.n, self.c, self.b)
+        a = np.empty(shape, dtype)
+        self._call("download", self.h, int(buf), a.ctypes.data_as(C.c_void_p), a.nbytes)
+        return a
+
+    def TransferToCL(self, buffers: dict):
+        for k, v in buffers.items():
+            self.upload(k, v)
+
+    def TransferFromCL(self, which=None) -> dict:
+        ids = which if which is not None else F.PARTICLE_BUFFERS
+        return {k: self.download(k) for k in ids}
+
+    def load_scene(self, scene):
+        self.set_grid(scene.grid)
+        self.set_params(scene.params)
+        self.set_genome(scene.genome)
+        self.TransferToCL(scene.buffers)
+
+    def counts(self):
+        a, b = C.c_uint32(), C.c_uint32()
+        self._call("get_counts", self.h, C.byref(a), C.byref(b))
+        return int(a.value), int(b.value)
+
+    def sync(self):
+        self._call("sync", self.h)
+
+    # ------------------------------------------------------------------ phases
+    def InsertParticlesCL(self, readback: bool = False):
+        if not readback:
+            self._call("insert_particles", self.h, None, None, None)
+            return None
+        gcell = np.empty(self.n, np.uint32)
+        gndx = np.empty(self.n, np.uint32)
+        gcnt = np.empty(self.c, np.uint32)
+        P = C.POINTER(C.c_uint32)
+        self._call("insert_particles", self.h, gcell.ctypes.data_as(P), gndx.ctypes.data_as(P),
+                   gcnt.ctypes.data_as(P))
+        return gcell, gndx, gcnt
+
+    def PrefixSumCellsCL(self, zero_offsets: int = 1):
+        self._call("prefix_sum_cells", self.h, int(zero_offsets))
+
+    def CountingSortFullCL(self):
+        self._call("counting_sort_full", self.h)
+
+    def ComputePressureCL(self):
+        self._call("compute_pressure", self.h)
+
+    def ComputeForceCL(self):
+        self._call("compute_force", self.h)
+
+    def AdvanceCL(self, time: float, dt: float, ss: float = 1.0):
+        self._call("advance", self.h, float(time), float(dt), float(ss))
+
+    def ComputeDiffusionCL(self):
+        self._call("compute_diffusion", self.h)
+
+    def ComputeGenesCL(self):
+        self._call("compute_genes", self.h)
+
+    def ZeroVelCL(self):
+        self._call("zero_vel", self.h)
+
+    def Run2PhysicalSort(self):
+        self._call("physical_sort", self.h)
+
+    def Run2InnerPhysicalLoop(self, dt: float):
+        self._call("inner_physical_loop", self.h, float(self.m_Time), float(dt))
+        self.m_Time += dt
+
+    def Run2GeneAction(self):
+        self._call("gene_action", self.h)
+
+    def step(self, nsteps: int, dt: float, flags: int = 0):
+        self._call("step", self.h, int(nsteps), float(dt), int(flags))
+        self.m_Time += nsteps * dt

@@ -1,0 +1,229 @@
+"""GPU parity: the HIP library (through its C-ABI) against the FP32 CPU oracle on the same
+seeded inputs. Integer/index results (cells, counts, offsets, order, bond indices, gene
+states) must be bit-exact; floating-point state is compared bit-exactly too, which is
+stronger than the 1e-5 relative bar (both sides evaluate the same expression trees in the
+same order with IEEE division/sqrt and no FMA contraction)."""
+import ctypes as C
+import sys
+
+import numpy as np
+import pytest
+
+from morphogenesis_amd import _ffi as F
+from morphogenesis_amd import scenes
+from tests.helpers import assert_bits_equal, mini_scene, per_cell_sets
+
+pytestmark = pytest.mark.gpu
+
+COMPARE = [F.FPOS, F.FVEL, F.FVEVAL, F.FFORCE, F.FPRESS, F.FDENSITY, F.FAGE, F.FCOLOR, F.FGCELL,
+           F.FGNDX, F.FELASTIDX, F.FNERVEIDX, F.FCONC, F.FEPIGEN, F.FBIN, F.FPARTICLEIDX,
+           F.FPARTICLE_ID, F.FMASS_RADIUS, F.FBIN_COUNT, F.FBIN_OFFSET,
+           F.FBIN_COUNT_ACTIVE_GENES, F.FBIN_OFFSET_ACTIVE_GENES, F.FDENSE_LIST_LENGTHS,
+           F.FDENSE_LISTS]
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def pair(scene):
+    from morphogenesis_amd.fluid_system import FluidSystem
+    from oracle.mg_oracle import Oracle
+
+    g = FluidSystem(scene.n, scene.b)
+    o = Oracle(scene.n, scene.b)
+    g.load_scene(scene)
+    o.load_scene(scene)
+    return g, o
+
+
+def compare_all(g, o, what, bufs=COMPARE):
+    assert g.counts() == o.counts(), (what, g.counts(), o.counts())
+    for b in bufs:
+        assert_bits_equal(g.download(b), o.download(b), f"{what}: {F.BUFFER_NAMES[b]}")
+
+
+def tricky_tissue(seed=5, n_side=10, b=16):
+    """Tissue with reserve (null) particles, particles outside the scan window and genes."""
+    sc = scenes.tissue(n_side=n_side, b=b, seed=seed, reserve=29)
+    sc.buffers[F.FPOS][:7, 0] = 0.4        # outside the window: dropped at the first sort
+    sc.buffers[F.FVEL][7:40, :3] = np.random.default_rng(seed).normal(0, 0.3, (33, 3))
+    return sc
+
+
+def test_phase_by_phase(built):
+    sc = tricky_tissue()
+    say("case: phase_by_phase", sc.name, sc.n)
+    g, o = pair(sc)
+    compare_all(g, o, "upload")
+    gi = g.InsertParticlesCL(readback=True)
+    oi = o.InsertParticlesCL(readback=True)
+    np.testing.assert_array_equal(gi[0], oi[0])          # FGCELL
+    np.testing.assert_array_equal(gi[2], oi[2])          # FBIN_COUNT
+    acc = gi[0] != F.GRID_UNDEF
+    # FGNDX after insert is an atomicAdd slot (any order); per-cell it is a permutation
+    for c in np.unique(gi[0][acc])[:300]:
+        m = gi[0] == c
+        assert sorted(gi[1][m].tolist()) == list(range(int(m.sum())))
+    for name, fg, fo in (("prefix", g.PrefixSumCellsCL, o.PrefixSumCellsCL),
+                         ("sort", g.CountingSortFullCL, o.CountingSortFullCL),
+                         ("pressure", g.ComputePressureCL, o.ComputePressureCL),
+                         ("force", g.ComputeForceCL, o.ComputeForceCL)):
+        say("phase:", name)
+        fg(); fo()
+        bufs = COMPARE if name != "prefix" else [F.FBIN_COUNT, F.FBIN_OFFSET,
+                                                 F.FBIN_COUNT_ACTIVE_GENES,
+                                                 F.FBIN_OFFSET_ACTIVE_GENES,
+                                                 F.FDENSE_LIST_LENGTHS]
+        compare_all(g, o, name, bufs)
+    say("phase: advance")
+    g.AdvanceCL(0.0, sc.dt, 1.0); o.AdvanceCL(0.0, sc.dt, 1.0)
+    compare_all(g, o, "advance")
+    say("phase: diffusion")
+    g.ComputeDiffusionCL(); o.ComputeDiffusionCL()
+    compare_all(g, o, "diffusion")
+    say("phase: genes")
+    g.ComputeGenesCL(); o.ComputeGenesCL()
+    compare_all(g, o, "genes")
+    say("phase: second sort")
+    g.Run2PhysicalSort(); o.Run2PhysicalSort()
+    compare_all(g, o, "sort2")
+    # force accumulates across inner steps until the next sort (src/fluid_system_cuda.cu:225)
+    for k in range(2):
+        g.ComputePressureCL(); o.ComputePressureCL()
+        g.ComputeForceCL(); o.ComputeForceCL()
+    compare_all(g, o, "force accumulated twice")
+    g.ZeroVelCL(); o.ZeroVelCL()
+    compare_all(g, o, "zero vel")
+
+
+@pytest.mark.parametrize("seed,n_side,b,steps", [(1, 12, 32, 100), (2, 9, 6, 60)])
+def test_tissue_steps_bit_exact(built, seed, n_side, b, steps):
+    sc = scenes.tissue(n_side=n_side, b=b, seed=seed)
+    say("case: tissue steps", sc.name, "seed", seed, "steps", steps)
+    g, o = pair(sc)
+    for sim in (g, o):
+        sim.Run2PhysicalSort()
+    for chunk in range(steps // 20):
+        g.step(20, sc.dt, sc.flags)
+        o.step(20, sc.dt, sc.flags)
+        compare_all(g, o, f"after {20 * (chunk + 1)} steps")
+
+
+def test_fluid_block_100_steps(built):
+    sc = scenes.fluid_block(32, seed=1)
+    say("case: fluid block 32^3, 100 steps")
+    g, o = pair(sc)
+    for sim in (g, o):
+        sim.Run2PhysicalSort()
+        sim.step(100, sc.dt, 0)
+    compare_all(g, o, "fluid block 100 steps")
+
+
+def test_escaping_and_breaking_bonds(built):
+    # stretched bonds beyond the elastic limit break; fast particles leave the window
+    sc = scenes.tissue(n_side=8, b=8, seed=9, limit=0.62)
+    rng = np.random.default_rng(3)
+    sc.buffers[F.FVEL][:, :3] = rng.normal(0, 2.5, (sc.n, 3)).astype(np.float32)
+    sc.buffers[F.FVEVAL][:, :3] = sc.buffers[F.FVEL][:, :3]
+    say("case: breaking bonds / escaping particles")
+    g, o = pair(sc)
+    for sim in (g, o):
+        sim.Run2PhysicalSort()
+    for k in range(4):
+        g.step(10, 1e-3, sc.flags)
+        o.step(10, 1e-3, sc.flags)
+        compare_all(g, o, f"escape chunk {k}")
+    rest = g.download(F.FELASTIDX)[:, :, 2].view(np.float32)
+    assert (rest == 0).any(), "no bond broke"
+
+
+def test_gpu_deterministic(built):
+    from morphogenesis_amd.fluid_system import FluidSystem
+
+    sc = scenes.tissue(n_side=12, b=32, seed=7)
+    say("case: determinism")
+    outs = []
+    for _ in range(2):
+        g = FluidSystem(sc.n, sc.b)
+        g.load_scene(sc)
+        g.Run2PhysicalSort()
+        g.step(30, sc.dt, sc.flags)
+        outs.append({b: g.download(b) for b in (F.FPOS, F.FVEL, F.FCONC, F.FELASTIDX, F.FGCELL)})
+        g.close()
+    for b in outs[0]:
+        assert_bits_equal(outs[0][b], outs[1][b], F.BUFFER_NAMES[b])
+
+
+def test_edge_cases(built):
+    from morphogenesis_amd.fluid_system import FluidSystem
+    from oracle.mg_oracle import Oracle
+
+    cases = {
+        "all_outside": mini_scene(np.full((5, 3), 0.2), b=2),
+        "single": mini_scene([[5.0, 5.0, 5.0]], b=0),
+        "one_cell_crowd": mini_scene(np.random.default_rng(1).uniform(5.0, 5.99, (70, 3)), b=3),
+        "nan_particle": mini_scene([[5.0, 5.0, 5.0], [np.nan, 5.0, 5.0], [5.5, 5.2, 5.0]], b=1),
+    }
+    for name, sc in cases.items():
+        say("case: edge", name)
+        g, o = pair(sc)
+        for sim in (g, o):
+            sim.Run2PhysicalSort()
+            sim.step(3, 1e-4, 0)
+        compare_all(g, o, name)
+
+
+def test_full_size_headline_workload(built):
+    """The 262,144-particle bonded tissue (B = 32) at its real size: the first sort's counts,
+    offsets and order are bit-exact, and 3 full steps match the oracle bit for bit."""
+    sc = scenes.tissue(64, 32, seed=1)
+    say("case: full-size tissue 262k")
+    g, o = pair(sc)
+    for sim in (g, o):
+        sim.Run2PhysicalSort()
+    compare_all(g, o, "initial sort", [F.FBIN_COUNT, F.FBIN_OFFSET, F.FGCELL, F.FGNDX,
+                                       F.FPARTICLE_ID, F.FELASTIDX, F.FPARTICLEIDX])
+    g.step(3, sc.dt, sc.flags)
+    o.step(3, sc.dt, sc.flags)
+    compare_all(g, o, "3 steps", [F.FPOS, F.FVEL, F.FVEVAL, F.FPRESS, F.FDENSITY, F.FCONC,
+                                  F.FEPIGEN, F.FELASTIDX, F.FGCELL, F.FBIN_OFFSET])
+
+
+def test_cpp_facade_matches_capi(built):
+    """The C++ FluidSystem facade (Run2PhysicalSort + n x [Run2InnerPhysicalLoop,
+    Run2GeneAction, Run2PhysicalSort]) gives the same state as mg_step."""
+    from morphogenesis_amd.fluid_system import FluidSystem
+
+    lib = F.load_library()
+    lib.mgfs_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_int]
+    lib.mgfs_set_host.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t]
+    lib.mgfs_get_host.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t]
+    lib.mgfs_setup.argtypes = [C.c_void_p, C.POINTER(F.MgGrid), C.POINTER(F.MgParams),
+                               C.POINTER(F.MgGenome), C.c_double]
+    lib.mgfs_transfer_to.argtypes = [C.c_void_p]
+    lib.mgfs_transfer_from.argtypes = [C.c_void_p]
+    lib.mgfs_run.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    lib.mgfs_destroy.argtypes = [C.c_void_p]
+    sc = scenes.tissue(n_side=10, b=16, seed=12)
+    say("case: C++ facade")
+    fh = C.c_void_p()
+    assert lib.mgfs_create(C.byref(fh), sc.n, sc.b) == 0
+    assert lib.mgfs_setup(fh, C.byref(sc.grid), C.byref(sc.params), C.byref(sc.genome), sc.dt) == 0
+    for b, arr in sc.buffers.items():
+        a = np.ascontiguousarray(arr)
+        assert lib.mgfs_set_host(fh, b, a.ctypes.data_as(C.c_void_p), a.nbytes) == 0
+    assert lib.mgfs_transfer_to(fh) == 0
+    assert lib.mgfs_run(fh, 15, 1) == 0
+    assert lib.mgfs_transfer_from(fh) == 0
+    g = FluidSystem(sc.n, sc.b)
+    g.load_scene(sc)
+    g.Run2PhysicalSort()
+    g.step(15, sc.dt, sc.flags)
+    for b in (F.FPOS, F.FVEL, F.FCONC, F.FEPIGEN, F.FELASTIDX, F.FGCELL):
+        shape, dt = F.buffer_shape(b, sc.n, sc.c, sc.b)
+        a = np.empty(shape, dt)
+        assert lib.mgfs_get_host(fh, b, a.ctypes.data_as(C.c_void_p), a.nbytes) == 0
+        assert_bits_equal(a, g.download(b), f"facade {F.BUFFER_NAMES[b]}")
+    lib.mgfs_destroy(fh)

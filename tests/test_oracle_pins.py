@@ -1,0 +1,434 @@
+"""Pins of the CPU oracle against what the mathematics fixes (closed forms, brute force,
+invariants, special cases). The reference ships no golden vectors or known-answer tests
+for this path (it has no test suite), so these checks are what make the oracle trustworthy;
+each one is chosen so that a plausible slip in the restatement (a dropped term, a wrong
+sign, a transposed index) fails it."""
+import math
+
+import numpy as np
+import pytest
+
+from morphogenesis_amd import _ffi as F
+from morphogenesis_amd import scenes
+from oracle.mg_oracle import Oracle
+from tests.helpers import mini_scene, per_cell_sets
+
+
+def make(scene, precision=32):
+    o = Oracle(scene.n, scene.b, precision=precision)
+    o.load_scene(scene)
+    return o
+
+
+# ------------------------------------------------------------------ insert / scan
+def expected_cells(pos, grid):
+    """Brute-force binning in float32: gc = trunc((pos - min) * delta), accepted iff
+    1 <= gc <= scan_max on every axis (src/fluid_system_cuda.cu:72-79)."""
+    f = np.float32
+    g = np.empty((pos.shape[0], 3), np.float64)
+    for k in range(3):
+        g[:, k] = (pos[:, k].astype(f) - f(grid.gmin[k])) * f(grid.delta[k])
+    ok = np.all(np.isfinite(g), axis=1)
+    gi = np.zeros_like(g, dtype=np.int64)
+    gi[ok] = np.trunc(g[ok]).astype(np.int64)
+    sm = np.array(list(grid.scan_max))
+    ok &= np.all((gi >= 1) & (gi <= sm), axis=1)
+    res = list(grid.res)
+    cell = (gi[:, 1] * res[2] + gi[:, 2]) * res[0] + gi[:, 0]
+    return np.where(ok, cell, F.GRID_UNDEF).astype(np.uint32)
+
+
+def test_insert_matches_bruteforce_binning(built):
+    rng = np.random.default_rng(5)
+    pos = rng.uniform(-1.0, 13.0, size=(3000, 3)).astype(np.float32)
+    pos[:50] = np.floor(pos[:50])               # exactly on cell faces
+    pos[50:60, 1] = np.nan                      # NaN: never accepted
+    pos[60:70, 0] = 1e30                        # far outside
+    pos[70:80] = 0.999999                       # just below cell 1
+    sc = mini_scene(pos, b=0)
+    o = make(sc)
+    gcell, gndx, gcnt = o.InsertParticlesCL(readback=True)
+    exp = expected_cells(pos, sc.grid)
+    np.testing.assert_array_equal(gcell, exp)
+    acc = exp != F.GRID_UNDEF
+    np.testing.assert_array_equal(gcnt, np.bincount(exp[acc], minlength=sc.c))
+    # gndx is the stable rank of the particle inside its cell
+    for c in np.unique(exp[acc])[:200]:
+        idx = np.nonzero(exp == c)[0]
+        np.testing.assert_array_equal(gndx[idx], np.arange(idx.size))
+    assert gcnt.sum() == acc.sum()
+
+
+def test_scan_is_exclusive_prefix_sum(built):
+    sc = scenes.tissue(n_side=9, b=8, seed=2, genes=True)
+    o = make(sc)
+    o.InsertParticlesCL()
+    o.PrefixSumCellsCL(1)
+    cnt, off = o.download(F.FBIN_COUNT), o.download(F.FBIN_OFFSET)
+    np.testing.assert_array_equal(off, np.concatenate([[0], np.cumsum(cnt)[:-1]]).astype(np.uint32))
+    assert np.all(off[:-1] + cnt[:-1] == off[1:])
+    gcnt, goff = o.download(F.FBIN_COUNT_ACTIVE_GENES), o.download(F.FBIN_OFFSET_ACTIVE_GENES)
+    for g in range(F.NUM_GENES):
+        np.testing.assert_array_equal(goff[g], np.concatenate([[0], np.cumsum(gcnt[g])[:-1]]))
+    lens = o.download(F.FDENSE_LIST_LENGTHS)
+    np.testing.assert_array_equal(lens, gcnt.sum(axis=1))
+    epi = sc.buffers[F.FEPIGEN]
+    cells = expected_cells(sc.buffers[F.FPOS][:, :3], sc.grid)
+    acc = cells != F.GRID_UNDEF
+    np.testing.assert_array_equal(lens, (epi[:, acc] > 0).sum(axis=1))
+
+
+# ------------------------------------------------------------------ counting sort
+def sorted_tissue(reserve=0, escape=0, seed=4, n_side=9):
+    sc = scenes.tissue(n_side=n_side, b=16, seed=seed, reserve=reserve)
+    if escape:
+        # push a few bonded particles outside the scan window: they leave the model
+        sc.buffers[F.FPOS][:escape, 0] = 0.5
+    o = make(sc)
+    o.Run2PhysicalSort()
+    return sc, o
+
+
+@pytest.mark.parametrize("reserve,escape", [(0, 0), (37, 0), (0, 11), (20, 5)])
+def test_sort_invariants(built, reserve, escape):
+    sc, o = sorted_tissue(reserve, escape)
+    nump, na = o.counts()
+    cells = expected_cells(sc.buffers[F.FPOS][:, :3], sc.grid)
+    acc = cells != F.GRID_UNDEF
+    assert nump == acc.sum()
+    gcell = o.download(F.FGCELL)
+    pid = o.download(F.FPARTICLE_ID)
+    assert np.all(np.diff(gcell[:nump].astype(np.int64)) >= 0), "not cell-ordered"
+    # multiset of particle IDs preserved, dropped ones gone, tail reset to the sentinel
+    np.testing.assert_array_equal(np.sort(pid[:nump]), np.sort(sc.buffers[F.FPARTICLE_ID][acc]))
+    assert np.all(pid[nump:] == F.NONE)
+    # per-cell sets equal the insert output
+    exp_sets = {}
+    for c, p in zip(cells[acc].tolist(), sc.buffers[F.FPARTICLE_ID][acc].tolist()):
+        exp_sets.setdefault(c, set()).add(p)
+    assert per_cell_sets(gcell, pid, nump) == exp_sets
+    # stable order: inside a cell, ascending original index (IDs are index + 1)
+    for c in list(exp_sets)[:100]:
+        seg = pid[:nump][gcell[:nump] == c]
+        assert np.all(np.diff(seg.astype(np.int64)) > 0)
+    # FBIN is the identity on sorted slots; FGNDX is the rank in the cell
+    np.testing.assert_array_equal(o.download(F.FBIN)[:nump], np.arange(nump))
+    off = o.download(F.FBIN_OFFSET)
+    np.testing.assert_array_equal(o.download(F.FGNDX)[:nump], np.arange(nump) - off[gcell[:nump]])
+    # bond graph isomorphic: partner index resolves to the partner ID stored in slot [5]
+    rec = o.download(F.FELASTIDX)
+    j = rec[:nump, :, 0]
+    live = j != F.NONE
+    np.testing.assert_array_equal(pid[j[live]], rec[:nump, :, 5][live])
+    # bonds whose partner left the model point to the sentinel but keep their partner ID
+    orig = sc.buffers[F.FELASTIDX]
+    n_orig_bonds = (orig[acc][:, :, 0] != F.NONE).sum()
+    assert live.sum() <= n_orig_bonds
+    # reciprocity of the incoming lists (src/fluid_system_cuda.cu:1892-1895, 1914-1917)
+    pix = o.download(F.FPARTICLEIDX)
+    k, b = pix[:nump, :, 0], pix[:nump, :, 1]
+    m = k != F.NONE
+    jj = np.broadcast_to(np.arange(nump)[:, None], k.shape)
+    np.testing.assert_array_equal(rec[k[m], b[m], 0], jj[m])
+    # reset tail
+    vmax = max(sc.grid.gmax)
+    pos = o.download(F.FPOS)
+    assert np.all(pos[nump:, :3] == np.float32(vmax))
+    assert np.all(rec[nump:] == F.NONE)
+    assert np.all(o.download(F.FCONC)[nump:] == 0)
+    assert np.all(o.download(F.FEPIGEN)[:, nump:] == 0)
+    # gene dense lists: ascending sorted indices of particles with a nonzero state
+    epi = o.download(F.FEPIGEN)
+    lens = o.download(F.FDENSE_LIST_LENGTHS)
+    lists = o.download(F.FDENSE_LISTS)
+    for g in range(F.NUM_GENES):
+        exp = np.nonzero(epi[g, :nump] > 0)[0]
+        assert lens[g] == exp.size
+        np.testing.assert_array_equal(lists[g, :exp.size], exp)
+
+
+def test_active_points_excludes_scan_max_corner_cell(built):
+    # mActivePoints = FBIN_OFFSET[grid_ScanMax] (src/host_CUDA.cpp:501-505): particles in the
+    # corner cell (scan_max.x, scan_max.y, scan_max.z) are sorted but not active.
+    sc = mini_scene([[1.5, 1.5, 1.5], [9.5, 9.5, 9.5], [9.6, 9.5, 9.5], [5.0, 5.0, 5.0]], b=0, vmax=12)
+    assert list(sc.grid.scan_max) == [9, 9, 9]
+    o = make(sc)
+    o.Run2PhysicalSort()
+    nump, na = o.counts()
+    assert (nump, na) == (4, 2)
+
+
+# ------------------------------------------------------------------ pressure
+def test_wendland_normalisation_constant():
+    from scipy.integrate import quad
+
+    integral, _ = quad(lambda q: 4 * math.pi * q * q * (1 - q) ** 4 * (4 * q + 1), 0.0, 1.0)
+    assert abs(integral - 2 * math.pi / 21) < 1e-12
+    p = scenes.make_params((0, 0, 0), (10, 10, 10))
+    assert p.wendlandC2kern == np.float32(21 / (2 * np.float32(3.141592)))
+    assert abs(p.wendlandC2kern - 1 / integral) < 1e-6
+
+
+def test_pressure_matches_bruteforce(built):
+    sc = scenes.fluid_block(n_side=9, seed=8)
+    o = make(sc)
+    o.Run2PhysicalSort()
+    o.ComputePressureCL()
+    nump, na = o.counts()
+    pos = o.download(F.FPOS)[:na, :3].astype(np.float64)
+    d2 = ((pos[:, None, :] - pos[None, :, :]) ** 2).sum(-1)
+    r = np.sqrt(d2)
+    m = (d2 < 1.0) & (d2 > 0)
+    q = np.where(m, r, 0.0)
+    w = np.where(m, (1 - q) ** 4 * (4 * q + 1), 0.0).sum(1)
+    p = sc.params
+    dsum = w * p.pmass * p.wendlandC2kern
+    dsum[dsum == 0] = 1.0
+    dens = o.download(F.FDENSITY)[:na]
+    press = o.download(F.FPRESS)[:na]
+    np.testing.assert_allclose(1.0 / dens, dsum, rtol=2e-6)
+    np.testing.assert_allclose(press, (dsum - p.prest_dens) * p.pintstiff, rtol=1e-4,
+                               atol=2e-6 * p.prest_dens)
+
+
+def test_isolated_particle_density_rule(built):
+    sc = mini_scene([[5.0, 5.0, 5.0], [8.0, 8.0, 8.0]], b=0)
+    o = make(sc)
+    o.Run2PhysicalSort()
+    o.ComputePressureCL()
+    dens, press = o.download(F.FDENSITY), o.download(F.FPRESS)
+    np.testing.assert_array_equal(dens, np.float32(1.0))
+    np.testing.assert_array_equal(press, np.float32((1.0 - np.float32(sc.params.prest_dens)) * np.float32(sc.params.pintstiff)))
+
+
+# ------------------------------------------------------------------ force
+def test_force_conserves_momentum(built):
+    sc = scenes.tissue(n_side=8, b=16, seed=11, genes=False)
+    o = make(sc, precision=64)
+    o.Run2PhysicalSort()
+    o.ComputePressureCL()
+    o.ComputeForceCL()
+    nump, na = o.counts()
+    frc = o.download(F.FFORCE)[:na, :3].astype(np.float64)
+    total = frc.sum(0)
+    scale = np.abs(frc).sum(0)
+    assert np.all(np.abs(total) <= 1e-6 * scale + 1e-9), (total, scale)
+
+
+def pair_force(d, rest, k, c, dv, m):
+    d = np.asarray(d, np.float64)
+    r = np.linalg.norm(d)
+    strain = max(0.0, (r - rest) / rest)
+    return (d / r * strain * k - c * np.asarray(dv, np.float64)) / m, strain
+
+
+def test_two_particle_spring_analytic(built):
+    # far enough apart that no fluid term acts (dsq >= 1), one bond 0 -> 1
+    p0, p1 = np.array([5.0, 5.0, 5.0]), np.array([5.0, 6.3, 5.4])
+    v0, v1 = np.array([0.1, -0.2, 0.3]), np.array([-0.05, 0.02, 0.1])
+    rest, k, c, lim = 0.9, 7.0, 0.01, 5.0
+    sc = mini_scene([p0, p1], b=2, bonds=[(0, 1, rest, k, c, lim)], vel=[v0, v1])
+    o = make(sc, precision=64)
+    o.Run2PhysicalSort()
+    o.ComputePressureCL()
+    o.ComputeForceCL()
+    frc = o.download(F.FFORCE)[:, :3]
+    e, strain = pair_force(p0 - p1, rest, k, c, v1 - v0, sc.params.pmass)
+    np.testing.assert_allclose(frc[0], -e, rtol=2e-6)
+    np.testing.assert_allclose(frc[1], e, rtol=2e-6)
+    rec = o.download(F.FELASTIDX)
+    assert abs(rec[0, 0, 6].view(np.float32) - strain ** 2) < 3e-6 * strain ** 2
+    assert abs(rec[0, 0, 7].view(np.float32) - strain) < 3e-6 * strain
+    # FP32: the pair forces are exact negatives of each other
+    o32 = make(sc)
+    o32.Run2PhysicalSort(); o32.ComputePressureCL(); o32.ComputeForceCL()
+    f32 = o32.download(F.FFORCE)[:, :3]
+    np.testing.assert_array_equal(f32[0], -f32[1])
+
+
+def test_slack_bond_and_breaking(built):
+    base = np.array([5.0, 5.0, 5.0])
+    # slack: distance below rest length -> tension-only spring gives zero elastic force
+    sc = mini_scene([base, base + [0, 1.2, 0]], b=1, bonds=[(0, 1, 1.5, 9.0, 0.0, 5.0)])
+    o = make(sc); o.Run2PhysicalSort(); o.ComputePressureCL(); o.ComputeForceCL()
+    assert np.all(o.download(F.FFORCE)[:, :3] == 0)
+    # beyond the elastic limit: no force, and the bond breaks (rest length := 0) unless frozen
+    for freeze, expect_rest in ((False, 0.0), (True, 1.0)):
+        sc = mini_scene([base, base + [0, 2.5, 0]], b=1, bonds=[(0, 1, 1.0, 9.0, 0.0, 2.0)])
+        o = make(sc)
+        o.setFreeze(freeze)
+        o.Run2PhysicalSort(); o.ComputePressureCL(); o.ComputeForceCL()
+        assert np.all(o.download(F.FFORCE)[:, :3] == 0)
+        assert o.download(F.FELASTIDX)[0, 0, 2].view(np.float32) == np.float32(expect_rest)
+
+
+# ------------------------------------------------------------------ advance
+def run_inner(o, steps, dt):
+    for _ in range(steps):
+        o.ComputePressureCL()
+        o.ComputeForceCL()
+        o.AdvanceCL(0.0, dt, 1.0)
+        # FFORCE accumulates across inner steps without a sort (src/fluid_system_cuda.cu:225);
+        # zero it through a sort to get independent steps
+        o.Run2PhysicalSort()
+
+
+def test_free_fall_closed_form(built):
+    sc = mini_scene([[6.0, 7.0, 6.0]], b=0)
+    o = make(sc, precision=64)
+    o.Run2PhysicalSort()
+    dt, n = 1e-3, 40
+    run_inner(o, n, dt)
+    g = np.array(sc.params.pgravity, np.float64)
+    v = o.download(F.FVEL)[0, :3]
+    x = o.download(F.FPOS)[0, :3]
+    np.testing.assert_allclose(v, n * g * np.float32(dt), rtol=1e-6, atol=1e-12)
+    x_exp = np.array([6.0, 7.0, 6.0]) + g * np.float32(dt) ** 2 * n * (n + 1) / 2
+    np.testing.assert_allclose(x, x_exp, rtol=1e-6)
+    ve = o.download(F.FVEVAL)[0, :3]
+    np.testing.assert_allclose(ve, (n - 0.5) * g * np.float32(dt), rtol=1e-6)
+
+
+@pytest.mark.parametrize("axis,side", [(0, -1), (0, 1), (1, -1), (1, 1), (2, -1), (2, 1)])
+def test_each_wall_in_isolation(built, axis, side):
+    p = scenes.make_params((0, 0, 0), (12, 12, 12))
+    pos = np.array([6.0, 6.0, 6.0])
+    depth = 0.3
+    pos[axis] = (p.pboundmin[axis] + p.pradius - depth) if side < 0 else (p.pboundmax[axis] - p.pradius + depth)
+    vel = np.array([0.02, -0.03, 0.01])
+    sc = mini_scene([pos], b=0, vel=[vel], params_over={"al": 1e9})
+    o = make(sc, precision=64)
+    o.Run2PhysicalSort()
+    o.ComputePressureCL(); o.ComputeForceCL()
+    dt = 1e-5
+    o.AdvanceCL(0.0, dt, 1.0)
+    v1 = o.download(F.FVEL)[0, :3].astype(np.float64)
+    sp = sc.params
+    diff = sp.pradius - ((pos[axis] - sp.pboundmin[axis]) if side < 0 else (sp.pboundmax[axis] - pos[axis]))
+    norm = np.zeros(3); norm[axis] = -side
+    scale = 1.0
+    if axis == 0:
+        scale = (sp.pforce_min if side < 0 else sp.pforce_max) + 1.0
+    if axis == 1 and side < 0:
+        norm = np.array([-sp.pground_slope, 1.0 - sp.pground_slope, 0.0])
+    adj = scale * sp.pextstiff * diff - sp.pdamp * norm.dot(vel)
+    accel = norm * adj + np.array(sp.pgravity)
+    np.testing.assert_allclose(v1, vel + accel * dt, rtol=1e-5)
+
+
+def test_accel_clamp_and_velocity_limit_noop(built):
+    p = scenes.make_params((0, 0, 0), (12, 12, 12))
+    pos = [p.pboundmin[0] + 0.2, 6.0, 6.0]       # deep in the x wall: huge penalty accel
+    vel = [4.0, 2.0, 1.0]                           # above VL = 3: the limit is a no-op
+    sc = mini_scene([pos], b=0, vel=[vel])
+    o = make(sc, precision=64)
+    o.Run2PhysicalSort(); o.ComputePressureCL(); o.ComputeForceCL()
+    dt = 1e-4
+    o.AdvanceCL(0.0, dt, 1.0)
+    v1 = o.download(F.FVEL)[0, :3].astype(np.float64)
+    acc = (v1 - np.array(vel)) / np.float32(dt)
+    assert abs(np.linalg.norm(acc) - sc.params.AL) < 1e-2 * sc.params.AL
+
+
+# ------------------------------------------------------------------ diffusion
+def diffusion_scene(seed=3):
+    # a box large enough that no particle sits in the excluded corner cell, so every
+    # particle that exchanges also updates
+    sc = scenes.tissue(n_side=7, b=4, seed=seed, genes=True, diff_scale=0.01, vmax=12.0)
+    return sc
+
+
+def test_diffusion_conserves_and_freezes_nondiffusing(built):
+    sc = diffusion_scene()
+    o = make(sc, precision=64)
+    o.Run2PhysicalSort()
+    nump, na = o.counts()
+    assert nump == na
+    c0 = o.download(F.FCONC)[:na].astype(np.float64)
+    o.ComputeDiffusionCL()
+    c1 = o.download(F.FCONC)[:na].astype(np.float64)
+    diffusing = np.array([sc.genome.tf_diffusability[t] != 0 for t in range(F.NUM_TF)])
+    assert diffusing.any() and (~diffusing).any()
+    np.testing.assert_allclose(c1.sum(0), c0.sum(0), rtol=1e-6)
+    np.testing.assert_array_equal(c1[:, ~diffusing], c0[:, ~diffusing])
+    assert np.abs(c1[:, diffusing] - c0[:, diffusing]).max() > 1e-4
+
+
+def test_two_particle_diffusion_analytic(built):
+    g = scenes.empty_genome()
+    g.tf_diffusability[3] = 2
+    conc = np.zeros((2, F.NUM_TF), np.float32)
+    conc[0, 3], conc[1, 3] = 1.0, 0.25
+    pos = np.array([[5.0, 5.0, 5.0], [5.0, 5.6, 5.0]])
+    sc = mini_scene(pos, b=0, genome=g, conc=conc, params_over={})
+    sc.params.diff_scale = 0.5
+    o = make(sc, precision=64)
+    o.Run2PhysicalSort(); o.ComputeDiffusionCL()
+    c = o.download(F.FCONC)[:, 3].astype(np.float64)
+    cc = 1.0 - 0.6 ** 2
+    D = 2 * 0.5
+    np.testing.assert_allclose(c, [1.0 + D * cc * (0.25 - 1.0), 0.25 + D * cc * (1.0 - 0.25)], rtol=1e-6)
+
+
+# ------------------------------------------------------------------ gene action
+def test_example_genome_single_particle(built):
+    """src/fluid_system.cpp:2415-2436: gene 0 secretes TF 1 (rate 1) and TF 3 (rate 4) and
+    activates gene 5 when its activity exceeds 6; sensitivity[i][j] = j."""
+    g = scenes.example_genome()
+    conc = np.zeros((1, F.NUM_TF), np.float32)
+    conc[0, :] = np.linspace(0.0, 1.0, F.NUM_TF)
+    epi = np.zeros((F.NUM_GENES, 1), np.uint32)
+    epi[0, 0] = scenes.INT_MAX
+    epi[2, 0] = scenes.INT_MAX  # living list: the tally applies
+    sc = mini_scene([[5, 5, 5]], b=0, genome=g, conc=conc, epigen=epi)
+    sc.params.secrete_scale = 0.5
+    o = make(sc, precision=64)
+    o.Run2PhysicalSort()
+    o.ComputeGenesCL()
+    c = o.download(F.FCONC)[0].astype(np.float64)
+    e = o.download(F.FEPIGEN)[:, 0]
+    act0 = float(np.dot(np.arange(F.NUM_TF), conc[0].astype(np.float64)))
+    act2 = act0  # gene 2 has the same sensitivities and secretes nothing
+    exp = conc[0].astype(np.float64).copy()
+    exp[1] += 1 * act0 * 0.5
+    exp[3] += 4 * act0 * 0.5
+    np.testing.assert_allclose(c, exp, rtol=1e-6)
+    assert act0 > 6 and e[5] == 1 and e[0] == scenes.INT_MAX and e[2] == scenes.INT_MAX
+    assert act2 == act0
+
+
+def test_gene_countdown_and_living_list(built):
+    g = scenes.example_genome()
+    for i in range(F.NUM_GENES):
+        g.delay[i] = 7
+    epi = np.zeros((F.NUM_GENES, 2), np.uint32)
+    epi[4, :] = 1          # counts down to 0 and starts gene 5's countdown
+    epi[5, :] = 123        # active gene 5 -> set to delay[5] = 7 (then not counted again)
+    epi[9, :] = 3          # 3 -> 2
+    epi[2, 0] = 1          # particle 0 is living (and gene 2 counts down 1 -> 0)
+    conc = np.ones((2, F.NUM_TF), np.float32)
+    sc = mini_scene([[5, 5, 5], [8, 8, 8]], b=0, genome=g, conc=conc, epigen=epi)
+    o = make(sc)
+    o.Run2PhysicalSort()
+    o.ComputeGenesCL()
+    e = o.download(F.FEPIGEN)
+    # gene 4: 1 -> 0 and gene 5 := delay[5]; gene 5 is then processed with delay 7 -> 6
+    assert list(e[4]) == [0, 0] and list(e[5]) == [6, 6] and list(e[9]) == [2, 2]
+    assert e[2, 0] == 0
+    c = o.download(F.FCONC)
+    # no gene in the list secretes; the living particle's conc is unchanged, as is the other
+    np.testing.assert_array_equal(c, conc)
+
+
+# ------------------------------------------------------------------ whole step
+def test_oracle_deterministic_and_fp64_drift_small(built):
+    sc = scenes.tissue(n_side=8, b=16, seed=21)
+    runs = []
+    for prec in (32, 32, 64):
+        o = make(sc, precision=prec)
+        o.Run2PhysicalSort()
+        o.step(20, sc.dt, sc.flags)
+        runs.append(o.download(F.FPOS)[:, :3])
+    np.testing.assert_array_equal(runs[0], runs[1])
+    rel = np.abs(runs[0].astype(np.float64) - runs[2]).max() / np.abs(runs[2]).max()
+    assert rel < 1e-5, rel
