@@ -208,12 +208,12 @@ struct mg_handle {
     MgGenome gen;
     bool have_params = false, have_grid = false;
     hipStream_t st = nullptr;
-    /* sorted ping-pong halves: index `cur` is current for P/V/VEL/HOME/GCELL/GNDX/GMASK,
-     * `ccur` for CONC (diffusion flips it on its own) */
+    /* sorted ping-pong halves; current-half indices: kcur for P/V/VEL, icur for
+     * HOME/GCELL/GNDX/GMASK, ccur for CONC (each group flips on its own) */
     float4 *P[2] = {}, *V[2] = {}, *VEL[2] = {};
     float* CONC[2] = {};
     uint32_t *HOME[2] = {}, *GCELL[2] = {}, *GNDX[2] = {}, *GMASK[2] = {};
-    int cur = 0, ccur = 0;
+    int kcur = 0, icur = 0, ccur = 0, gcur = 0;
     float4* F = nullptr;
     bool force_zero = true;
     /* home-indexed */
@@ -226,8 +226,10 @@ struct mg_handle {
     uint32_t *INC_OFF = nullptr, *INC = nullptr;
     size_t inc_cap = 0;
     /* grid */
-    uint32_t *CNT = nullptr, *OFF = nullptr, *GCNT = nullptr, *GOFF = nullptr, *LENS = nullptr;
-    uint32_t *LISTS = nullptr, *PERM = nullptr;
+    uint32_t *CNT[2] = {}, *OFF[2] = {};  /* gcur is current; the fused step bins into 1-gcur */
+    uint32_t *GCNT = nullptr, *GOFF = nullptr, *LENS = nullptr;
+    uint32_t *LISTS = nullptr, *PERM = nullptr, *GMN = nullptr;
+    bool gene_grid_stale = false;         /* gene counts/offsets/lists not rebuilt since a fused step */
     uint32_t* ctr = nullptr;
     void* staging = nullptr;
     size_t staging_bytes = 0;
@@ -243,6 +245,13 @@ struct mg_handle {
     std::vector<Mark> marks;
     std::vector<hipEvent_t> pool;
     size_t pool_used = 0;
+    /* two captured fused steps (mg_step with MG_STEP_GRAPH) and what they were captured for */
+    hipGraphExec_t graph = nullptr;
+    uint64_t epoch = 1;  /* bumped whenever kernel arguments or buffer addresses change */
+    uint64_t graph_epoch = 0;
+    float graph_dt = 0.f;
+    unsigned graph_flags = 0;
+    int graph_parity = -1;
 };
 
 namespace {
@@ -311,24 +320,34 @@ int staging(mg_handle* h, size_t bytes)
     h->staging_bytes = 0;
     CHK(dalloc(h, (uint8_t**)&h->staging, bytes));
     h->staging_bytes = bytes;
+    h->epoch++;
     return MG_OK;
 }
 
-DS make_ds(mg_handle* h)
+/* Kernel view of the handle. `kin` / `conc` select which half is read as current (0) or
+ * as the other half (1): the fused step reads the advanced kinematics and the diffused
+ * concentrations from the "other" halves. */
+DS make_ds(mg_handle* h, int kin = 0, int conc = 0)
 {
     DS d;
-    const int c = h->cur, o = 1 - h->cur;
-    d.P = h->P[c]; d.V = h->V[c]; d.VEL = h->VEL[c]; d.F = h->F;
-    d.CONC = h->CONC[h->ccur];
-    d.HOME = h->HOME[c]; d.GCELL = h->GCELL[c]; d.GNDX = h->GNDX[c]; d.GMASK = h->GMASK[c];
-    d.P2 = h->P[o]; d.V2 = h->V[o]; d.VEL2 = h->VEL[o];
-    d.CONC2 = h->CONC[1 - h->ccur];
-    d.HOME2 = h->HOME[o]; d.GCELL2 = h->GCELL[o]; d.GNDX2 = h->GNDX[o]; d.GMASK2 = h->GMASK[o];
+    const int kc = kin ? 1 - h->kcur : h->kcur, ko = 1 - kc;
+    const int cc = conc ? 1 - h->ccur : h->ccur;
+    const int ic = h->icur, io = 1 - h->icur;
+    d.P = h->P[kc]; d.V = h->V[kc]; d.VEL = h->VEL[kc]; d.F = h->F;
+    d.P2 = h->P[ko]; d.V2 = h->V[ko]; d.VEL2 = h->VEL[ko];
+    d.CONC = h->CONC[cc]; d.CONC2 = h->CONC[1 - cc];
+    d.HOME = h->HOME[ic]; d.GCELL = h->GCELL[ic]; d.GNDX = h->GNDX[ic]; d.GMASK = h->GMASK[ic];
+    d.HOME2 = h->HOME[io]; d.GCELL2 = h->GCELL[io]; d.GNDX2 = h->GNDX[io]; d.GMASK2 = h->GMASK[io];
     d.SLOT = h->SLOT; d.EPIGEN = h->EPIGEN; d.BPARTNER = h->BPARTNER;
     d.BPAR = h->BPAR; d.BINT = h->BINT;
     d.INC_OFF = h->INC_OFF; d.INC = h->INC;
-    d.CNT = h->CNT; d.OFF = h->OFF; d.GCNT = h->GCNT; d.GOFF = h->GOFF;
+    d.CNT = h->CNT[h->gcur]; d.OFF = h->OFF[h->gcur];
+    d.CNTN = h->CNT[1 - h->gcur]; d.OFFN = h->OFF[1 - h->gcur];
+    d.GCNT = h->GCNT; d.GOFF = h->GOFF;
     d.PERM = h->PERM; d.LISTS = h->LISTS; d.LENS = h->LENS;
+    d.GMN = nullptr;
+    d.ZERO = nullptr;
+    d.zero_n = 0;
     d.ctr = h->ctr;
     d.N = h->N; d.B = h->B; d.C = h->C;
     return d;
@@ -374,6 +393,7 @@ int rebuild_inc(mg_handle* h)
         CHK(dalloc(h, &h->INC, ent.size()));
         h->inc_cap = ent.size();
     }
+    h->epoch++;
     HIPCHK(hipMemcpyAsync(h->INC_OFF, off.data(), off.size() * 4, hipMemcpyHostToDevice, h->st));
     HIPCHK(hipMemcpyAsync(h->INC, ent.data(), ent.size() * 4, hipMemcpyHostToDevice, h->st));
     HIPCHK(hipStreamSynchronize(h->st));
@@ -386,7 +406,7 @@ int normalize(mg_handle* h)
 {
     if (h->normalized) return MG_OK;
     const int N = h->N, B = h->B;
-    const uint32_t* HOME = h->HOME[h->cur];
+    const uint32_t* HOME = h->HOME[h->icur];
     auto regather = [&](uint32_t** field, int keep_stale, uint32_t reset) -> int {
         uint32_t* nf = nullptr;
         CHK(dalloc(h, &nf, N));
@@ -428,13 +448,14 @@ int normalize(mg_handle* h)
         dfree(h, h->BPARTNER); dfree(h, h->BPAR); dfree(h, h->BINT); dfree(h, h->BCOLD); dfree(h, h->PIDX);
         h->BPARTNER = nP; h->BPAR = nPar; h->BINT = nInt; h->BCOLD = nCold; h->PIDX = nPidx;
     }
-    hipLaunchKernelGGL(k_iota, dim3(nblk(N)), dim3(256), 0, h->st, h->HOME[h->cur], N);
+    hipLaunchKernelGGL(k_iota, dim3(nblk(N)), dim3(256), 0, h->st, h->HOME[h->icur], N);
     hipLaunchKernelGGL(k_iota, dim3(nblk(N)), dim3(256), 0, h->st, h->SLOT, N);
     HIPCHK(hipGetLastError());
     uint32_t tail = (uint32_t)N;
     HIPCHK(hipMemcpyAsync(h->ctr + MG_C_TAIL, &tail, 4, hipMemcpyHostToDevice, h->st));
     HIPCHK(hipStreamSynchronize(h->st));
     h->normalized = true;
+    h->epoch++;
     return rebuild_inc(h);
 }
 
@@ -444,8 +465,8 @@ int do_insert(mg_handle* h, int gene_counts)
     DS d = make_ds(h);
     {
         PhaseScope ps(h, MG_PH_FILL);
-        CHK(fill_u32(h, h->CNT, 0, h->C));
-        CHK(fill_u32(h, h->OFF, 0, h->C));
+        CHK(fill_u32(h, h->CNT[h->gcur], 0, h->C));
+        CHK(fill_u32(h, h->OFF[h->gcur], 0, h->C));
         if (gene_counts) {
             CHK(fill_u32(h, h->GCNT, 0, (size_t)MG_NUM_GENES * h->C));
             CHK(fill_u32(h, h->GOFF, 0, (size_t)MG_NUM_GENES * h->C));
@@ -465,7 +486,7 @@ int grid_scan_max_index(const MgGrid& g)
 int do_scan(mg_handle* h, int genes)
 {
     PhaseScope ps(h, MG_PH_SCAN);
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, h->st, h->CNT, h->OFF, h->C, h->ctr, 1,
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, h->st, h->CNT[h->gcur], h->OFF[h->gcur], h->C, h->ctr, 1,
                        grid_scan_max_index(h->g), (uint32_t*)nullptr);
     HIPCHK(hipGetLastError());
     if (genes) {
@@ -488,10 +509,10 @@ int do_sort(mg_handle* h, int lists)
         PhaseScope ps(h, MG_PH_SORT);
         const float vmax = std::max(std::max(h->g.gmax[0], h->g.gmax[1]), h->g.gmax[2]);
         hipLaunchKernelGGL(k_sort_gather, dim3(nblk(h->N)), dim3(256), 0, h->st, d, vmax);
-        hipLaunchKernelGGL(k_sort_finish, dim3(1), dim3(64), 0, h->st, h->ctr);
     }
     HIPCHK(hipGetLastError());
-    h->cur = 1 - h->cur;
+    h->kcur = 1 - h->kcur;
+    h->icur = 1 - h->icur;
     h->ccur = 1 - h->ccur;
     h->force_zero = true;
     h->normalized = false;
@@ -502,6 +523,121 @@ int do_sort(mg_handle* h, int lists)
         hipLaunchKernelGGL(k_dense_lists, dim3(nblk(h->C)), dim3(256), 0, h->st, d2);
         HIPCHK(hipGetLastError());
     }
+    return MG_OK;
+}
+
+/* Rebuild FBIN_COUNT_ACTIVE_GENES / FBIN_OFFSET_ACTIVE_GENES / FDENSE_LIST_LENGTHS /
+ * FDENSE_LISTS from the sorted state after fused steps (which skip them): the values the
+ * reference's insert + scan + dense-list kernels leave after a sort. */
+int materialize_gene_grid(mg_handle* h)
+{
+    if (!h->gene_grid_stale) return MG_OK;
+    PhaseScope ps(h, MG_PH_LISTS);
+    DS d = make_ds(h);
+    CHK(fill_u32(h, h->GCNT, 0, (size_t)MG_NUM_GENES * h->C));
+    hipLaunchKernelGGL(k_gene_counts_sorted, dim3(nblk(h->N)), dim3(256), 0, h->st, d);
+    hipLaunchKernelGGL(k_scan, dim3(MG_NUM_GENES), dim3(1024), 0, h->st, h->GCNT, h->GOFF, h->C,
+                       h->ctr, 0, 0, h->LENS);
+    CHK(fill_u32(h, h->LISTS, MG_NONE, (size_t)MG_NUM_GENES * h->N));
+    hipLaunchKernelGGL(k_dense_lists, dim3(nblk(h->C)), dim3(256), 0, h->st, d);
+    HIPCHK(hipGetLastError());
+    h->gene_grid_stale = false;
+    return MG_OK;
+}
+
+/* One step of the hot loop with the fused kernels. Starts and ends in the sorted state with
+ * FFORCE = 0 (as after CountingSortFull). */
+int do_fused_step(mg_handle* h, float dt, unsigned flags)
+{
+    const int genes = (flags & MG_STEP_GENES) ? 1 : 0;
+    const unsigned nb = nblk(h->N);
+    {
+        PhaseScope ps(h, MG_PH_PRESSURE);
+        DS d = make_ds(h);
+        d.ZERO = h->CNT[1 - h->gcur];
+        d.zero_n = (uint32_t)h->C;
+        hipLaunchKernelGGL(k_pressure, dim3(nb), dim3(256), 0, h->st, d, h->p, h->g);
+    }
+    {
+        PhaseScope ps(h, MG_PH_FORCE);
+        hipLaunchKernelGGL(k_force_advance, dim3(nb), dim3(256), 0, h->st, make_ds(h), h->p, h->g, dt);
+    }
+    {
+        PhaseScope ps(h, MG_PH_DIFFUSION);
+        DS d = make_ds(h, 1, 0);
+        d.GMN = h->GMN;
+        hipLaunchKernelGGL(k_diffuse_genes_insert, dim3(nb), dim3(256), 0, h->st, d, h->p, h->g,
+                           h->gen, genes);
+    }
+    {
+        PhaseScope ps(h, MG_PH_SCAN);
+        hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, h->st, h->CNT[1 - h->gcur],
+                           h->OFF[1 - h->gcur], h->C, h->ctr, 1, grid_scan_max_index(h->g),
+                           (uint32_t*)nullptr);
+    }
+    {
+        PhaseScope ps(h, MG_PH_PLACE);
+        DS d = make_ds(h);
+        d.CNT = d.CNTN;
+        d.OFF = d.OFFN;
+        hipLaunchKernelGGL(k_place, dim3(nb), dim3(256), 0, h->st, d);
+        hipLaunchKernelGGL(k_cellsort, dim3(nblk(h->C)), dim3(256), 0, h->st, d);
+    }
+    {
+        PhaseScope ps(h, MG_PH_SORT);
+        DS d = make_ds(h, 1, 1);
+        d.GMN = h->GMN;
+        const float vmax = std::max(std::max(h->g.gmax[0], h->g.gmax[1]), h->g.gmax[2]);
+        hipLaunchKernelGGL(k_sort_gather, dim3(nb), dim3(256), 0, h->st, d, vmax);
+    }
+    HIPCHK(hipGetLastError());
+    h->icur = 1 - h->icur;
+    h->gcur = 1 - h->gcur;
+    h->force_zero = true;
+    h->normalized = false;
+    h->gene_grid_stale = true;
+    return MG_OK;
+}
+
+/* Two fused steps captured once into a hipGraph and replayed; the step has a two-step
+ * period in its buffer roles (icur, gcur flip every step), so a two-step graph leaves the
+ * handle's indices where they started. Recaptured when dt, flags, the parities or the
+ * handle's epoch change. */
+int graph_steps(mg_handle* h, int pairs, float dt, unsigned flags)
+{
+    const int parity = (h->kcur << 3) | (h->icur << 2) | (h->ccur << 1) | h->gcur;
+    if (!h->graph || h->graph_epoch != h->epoch || h->graph_dt != dt || h->graph_flags != flags ||
+        h->graph_parity != parity) {
+        if (h->graph) {
+            (void)hipGraphExecDestroy(h->graph);
+            h->graph = nullptr;
+        }
+        hipGraph_t g = nullptr;
+        HIPCHK(hipStreamBeginCapture(h->st, hipStreamCaptureModeThreadLocal));
+        int r1 = do_fused_step(h, dt, flags);
+        int r2 = (r1 == MG_OK) ? do_fused_step(h, dt, flags) : r1;
+        hipError_t ec = hipStreamEndCapture(h->st, &g);
+        if (r1 != MG_OK || r2 != MG_OK) return r1 != MG_OK ? r1 : r2;
+        if (ec != hipSuccess) {
+            set_err("hipStreamEndCapture failed: %s", hipGetErrorString(ec));
+            return MG_ERR_HIP;
+        }
+        hipError_t ei = hipGraphInstantiate(&h->graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ei != hipSuccess) {
+            h->graph = nullptr;
+            set_err("hipGraphInstantiate failed: %s", hipGetErrorString(ei));
+            return MG_ERR_HIP;
+        }
+        h->graph_epoch = h->epoch;
+        h->graph_dt = dt;
+        h->graph_flags = flags;
+        h->graph_parity = parity;
+    }
+    for (int k = 0; k < pairs; k++) HIPCHK(hipGraphLaunch(h->graph, h->st));
+    h->force_zero = true;
+    h->normalized = false;
+    h->gene_grid_stale = true;
     return MG_OK;
 }
 
@@ -620,6 +756,7 @@ int mg_create(mg_handle** out, int max_points, int bonds_per_particle)
     if (r == MG_OK) r = dalloc(h, &h->FBIN, N);
     if (r == MG_OK) r = dalloc(h, &h->INC_OFF, N + 1);
     if (r == MG_OK) r = dalloc(h, &h->PERM, N);
+    if (r == MG_OK) r = dalloc(h, &h->GMN, N);
     if (r == MG_OK) r = dalloc(h, &h->LISTS, N * MG_NUM_GENES);
     if (r == MG_OK) r = dalloc(h, &h->LENS, MG_NUM_GENES);
     if (r == MG_OK) r = dalloc(h, &h->ctr, MG_C_COUNT);
@@ -675,6 +812,7 @@ int mg_destroy(mg_handle* h)
     h->allocs.clear();
     for (hipEvent_t e : h->pool) (void)hipEventDestroy(e);
     h->pool.clear();
+    if (h->graph) (void)hipGraphExecDestroy(h->graph);
     if (h->st) (void)hipStreamDestroy(h->st);
     delete h;
     return MG_OK;
@@ -696,22 +834,26 @@ int mg_set_grid(mg_handle* h, const MgGrid* g)
         return MG_ERR_ARG;
     }
     if (h->have_grid && g->total != h->C) {
-        dfree(h, h->CNT); dfree(h, h->OFF); dfree(h, h->GCNT); dfree(h, h->GOFF);
+        for (int k = 0; k < 2; k++) { dfree(h, h->CNT[k]); dfree(h, h->OFF[k]); }
+        dfree(h, h->GCNT); dfree(h, h->GOFF);
         h->have_grid = false;
     }
     if (!h->have_grid) {
         h->C = g->total;
-        CHK(dalloc(h, &h->CNT, h->C));
-        CHK(dalloc(h, &h->OFF, h->C));
+        for (int k = 0; k < 2; k++) {
+            CHK(dalloc(h, &h->CNT[k], h->C));
+            CHK(dalloc(h, &h->OFF[k], h->C));
+            CHK(fill_u32(h, h->CNT[k], 0, h->C));
+            CHK(fill_u32(h, h->OFF[k], 0, h->C));
+        }
         CHK(dalloc(h, &h->GCNT, (size_t)MG_NUM_GENES * h->C));
         CHK(dalloc(h, &h->GOFF, (size_t)MG_NUM_GENES * h->C));
-        CHK(fill_u32(h, h->CNT, 0, h->C));
-        CHK(fill_u32(h, h->OFF, 0, h->C));
         CHK(fill_u32(h, h->GCNT, 0, (size_t)MG_NUM_GENES * h->C));
         CHK(fill_u32(h, h->GOFF, 0, (size_t)MG_NUM_GENES * h->C));
     }
     h->g = *g;
     h->have_grid = true;
+    h->epoch++;
     return MG_OK;
 }
 
@@ -733,6 +875,7 @@ int mg_set_params(mg_handle* h, const MgParams* p)
     if (!h || !p) { set_err("mg_set_params: null argument"); return MG_ERR_ARG; }
     h->p = *p;
     h->have_params = true;
+    h->epoch++;
     return MG_OK;
 }
 
@@ -757,6 +900,7 @@ int mg_set_genome(mg_handle* h, const MgGenome* g)
 {
     if (!h || !g) { set_err("mg_set_genome: null argument"); return MG_ERR_ARG; }
     h->gen = *g;
+    h->epoch++;
     return MG_OK;
 }
 
@@ -764,6 +908,7 @@ int mg_set_freeze(mg_handle* h, int freeze)
 {
     if (!h) { set_err("mg_set_freeze: null handle"); return MG_ERR_ARG; }
     h->p.freeze = freeze ? 1 : 0;
+    h->epoch++;
     return MG_OK;
 }
 
@@ -780,9 +925,10 @@ int mg_upload(mg_handle* h, int buf_id, const void* src, size_t bytes)
     if (bytes != need) { set_err("mg_upload: buffer %d needs %zu bytes, got %zu", buf_id, need, bytes); return MG_ERR_SIZE; }
     if (need == 0) return MG_OK;
     CHK(normalize(h));
+    if (buf_id >= MG_FBIN_COUNT_ACTIVE_GENES && buf_id <= MG_FDENSE_LISTS) CHK(materialize_gene_grid(h));
     CHK(staging(h, bytes));
     HIPCHK(hipMemcpyAsync(h->staging, src, bytes, hipMemcpyHostToDevice, h->st));
-    const int c = h->cur;
+    const int c = h->kcur, ic = h->icur;
     const unsigned nb = nblk(N);
     void* stg = h->staging;
     switch (buf_id) {
@@ -797,8 +943,8 @@ int mg_upload(mg_handle* h, int buf_id, const void* src, size_t bytes)
     case MG_FDENSITY: hipLaunchKernelGGL(k_set_w, dim3(nb), dim3(256), 0, h->st, h->V[c], (const float*)stg, N); break;
     case MG_FAGE: HIPCHK(hipMemcpyAsync(h->AGE, stg, bytes, hipMemcpyDeviceToDevice, h->st)); break;
     case MG_FCOLOR: HIPCHK(hipMemcpyAsync(h->COLOR, stg, bytes, hipMemcpyDeviceToDevice, h->st)); break;
-    case MG_FGCELL: HIPCHK(hipMemcpyAsync(h->GCELL[c], stg, bytes, hipMemcpyDeviceToDevice, h->st)); break;
-    case MG_FGNDX: HIPCHK(hipMemcpyAsync(h->GNDX[c], stg, bytes, hipMemcpyDeviceToDevice, h->st)); break;
+    case MG_FGCELL: HIPCHK(hipMemcpyAsync(h->GCELL[ic], stg, bytes, hipMemcpyDeviceToDevice, h->st)); break;
+    case MG_FGNDX: HIPCHK(hipMemcpyAsync(h->GNDX[ic], stg, bytes, hipMemcpyDeviceToDevice, h->st)); break;
     case MG_FNERVEIDX: HIPCHK(hipMemcpyAsync(h->NERVE, stg, bytes, hipMemcpyDeviceToDevice, h->st)); break;
     case MG_FPARTICLE_ID: HIPCHK(hipMemcpyAsync(h->PID, stg, bytes, hipMemcpyDeviceToDevice, h->st)); break;
     case MG_FMASS_RADIUS: HIPCHK(hipMemcpyAsync(h->MRAD, stg, bytes, hipMemcpyDeviceToDevice, h->st)); break;
@@ -823,8 +969,8 @@ int mg_upload(mg_handle* h, int buf_id, const void* src, size_t bytes)
         if (B > 0)
             hipLaunchKernelGGL(k_upload_pidx, dim3(nb, B), dim3(256), 0, h->st, h->PIDX, (const uint32_t*)stg, N, B);
         break;
-    case MG_FBIN_COUNT: HIPCHK(hipMemcpyAsync(h->CNT, stg, bytes, hipMemcpyDeviceToDevice, h->st)); break;
-    case MG_FBIN_OFFSET: HIPCHK(hipMemcpyAsync(h->OFF, stg, bytes, hipMemcpyDeviceToDevice, h->st)); break;
+    case MG_FBIN_COUNT: HIPCHK(hipMemcpyAsync(h->CNT[h->gcur], stg, bytes, hipMemcpyDeviceToDevice, h->st)); break;
+    case MG_FBIN_OFFSET: HIPCHK(hipMemcpyAsync(h->OFF[h->gcur], stg, bytes, hipMemcpyDeviceToDevice, h->st)); break;
     case MG_FBIN_COUNT_ACTIVE_GENES: HIPCHK(hipMemcpyAsync(h->GCNT, stg, bytes, hipMemcpyDeviceToDevice, h->st)); break;
     case MG_FBIN_OFFSET_ACTIVE_GENES: HIPCHK(hipMemcpyAsync(h->GOFF, stg, bytes, hipMemcpyDeviceToDevice, h->st)); break;
     case MG_FDENSE_LIST_LENGTHS: HIPCHK(hipMemcpyAsync(h->LENS, stg, bytes, hipMemcpyDeviceToDevice, h->st)); break;
@@ -848,11 +994,12 @@ int mg_download(mg_handle* h, int buf_id, void* dst, size_t bytes)
     const size_t need = mg_buffer_bytes(buf_id, N, h->C, B);
     if (bytes != need) { set_err("mg_download: buffer %d needs %zu bytes, got %zu", buf_id, need, bytes); return MG_ERR_SIZE; }
     if (need == 0) return MG_OK;
+    if (buf_id >= MG_FBIN_COUNT_ACTIVE_GENES && buf_id <= MG_FDENSE_LISTS) CHK(materialize_gene_grid(h));
     CHK(staging(h, bytes));
-    const int c = h->cur;
+    const int c = h->kcur, ic = h->icur;
     const unsigned nb = nblk(N);
     void* stg = h->staging;
-    const uint32_t* HOME = h->HOME[c];
+    const uint32_t* HOME = h->HOME[ic];
     const void* direct = nullptr;
     switch (buf_id) {
     case MG_FPOS: hipLaunchKernelGGL(k_get_xyz, dim3(nb), dim3(256), 0, h->st, (float4*)stg, h->P[c], N, 0); break;
@@ -866,8 +1013,8 @@ int mg_download(mg_handle* h, int buf_id, void* dst, size_t bytes)
     case MG_FNERVEIDX: hipLaunchKernelGGL(k_gather_home, dim3(nb), dim3(256), 0, h->st, (uint32_t*)stg, h->NERVE, HOME, h->ctr, N, 1, 0u); break;
     case MG_FMASS_RADIUS: hipLaunchKernelGGL(k_gather_home, dim3(nb), dim3(256), 0, h->st, (uint32_t*)stg, h->MRAD, HOME, h->ctr, N, 1, 0u); break;
     case MG_FPARTICLE_ID: hipLaunchKernelGGL(k_gather_home, dim3(nb), dim3(256), 0, h->st, (uint32_t*)stg, h->PID, HOME, h->ctr, N, 0, MG_NONE); break;
-    case MG_FGCELL: direct = h->GCELL[c]; break;
-    case MG_FGNDX: direct = h->GNDX[c]; break;
+    case MG_FGCELL: direct = h->GCELL[ic]; break;
+    case MG_FGNDX: direct = h->GNDX[ic]; break;
     case MG_FBIN: hipLaunchKernelGGL(k_download_fbin, dim3(nb), dim3(256), 0, h->st, (uint32_t*)stg, h->FBIN, h->ctr, N); break;
     case MG_FCONC: direct = h->CONC[h->ccur]; break;
     case MG_FEPIGEN:
@@ -883,8 +1030,8 @@ int mg_download(mg_handle* h, int buf_id, void* dst, size_t bytes)
             hipLaunchKernelGGL(k_download_pidx, dim3(nb, B), dim3(256), 0, h->st, (uint32_t*)stg, h->PIDX, HOME,
                                h->SLOT, h->ctr, N, B);
         break;
-    case MG_FBIN_COUNT: direct = h->CNT; break;
-    case MG_FBIN_OFFSET: direct = h->OFF; break;
+    case MG_FBIN_COUNT: direct = h->CNT[h->gcur]; break;
+    case MG_FBIN_OFFSET: direct = h->OFF[h->gcur]; break;
     case MG_FBIN_COUNT_ACTIVE_GENES: direct = h->GCNT; break;
     case MG_FBIN_OFFSET_ACTIVE_GENES: direct = h->GOFF; break;
     case MG_FDENSE_LIST_LENGTHS: direct = h->LENS; break;
@@ -918,10 +1065,11 @@ int mg_sync(mg_handle* h)
 int mg_insert_particles(mg_handle* h, uint32_t* gcell, uint32_t* gndx, uint32_t* gcnt)
 {
     CHK(ready(h));
+    CHK(materialize_gene_grid(h));
     CHK(do_insert(h, 1));
-    if (gcell) HIPCHK(hipMemcpyAsync(gcell, h->GCELL[h->cur], (size_t)h->N * 4, hipMemcpyDeviceToHost, h->st));
-    if (gndx) HIPCHK(hipMemcpyAsync(gndx, h->GNDX[h->cur], (size_t)h->N * 4, hipMemcpyDeviceToHost, h->st));
-    if (gcnt) HIPCHK(hipMemcpyAsync(gcnt, h->CNT, (size_t)h->C * 4, hipMemcpyDeviceToHost, h->st));
+    if (gcell) HIPCHK(hipMemcpyAsync(gcell, h->GCELL[h->icur], (size_t)h->N * 4, hipMemcpyDeviceToHost, h->st));
+    if (gndx) HIPCHK(hipMemcpyAsync(gndx, h->GNDX[h->icur], (size_t)h->N * 4, hipMemcpyDeviceToHost, h->st));
+    if (gcnt) HIPCHK(hipMemcpyAsync(gcnt, h->CNT[h->gcur], (size_t)h->C * 4, hipMemcpyDeviceToHost, h->st));
     if (gcell || gndx || gcnt) HIPCHK(hipStreamSynchronize(h->st));
     return MG_OK;
 }
@@ -933,12 +1081,14 @@ int mg_prefix_sum_cells(mg_handle* h, int zero_offsets)
         set_err("mg_prefix_sum_cells: only zero_offsets=1 (exclusive offsets) is used by the step");
         return MG_ERR_UNSUPPORTED;
     }
+    CHK(materialize_gene_grid(h));
     return do_scan(h, 1);
 }
 
 int mg_counting_sort_full(mg_handle* h)
 {
     CHK(ready(h));
+    CHK(materialize_gene_grid(h));
     return do_sort(h, 1);
 }
 
@@ -963,6 +1113,7 @@ int mg_zero_vel(mg_handle* h)
 int mg_physical_sort(mg_handle* h)
 {
     CHK(ready(h));
+    CHK(materialize_gene_grid(h));
     CHK(do_insert(h, 1));
     CHK(do_scan(h, 1));
     return do_sort(h, 1);
@@ -988,7 +1139,21 @@ int mg_step(mg_handle* h, int nsteps, float dt, unsigned flags)
 {
     CHK(ready(h));
     if (nsteps < 0) { set_err("mg_step: nsteps < 0"); return MG_ERR_ARG; }
-    for (int s = 0; s < nsteps; s++) {
+    int s = 0;
+    if ((flags & MG_STEP_GRAPH) && h->force_zero && nsteps >= 2) {
+        const int pairs = nsteps / 2;
+        CHK(graph_steps(h, pairs, dt, flags & MG_STEP_GENES));
+        h->time += 2.0 * pairs * dt;
+        s = 2 * pairs;
+    }
+    for (; s < nsteps; s++) {
+        if (h->force_zero) {
+            /* the usual case: the previous operation was a sort */
+            CHK(do_fused_step(h, dt, flags));
+            h->time += dt;
+            continue;
+        }
+        CHK(materialize_gene_grid(h));
         CHK(do_pressure(h));
         CHK(do_force(h));
         CHK(do_advance(h, dt));

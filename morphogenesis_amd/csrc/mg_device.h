@@ -49,8 +49,14 @@ struct DS {
     float4* BPAR;
     float2* BINT;
     uint32_t *INC_OFF, *INC;
-    /* grid */
+    /* grid (current) and the grid the fused step bins into */
     uint32_t *CNT, *OFF, *GCNT, *GOFF, *PERM, *LISTS, *LENS;
+    uint32_t *CNTN, *OFFN;
+    /* post-gene membership recorded for the coming sort (fused step), or null */
+    uint32_t* GMN;
+    /* cleared by k_pressure when set (zero_n words) */
+    uint32_t* ZERO;
+    uint32_t zero_n;
     uint32_t* ctr;
     int N, B, C;
 };

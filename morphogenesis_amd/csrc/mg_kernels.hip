@@ -5,6 +5,14 @@
  * reads the live particle counts from device counters, so a step needs no host sync.
  * The arithmetic follows the reference kernels in src/fluid_system_cuda.cu (cited per
  * kernel) with the deterministic order described in mg_device.h.
+ *
+ * Two launch sequences share the per-particle device functions below, so they produce
+ * identical bits:
+ *   phase kernels (k_insert, k_scan, k_place, k_cellsort, k_sort_gather, k_pressure,
+ *   k_force, k_advance, k_diffusion, k_genes), one per reference phase, used by the
+ *   FluidSystem-style entry points;
+ *   fused kernels (k_force_advance, k_diffuse_genes_insert), the hot loop of mg_step:
+ *   pressure -> force+advance -> diffusion+genes+insert -> scan -> place -> cellsort -> gather.
  */
 #include "mg_device.h"
 
@@ -22,19 +30,18 @@ __global__ LAUNCH_BOUNDS_256 void k_fill_u32(uint32_t* p, uint32_t v, size_t n)
 }
 
 /* -------------------------------------------------------------- insert (a4)
- * insertParticles, src/fluid_system_cuda.cu:53-100. Lanes of one wavefront that fall in the
- * same cell (the input is cell-sorted from the previous step, so they are contiguous)
- * are counted with one atomicAdd per run: the run head adds the run length and
- * broadcasts the base. FGNDX receives a slot in [0, count) exactly as the reference's
- * atomicAdd does; k_cellsort later replaces it with the stable rank. */
-__global__ LAUNCH_BOUNDS_256 void k_insert(DS d, MgGrid g, int gene_counts)
+ * insertParticles, src/fluid_system_cuda.cu:53-100, for the particle in slot s (all lanes of
+ * the wavefront must call it). Lanes of one wavefront that fall in the same cell (the input
+ * is cell-sorted from the previous step, so they are contiguous) are counted with one
+ * atomicAdd per run: the run head adds the run length and broadcasts the base. FGNDX
+ * receives a slot in [0, count) as the reference's atomicAdd does; k_cellsort later
+ * replaces it with the stable rank. */
+__device__ __forceinline__ void insert_one(const DS& d, const MgGrid& g, uint32_t s, bool valid,
+                                           float4 p, uint32_t* CNT, uint32_t* GCNT)
 {
-    const uint32_t s = xcd_block() * blockDim.x + threadIdx.x;
-    const bool valid = s < (uint32_t)d.N;
     const uint32_t lane = __lane_id();
     uint32_t key = 0xFFFFFFFFu;
     if (valid) {
-        float4 p = d.P[s];
         float gx = (p.x - g.gmin[0]) * g.delta[0];
         float gy = (p.y - g.gmin[1]) * g.delta[1];
         float gz = (p.z - g.gmin[2]) * g.delta[2];
@@ -47,7 +54,6 @@ __global__ LAUNCH_BOUNDS_256 void k_insert(DS d, MgGrid g, int gene_counts)
             if (s < d.ctr[MG_C_TAIL]) d.SLOT[d.HOME[s]] = MG_NONE; /* particle leaves the model */
         }
     }
-    /* run-length aggregation of equal keys across the wavefront */
     uint32_t prev = __shfl_up(key, 1);
     bool head = (lane == 0) || (prev != key);
     uint64_t heads = __ballot(head);
@@ -56,18 +62,26 @@ __global__ LAUNCH_BOUNDS_256 void k_insert(DS d, MgGrid g, int gene_counts)
     uint64_t gt = heads & ~((2ull << lane) - 1ull);
     uint32_t next = gt ? (uint32_t)(__ffsll((long long)gt) - 1) : 64u;
     uint32_t base = 0;
-    if (head && key != 0xFFFFFFFFu) base = atomicAdd(&d.CNT[key], next - start);
+    if (head && key != 0xFFFFFFFFu) base = atomicAdd(&CNT[key], next - start);
     base = __shfl(base, (int)start);
     if (key != 0xFFFFFFFFu) {
         d.GCELL[s] = key;
         d.GNDX[s] = base + (lane - start);
-        if (gene_counts) {
+        if (GCNT) {
             const uint32_t h = d.HOME[s];
             for (int gene = 0; gene < MG_NUM_GENES; gene++)
                 if (d.EPIGEN[(size_t)gene * d.N + h] > 0u)
-                    atomicAdd(&d.GCNT[(size_t)gene * d.C + key], 1u);
+                    atomicAdd(&GCNT[(size_t)gene * d.C + key], 1u);
         }
     }
+}
+
+__global__ LAUNCH_BOUNDS_256 void k_insert(DS d, MgGrid g, int gene_counts)
+{
+    const uint32_t s = xcd_block() * blockDim.x + threadIdx.x;
+    const bool valid = s < (uint32_t)d.N;
+    float4 p = valid ? d.P[s] : make_float4(0.f, 0.f, 0.f, 0.f);
+    insert_one(d, g, s, valid, p, d.CNT, gene_counts ? d.GCNT : nullptr);
 }
 
 /* ----------------------------------------------------------- scan (a5, a6)
@@ -75,25 +89,23 @@ __global__ LAUNCH_BOUNDS_256 void k_insert(DS d, MgGrid g, int gene_counts)
  * src/fluid_system_cuda.cu:136-183, driven by src/host_CUDA.cpp:307-381). One 1024-thread
  * block per segment (blockIdx.x); each thread owns a contiguous run of the segment.
  * Segment 0 with `cell_counters` set also records OFF[C-1] and OFF[grid_ScanMax], which the
- * sort commits as mNumPoints and mActivePoints (src/host_CUDA.cpp:501-511). `lens`, when given,
- * receives count[C-1] + offset[C-1] per segment (tally_denselist_lengths,
+ * sort commits as mNumPoints and mActivePoints (src/host_CUDA.cpp:501-511). `lens`, when
+ * given, receives count[C-1] + offset[C-1] per segment (tally_denselist_lengths,
  * src/fluid_system_cuda.cu:185-194). */
 __global__ __launch_bounds__(1024) void k_scan(const uint32_t* in, uint32_t* out, int n,
                                                uint32_t* ctr, int cell_counters, int gsm,
                                                uint32_t* lens)
 {
     __shared__ uint32_t wsum[16];
-    __shared__ uint32_t carry_s;
     const int seg = blockIdx.x;
     in += (size_t)seg * n;
     out += (size_t)seg * n;
     const int t = threadIdx.x;
     const int per = (n + 1023) / 1024;
-    const int b0 = t * per;
+    const int b0 = min(n, t * per);
     const int b1 = min(n, b0 + per);
     uint32_t local = 0;
     for (int i = b0; i < b1; i++) local += in[i];
-    /* block exclusive scan of the per-thread sums */
     const int lane = t & 63, w = t >> 6;
     uint32_t x = local;
     for (int o = 1; o < 64; o <<= 1) {
@@ -105,7 +117,6 @@ __global__ __launch_bounds__(1024) void k_scan(const uint32_t* in, uint32_t* out
     if (t == 0) {
         uint32_t run = 0;
         for (int k = 0; k < 16; k++) { uint32_t v = wsum[k]; wsum[k] = run; run += v; }
-        carry_s = run;
     }
     __syncthreads();
     uint32_t run = wsum[w] + x - local;
@@ -118,7 +129,6 @@ __global__ __launch_bounds__(1024) void k_scan(const uint32_t* in, uint32_t* out
             ctr[MG_C_SCAN_ACTIVE] = out[gsm];
         }
     }
-    (void)carry_s;
 }
 
 /* ------------------------------------------------------ stable placement
@@ -151,17 +161,25 @@ __global__ LAUNCH_BOUNDS_256 void k_cellsort(DS d)
 
 /* ------------------------------------------------------- counting sort (a7)
  * countingSortFull (src/fluid_system_cuda.cu:196-296) with the host resets of
- * src/host_CL.cpp:1156-1206, written as a gather over destination slots: slot d < mNumPoints
- * takes the particle PERM[d]; slots past mNumPoints get the reset values (FPOS filled with
- * max(VOLMAX), FCONC zero) and keep the stale ones the reference leaves in place. Bond
- * records and incoming lists do not move: they are addressed by home index, and the
- * partner indices the reference rewrites here are resolved through SLOT. */
+ * src/host_CL.cpp:1156-1206, written as a gather over destination slots: slot dd <
+ * mNumPoints takes the particle PERM[dd]; slots past mNumPoints get the reset values (FPOS
+ * filled with max(VOLMAX), FCONC zero) and keep the stale ones the reference leaves in
+ * place. Bond records and incoming lists do not move: they are addressed by home index,
+ * and the partner indices the reference rewrites here are resolved through SLOT.
+ * Reads P/V/VEL/CONC/HOME/GCELL/GNDX, writes their "2" halves. The gene membership comes
+ * from GMN (precomputed) when given, else from EPIGEN. Block 0 commits the counters. */
 __global__ LAUNCH_BOUNDS_256 void k_sort_gather(DS d, float vmax)
 {
     const uint32_t dd = xcd_block() * blockDim.x + threadIdx.x;
-    if (dd >= (uint32_t)d.N) return;
     const uint32_t nump = d.ctr[MG_C_SCAN_NUM];
-    const float4* csrc;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        d.ctr[MG_C_NUM_POINTS] = nump;
+        d.ctr[MG_C_ACTIVE] = d.ctr[MG_C_SCAN_ACTIVE];
+        d.ctr[MG_C_TAIL] = nump;
+        if (nump > d.ctr[MG_C_HWM]) d.ctr[MG_C_HWM] = nump;
+        d.ctr[MG_C_LISTS] = 1u;
+    }
+    if (dd >= (uint32_t)d.N) return;
     float4* cdst = (float4*)(d.CONC2 + (size_t)dd * MG_NUM_TF);
     if (dd < nump) {
         const uint32_t s = d.PERM[dd];
@@ -169,14 +187,18 @@ __global__ LAUNCH_BOUNDS_256 void k_sort_gather(DS d, float vmax)
         d.P2[dd] = d.P[s];
         d.V2[dd] = d.V[s];
         d.VEL2[dd] = d.VEL[s];
-        csrc = (const float4*)(d.CONC + (size_t)s * MG_NUM_TF);
+        const float4* csrc = (const float4*)(d.CONC + (size_t)s * MG_NUM_TF);
         cdst[0] = csrc[0]; cdst[1] = csrc[1]; cdst[2] = csrc[2]; cdst[3] = csrc[3];
         d.HOME2[dd] = h;
         d.GCELL2[dd] = d.GCELL[s];
         d.GNDX2[dd] = d.GNDX[s];
         uint32_t m = 0;
-        for (int gene = 0; gene < MG_NUM_GENES; gene++)
-            if (d.EPIGEN[(size_t)gene * d.N + h] > 0u) m |= 1u << gene;
+        if (d.GMN) {
+            m = d.GMN[s];
+        } else {
+            for (int gene = 0; gene < MG_NUM_GENES; gene++)
+                if (d.EPIGEN[(size_t)gene * d.N + h] > 0u) m |= 1u << gene;
+        }
         d.GMASK2[dd] = m;
         d.SLOT[h] = dd;
     } else {
@@ -190,18 +212,6 @@ __global__ LAUNCH_BOUNDS_256 void k_sort_gather(DS d, float vmax)
         d.GCELL2[dd] = d.GCELL[dd];
         d.GNDX2[dd] = d.GNDX[dd];
         d.GMASK2[dd] = 0u;
-    }
-}
-
-__global__ void k_sort_finish(uint32_t* ctr)
-{
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        const uint32_t nump = ctr[MG_C_SCAN_NUM];
-        ctr[MG_C_NUM_POINTS] = nump;
-        ctr[MG_C_ACTIVE] = ctr[MG_C_SCAN_ACTIVE];
-        ctr[MG_C_TAIL] = nump;
-        if (nump > ctr[MG_C_HWM]) ctr[MG_C_HWM] = nump;
-        ctr[MG_C_LISTS] = 1u;
     }
 }
 
@@ -227,12 +237,28 @@ __global__ LAUNCH_BOUNDS_256 void k_dense_lists(DS d)
     }
 }
 
+/* Gene counts per cell from the sorted state (rebuilds the gene grid after fused steps). */
+__global__ LAUNCH_BOUNDS_256 void k_gene_counts_sorted(DS d)
+{
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= d.ctr[MG_C_NUM_POINTS]) return;
+    const uint32_t c = d.GCELL[q];
+    if (c == MG_GRID_UNDEF) return;
+    const uint32_t m = d.GMASK[q];
+    for (int g = 0; g < MG_NUM_GENES; g++)
+        if (m & (1u << g)) atomicAdd(&d.GCNT[(size_t)g * d.C + c], 1u);
+}
+
 /* ------------------------------------------------------------ pressure (a9)
  * contributePressure/computePressure, src/fluid_system_cuda.cu:620-712. Per cell partial
  * sums are added to the running total in gridAdj order. Writes FPRESS into P.w and
- * FDENSITY (= 1/density) into V.w. */
+ * FDENSITY (= 1/density) into V.w. Also clears d.ZERO (the next grid's counters) when set. */
 __global__ LAUNCH_BOUNDS_256 void k_pressure(DS d, MgParams p, MgGrid g)
 {
+    if (d.ZERO) {
+        for (uint32_t z = blockIdx.x * blockDim.x + threadIdx.x; z < d.zero_n; z += gridDim.x * blockDim.x)
+            d.ZERO[z] = 0u;
+    }
     const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= d.ctr[MG_C_ACTIVE]) return;
     uint32_t gc = d.GCELL[i];
@@ -269,20 +295,16 @@ __global__ LAUNCH_BOUNDS_256 void k_pressure(DS d, MgParams p, MgGrid g)
 }
 
 /* --------------------------------------------------------------- force (a10)
- * contributeForce/computeForce, src/fluid_system_cuda.cu:3167-3358. The owner subtracts its
- * bond terms in slot order, adds pmass * (sum over cells of the per-cell fluid sums); the
- * atomicAdd scatter of bond terms to partners is replaced by each particle gathering the
- * terms of its incoming bonds (same expression, same operands, hence the same bits) in
- * (owner home, slot) order after its own total:
- *   FFORCE[i] = ((FFORCE[i] + own) + in_1) + in_2 ...
- * `accumulate` = 0 starts from zero (FFORCE was reset by the sort). */
-__global__ LAUNCH_BOUNDS_256 void k_force(DS d, MgParams p, MgGrid g, int accumulate)
+ * contributeForce/computeForce, src/fluid_system_cuda.cu:3167-3358, for active particle i in
+ * cell gc (not GRID_UNDEF). The owner subtracts its bond terms in slot order and adds pmass *
+ * (sum over cells of the per-cell fluid sums); the atomicAdd scatter of bond terms to
+ * partners is replaced by each particle gathering the terms of its incoming bonds (same
+ * expression, same operands, hence the same bits) in (owner home, slot) order:
+ *   FFORCE[i] = ((F0 + own) + in_1) + in_2 ...
+ * Updates the bond integrators and breaks bonds as the reference does. */
+__device__ __forceinline__ f3 force_one(const DS& d, const MgParams& p, const MgGrid& g,
+                                        uint32_t i, uint32_t na, uint32_t gc, f3 F0)
 {
-    const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
-    const uint32_t na = d.ctr[MG_C_ACTIVE];
-    if (i >= na) return;
-    uint32_t gc = d.GCELL[i];
-    if (gc == MG_GRID_UNDEF) return;
     gc -= (uint32_t)((1 * g.res[2] + 1) * g.res[0] + 1);
     const int N = d.N, B = d.B;
     const uint32_t h = d.HOME[i];
@@ -332,9 +354,7 @@ __global__ LAUNCH_BOUNDS_256 void k_force(DS d, MgParams p, MgGrid g, int accumu
     }
     force = force + fsum * p.pmass;
 
-    f3 F = accumulate ? xyz(d.F[i]) : mk3(0.0f, 0.0f, 0.0f);
-    F = F + force;
-    /* incoming bonds */
+    f3 F = F0 + force;
     const uint32_t e0 = d.INC_OFF[h], e1 = d.INC_OFF[h + 1];
     for (uint32_t e = e0; e < e1; e++) {
         const uint32_t packed = d.INC[e];
@@ -347,20 +367,28 @@ __global__ LAUNCH_BOUNDS_256 void k_force(DS d, MgParams p, MgGrid g, int accumu
         f3 eterm = bond_term(xyz(d.P[k]), xyz(d.V[k]), ipos, ivev, bp, p.pmass, &strain, &abs_dist);
         if (!has_nan3(eterm)) F = F + eterm;
     }
+    return F;
+}
+
+/* ComputeForceCL: FFORCE[i] += force (accumulate) or = 0 + force. */
+__global__ LAUNCH_BOUNDS_256 void k_force(DS d, MgParams p, MgGrid g, int accumulate)
+{
+    const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
+    const uint32_t na = d.ctr[MG_C_ACTIVE];
+    if (i >= na) return;
+    const uint32_t gc = d.GCELL[i];
+    if (gc == MG_GRID_UNDEF) return;
+    f3 F0 = accumulate ? xyz(d.F[i]) : mk3(0.0f, 0.0f, 0.0f);
+    f3 F = force_one(d, p, g, i, na, gc, F0);
     d.F[i] = make_float4(F.x, F.y, F.z, 0.0f);
 }
 
 /* ------------------------------------------------------------- advance (a11)
- * advanceParticles, src/fluid_system_cuda.cu:3470-3614, in place. With the copies of
- * src/host_CL.cpp:199-226 around it, a GRID_UNDEF particle ends unchanged. */
-__global__ LAUNCH_BOUNDS_256 void k_advance(DS d, MgParams p, float dt)
+ * advanceParticles, src/fluid_system_cuda.cu:3470-3614, for one particle with acceleration
+ * `accel` (FFORCE). Returns the new position, veval and velocity. */
+__device__ __forceinline__ void advance_one(const MgParams& p, float dt, f3 pos, f3 vev, f3 v,
+                                            f3 accel, f3* np_out, f3* ve_out, f3* vnext_out)
 {
-    const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
-    if (i >= d.ctr[MG_C_ACTIVE]) return;
-    if (d.GCELL[i] == MG_GRID_UNDEF) return;
-    const float4 P_i = d.P[i], V_i = d.V[i];
-    f3 pos = xyz(P_i), vev = xyz(V_i);
-    f3 accel = xyz(d.F[i]);
     const float eps = MG_EPSILON, prad = p.pradius, slope = p.pground_slope;
     const float estiff = p.pextstiff, damp = p.pdamp;
     const f3 bmin = mk3(p.pboundmin[0], p.pboundmin[1], p.pboundmin[2]);
@@ -413,47 +441,88 @@ __global__ LAUNCH_BOUNDS_256 void k_advance(DS d, MgParams p, float dt)
     accel = accel + mk3(p.pgravity[0], p.pgravity[1], p.pgravity[2]);
     speed = accel.x * accel.x + accel.y * accel.y + accel.z * accel.z;
     if (speed > p.AL2) accel = accel * (p.AL / sqrtf(speed));
-    f3 v = xyz(d.VEL[i]);
     speed = v.x * v.x + v.y * v.y + v.z * v.z;
     if (speed > p.VL2) {
         speed = p.VL2;
-        v = v * (p.VL / sqrtf(speed));
+        v = v * (p.VL / sqrtf(speed)); /* no-op limit, src/fluid_system_cuda.cu:3577-3579 */
     }
     f3 vnext = accel * dt + v;
-    f3 ve = (v + vnext) * 0.5f;
-    f3 np = pos + vnext * dt;
-    d.P[i] = make_float4(np.x, np.y, np.z, P_i.w);
-    d.V[i] = make_float4(ve.x, ve.y, ve.z, V_i.w);
-    d.VEL[i] = make_float4(vnext.x, vnext.y, vnext.z, 0.0f);
+    *ve_out = (v + vnext) * 0.5f;
+    *np_out = pos + vnext * dt;
+    *vnext_out = vnext;
 }
 
-/* ----------------------------------------------------------- diffusion (a12)
- * contributeDiffusion/computeDiffusion, src/fluid_system_cuda.cu:3026-3105, as a Jacobi
- * update CONC -> CONC2 (every slot is written; slots that do not diffuse copy through). */
-__global__ LAUNCH_BOUNDS_256 void k_diffusion(DS d, MgParams p, MgGrid g, MgGenome gen)
+/* AdvanceCL in place. With the copies of src/host_CL.cpp:199-226 around it, a GRID_UNDEF
+ * particle ends unchanged. */
+__global__ LAUNCH_BOUNDS_256 void k_advance(DS d, MgParams p, float dt)
+{
+    const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
+    if (i >= d.ctr[MG_C_ACTIVE]) return;
+    if (d.GCELL[i] == MG_GRID_UNDEF) return;
+    const float4 P_i = d.P[i], V_i = d.V[i];
+    f3 np, ve, vn;
+    advance_one(p, dt, xyz(P_i), xyz(V_i), xyz(d.VEL[i]), xyz(d.F[i]), &np, &ve, &vn);
+    d.P[i] = make_float4(np.x, np.y, np.z, P_i.w);
+    d.V[i] = make_float4(ve.x, ve.y, ve.z, V_i.w);
+    d.VEL[i] = make_float4(vn.x, vn.y, vn.z, 0.0f);
+}
+
+/* Fused ComputeForceCL + AdvanceCL for a step that starts from a sort (FFORCE = 0): the force
+ * stays in registers and the advanced state goes to the other half of the ping-pong
+ * (P2/V2/VEL2) so neighbours keep reading the pre-step state. Inactive slots copy through. */
+__global__ LAUNCH_BOUNDS_256 void k_force_advance(DS d, MgParams p, MgGrid g, float dt)
 {
     const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= (uint32_t)d.N) return;
-    const float4* src = (const float4*)(d.CONC + (size_t)i * MG_NUM_TF);
-    float4* dst = (float4*)(d.CONC2 + (size_t)i * MG_NUM_TF);
-    float cur[MG_NUM_TF];
-    {
-        float4 a = src[0], b = src[1], c = src[2], e = src[3];
-        cur[0] = a.x; cur[1] = a.y; cur[2] = a.z; cur[3] = a.w;
-        cur[4] = b.x; cur[5] = b.y; cur[6] = b.z; cur[7] = b.w;
-        cur[8] = c.x; cur[9] = c.y; cur[10] = c.z; cur[11] = c.w;
-        cur[12] = e.x; cur[13] = e.y; cur[14] = e.z; cur[15] = e.w;
-    }
-    uint32_t gc = (i < d.ctr[MG_C_ACTIVE]) ? d.GCELL[i] : MG_GRID_UNDEF;
+    const uint32_t na = d.ctr[MG_C_ACTIVE];
+    const float4 P_i = d.P[i], V_i = d.V[i], VEL_i = d.VEL[i];
+    const uint32_t gc = (i < na) ? d.GCELL[i] : MG_GRID_UNDEF;
     if (gc == MG_GRID_UNDEF) {
-        dst[0] = src[0]; dst[1] = src[1]; dst[2] = src[2]; dst[3] = src[3];
+        d.P2[i] = P_i;
+        d.V2[i] = V_i;
+        d.VEL2[i] = VEL_i;
         return;
     }
+    f3 F = force_one(d, p, g, i, na, gc, mk3(0.0f, 0.0f, 0.0f));
+    f3 np, ve, vn;
+    advance_one(p, dt, xyz(P_i), xyz(V_i), xyz(VEL_i), F, &np, &ve, &vn);
+    d.P2[i] = make_float4(np.x, np.y, np.z, P_i.w);
+    d.V2[i] = make_float4(ve.x, ve.y, ve.z, V_i.w);
+    d.VEL2[i] = make_float4(vn.x, vn.y, vn.z, 0.0f);
+}
+
+/* ----------------------------------------------------------- diffusion (a12)
+ * contributeDiffusion/computeDiffusion, src/fluid_system_cuda.cu:3026-3105, as a Jacobi update
+ * of particle i (cell gc, not GRID_UNDEF) from the concentrations CONC (read-only). */
+__device__ __forceinline__ void load16(const float* src, float v[16])
+{
+    const float4* s4 = (const float4*)src;
+    float4 a = s4[0], b = s4[1], c = s4[2], e = s4[3];
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+    v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+    v[8] = c.x; v[9] = c.y; v[10] = c.z; v[11] = c.w;
+    v[12] = e.x; v[13] = e.y; v[14] = e.z; v[15] = e.w;
+}
+
+__device__ __forceinline__ void store16(float* dst, const float v[16])
+{
+    float4* d4 = (float4*)dst;
+    d4[0] = make_float4(v[0], v[1], v[2], v[3]);
+    d4[1] = make_float4(v[4], v[5], v[6], v[7]);
+    d4[2] = make_float4(v[8], v[9], v[10], v[11]);
+    d4[3] = make_float4(v[12], v[13], v[14], v[15]);
+}
+
+__device__ __forceinline__ void diffuse_one(const DS& d, const MgParams& p, const MgGrid& g,
+                                            const MgGenome& gen, uint32_t i, uint32_t gc,
+                                            const float cur[MG_NUM_TF], float out[MG_NUM_TF])
+{
     gc -= (uint32_t)((1 * g.res[2] + 1) * g.res[0] + 1);
     const float d2 = p.psimscale * p.psimscale;
     const float r2 = p.r2 / d2;
     float D[MG_NUM_TF], nw[MG_NUM_TF];
     uint32_t dmask = 0;
+#pragma unroll
     for (int t = 0; t < MG_NUM_TF; t++) {
         D[t] = (float)gen.tf_diffusability[t] * p.diff_scale;
         nw[t] = 0.0f;
@@ -470,45 +539,48 @@ __global__ LAUNCH_BOUNDS_256 void k_diffusion(DS d, MgParams p, MgGrid g, MgGeno
             float dsq = dist.x * dist.x + dist.y * dist.y + dist.z * dist.z;
             if (dsq < r2 && dsq > 0.0f) {
                 const float cc = (r2 - dsq) * d2;
-                const float* cj = d.CONC + (size_t)j * MG_NUM_TF;
+                float cn[MG_NUM_TF];
+                load16(d.CONC + (size_t)j * MG_NUM_TF, cn);
 #pragma unroll
                 for (int t = 0; t < MG_NUM_TF; t++)
-                    if (dmask & (1u << t)) nw[t] += D[t] * cc * (cj[t] - cur[t]);
+                    if (dmask & (1u << t)) nw[t] += D[t] * cc * (cn[t] - cur[t]);
             }
         }
     }
-    float out[MG_NUM_TF];
 #pragma unroll
     for (int t = 0; t < MG_NUM_TF; t++) out[t] = (float)gen.tf_breakdown_rate[t] * (cur[t] + nw[t]);
-    dst[0] = make_float4(out[0], out[1], out[2], out[3]);
-    dst[1] = make_float4(out[4], out[5], out[6], out[7]);
-    dst[2] = make_float4(out[8], out[9], out[10], out[11]);
-    dst[3] = make_float4(out[12], out[13], out[14], out[15]);
+}
+
+/* ComputeDiffusionCL: CONC -> CONC2 for every slot (slots that do not diffuse copy through). */
+__global__ LAUNCH_BOUNDS_256 void k_diffusion(DS d, MgParams p, MgGrid g, MgGenome gen)
+{
+    const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
+    if (i >= (uint32_t)d.N) return;
+    float cur[MG_NUM_TF], out[MG_NUM_TF];
+    load16(d.CONC + (size_t)i * MG_NUM_TF, cur);
+    const uint32_t gc = (i < d.ctr[MG_C_ACTIVE]) ? d.GCELL[i] : MG_GRID_UNDEF;
+    if (gc == MG_GRID_UNDEF) {
+        store16(d.CONC2 + (size_t)i * MG_NUM_TF, cur);
+        return;
+    }
+    diffuse_one(d, p, g, gen, i, gc, cur, out);
+    store16(d.CONC2 + (size_t)i * MG_NUM_TF, out);
 }
 
 /* --------------------------------------------------------------- genes (a13)
- * computeGeneAction + tallyGeneAction (src/fluid_system_cuda.cu:714-772) fused per
- * particle: genes in order 0..15 for the genes the particle's sort-time membership holds
- * (the dense lists), secretion/activation deltas kept in registers, then the tally for
- * members of the living list (gene 2) below mActivePoints. Fixed semantics are those of
- * the oracle's genes() (float FCONC in AoS, one tally, bounded countdown). */
-__global__ LAUNCH_BOUNDS_256 void k_genes(DS d, MgParams p, MgGenome gen)
+ * computeGeneAction + tallyGeneAction (src/fluid_system_cuda.cu:714-772) for one particle with
+ * sort-time membership m (the dense lists): genes in order 0..15, secretion/activation deltas
+ * in registers, then the tally when the particle is a member of the living list (gene 2)
+ * below mActivePoints. Fixed semantics are those of the oracle's genes() (float FCONC in
+ * AoS, one tally, bounded countdown). Updates epi[] and conc[] in place. */
+__device__ __forceinline__ void genes_one(const MgParams& p, const MgGenome& gen, uint32_t m,
+                                          bool tally, uint32_t epi[MG_NUM_GENES],
+                                          float conc[MG_NUM_TF])
 {
-    const uint32_t s = xcd_block() * blockDim.x + threadIdx.x;
-    if (!d.ctr[MG_C_LISTS]) return;
-    if (s >= d.ctr[MG_C_NUM_POINTS]) return;
-    const uint32_t m = d.GMASK[s];
-    if (m == 0) return;
-    const uint32_t h = d.HOME[s];
-    const int N = d.N;
-    uint32_t epi[MG_NUM_GENES];
-#pragma unroll
-    for (int g = 0; g < MG_NUM_GENES; g++) epi[g] = d.EPIGEN[(size_t)g * N + h];
-    float conc[MG_NUM_TF], dconc[MG_NUM_TF];
+    float dconc[MG_NUM_TF];
     uint32_t depi[MG_NUM_GENES];
-    float* cp = d.CONC + (size_t)s * MG_NUM_TF;
 #pragma unroll
-    for (int t = 0; t < MG_NUM_TF; t++) { conc[t] = cp[t]; dconc[t] = 0.0f; depi[t] = 0u; }
+    for (int t = 0; t < MG_NUM_TF; t++) { dconc[t] = 0.0f; depi[t] = 0u; }
     for (int gene = 0; gene < MG_NUM_GENES; gene++) {
         if (!(m & (1u << gene))) continue;
         const int delay = (int)epi[gene];
@@ -534,12 +606,76 @@ __global__ LAUNCH_BOUNDS_256 void k_genes(DS d, MgParams p, MgGenome gen)
             if ((float)threshold < activity) depi[other] += 1u;
         }
     }
-    if ((m & 4u) && s < d.ctr[MG_C_ACTIVE]) {
-        for (int t = 0; t < MG_NUM_TF; t++) cp[t] = conc[t] + dconc[t];
+    if (tally) {
+        for (int t = 0; t < MG_NUM_TF; t++) conc[t] = conc[t] + dconc[t];
         for (int g = 0; g < MG_NUM_GENES; g++) epi[g] += depi[g];
     }
+}
+
+/* ComputeGenesCL in place (EPIGEN by home, CONC by slot). */
+__global__ LAUNCH_BOUNDS_256 void k_genes(DS d, MgParams p, MgGenome gen)
+{
+    const uint32_t s = xcd_block() * blockDim.x + threadIdx.x;
+    if (!d.ctr[MG_C_LISTS]) return;
+    if (s >= d.ctr[MG_C_NUM_POINTS]) return;
+    const uint32_t m = d.GMASK[s];
+    if (m == 0) return;
+    const uint32_t h = d.HOME[s];
+    uint32_t epi[MG_NUM_GENES];
 #pragma unroll
-    for (int g = 0; g < MG_NUM_GENES; g++) d.EPIGEN[(size_t)g * N + h] = epi[g];
+    for (int g = 0; g < MG_NUM_GENES; g++) epi[g] = d.EPIGEN[(size_t)g * d.N + h];
+    float conc[MG_NUM_TF];
+    float* cp = d.CONC + (size_t)s * MG_NUM_TF;
+    load16(cp, conc);
+    const bool tally = (m & 4u) && s < d.ctr[MG_C_ACTIVE];
+    genes_one(p, gen, m, tally, epi, conc);
+    if (tally) store16(cp, conc);
+#pragma unroll
+    for (int g = 0; g < MG_NUM_GENES; g++) d.EPIGEN[(size_t)g * d.N + h] = epi[g];
+}
+
+/* Fused ComputeDiffusionCL + ComputeGenesCL + InsertParticlesCL of mg_step. Reads positions
+ * after the advance (d.P), the pre-step concentrations (d.CONC) and the pre-step grid
+ * (d.CNT / d.OFF); writes CONC2, updates EPIGEN, records the post-gene membership in GMN for
+ * the coming sort and bins the particle into the next grid (d.CNTN). */
+__global__ LAUNCH_BOUNDS_256 void k_diffuse_genes_insert(DS d, MgParams p, MgGrid g, MgGenome gen,
+                                                         int genes)
+{
+    const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
+    const bool valid = i < (uint32_t)d.N;
+    float4 P_i = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (valid) {
+        P_i = d.P[i];
+        float cur[MG_NUM_TF], out[MG_NUM_TF];
+        load16(d.CONC + (size_t)i * MG_NUM_TF, cur);
+        const uint32_t gc = (i < d.ctr[MG_C_ACTIVE]) ? d.GCELL[i] : MG_GRID_UNDEF;
+        if (gc == MG_GRID_UNDEF || !genes) {
+#pragma unroll
+            for (int t = 0; t < MG_NUM_TF; t++) out[t] = cur[t];
+        } else {
+            diffuse_one(d, p, g, gen, i, gc, cur, out);
+        }
+        uint32_t mnext = 0;
+        if (i < d.ctr[MG_C_NUM_POINTS]) {
+            const uint32_t h = d.HOME[i];
+            uint32_t epi[MG_NUM_GENES];
+#pragma unroll
+            for (int k = 0; k < MG_NUM_GENES; k++) epi[k] = d.EPIGEN[(size_t)k * d.N + h];
+            const uint32_t m = d.GMASK[i];
+            if (genes && m != 0 && d.ctr[MG_C_LISTS]) {
+                const bool tally = (m & 4u) && i < d.ctr[MG_C_ACTIVE];
+                genes_one(p, gen, m, tally, epi, out);
+#pragma unroll
+                for (int k = 0; k < MG_NUM_GENES; k++) d.EPIGEN[(size_t)k * d.N + h] = epi[k];
+            }
+#pragma unroll
+            for (int k = 0; k < MG_NUM_GENES; k++)
+                if (epi[k] > 0u) mnext |= 1u << k;
+        }
+        d.GMN[i] = mnext;
+        store16(d.CONC2 + (size_t)i * MG_NUM_TF, out);
+    }
+    insert_one(d, g, i, valid, P_i, d.CNTN, nullptr);
 }
 
 /* ZeroVelCL (src/host_CL.cpp:2175-2180) over the whole FVEL / FVEVAL buffers. */

@@ -227,3 +227,26 @@ def test_cpp_facade_matches_capi(built):
         assert lib.mgfs_get_host(fh, b, a.ctypes.data_as(C.c_void_p), a.nbytes) == 0
         assert_bits_equal(a, g.download(b), f"facade {F.BUFFER_NAMES[b]}")
     lib.mgfs_destroy(fh)
+
+
+def test_graph_replay_matches_eager(built):
+    """mg_step with MG_STEP_GRAPH (two fused steps captured once, replayed) equals eager
+    stepping bit for bit, also with eager work, downloads and syncs between replays."""
+    from morphogenesis_amd.fluid_system import FluidSystem
+
+    sc = scenes.tissue(n_side=12, b=32, seed=17)
+    say("case: graph replay")
+    sims = []
+    for _ in range(2):
+        s = FluidSystem(sc.n, sc.b)
+        s.load_scene(sc)
+        s.Run2PhysicalSort()
+        sims.append(s)
+    eager, graph = sims
+    eager.step(40, sc.dt, sc.flags)
+    for chunk in (10, 7, 1, 2, 20):
+        graph.step(chunk, sc.dt, sc.flags | F.STEP_GRAPH)
+        _ = graph.download(F.FPOS)  # eager work and a sync between replays
+        graph.sync()
+    for b in COMPARE:
+        assert_bits_equal(graph.download(b), eager.download(b), f"graph {F.BUFFER_NAMES[b]}")
