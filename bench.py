@@ -24,45 +24,49 @@ METRIC = ("particle-steps/sec + % HBM peak, 262k bonded tissue; learned 4×dt vs
 HBM_PEAK_GBS = 8000.0
 PHASES = ["pressure", "force", "advance", "diffusion", "genes", "insert", "scan", "place",
           "sort", "lists", "fill"]
-KERNEL_OF_PHASE = {"pressure": "k_pressure", "force": "k_force", "advance": "k_advance",
-                   "diffusion": "k_diffusion", "genes": "k_genes", "insert": "k_insert",
-                   "scan": "k_scan", "place": "k_place+k_cellsort", "sort": "k_sort_gather",
+# kernels of the fused step (mg_step) behind each timed phase
+KERNEL_OF_PHASE = {"pressure": "k_pressure", "force": "k_force_advance",
+                   "diffusion": "k_diffuse_genes_insert", "scan": "k_scan_lookback",
+                   "place": "k_place+k_cellsort", "sort": "k_sort_gather",
                    "lists": "k_dense_lists", "fill": "k_fill_u32"}
 
 
 def algorithmic_bytes(phase: str, scene, stats: dict) -> float:
-    """Compulsory HBM bytes per launch of a phase's kernel(s) in this build's layout, with
-    every array read or written once (neighbour data counted once, as the particle's own
-    read). n = particles, C = cells, B = bond slots, bo / bi = mean outgoing / incoming
-    bonds per particle (the scene's, from its bond table)."""
+    """Compulsory HBM bytes per launch of the fused step's kernels in this build's layout:
+    every array element read or written once, neighbour data counted once (as that
+    particle's own read). n = active particles, C = cells, B = bond slots, bo / bi = mean
+    outgoing / incoming bonds per particle, L = mean neighbour-list length (entries within
+    the list radius, the particle itself included)."""
     n, c, b = stats["n_active"], scene.c, scene.b
-    bo, bi = stats["bonds_out"], stats["bonds_in"]
-    if phase == "pressure":   # gcell 4, P 16 read, press/dens 8 written; CNT/OFF once
-        return n * (4 + 16 + 8) + 8 * c
-    if phase == "force":      # gcell, home, P, V: 40; BPARTNER all B slots; per bond BPAR 16,
-        # SLOT 4, BINT 8 r + 8 w; INC_OFF 8, INC 4 per incoming bond; F 16 written
-        return n * (40 + 4 * b + bo * (16 + 4 + 16) + 8 + 4 * bi + 16) + 8 * c
-    if phase == "advance":    # gcell 4, F 16, P 16, V 16, VEL 16 read; P, V, VEL 48 written
-        return n * (4 + 64 + 48)
-    if phase == "diffusion":  # gcell 4, P 16, CONC 64 read; CONC 64 written
-        return n * (4 + 16 + 64 + 64) + 8 * c
-    if phase == "genes":      # GMASK 4, HOME 4, EPIGEN 64 r/w, CONC 64 r/w
-        return n * (8 + 128 + 128)
-    if phase == "insert":     # P 16 read; GCELL, GNDX written; CNT atomics
-        return n * (16 + 8) + 4 * c
+    bo, bi, L = stats["bonds_out"], stats["bonds_in"], stats["list_len"]
+    genes = bool(scene.flags & 1)
+    if phase == "pressure":
+        # GCELL 4, P 16 read; press, 1/density 8 written; list L*4 + count 4 written;
+        # CNT/OFF read once, next grid's CNT cleared
+        return n * (4 + 16 + 8 + 4 * L + 4) + 12 * c
+    if phase == "force":
+        # force: GCELL 4, HOME 4, P 16, V 16 (+VEL 16 for the advance); BPARTNER 4 per slot;
+        # per bond BPAR 16 + SLOT 4 + BINT 8 read + 8 written; INC_OFF 8, INC 4 per incoming
+        # bond; list count 4 + L*4 read; advance writes P, V, VEL: 48
+        return n * (56 + 4 * b + bo * 36 + 8 + 4 * bi + 4 + 4 * L + 48)
+    if phase == "diffusion":
+        # P 16, CONC 64 read, CONC 64 written, GCELL 4 read/written, GNDX 4, GMASK 4, HOME 4,
+        # EPIGEN 64 read (+64 written with genes), GMN 4, list 4 + 4*L
+        return n * (16 + 128 + 8 + 4 + 4 + 4 + 64 + (64 if genes else 0) + 4 + 4 + 4 * L) + 4 * c
     if phase == "scan":
-        return 8 * c * (1 + 16) + 16 * 4
-    if phase == "place":      # GCELL, GNDX read, PERM scatter; cellsort PERM r/w, GNDX w
+        return 8 * c
+    if phase == "place":  # k_place: GCELL, GNDX read, PERM written; k_cellsort: PERM r/w, GNDX w
         return n * (8 + 4 + 8 + 4) + 8 * c
-    if phase == "sort":       # PERM 4; P V VEL 48, CONC 64, HOME GCELL GNDX 12 read and written
-        return n * (4 + 2 * (48 + 64 + 12) + 64 + 4 + 4)  # + EPIGEN 64 read, GMASK 4, SLOT 4
-    if phase == "lists":
-        return n * (4 + 16 * 4 * 0.3) + 8 * c * 17
+    if phase == "sort":
+        # PERM 4; P, V, VEL 48, CONC 64, HOME/GCELL/GNDX 12 read and written; GMN 4 read,
+        # GMASK 4 and SLOT 4 written
+        return n * (4 + 2 * (48 + 64 + 12) + 4 + 4 + 4)
     return 0.0
 
 
 def scene_stats(scene) -> dict:
     import numpy as np
+    from scipy.spatial import cKDTree
 
     from morphogenesis_amd import _ffi as F
 
@@ -70,11 +74,17 @@ def scene_stats(scene) -> dict:
     if rec is None or scene.b == 0:
         bo = bi = 0.0
     else:
-        j = rec[:, :, 0]
-        valid = j != F.NONE
+        valid = rec[:, :, 0] != F.NONE
         bo = float(valid.sum()) / scene.n
         bi = bo
-    return {"bonds_out": bo, "bonds_in": bi}
+    pos = scene.buffers[F.FPOS][:, :3].astype(np.float64)
+    rng = np.random.default_rng(0)
+    idx = rng.choice(pos.shape[0], size=min(4096, pos.shape[0]), replace=False)
+    tree = cKDTree(pos)
+    # list radius: max(1, r2 / simscale^2) + 0.1 skin (mg_set_params)
+    r = max(1.0, float(np.sqrt(scene.params.r2 / scene.params.psimscale ** 2))) + 0.1
+    L = float(np.mean([len(x) for x in tree.query_ball_point(pos[idx], r)]))
+    return {"bonds_out": bo, "bonds_in": bi, "list_len": L}
 
 
 def cpu_baseline(scene, budget_s: float = 15.0) -> dict:
@@ -128,6 +138,7 @@ def main():
     ap.add_argument("--workload", default="tissue_262k")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--timed-steps", type=int, default=20)
+    ap.add_argument("--no-graph", action="store_true", help="launch the step kernels eagerly")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -150,11 +161,12 @@ def main():
     sim = FluidSystem(scene.n, scene.b)
     sim.load_scene(scene)
     sim.Run2PhysicalSort()
-    sim.step(args.warmup, scene.dt, scene.flags)
+    flags = scene.flags | (0 if args.no_graph else F.STEP_GRAPH)
+    sim.step(args.warmup, scene.dt, flags)
     sim.sync()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    sim.step(args.steps, scene.dt, scene.flags)
+    sim.step(args.steps, scene.dt, flags)
     sim.sync()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
@@ -170,17 +182,19 @@ def main():
     r = lib.mg_step_timed(sim.h, args.timed_steps, scene.dt, scene.flags, ms, calls)
     if r != 0:
         raise F.MgError(sim.last_error())
-    per_kernel = {}
-    for i, ph in enumerate(PHASES):
-        if calls[i]:
-            per_kernel[ph] = {"avg_ms": ms[i] / calls[i] * (1 if ph not in ("place", "sort") else 1),
-                              "calls": calls[i]}
-    # launches per step of each phase: place/sort scopes bracket two kernels
-    dom = max((p for p in per_kernel if p != "fill"), key=lambda p: per_kernel[p]["avg_ms"])
-    dom_ms = per_kernel[dom]["avg_ms"] / (calls[PHASES.index(dom)] / args.timed_steps)
+    per_step = {ph: ms[i] / args.timed_steps for i, ph in enumerate(PHASES) if calls[i]}
+    dom = max((p for p in per_step if p in KERNEL_OF_PHASE and p not in ("fill", "place")),
+              key=lambda p: per_step[p])
+    dom_ms = per_step[dom]  # one launch per step
     alg = algorithmic_bytes(dom, scene, stats)
     achieved = alg / (dom_ms * 1e-3) / 1e9
-    breakdown = {p: round(v["avg_ms"] * v["calls"] / args.timed_steps, 5) for p, v in per_kernel.items()}
+    breakdown = {p: round(v, 5) for p, v in per_step.items()}
+    kernels = {}
+    for p, v in per_step.items():
+        if p in KERNEL_OF_PHASE and algorithmic_bytes(p, scene, stats) > 0:
+            bts = algorithmic_bytes(p, scene, stats)
+            kernels[KERNEL_OF_PHASE[p]] = {"ms": round(v, 5), "alg_bytes": int(bts),
+                                           "GB/s": round(bts / (v * 1e-3) / 1e9, 1)}
     ms_per_step = elapsed / args.steps * 1e3
     value = scene.n * args.steps / elapsed
     out = {
@@ -199,12 +213,15 @@ def main():
         "config": {"workload": args.workload, "particles": scene.n, "active": na,
                    "bonds_per_particle": scene.b, "bonds_per_particle_mean": stats["bonds_out"] * 2,
                    "cells": scene.c, "dt": scene.dt, "genes": bool(scene.flags & F.STEP_GENES),
+                   "hipgraph": not args.no_graph,
                    "step": "pressure+force+advance+diffusion+genes+insert+scan+sort"},
         "roofline": {"bound": "hbm", "kernel": KERNEL_OF_PHASE[dom], "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": load_traffic(KERNEL_OF_PHASE[dom], args.workload),
                      "algorithmic_bytes_per_launch": alg, "avg_launch_ms": dom_ms},
         "step_breakdown_ms": breakdown,
+        "kernels": kernels,
+        "mean_list_len": round(stats["list_len"], 2),
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(scene)

@@ -9,6 +9,7 @@
 #include "mg_kernels.hip"
 
 #include <algorithm>
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -230,7 +231,14 @@ struct mg_handle {
     uint32_t *GCNT = nullptr, *GOFF = nullptr, *LENS = nullptr;
     uint32_t *LISTS = nullptr, *PERM = nullptr, *GMN = nullptr;
     bool gene_grid_stale = false;         /* gene counts/offsets/lists not rebuilt since a fused step */
+    /* neighbour lists (built by pressure, used by force and by the fused diffusion) */
+    uint32_t *NBR = nullptr, *NBRN = nullptr;
+    uint32_t nbr_cap = 64;
+    float nbr_r2 = 0.f, nbr_slack = 0.f;
+    bool nbr_fresh = false; /* lists match the current positions (pressure ran since they moved) */
     uint32_t* ctr = nullptr;
+    unsigned long long* scan_status = nullptr;  /* decoupled look-back status, one word per tile */
+    uint32_t* scan_ticket = nullptr;
     void* staging = nullptr;
     size_t staging_bytes = 0;
     bool normalized = true;
@@ -348,8 +356,20 @@ DS make_ds(mg_handle* h, int kin = 0, int conc = 0)
     d.GMN = nullptr;
     d.ZERO = nullptr;
     d.zero_n = 0;
+    d.NBR = nullptr;
+    d.NBRN = nullptr;
+    d.nbr_cap = h->nbr_cap;
+    d.nbr_r2 = h->nbr_r2;
+    d.nbr_slack = h->nbr_slack;
     d.ctr = h->ctr;
     d.N = h->N; d.B = h->B; d.C = h->C;
+    return d;
+}
+
+DS with_lists(mg_handle* h, DS d)
+{
+    d.NBR = h->NBR;
+    d.NBRN = h->NBRN;
     return d;
 }
 
@@ -515,6 +535,7 @@ int do_sort(mg_handle* h, int lists)
     h->icur = 1 - h->icur;
     h->ccur = 1 - h->ccur;
     h->force_zero = true;
+    h->nbr_fresh = false;
     h->normalized = false;
     if (lists) {
         PhaseScope ps(h, MG_PH_LISTS);
@@ -553,27 +574,28 @@ int do_fused_step(mg_handle* h, float dt, unsigned flags)
     const unsigned nb = nblk(h->N);
     {
         PhaseScope ps(h, MG_PH_PRESSURE);
-        DS d = make_ds(h);
+        DS d = with_lists(h, make_ds(h));
         d.ZERO = h->CNT[1 - h->gcur];
         d.zero_n = (uint32_t)h->C;
         hipLaunchKernelGGL(k_pressure, dim3(nb), dim3(256), 0, h->st, d, h->p, h->g);
     }
     {
         PhaseScope ps(h, MG_PH_FORCE);
-        hipLaunchKernelGGL(k_force_advance, dim3(nb), dim3(256), 0, h->st, make_ds(h), h->p, h->g, dt);
+        hipLaunchKernelGGL(k_force_advance, dim3(nb), dim3(256), 0, h->st, with_lists(h, make_ds(h)), h->p, h->g, dt);
     }
     {
         PhaseScope ps(h, MG_PH_DIFFUSION);
-        DS d = make_ds(h, 1, 0);
+        DS d = with_lists(h, make_ds(h, 1, 0));
         d.GMN = h->GMN;
         hipLaunchKernelGGL(k_diffuse_genes_insert, dim3(nb), dim3(256), 0, h->st, d, h->p, h->g,
                            h->gen, genes);
     }
     {
         PhaseScope ps(h, MG_PH_SCAN);
-        hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, h->st, h->CNT[1 - h->gcur],
-                           h->OFF[1 - h->gcur], h->C, h->ctr, 1, grid_scan_max_index(h->g),
-                           (uint32_t*)nullptr);
+        const unsigned tiles = (unsigned)((h->C + SCAN_TILE - 1) / SCAN_TILE);
+        hipLaunchKernelGGL(k_scan_lookback, dim3(tiles), dim3(256), 0, h->st, h->CNT[1 - h->gcur],
+                           h->OFF[1 - h->gcur], h->C, h->scan_status, h->scan_ticket, h->ctr,
+                           grid_scan_max_index(h->g));
     }
     {
         PhaseScope ps(h, MG_PH_PLACE);
@@ -596,6 +618,7 @@ int do_fused_step(mg_handle* h, float dt, unsigned flags)
     h->force_zero = true;
     h->normalized = false;
     h->gene_grid_stale = true;
+    h->nbr_fresh = false;
     return MG_OK;
 }
 
@@ -654,8 +677,9 @@ int ensure_force(mg_handle* h)
 int do_pressure(mg_handle* h)
 {
     PhaseScope ps(h, MG_PH_PRESSURE);
-    hipLaunchKernelGGL(k_pressure, dim3(nblk(h->N)), dim3(256), 0, h->st, make_ds(h), h->p, h->g);
+    hipLaunchKernelGGL(k_pressure, dim3(nblk(h->N)), dim3(256), 0, h->st, with_lists(h, make_ds(h)), h->p, h->g);
     HIPCHK(hipGetLastError());
+    h->nbr_fresh = true;
     return MG_OK;
 }
 
@@ -663,7 +687,8 @@ int do_force(mg_handle* h)
 {
     CHK(ensure_force(h));
     PhaseScope ps(h, MG_PH_FORCE);
-    hipLaunchKernelGGL(k_force, dim3(nblk(h->N)), dim3(256), 0, h->st, make_ds(h), h->p, h->g, 1);
+    DS d = h->nbr_fresh ? with_lists(h, make_ds(h)) : make_ds(h);
+    hipLaunchKernelGGL(k_force, dim3(nblk(h->N)), dim3(256), 0, h->st, d, h->p, h->g, 1);
     HIPCHK(hipGetLastError());
     return MG_OK;
 }
@@ -674,6 +699,7 @@ int do_advance(mg_handle* h, float dt)
     PhaseScope ps(h, MG_PH_ADVANCE);
     hipLaunchKernelGGL(k_advance, dim3(nblk(h->N)), dim3(256), 0, h->st, make_ds(h), h->p, dt);
     HIPCHK(hipGetLastError());
+    h->nbr_fresh = false;
     return MG_OK;
 }
 
@@ -757,6 +783,8 @@ int mg_create(mg_handle** out, int max_points, int bonds_per_particle)
     if (r == MG_OK) r = dalloc(h, &h->INC_OFF, N + 1);
     if (r == MG_OK) r = dalloc(h, &h->PERM, N);
     if (r == MG_OK) r = dalloc(h, &h->GMN, N);
+    if (r == MG_OK) r = dalloc(h, &h->NBR, N * h->nbr_cap);
+    if (r == MG_OK) r = dalloc(h, &h->NBRN, N);
     if (r == MG_OK) r = dalloc(h, &h->LISTS, N * MG_NUM_GENES);
     if (r == MG_OK) r = dalloc(h, &h->LENS, MG_NUM_GENES);
     if (r == MG_OK) r = dalloc(h, &h->ctr, MG_C_COUNT);
@@ -846,6 +874,11 @@ int mg_set_grid(mg_handle* h, const MgGrid* g)
             CHK(fill_u32(h, h->CNT[k], 0, h->C));
             CHK(fill_u32(h, h->OFF[k], 0, h->C));
         }
+        if (h->scan_status) { dfree(h, h->scan_status); h->scan_status = nullptr; }
+        if (!h->scan_ticket) CHK(dalloc(h, &h->scan_ticket, 1));
+        CHK(dalloc(h, &h->scan_status, (size_t)(h->C + SCAN_TILE - 1) / SCAN_TILE));
+        CHK(fill_u32(h, h->scan_status, 0, 2 * ((size_t)(h->C + SCAN_TILE - 1) / SCAN_TILE)));
+        CHK(fill_u32(h, h->scan_ticket, 0, 1));
         CHK(dalloc(h, &h->GCNT, (size_t)MG_NUM_GENES * h->C));
         CHK(dalloc(h, &h->GOFF, (size_t)MG_NUM_GENES * h->C));
         CHK(fill_u32(h, h->GCNT, 0, (size_t)MG_NUM_GENES * h->C));
@@ -875,6 +908,16 @@ int mg_set_params(mg_handle* h, const MgParams* p)
     if (!h || !p) { set_err("mg_set_params: null argument"); return MG_ERR_ARG; }
     h->p = *p;
     h->have_params = true;
+    /* neighbour-list radius: covers the force range (dsq < 1) and the diffusion range
+     * (dsq < r2 / simscale^2) plus a skin; diffusion, which runs after the advance, uses the
+     * lists only while no particle moved more than half the skin */
+    const float d2f = p->psimscale * p->psimscale;
+    const float r2f = p->r2 / d2f;
+    const double rb = std::max(1.0, std::sqrt((double)r2f));
+    const double skin = 0.1;
+    h->nbr_r2 = (float)((rb + skin) * (rb + skin) * (1.0 + 1e-5));
+    h->nbr_slack = (float)(0.5 * skin * 0.99);
+    if (!(h->nbr_r2 > 0.f) || !std::isfinite(h->nbr_r2)) h->nbr_r2 = 0.f, h->nbr_slack = -1.f;
     h->epoch++;
     return MG_OK;
 }
@@ -925,6 +968,7 @@ int mg_upload(mg_handle* h, int buf_id, const void* src, size_t bytes)
     if (bytes != need) { set_err("mg_upload: buffer %d needs %zu bytes, got %zu", buf_id, need, bytes); return MG_ERR_SIZE; }
     if (need == 0) return MG_OK;
     CHK(normalize(h));
+    h->nbr_fresh = false;
     if (buf_id >= MG_FBIN_COUNT_ACTIVE_GENES && buf_id <= MG_FDENSE_LISTS) CHK(materialize_gene_grid(h));
     CHK(staging(h, bytes));
     HIPCHK(hipMemcpyAsync(h->staging, src, bytes, hipMemcpyHostToDevice, h->st));
@@ -1047,18 +1091,22 @@ int mg_download(mg_handle* h, int buf_id, void* dst, size_t bytes)
 int mg_get_counts(mg_handle* h, uint32_t* num_points, uint32_t* active_points)
 {
     if (!h) { set_err("mg_get_counts: null handle"); return MG_ERR_ARG; }
-    uint32_t c[2];
-    HIPCHK(hipMemcpyAsync(c, h->ctr, 8, hipMemcpyDeviceToHost, h->st));
+    uint32_t c[MG_C_COUNT];
+    HIPCHK(hipMemcpyAsync(c, h->ctr, sizeof(c), hipMemcpyDeviceToHost, h->st));
     HIPCHK(hipStreamSynchronize(h->st));
-    if (num_points) *num_points = c[0];
-    if (active_points) *active_points = c[1];
+    if (c[MG_C_ERR]) { set_err("device error flag set (a bounded wait expired)"); return MG_ERR_STATE; }
+    if (num_points) *num_points = c[MG_C_NUM_POINTS];
+    if (active_points) *active_points = c[MG_C_ACTIVE];
     return MG_OK;
 }
 
 int mg_sync(mg_handle* h)
 {
     if (!h) { set_err("mg_sync: null handle"); return MG_ERR_ARG; }
+    uint32_t err = 0;
+    HIPCHK(hipMemcpyAsync(&err, h->ctr + MG_C_ERR, 4, hipMemcpyDeviceToHost, h->st));
     HIPCHK(hipStreamSynchronize(h->st));
+    if (err) { set_err("device error flag set (a bounded wait expired)"); return MG_ERR_STATE; }
     return MG_OK;
 }
 

@@ -131,6 +131,104 @@ __global__ __launch_bounds__(1024) void k_scan(const uint32_t* in, uint32_t* out
     }
 }
 
+/* Single-pass exclusive scan with decoupled look-back (Merrill & Garland, "Single-pass
+ * Parallel Prefix Scan with Decoupled Look-back", 2016) for the cell counts of the fused
+ * step. Tiles of 4096 counts (256 threads x 16); each tile publishes its aggregate, then
+ * walks back over predecessors' published aggregates / inclusive prefixes until it meets an
+ * inclusive one. Tile ids come from a monotonic ticket taken at block start, so a tile only
+ * ever waits on tiles that started before it (no deadlock), and the launch number
+ * (ticket / tiles) tags every status word, so status words from earlier launches read as
+ * "not ready" without any reset (also under hipGraph replay).
+ * status word: [63:34] launch epoch, [33:32] flag (1 aggregate, 2 inclusive), [31:0] value. */
+#define SCAN_TILE 4096
+__global__ __launch_bounds__(256) void k_scan_lookback(const uint32_t* __restrict__ in,
+                                                       uint32_t* __restrict__ out, int n,
+                                                       unsigned long long* status,
+                                                       uint32_t* ticket, uint32_t* ctr, int gsm)
+{
+    __shared__ uint32_t s_ticket, s_excl;
+    __shared__ uint32_t wsum[4];
+    const int t = threadIdx.x;
+    if (t == 0) s_ticket = atomicAdd(ticket, 1u);
+    __syncthreads();
+    const uint32_t tiles = gridDim.x;
+    const uint32_t tile = s_ticket % tiles;
+    const unsigned long long epoch = (unsigned long long)(s_ticket / tiles + 1u) & 0x3FFFFFFFull;
+    const int base = (int)tile * SCAN_TILE + t * 16;
+    uint32_t v[16];
+    if (base + 15 < n) {
+        const uint4* q = (const uint4*)(in + base);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            uint4 w = q[k];
+            v[4 * k] = w.x; v[4 * k + 1] = w.y; v[4 * k + 2] = w.z; v[4 * k + 3] = w.w;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 16; k++) v[k] = (base + k < n) ? in[base + k] : 0u;
+    }
+    uint32_t local = 0;
+#pragma unroll
+    for (int k = 0; k < 16; k++) local += v[k];
+    const int lane = t & 63, w = t >> 6;
+    uint32_t x = local;
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    uint32_t wbase = 0, total = 0;
+    for (int k = 0; k < 4; k++) {
+        if (k < w) wbase += wsum[k];
+        total += wsum[k];
+    }
+    if (t == 0) {
+        uint32_t excl = 0;
+        unsigned long long* st = status + tile;
+        if (tile == 0) {
+            __hip_atomic_store(st, (epoch << 34) | (2ull << 32) | total, __ATOMIC_RELEASE,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            __hip_atomic_store(st, (epoch << 34) | (1ull << 32) | total, __ATOMIC_RELEASE,
+                               __HIP_MEMORY_SCOPE_AGENT);
+            int pidx = (int)tile - 1;
+            uint32_t spins = 0;
+            for (;;) {
+                unsigned long long wd = __hip_atomic_load(status + pidx, __ATOMIC_ACQUIRE,
+                                                          __HIP_MEMORY_SCOPE_AGENT);
+                if ((wd >> 34) != epoch || ((wd >> 32) & 3ull) == 0ull) {
+                    /* bounded wait: predecessors always publish, but never spin forever */
+                    if (++spins > (1u << 24)) {
+                        atomicOr(&ctr[MG_C_ERR], 1u);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                excl += (uint32_t)wd;
+                if (((wd >> 32) & 3ull) == 2ull) break;
+                pidx--;
+            }
+            __hip_atomic_store(st, (epoch << 34) | (2ull << 32) | (uint32_t)(excl + total),
+                               __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        s_excl = excl;
+    }
+    __syncthreads();
+    uint32_t run = s_excl + wbase + x - local;
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        const int idx = base + k;
+        if (idx < n) {
+            out[idx] = run;
+            if (idx == n - 1) ctr[MG_C_SCAN_NUM] = run;
+            if (idx == gsm) ctr[MG_C_SCAN_ACTIVE] = run;
+        }
+        run += v[k];
+    }
+}
+
 /* ------------------------------------------------------ stable placement
  * PERM[OFF[cell] + slot] = s, then each cell's run of PERM is sorted ascending so the
  * intra-cell order is the pre-sort index order; FGNDX becomes that rank. */
@@ -259,6 +357,7 @@ __global__ LAUNCH_BOUNDS_256 void k_pressure(DS d, MgParams p, MgGrid g)
         for (uint32_t z = blockIdx.x * blockDim.x + threadIdx.x; z < d.zero_n; z += gridDim.x * blockDim.x)
             d.ZERO[z] = 0u;
     }
+    if (blockIdx.x == 0 && threadIdx.x == 0) d.ctr[MG_C_MAXDISP] = 0u;
     const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= d.ctr[MG_C_ACTIVE]) return;
     uint32_t gc = d.GCELL[i];
@@ -266,6 +365,11 @@ __global__ LAUNCH_BOUNDS_256 void k_pressure(DS d, MgParams p, MgGrid g)
     gc -= (uint32_t)((1 * g.res[2] + 1) * g.res[0] + 1);
     const f3 pi = xyz(d.P[i]);
     const float r2 = p.r2, sr = p.psmoothradius;
+    const bool lists = d.NBR != nullptr;
+    const float lr2 = d.nbr_r2;
+    const uint32_t cap = d.nbr_cap;
+    uint32_t ne = 0;          /* entries appended */
+    uint32_t pend = 0;        /* last appended entry, written once its group flag is known */
     float sum = 0.0f;
     for (int c = 0; c < g.adj_cnt; c++) {
         const uint32_t cell = gc + (uint32_t)g.adj[c];
@@ -273,6 +377,7 @@ __global__ LAUNCH_BOUNDS_256 void k_pressure(DS d, MgParams p, MgGrid g)
         float csum = 0.0f;
         if (n != 0) {
             const uint32_t o = d.OFF[cell];
+            const uint32_t ne0 = ne;
             for (uint32_t j = o; j < o + n; j++) {
                 f3 dist = pi - xyz(d.P[j]);
                 float dsq = dist.x * dist.x + dist.y * dist.y + dist.z * dist.z;
@@ -284,9 +389,19 @@ __global__ LAUNCH_BOUNDS_256 void k_pressure(DS d, MgParams p, MgGrid g)
                     b *= b;
                     csum += b * (4.0f * q + 1.0f);
                 }
+                if (lists && dsq < lr2) {
+                    if (ne > 0 && ne <= cap) d.NBR[(size_t)(ne - 1) * d.N + i] = pend;
+                    pend = j;
+                    ne++;
+                }
             }
+            if (ne > ne0) pend |= 0x80000000u; /* last entry of this cell's group */
         }
         sum += csum;
+    }
+    if (lists) {
+        if (ne > 0 && ne <= cap) d.NBR[(size_t)(ne - 1) * d.N + i] = pend;
+        d.NBRN[i] = (ne <= cap) ? ne : MG_NONE;
     }
     sum = sum * p.pmass * p.wendlandC2kern;
     if (sum == 0.0f) sum = 1.0f;
@@ -303,7 +418,8 @@ __global__ LAUNCH_BOUNDS_256 void k_pressure(DS d, MgParams p, MgGrid g)
  *   FFORCE[i] = ((F0 + own) + in_1) + in_2 ...
  * Updates the bond integrators and breaks bonds as the reference does. */
 __device__ __forceinline__ f3 force_one(const DS& d, const MgParams& p, const MgGrid& g,
-                                        uint32_t i, uint32_t na, uint32_t gc, f3 F0)
+                                        uint32_t i, uint32_t na, uint32_t gc, f3 F0,
+                                        bool use_list)
 {
     gc -= (uint32_t)((1 * g.res[2] + 1) * g.res[0] + 1);
     const int N = d.N, B = d.B;
@@ -331,26 +447,50 @@ __device__ __forceinline__ f3 force_one(const DS& d, const MgParams& p, const Mg
         if (abs_dist >= bp.x && !freeze) ((float*)&d.BPAR[bi])[1] = 0.0f;
     }
 
+    /* Fluid terms. A cell with no neighbour in range adds (+0,+0,+0) to fsum, which never
+     * changes it (fsum starts at +0 and round-to-nearest sums are -0 only when both operands
+     * are), so walking only the cell groups of the neighbour list gives the same bits. */
     const float sr = p.psmoothradius, st = p.psurface_t;
     f3 fsum = mk3(0.0f, 0.0f, 0.0f);
-    for (int c = 0; c < g.adj_cnt; c++) {
-        const uint32_t cell = gc + (uint32_t)g.adj[c];
-        const uint32_t n = d.CNT[cell];
+    const uint32_t nl = use_list ? d.NBRN[i] : MG_NONE;
+    if (nl != MG_NONE) {
         f3 cf = mk3(0.0f, 0.0f, 0.0f);
-        if (n != 0) {
-            const uint32_t o = d.OFF[cell];
-            for (uint32_t j = o; j < o + n; j++) {
-                const float4 Pj = d.P[j];
-                f3 dist = ipos - xyz(Pj);
-                float dsq = dist.x * dist.x + dist.y * dist.y + dist.z * dist.z;
+        for (uint32_t e = 0; e < nl; e++) {
+            const uint32_t w = d.NBR[(size_t)e * N + i];
+            const uint32_t j = w & 0x7FFFFFFFu;
+            const float4 Pj = d.P[j];
+            f3 dist = ipos - xyz(Pj);
+            float dsq = dist.x * dist.x + dist.y * dist.y + dist.z * dist.z;
+            if (dsq < 1.0f && dsq > 0.0f) {
                 float r = sqrtf(dsq);
-                if (dsq < 1.0f && dsq > 0.0f) {
-                    const float4 Vj = d.V[j];
-                    cf = cf + fluid_term(dist, r, sr, ipress, Pj.w, idens, Vj.w, ivev, xyz(Vj), st);
-                }
+                const float4 Vj = d.V[j];
+                cf = cf + fluid_term(dist, r, sr, ipress, Pj.w, idens, Vj.w, ivev, xyz(Vj), st);
+            }
+            if (w & 0x80000000u) {
+                fsum = fsum + cf;
+                cf = mk3(0.0f, 0.0f, 0.0f);
             }
         }
-        fsum = fsum + cf;
+    } else {
+        for (int c = 0; c < g.adj_cnt; c++) {
+            const uint32_t cell = gc + (uint32_t)g.adj[c];
+            const uint32_t n = d.CNT[cell];
+            f3 cf = mk3(0.0f, 0.0f, 0.0f);
+            if (n != 0) {
+                const uint32_t o = d.OFF[cell];
+                for (uint32_t j = o; j < o + n; j++) {
+                    const float4 Pj = d.P[j];
+                    f3 dist = ipos - xyz(Pj);
+                    float dsq = dist.x * dist.x + dist.y * dist.y + dist.z * dist.z;
+                    if (dsq < 1.0f && dsq > 0.0f) {
+                        float r = sqrtf(dsq);
+                        const float4 Vj = d.V[j];
+                        cf = cf + fluid_term(dist, r, sr, ipress, Pj.w, idens, Vj.w, ivev, xyz(Vj), st);
+                    }
+                }
+            }
+            fsum = fsum + cf;
+        }
     }
     force = force + fsum * p.pmass;
 
@@ -379,7 +519,7 @@ __global__ LAUNCH_BOUNDS_256 void k_force(DS d, MgParams p, MgGrid g, int accumu
     const uint32_t gc = d.GCELL[i];
     if (gc == MG_GRID_UNDEF) return;
     f3 F0 = accumulate ? xyz(d.F[i]) : mk3(0.0f, 0.0f, 0.0f);
-    f3 F = force_one(d, p, g, i, na, gc, F0);
+    f3 F = force_one(d, p, g, i, na, gc, F0, d.NBR != nullptr);
     d.F[i] = make_float4(F.x, F.y, F.z, 0.0f);
 }
 
@@ -473,22 +613,32 @@ __global__ LAUNCH_BOUNDS_256 void k_advance(DS d, MgParams p, float dt)
 __global__ LAUNCH_BOUNDS_256 void k_force_advance(DS d, MgParams p, MgGrid g, float dt)
 {
     const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
-    if (i >= (uint32_t)d.N) return;
-    const uint32_t na = d.ctr[MG_C_ACTIVE];
-    const float4 P_i = d.P[i], V_i = d.V[i], VEL_i = d.VEL[i];
-    const uint32_t gc = (i < na) ? d.GCELL[i] : MG_GRID_UNDEF;
-    if (gc == MG_GRID_UNDEF) {
-        d.P2[i] = P_i;
-        d.V2[i] = V_i;
-        d.VEL2[i] = VEL_i;
-        return;
+    uint32_t bits = 0; /* |displacement| as float bits (positive floats order like uints) */
+    if (i < (uint32_t)d.N) {
+        const uint32_t na = d.ctr[MG_C_ACTIVE];
+        const float4 P_i = d.P[i], V_i = d.V[i], VEL_i = d.VEL[i];
+        const uint32_t gc = (i < na) ? d.GCELL[i] : MG_GRID_UNDEF;
+        if (gc == MG_GRID_UNDEF) {
+            d.P2[i] = P_i;
+            d.V2[i] = V_i;
+            d.VEL2[i] = VEL_i;
+        } else {
+            f3 F = force_one(d, p, g, i, na, gc, mk3(0.0f, 0.0f, 0.0f), d.NBR != nullptr);
+            f3 np, ve, vn;
+            advance_one(p, dt, xyz(P_i), xyz(V_i), xyz(VEL_i), F, &np, &ve, &vn);
+            d.P2[i] = make_float4(np.x, np.y, np.z, P_i.w);
+            d.V2[i] = make_float4(ve.x, ve.y, ve.z, V_i.w);
+            d.VEL2[i] = make_float4(vn.x, vn.y, vn.z, 0.0f);
+            f3 dx = np - xyz(P_i);
+            float disp = sqrtf(dx.x * dx.x + dx.y * dx.y + dx.z * dx.z);
+            /* NaN/inf map above every finite bound and disable the lists for diffusion */
+            bits = (disp == disp) ? __float_as_uint(disp) & 0x7FFFFFFFu : 0x7F800000u;
+        }
     }
-    f3 F = force_one(d, p, g, i, na, gc, mk3(0.0f, 0.0f, 0.0f));
-    f3 np, ve, vn;
-    advance_one(p, dt, xyz(P_i), xyz(V_i), xyz(VEL_i), F, &np, &ve, &vn);
-    d.P2[i] = make_float4(np.x, np.y, np.z, P_i.w);
-    d.V2[i] = make_float4(ve.x, ve.y, ve.z, V_i.w);
-    d.VEL2[i] = make_float4(vn.x, vn.y, vn.z, 0.0f);
+    if (d.NBR) {
+        for (int o = 32; o > 0; o >>= 1) bits = max(bits, (uint32_t)__shfl_xor((int)bits, o));
+        if (__lane_id() == 0 && bits) atomicMax(&d.ctr[MG_C_MAXDISP], bits);
+    }
 }
 
 /* ----------------------------------------------------------- diffusion (a12)
@@ -515,7 +665,8 @@ __device__ __forceinline__ void store16(float* dst, const float v[16])
 
 __device__ __forceinline__ void diffuse_one(const DS& d, const MgParams& p, const MgGrid& g,
                                             const MgGenome& gen, uint32_t i, uint32_t gc,
-                                            const float cur[MG_NUM_TF], float out[MG_NUM_TF])
+                                            const float cur[MG_NUM_TF], float out[MG_NUM_TF],
+                                            bool use_list)
 {
     gc -= (uint32_t)((1 * g.res[2] + 1) * g.res[0] + 1);
     const float d2 = p.psimscale * p.psimscale;
@@ -529,22 +680,28 @@ __device__ __forceinline__ void diffuse_one(const DS& d, const MgParams& p, cons
         if (gen.tf_diffusability[t]) dmask |= 1u << t;
     }
     const f3 pi = xyz(d.P[i]);
-    for (int c = 0; c < g.adj_cnt; c++) {
-        const uint32_t cell = gc + (uint32_t)g.adj[c];
-        const uint32_t n = d.CNT[cell];
-        if (n == 0) continue;
-        const uint32_t o = d.OFF[cell];
-        for (uint32_t j = o; j < o + n; j++) {
-            f3 dist = pi - xyz(d.P[j]);
-            float dsq = dist.x * dist.x + dist.y * dist.y + dist.z * dist.z;
-            if (dsq < r2 && dsq > 0.0f) {
-                const float cc = (r2 - dsq) * d2;
-                float cn[MG_NUM_TF];
-                load16(d.CONC + (size_t)j * MG_NUM_TF, cn);
+    /* one neighbour candidate j (same order with or without the list) */
+    auto visit = [&](uint32_t j) {
+        f3 dist = pi - xyz(d.P[j]);
+        float dsq = dist.x * dist.x + dist.y * dist.y + dist.z * dist.z;
+        if (dsq < r2 && dsq > 0.0f) {
+            const float cc = (r2 - dsq) * d2;
+            float cn[MG_NUM_TF];
+            load16(d.CONC + (size_t)j * MG_NUM_TF, cn);
 #pragma unroll
-                for (int t = 0; t < MG_NUM_TF; t++)
-                    if (dmask & (1u << t)) nw[t] += D[t] * cc * (cn[t] - cur[t]);
-            }
+            for (int t = 0; t < MG_NUM_TF; t++)
+                if (dmask & (1u << t)) nw[t] += D[t] * cc * (cn[t] - cur[t]);
+        }
+    };
+    const uint32_t nl = use_list ? d.NBRN[i] : MG_NONE;
+    if (nl != MG_NONE) {
+        for (uint32_t e = 0; e < nl; e++) visit(d.NBR[(size_t)e * d.N + i] & 0x7FFFFFFFu);
+    } else {
+        for (int c = 0; c < g.adj_cnt; c++) {
+            const uint32_t cell = gc + (uint32_t)g.adj[c];
+            const uint32_t n = d.CNT[cell];
+            const uint32_t o = d.OFF[cell];
+            for (uint32_t j = o; j < o + n; j++) visit(j);
         }
     }
 #pragma unroll
@@ -563,7 +720,7 @@ __global__ LAUNCH_BOUNDS_256 void k_diffusion(DS d, MgParams p, MgGrid g, MgGeno
         store16(d.CONC2 + (size_t)i * MG_NUM_TF, cur);
         return;
     }
-    diffuse_one(d, p, g, gen, i, gc, cur, out);
+    diffuse_one(d, p, g, gen, i, gc, cur, out, false);
     store16(d.CONC2 + (size_t)i * MG_NUM_TF, out);
 }
 
@@ -653,7 +810,10 @@ __global__ LAUNCH_BOUNDS_256 void k_diffuse_genes_insert(DS d, MgParams p, MgGri
 #pragma unroll
             for (int t = 0; t < MG_NUM_TF; t++) out[t] = cur[t];
         } else {
-            diffuse_one(d, p, g, gen, i, gc, cur, out);
+            /* the lists hold every j within sqrt(nbr_r2) of x_i before the advance; they cover
+             * all post-advance pairs within r2 while no particle moved more than nbr_slack */
+            const bool lists = d.NBR && __uint_as_float(d.ctr[MG_C_MAXDISP]) <= d.nbr_slack;
+            diffuse_one(d, p, g, gen, i, gc, cur, out, lists);
         }
         uint32_t mnext = 0;
         if (i < d.ctr[MG_C_NUM_POINTS]) {
