@@ -35,24 +35,23 @@ def algorithmic_bytes(phase: str, scene, stats: dict) -> float:
     """Compulsory HBM bytes per launch of the fused step's kernels in this build's layout:
     every array element read or written once, neighbour data counted once (as that
     particle's own read). n = active particles, C = cells, B = bond slots, bo / bi = mean
-    outgoing / incoming bonds per particle, L = mean neighbour-list length (entries within
-    the list radius, the particle itself included)."""
+    outgoing / incoming bonds per particle."""
     n, c, b = stats["n_active"], scene.c, scene.b
-    bo, bi, L = stats["bonds_out"], stats["bonds_in"], stats["list_len"]
+    bo, bi = stats["bonds_out"], stats["bonds_in"]
     genes = bool(scene.flags & 1)
     if phase == "pressure":
-        # GCELL 4, P 16 read; press, 1/density 8 written; list L*4 + count 4 written;
-        # CNT/OFF read once, next grid's CNT cleared
-        return n * (4 + 16 + 8 + 4 * L + 4) + 12 * c
+        # GCELL 4, P 16 read; press, 1/density 8 written; OFF read once, next grid's CNT
+        # cleared
+        return n * (4 + 16 + 8) + 8 * c
     if phase == "force":
         # force: GCELL 4, HOME 4, P 16, V 16 (+VEL 16 for the advance); BPARTNER 4 per slot;
         # per bond BPAR 16 + SLOT 4 + BINT 8 read + 8 written; INC_OFF 8, INC 4 per incoming
-        # bond; list count 4 + L*4 read; advance writes P, V, VEL: 48
-        return n * (56 + 4 * b + bo * 36 + 8 + 4 * bi + 4 + 4 * L + 48)
+        # bond; advance writes P, V, VEL: 48
+        return n * (56 + 4 * b + bo * 36 + 8 + 4 * bi + 48)
     if phase == "diffusion":
         # P 16, CONC 64 read, CONC 64 written, GCELL 4 read/written, GNDX 4, GMASK 4, HOME 4,
-        # EPIGEN 64 read (+64 written with genes), GMN 4, list 4 + 4*L
-        return n * (16 + 128 + 8 + 4 + 4 + 4 + 64 + (64 if genes else 0) + 4 + 4 + 4 * L) + 4 * c
+        # EPIGEN 64 read (+64 written with genes), GMN 4
+        return n * (16 + 128 + 8 + 4 + 4 + 4 + 64 + (64 if genes else 0) + 4) + 4 * c
     if phase == "scan":
         return 8 * c
     if phase == "place":  # k_place: GCELL, GNDX read, PERM written; k_cellsort: PERM r/w, GNDX w
@@ -81,10 +80,9 @@ def scene_stats(scene) -> dict:
     rng = np.random.default_rng(0)
     idx = rng.choice(pos.shape[0], size=min(4096, pos.shape[0]), replace=False)
     tree = cKDTree(pos)
-    # list radius: max(1, r2 / simscale^2) + 0.1 skin (mg_set_params)
-    r = max(1.0, float(np.sqrt(scene.params.r2 / scene.params.psimscale ** 2))) + 0.1
-    L = float(np.mean([len(x) for x in tree.query_ball_point(pos[idx], r)]))
-    return {"bonds_out": bo, "bonds_in": bi, "list_len": L}
+    # neighbours within the force radius (dsq < 1), the particle itself excluded
+    nbrs = float(np.mean([len(x) for x in tree.query_ball_point(pos[idx], 1.0)])) - 1.0
+    return {"bonds_out": bo, "bonds_in": bi, "neighbours": nbrs}
 
 
 def cpu_baseline(scene, budget_s: float = 15.0) -> dict:
@@ -139,6 +137,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--timed-steps", type=int, default=20)
     ap.add_argument("--no-graph", action="store_true", help="launch the step kernels eagerly")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="also run a plain device copy of this size (traffic counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -157,6 +157,15 @@ def main():
                 "fluid_block_32k": lambda: scenes.fluid_block(32, seed=1)}
     scene = builders[args.workload]()
     stats = scene_stats(scene)
+
+    if args.calib_copy_mib > 0:
+        n = args.calib_copy_mib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     sim = FluidSystem(scene.n, scene.b)
     sim.load_scene(scene)
@@ -221,7 +230,7 @@ def main():
                      "algorithmic_bytes_per_launch": alg, "avg_launch_ms": dom_ms},
         "step_breakdown_ms": breakdown,
         "kernels": kernels,
-        "mean_list_len": round(stats["list_len"], 2),
+        "mean_neighbours": round(stats["neighbours"], 2),
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(scene)

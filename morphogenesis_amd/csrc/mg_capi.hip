@@ -231,11 +231,14 @@ struct mg_handle {
     uint32_t *GCNT = nullptr, *GOFF = nullptr, *LENS = nullptr;
     uint32_t *LISTS = nullptr, *PERM = nullptr, *GMN = nullptr;
     bool gene_grid_stale = false;         /* gene counts/offsets/lists not rebuilt since a fused step */
-    /* neighbour lists (built by pressure, used by force and by the fused diffusion) */
-    uint32_t *NBR = nullptr, *NBRN = nullptr;
-    uint32_t nbr_cap = 64;
-    float nbr_r2 = 0.f, nbr_slack = 0.f;
-    bool nbr_fresh = false; /* lists match the current positions (pressure ran since they moved) */
+    /* OFF[gcur] is the exclusive prefix sum of CNT[gcur] (set by a scan, cleared by an insert
+     * or an upload of either); with a row-structured gridAdj the sweeps walk rows */
+    bool grid_scanned = false;
+    bool adj_rows = false;
+    /* slots sorted by cell by the last sort and positions unchanged since (no advance, no
+     * upload): sweeps may skip neighbour cells out of reach */
+    bool cells_fresh = false;
+    bool pos_binned = false; /* GCELL / CNT come from the current positions (insert since the last move) */
     uint32_t* ctr = nullptr;
     unsigned long long* scan_status = nullptr;  /* decoupled look-back status, one word per tile */
     uint32_t* scan_ticket = nullptr;
@@ -356,20 +359,10 @@ DS make_ds(mg_handle* h, int kin = 0, int conc = 0)
     d.GMN = nullptr;
     d.ZERO = nullptr;
     d.zero_n = 0;
-    d.NBR = nullptr;
-    d.NBRN = nullptr;
-    d.nbr_cap = h->nbr_cap;
-    d.nbr_r2 = h->nbr_r2;
-    d.nbr_slack = h->nbr_slack;
+    d.rows = (h->grid_scanned && h->adj_rows) ? 1 : 0;
+    d.cull = (d.rows && h->cells_fresh) ? 1 : 0;
     d.ctr = h->ctr;
     d.N = h->N; d.B = h->B; d.C = h->C;
-    return d;
-}
-
-DS with_lists(mg_handle* h, DS d)
-{
-    d.NBR = h->NBR;
-    d.NBRN = h->NBRN;
     return d;
 }
 
@@ -479,6 +472,34 @@ int normalize(mg_handle* h)
     return rebuild_inc(h);
 }
 
+/* The FCONC float4 groups the diffusion sweep reads and the per-TF coefficients
+ * tf_diffusability * diff_scale (the same float operations as the reference's loop). */
+DiffPlan diff_plan(const mg_handle* h)
+{
+    DiffPlan dp;
+    std::memset(&dp, 0, sizeof(dp));
+    int used[4], n = 0;
+    for (int q = 0; q < 4; q++) {
+        bool any = false;
+        for (int e = 0; e < 4; e++) any = any || h->gen.tf_diffusability[4 * q + e] != 0;
+        if (any) used[n++] = q;
+    }
+    if (n == 3) {
+        n = 4;
+        for (int q = 0; q < 4; q++) used[q] = q;
+    }
+    dp.ng = n;
+    for (int q = 0; q < 4; q++) dp.grp[q] = q < n ? used[q] : 0;
+    for (int q = 0; q < n; q++)
+        for (int e = 0; e < 4; e++) {
+            const int t = 4 * used[q] + e;
+            const uint32_t df = h->gen.tf_diffusability[t];
+            dp.D[4 * q + e] = (float)df * h->p.diff_scale;
+            if (df) dp.dm |= 1u << (4 * q + e);
+        }
+    return dp;
+}
+
 /* --------------------------------------------------------------- phases */
 int do_insert(mg_handle* h, int gene_counts)
 {
@@ -495,6 +516,9 @@ int do_insert(mg_handle* h, int gene_counts)
     PhaseScope ps(h, MG_PH_INSERT);
     hipLaunchKernelGGL(k_insert, dim3(nblk(h->N)), dim3(256), 0, h->st, d, h->g, gene_counts);
     HIPCHK(hipGetLastError());
+    h->grid_scanned = false;
+    h->pos_binned = true;
+    h->cells_fresh = false;
     return MG_OK;
 }
 
@@ -509,6 +533,7 @@ int do_scan(mg_handle* h, int genes)
     hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, h->st, h->CNT[h->gcur], h->OFF[h->gcur], h->C, h->ctr, 1,
                        grid_scan_max_index(h->g), (uint32_t*)nullptr);
     HIPCHK(hipGetLastError());
+    h->grid_scanned = true;
     if (genes) {
         hipLaunchKernelGGL(k_scan, dim3(MG_NUM_GENES), dim3(1024), 0, h->st, h->GCNT, h->GOFF,
                            h->C, h->ctr, 0, 0, h->LENS);
@@ -534,8 +559,8 @@ int do_sort(mg_handle* h, int lists)
     h->kcur = 1 - h->kcur;
     h->icur = 1 - h->icur;
     h->ccur = 1 - h->ccur;
+    h->cells_fresh = h->pos_binned;
     h->force_zero = true;
-    h->nbr_fresh = false;
     h->normalized = false;
     if (lists) {
         PhaseScope ps(h, MG_PH_LISTS);
@@ -574,21 +599,26 @@ int do_fused_step(mg_handle* h, float dt, unsigned flags)
     const unsigned nb = nblk(h->N);
     {
         PhaseScope ps(h, MG_PH_PRESSURE);
-        DS d = with_lists(h, make_ds(h));
+        DS d = make_ds(h);
         d.ZERO = h->CNT[1 - h->gcur];
         d.zero_n = (uint32_t)h->C;
         hipLaunchKernelGGL(k_pressure, dim3(nb), dim3(256), 0, h->st, d, h->p, h->g);
     }
     {
         PhaseScope ps(h, MG_PH_FORCE);
-        hipLaunchKernelGGL(k_force_advance, dim3(nb), dim3(256), 0, h->st, with_lists(h, make_ds(h)), h->p, h->g, dt);
+        hipLaunchKernelGGL(k_force_advance, dim3(nb), dim3(256), 0, h->st, make_ds(h), h->p, h->g, dt);
     }
     {
         PhaseScope ps(h, MG_PH_DIFFUSION);
-        DS d = with_lists(h, make_ds(h, 1, 0));
+        DS d = make_ds(h, 1, 0);
         d.GMN = h->GMN;
-        hipLaunchKernelGGL(k_diffuse_genes_insert, dim3(nb), dim3(256), 0, h->st, d, h->p, h->g,
-                           h->gen, genes);
+        const DiffPlan dp = diff_plan(h);
+        if (dp.ng <= 1)
+            hipLaunchKernelGGL(k_diffuse_genes_insert<1>, dim3(nb), dim3(256), 0, h->st, d, h->p, h->g, h->gen, dp, genes);
+        else if (dp.ng == 2)
+            hipLaunchKernelGGL(k_diffuse_genes_insert<2>, dim3(nb), dim3(256), 0, h->st, d, h->p, h->g, h->gen, dp, genes);
+        else
+            hipLaunchKernelGGL(k_diffuse_genes_insert<4>, dim3(nb), dim3(256), 0, h->st, d, h->p, h->g, h->gen, dp, genes);
     }
     {
         PhaseScope ps(h, MG_PH_SCAN);
@@ -618,7 +648,9 @@ int do_fused_step(mg_handle* h, float dt, unsigned flags)
     h->force_zero = true;
     h->normalized = false;
     h->gene_grid_stale = true;
-    h->nbr_fresh = false;
+    h->grid_scanned = true;
+    h->pos_binned = true;
+    h->cells_fresh = true;
     return MG_OK;
 }
 
@@ -628,7 +660,8 @@ int do_fused_step(mg_handle* h, float dt, unsigned flags)
  * handle's epoch change. */
 int graph_steps(mg_handle* h, int pairs, float dt, unsigned flags)
 {
-    const int parity = (h->kcur << 3) | (h->icur << 2) | (h->ccur << 1) | h->gcur;
+    const int parity = ((h->cells_fresh && h->grid_scanned) << 4) | (h->kcur << 3) | (h->icur << 2) |
+                       (h->ccur << 1) | h->gcur;
     if (!h->graph || h->graph_epoch != h->epoch || h->graph_dt != dt || h->graph_flags != flags ||
         h->graph_parity != parity) {
         if (h->graph) {
@@ -661,6 +694,9 @@ int graph_steps(mg_handle* h, int pairs, float dt, unsigned flags)
     h->force_zero = true;
     h->normalized = false;
     h->gene_grid_stale = true;
+    h->grid_scanned = true;
+    h->pos_binned = true;
+    h->cells_fresh = true;
     return MG_OK;
 }
 
@@ -677,9 +713,8 @@ int ensure_force(mg_handle* h)
 int do_pressure(mg_handle* h)
 {
     PhaseScope ps(h, MG_PH_PRESSURE);
-    hipLaunchKernelGGL(k_pressure, dim3(nblk(h->N)), dim3(256), 0, h->st, with_lists(h, make_ds(h)), h->p, h->g);
+    hipLaunchKernelGGL(k_pressure, dim3(nblk(h->N)), dim3(256), 0, h->st, make_ds(h), h->p, h->g);
     HIPCHK(hipGetLastError());
-    h->nbr_fresh = true;
     return MG_OK;
 }
 
@@ -687,7 +722,7 @@ int do_force(mg_handle* h)
 {
     CHK(ensure_force(h));
     PhaseScope ps(h, MG_PH_FORCE);
-    DS d = h->nbr_fresh ? with_lists(h, make_ds(h)) : make_ds(h);
+    DS d = make_ds(h);
     hipLaunchKernelGGL(k_force, dim3(nblk(h->N)), dim3(256), 0, h->st, d, h->p, h->g, 1);
     HIPCHK(hipGetLastError());
     return MG_OK;
@@ -699,14 +734,22 @@ int do_advance(mg_handle* h, float dt)
     PhaseScope ps(h, MG_PH_ADVANCE);
     hipLaunchKernelGGL(k_advance, dim3(nblk(h->N)), dim3(256), 0, h->st, make_ds(h), h->p, dt);
     HIPCHK(hipGetLastError());
-    h->nbr_fresh = false;
+    h->cells_fresh = false;
+    h->pos_binned = false;
     return MG_OK;
 }
 
 int do_diffusion(mg_handle* h)
 {
     PhaseScope ps(h, MG_PH_DIFFUSION);
-    hipLaunchKernelGGL(k_diffusion, dim3(nblk(h->N)), dim3(256), 0, h->st, make_ds(h), h->p, h->g, h->gen);
+    const DiffPlan dp = diff_plan(h);
+    const DS d = make_ds(h);
+    if (dp.ng <= 1)
+        hipLaunchKernelGGL(k_diffusion<1>, dim3(nblk(h->N)), dim3(256), 0, h->st, d, h->p, h->g, h->gen, dp);
+    else if (dp.ng == 2)
+        hipLaunchKernelGGL(k_diffusion<2>, dim3(nblk(h->N)), dim3(256), 0, h->st, d, h->p, h->g, h->gen, dp);
+    else
+        hipLaunchKernelGGL(k_diffusion<4>, dim3(nblk(h->N)), dim3(256), 0, h->st, d, h->p, h->g, h->gen, dp);
     HIPCHK(hipGetLastError());
     h->ccur = 1 - h->ccur;
     return MG_OK;
@@ -757,7 +800,7 @@ int mg_create(mg_handle** out, int max_points, int bonds_per_particle)
     const size_t N = (size_t)max_points, B = (size_t)bonds_per_particle;
     int r = MG_OK;
     for (int k = 0; k < 2 && r == MG_OK; k++) {
-        r = dalloc(h, &h->P[k], N);
+        r = dalloc(h, &h->P[k], N + MG_P_PAD);
         if (r == MG_OK) r = dalloc(h, &h->V[k], N);
         if (r == MG_OK) r = dalloc(h, &h->VEL[k], N);
         if (r == MG_OK) r = dalloc(h, &h->CONC[k], N * MG_NUM_TF);
@@ -783,8 +826,6 @@ int mg_create(mg_handle** out, int max_points, int bonds_per_particle)
     if (r == MG_OK) r = dalloc(h, &h->INC_OFF, N + 1);
     if (r == MG_OK) r = dalloc(h, &h->PERM, N);
     if (r == MG_OK) r = dalloc(h, &h->GMN, N);
-    if (r == MG_OK) r = dalloc(h, &h->NBR, N * h->nbr_cap);
-    if (r == MG_OK) r = dalloc(h, &h->NBRN, N);
     if (r == MG_OK) r = dalloc(h, &h->LISTS, N * MG_NUM_GENES);
     if (r == MG_OK) r = dalloc(h, &h->LENS, MG_NUM_GENES);
     if (r == MG_OK) r = dalloc(h, &h->ctr, MG_C_COUNT);
@@ -793,7 +834,7 @@ int mg_create(mg_handle** out, int max_points, int bonds_per_particle)
      * bonds), every slot its own home, no sort yet */
     const int n = max_points;
     for (int k = 0; k < 2; k++) {
-        fill_u32(h, h->P[k], 0, N * 4);
+        fill_u32(h, h->P[k], 0, (N + MG_P_PAD) * 4);
         fill_u32(h, h->V[k], 0, N * 4);
         fill_u32(h, h->VEL[k], 0, N * 4);
         fill_u32(h, h->CONC[k], 0, N * MG_NUM_TF);
@@ -861,6 +902,16 @@ int mg_set_grid(mg_handle* h, const MgGrid* g)
         set_err("mg_set_grid: total != res.x*res.y*res.z");
         return MG_ERR_ARG;
     }
+    {
+        /* every neighbour cell of an in-window particle must be a grid cell */
+        const long long gc0max = ((long long)(g->scan_max[1] - 1) * g->res[2] + (g->scan_max[2] - 1)) * g->res[0] +
+                                 (g->scan_max[0] - 1);
+        for (int c = 0; c < g->adj_cnt; c++)
+            if (g->adj[c] < 0 || gc0max + g->adj[c] >= (long long)g->total) {
+                set_err("mg_set_grid: gridAdj[%d] = %d leaves the grid", c, g->adj[c]);
+                return MG_ERR_ARG;
+            }
+    }
     if (h->have_grid && g->total != h->C) {
         for (int k = 0; k < 2; k++) { dfree(h, h->CNT[k]); dfree(h, h->OFF[k]); }
         dfree(h, h->GCNT); dfree(h, h->GOFF);
@@ -886,6 +937,14 @@ int mg_set_grid(mg_handle* h, const MgGrid* g)
     }
     h->g = *g;
     h->have_grid = true;
+    /* row walks need gridAdj[(y * srch + z) * srch + x] == (y * res.z + z) * res.x + x */
+    h->adj_rows = g->srch >= 1 && g->srch <= 4 && g->adj_cnt == g->srch * g->srch * g->srch;
+    for (int y = 0; h->adj_rows && y < g->srch; y++)
+        for (int z = 0; z < g->srch; z++)
+            for (int x = 0; x < g->srch; x++)
+                if (g->adj[(y * g->srch + z) * g->srch + x] != (y * g->res[2] + z) * g->res[0] + x)
+                    h->adj_rows = false;
+    h->grid_scanned = false;
     h->epoch++;
     return MG_OK;
 }
@@ -908,16 +967,6 @@ int mg_set_params(mg_handle* h, const MgParams* p)
     if (!h || !p) { set_err("mg_set_params: null argument"); return MG_ERR_ARG; }
     h->p = *p;
     h->have_params = true;
-    /* neighbour-list radius: covers the force range (dsq < 1) and the diffusion range
-     * (dsq < r2 / simscale^2) plus a skin; diffusion, which runs after the advance, uses the
-     * lists only while no particle moved more than half the skin */
-    const float d2f = p->psimscale * p->psimscale;
-    const float r2f = p->r2 / d2f;
-    const double rb = std::max(1.0, std::sqrt((double)r2f));
-    const double skin = 0.1;
-    h->nbr_r2 = (float)((rb + skin) * (rb + skin) * (1.0 + 1e-5));
-    h->nbr_slack = (float)(0.5 * skin * 0.99);
-    if (!(h->nbr_r2 > 0.f) || !std::isfinite(h->nbr_r2)) h->nbr_r2 = 0.f, h->nbr_slack = -1.f;
     h->epoch++;
     return MG_OK;
 }
@@ -968,7 +1017,8 @@ int mg_upload(mg_handle* h, int buf_id, const void* src, size_t bytes)
     if (bytes != need) { set_err("mg_upload: buffer %d needs %zu bytes, got %zu", buf_id, need, bytes); return MG_ERR_SIZE; }
     if (need == 0) return MG_OK;
     CHK(normalize(h));
-    h->nbr_fresh = false;
+    h->cells_fresh = false;
+    h->pos_binned = false;
     if (buf_id >= MG_FBIN_COUNT_ACTIVE_GENES && buf_id <= MG_FDENSE_LISTS) CHK(materialize_gene_grid(h));
     CHK(staging(h, bytes));
     HIPCHK(hipMemcpyAsync(h->staging, src, bytes, hipMemcpyHostToDevice, h->st));
@@ -1013,8 +1063,14 @@ int mg_upload(mg_handle* h, int buf_id, const void* src, size_t bytes)
         if (B > 0)
             hipLaunchKernelGGL(k_upload_pidx, dim3(nb, B), dim3(256), 0, h->st, h->PIDX, (const uint32_t*)stg, N, B);
         break;
-    case MG_FBIN_COUNT: HIPCHK(hipMemcpyAsync(h->CNT[h->gcur], stg, bytes, hipMemcpyDeviceToDevice, h->st)); break;
-    case MG_FBIN_OFFSET: HIPCHK(hipMemcpyAsync(h->OFF[h->gcur], stg, bytes, hipMemcpyDeviceToDevice, h->st)); break;
+    case MG_FBIN_COUNT:
+        HIPCHK(hipMemcpyAsync(h->CNT[h->gcur], stg, bytes, hipMemcpyDeviceToDevice, h->st));
+        h->grid_scanned = false;
+        break;
+    case MG_FBIN_OFFSET:
+        HIPCHK(hipMemcpyAsync(h->OFF[h->gcur], stg, bytes, hipMemcpyDeviceToDevice, h->st));
+        h->grid_scanned = false;
+        break;
     case MG_FBIN_COUNT_ACTIVE_GENES: HIPCHK(hipMemcpyAsync(h->GCNT, stg, bytes, hipMemcpyDeviceToDevice, h->st)); break;
     case MG_FBIN_OFFSET_ACTIVE_GENES: HIPCHK(hipMemcpyAsync(h->GOFF, stg, bytes, hipMemcpyDeviceToDevice, h->st)); break;
     case MG_FDENSE_LIST_LENGTHS: HIPCHK(hipMemcpyAsync(h->LENS, stg, bytes, hipMemcpyDeviceToDevice, h->st)); break;

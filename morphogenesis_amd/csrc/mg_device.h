@@ -32,7 +32,6 @@ enum {
     MG_C_LISTS = 4,       /* 1 once a sort has built the gene membership              */
     MG_C_SCAN_NUM = 5,    /* FBIN_OFFSET[C-1] of the last scan (committed by the sort) */
     MG_C_SCAN_ACTIVE = 6, /* FBIN_OFFSET[grid_ScanMax] of the last scan               */
-    MG_C_MAXDISP = 7,     /* float bits: largest |x_new - x_old| of the last advance    */
     MG_C_ERR = 8,         /* nonzero: a kernel gave up a bounded wait (result invalid)  */
     MG_C_COUNT = 16
 };
@@ -59,14 +58,13 @@ struct DS {
     /* cleared by k_pressure when set (zero_n words) */
     uint32_t* ZERO;
     uint32_t zero_n;
-    /* neighbour lists written by k_pressure: NBR[e * N + i] = neighbour slot j | (end of its
-     * cell group) << 31, for every j in the 27 cells with |x_i - x_j|^2 < nbr_r2, in the
-     * order the cells and their particles are visited; NBRN[i] = entry count or MG_NONE
-     * when the list did not fit in nbr_cap entries. Null when unused. */
-    uint32_t *NBR, *NBRN;
-    uint32_t nbr_cap;
-    float nbr_r2;
-    float nbr_slack;  /* diffusion may use the lists while the largest displacement <= this */
+    /* 1 when OFF is the exclusive prefix sum of CNT (the sorted state) and the adjacency
+     * list is made of rows of g.srch consecutive cells: sweeps then walk one slot range per
+     * row. 0 selects the per-cell walk (any CNT / OFF contents). */
+    int rows;
+    /* 1 when the slots are sorted by cell (CNT / OFF / GCELL of the last sort) and the
+     * positions have not changed since: sweeps may skip cells out of reach */
+    int cull;
     uint32_t* ctr;
     int N, B, C;
 };

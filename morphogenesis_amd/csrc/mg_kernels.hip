@@ -20,6 +20,9 @@
 #include <climits>
 
 #define LAUNCH_BOUNDS_256 __launch_bounds__(256)
+/* sweep kernels: at most 128 VGPRs, so four 256-thread blocks fit on a CU (every wave of a
+ * 262k-particle launch resident at once) */
+#define LAUNCH_BOUNDS_SWEEP __launch_bounds__(256, 4)
 
 /* ------------------------------------------------------------------ fills */
 __global__ LAUNCH_BOUNDS_256 void k_fill_u32(uint32_t* p, uint32_t v, size_t n)
@@ -347,62 +350,161 @@ __global__ LAUNCH_BOUNDS_256 void k_gene_counts_sorted(DS d)
         if (m & (1u << g)) atomicAdd(&d.GCNT[(size_t)g * d.C + c], 1u);
 }
 
+/* ------------------------------------------------------------ neighbour sweep
+ * The reference visits the srch^3 cells gc + gridAdj[c] in gridAdj order and, inside a cell,
+ * the slots OFF[cell] .. OFF[cell] + CNT[cell] - 1 in order. gridAdj is built y, z, x with
+ * x fastest, so the cells come in srch^2 rows of srch consecutive cell indices; in the
+ * sorted state (d.rows) cell k of a row starting at cell c0 holds the slots
+ * [OFF[c0 + k], OFF[c0 + k + 1]), so one row needs srch + 1 offsets. A sweep calls
+ *   visit(j, P[j], x_i - x_j, |x_i - x_j|^2, extra(j))  for candidates with 0 < dsq < lim,
+ *   end_cell()                                          after each cell,
+ * in exactly the reference's order, fetching up to K candidates of a cell at a time so their
+ * loads are in flight together; extra(j) is issued for all hits of a batch before the first
+ * visit. Callers use end_cell() to add a per-cell partial sum: for a cell without hits it
+ * adds +0, a no-op for sums that start at +0, so cells may also be skipped.
+ *
+ * With d.cull (slots sorted by cell and positions unchanged since the binning), cells whose
+ * box lies farther than sqrt(lim) from x_i are skipped: no candidate in them can hit. The
+ * box distance is shrunk by 1e-3 cell per axis, far above the rounding of the binning and
+ * of dsq, so the skip never drops a hit.
+ *
+ * P arrays carry MG_P_PAD slots past N, so a batch reads P[j0 .. j0 + K - 1] unclamped. */
+#define MG_P_PAD 16
+
+struct RowBounds {
+    uint32_t b[5]; /* cell k of the row: [b[k], b[k + 1]) */
+};
+
+__device__ __forceinline__ RowBounds row_bounds(const DS& d, const MgGrid& g, uint32_t gc0, int s)
+{
+    RowBounds r;
+    if (d.rows) {
+        const uint32_t c0 = gc0 + (uint32_t)g.adj[s * g.srch];
+#pragma unroll
+        for (int k = 0; k < 5; k++) r.b[k] = (k <= g.srch) ? d.OFF[c0 + (uint32_t)k] : 0u;
+    } else {
+        const uint32_t c = gc0 + (uint32_t)g.adj[s];
+        r.b[0] = d.OFF[c];
+        r.b[1] = r.b[0] + d.CNT[c];
+        r.b[2] = r.b[3] = r.b[4] = 0u;
+    }
+    /* never read past the particle arrays, whatever CNT / OFF hold */
+#pragma unroll
+    for (int k = 0; k < 5; k++) r.b[k] = min(r.b[k], (uint32_t)d.N);
+    return r;
+}
+
+/* squared distance from x_i to the box of the neighbour cell at offset (ox, oy, oz) from the
+ * particle's own cell, in world units, each axis gap shrunk by 1e-3 cell; fr = the particle's
+ * position inside its own cell in cell units */
+__device__ __forceinline__ float axis_gap(float fr, int o, float inv_delta)
+{
+    const float gap = o < 0 ? fr - (float)(o + 1) : (o > 0 ? (float)o - fr : 0.0f);
+    return fmaxf(gap - 1e-3f, 0.0f) * inv_delta;
+}
+
+template <int K, class Extra, class Visit, class EndCell>
+__device__ __forceinline__ void sweep(const DS& d, const MgGrid& g, uint32_t gc0, f3 pi, float lim,
+                                      Extra extra, Visit visit, EndCell end_cell)
+{
+    using X = decltype(extra(0u));
+    const bool rows = d.rows != 0;
+    const int nseg = rows ? g.srch * g.srch : g.adj_cnt;
+    const int ncell = rows ? g.srch : 1;
+    const bool cull = rows && d.cull;
+    /* position of x_i inside its own cell (cell units) and cell sizes, for the culling */
+    f3 fr = mk3(0.f, 0.f, 0.f), inv = mk3(0.f, 0.f, 0.f);
+    if (cull) {
+        const uint32_t own = gc0 + (uint32_t)((1 * g.res[2] + 1) * g.res[0] + 1);
+        const uint32_t cx = own % (uint32_t)g.res[0];
+        const uint32_t cz = (own / (uint32_t)g.res[0]) % (uint32_t)g.res[2];
+        const uint32_t cy = own / ((uint32_t)g.res[0] * (uint32_t)g.res[2]);
+        fr = mk3((pi.x - g.gmin[0]) * g.delta[0] - (float)cx, (pi.y - g.gmin[1]) * g.delta[1] - (float)cy,
+                 (pi.z - g.gmin[2]) * g.delta[2] - (float)cz);
+        inv = mk3(1.0f / g.delta[0], 1.0f / g.delta[1], 1.0f / g.delta[2]);
+    }
+    RowBounds nxt = row_bounds(d, g, gc0, 0);
+    for (int s = 0; s < nseg; s++) {
+        const RowBounds rb = nxt;
+        if (s + 1 < nseg) nxt = row_bounds(d, g, gc0, s + 1);
+        float gyz = 0.0f;
+        if (cull) {
+            const float gy = axis_gap(fr.y, s / g.srch - 1, inv.y);
+            const float gz = axis_gap(fr.z, s % g.srch - 1, inv.z);
+            gyz = gy * gy + gz * gz;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            bool live = k < ncell;
+            if (cull && live) {
+                const float gx = axis_gap(fr.x, k - 1, inv.x);
+                live = gyz + gx * gx < lim;
+            }
+            if (!live) continue;
+            const uint32_t lo = rb.b[k], hi = max(rb.b[k + 1], lo);
+            for (uint32_t j0 = lo; j0 < hi; j0 += K) {
+                const float4* pb = d.P + j0;
+                float4 pj[K];
+#pragma unroll
+                for (int q = 0; q < K; q++) pj[q] = pb[q];
+                bool hit[K];
+#pragma unroll
+                for (int q = 0; q < K; q++) {
+                    const f3 dist = pi - xyz(pj[q]);
+                    const float dsq = dist.x * dist.x + dist.y * dist.y + dist.z * dist.z;
+                    hit[q] = (j0 + (uint32_t)q < hi) && dsq < lim && dsq > 0.0f;
+                }
+                X ex[K];
+#pragma unroll
+                for (int q = 0; q < K; q++)
+                    if (hit[q]) ex[q] = extra(j0 + (uint32_t)q);
+#pragma unroll
+                for (int q = 0; q < K; q++) {
+                    if (hit[q]) {
+                        /* recomputed (same operations) rather than kept in registers */
+                        const f3 dist = pi - xyz(pj[q]);
+                        const float dsq = dist.x * dist.x + dist.y * dist.y + dist.z * dist.z;
+                        visit(j0 + (uint32_t)q, pj[q], dist, dsq, ex[q]);
+                    }
+                }
+            }
+            end_cell();
+        }
+    }
+}
+
 /* ------------------------------------------------------------ pressure (a9)
  * contributePressure/computePressure, src/fluid_system_cuda.cu:620-712. Per cell partial
  * sums are added to the running total in gridAdj order. Writes FPRESS into P.w and
  * FDENSITY (= 1/density) into V.w. Also clears d.ZERO (the next grid's counters) when set. */
-__global__ LAUNCH_BOUNDS_256 void k_pressure(DS d, MgParams p, MgGrid g)
+__global__ LAUNCH_BOUNDS_SWEEP void k_pressure(DS d, MgParams p, MgGrid g)
 {
     if (d.ZERO) {
         for (uint32_t z = blockIdx.x * blockDim.x + threadIdx.x; z < d.zero_n; z += gridDim.x * blockDim.x)
             d.ZERO[z] = 0u;
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) d.ctr[MG_C_MAXDISP] = 0u;
     const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= d.ctr[MG_C_ACTIVE]) return;
     uint32_t gc = d.GCELL[i];
     if (gc == MG_GRID_UNDEF) return;
     gc -= (uint32_t)((1 * g.res[2] + 1) * g.res[0] + 1);
     const f3 pi = xyz(d.P[i]);
-    const float r2 = p.r2, sr = p.psmoothradius;
-    const bool lists = d.NBR != nullptr;
-    const float lr2 = d.nbr_r2;
-    const uint32_t cap = d.nbr_cap;
-    uint32_t ne = 0;          /* entries appended */
-    uint32_t pend = 0;        /* last appended entry, written once its group flag is known */
-    float sum = 0.0f;
-    for (int c = 0; c < g.adj_cnt; c++) {
-        const uint32_t cell = gc + (uint32_t)g.adj[c];
-        const uint32_t n = d.CNT[cell];
-        float csum = 0.0f;
-        if (n != 0) {
-            const uint32_t o = d.OFF[cell];
-            const uint32_t ne0 = ne;
-            for (uint32_t j = o; j < o + n; j++) {
-                f3 dist = pi - xyz(d.P[j]);
-                float dsq = dist.x * dist.x + dist.y * dist.y + dist.z * dist.z;
-                if (dsq < r2 && dsq > 0.0f) {
-                    float r = sqrtf(dsq);
-                    float q = r / sr;
-                    float b = 1.0f - q;
-                    b *= b;
-                    b *= b;
-                    csum += b * (4.0f * q + 1.0f);
-                }
-                if (lists && dsq < lr2) {
-                    if (ne > 0 && ne <= cap) d.NBR[(size_t)(ne - 1) * d.N + i] = pend;
-                    pend = j;
-                    ne++;
-                }
-            }
-            if (ne > ne0) pend |= 0x80000000u; /* last entry of this cell's group */
-        }
-        sum += csum;
-    }
-    if (lists) {
-        if (ne > 0 && ne <= cap) d.NBR[(size_t)(ne - 1) * d.N + i] = pend;
-        d.NBRN[i] = (ne <= cap) ? ne : MG_NONE;
-    }
+    const float sr = p.psmoothradius;
+    float sum = 0.0f, csum = 0.0f;
+    sweep<8>(
+        d, g, gc, pi, p.r2, [](uint32_t) { return 0; },
+        [&](uint32_t, float4, f3, float dsq, int) {
+            float r = sqrtf(dsq);
+            float q = r / sr;
+            float b = 1.0f - q;
+            b *= b;
+            b *= b;
+            csum += b * (4.0f * q + 1.0f);
+        },
+        [&]() {
+            sum += csum;
+            csum = 0.0f;
+        });
     sum = sum * p.pmass * p.wendlandC2kern;
     if (sum == 0.0f) sum = 1.0f;
     ((float*)&d.P[i])[3] = (sum - p.prest_dens) * p.pintstiff;
@@ -416,102 +518,122 @@ __global__ LAUNCH_BOUNDS_256 void k_pressure(DS d, MgParams p, MgGrid g)
  * partners is replaced by each particle gathering the terms of its incoming bonds (same
  * expression, same operands, hence the same bits) in (owner home, slot) order:
  *   FFORCE[i] = ((F0 + own) + in_1) + in_2 ...
- * Updates the bond integrators and breaks bonds as the reference does. */
+ * Updates the bond integrators and breaks bonds as the reference does. Bond slots are
+ * scanned 32 at a time for live partners and the live ones fetched 4 at a time; incoming
+ * bonds are fetched 4 at a time. */
 __device__ __forceinline__ f3 force_one(const DS& d, const MgParams& p, const MgGrid& g,
-                                        uint32_t i, uint32_t na, uint32_t gc, f3 F0,
-                                        bool use_list)
+                                        uint32_t i, uint32_t na, uint32_t gc, f3 F0)
 {
     gc -= (uint32_t)((1 * g.res[2] + 1) * g.res[0] + 1);
     const int N = d.N, B = d.B;
     const uint32_t h = d.HOME[i];
+    const uint32_t e0 = d.INC_OFF[h], e1 = d.INC_OFF[h + 1];
     const float4 P_i = d.P[i], V_i = d.V[i];
     const f3 ipos = xyz(P_i), ivev = xyz(V_i);
     const float ipress = P_i.w, idens = V_i.w;
     const bool freeze = p.freeze != 0;
 
     f3 force = mk3(0.0f, 0.0f, 0.0f);
-    for (int a = 0; a < B; a++) {
-        const size_t bi = (size_t)a * N + h;
-        const uint32_t jh = d.BPARTNER[bi];
-        if (jh == MG_NONE) continue;
-        const uint32_t j = d.SLOT[jh];
-        float4 bp = d.BPAR[bi];
-        if (j >= na || (double)bp.y < 0.000000001) continue;
-        float strain, abs_dist;
-        f3 eterm = bond_term(ipos, ivev, xyz(d.P[j]), xyz(d.V[j]), bp, p.pmass, &strain, &abs_dist);
-        float2 in = d.BINT[bi];
-        in.x = in.x + strain * strain;
-        in.y = in.y + strain;
-        d.BINT[bi] = in;
-        if (!has_nan3(eterm)) force = force - eterm;
-        if (abs_dist >= bp.x && !freeze) ((float*)&d.BPAR[bi])[1] = 0.0f;
-    }
-
-    /* Fluid terms. A cell with no neighbour in range adds (+0,+0,+0) to fsum, which never
-     * changes it (fsum starts at +0 and round-to-nearest sums are -0 only when both operands
-     * are), so walking only the cell groups of the neighbour list gives the same bits. */
-    const float sr = p.psmoothradius, st = p.psurface_t;
-    f3 fsum = mk3(0.0f, 0.0f, 0.0f);
-    const uint32_t nl = use_list ? d.NBRN[i] : MG_NONE;
-    if (nl != MG_NONE) {
-        f3 cf = mk3(0.0f, 0.0f, 0.0f);
-        for (uint32_t e = 0; e < nl; e++) {
-            const uint32_t w = d.NBR[(size_t)e * N + i];
-            const uint32_t j = w & 0x7FFFFFFFu;
-            const float4 Pj = d.P[j];
-            f3 dist = ipos - xyz(Pj);
-            float dsq = dist.x * dist.x + dist.y * dist.y + dist.z * dist.z;
-            if (dsq < 1.0f && dsq > 0.0f) {
-                float r = sqrtf(dsq);
-                const float4 Vj = d.V[j];
-                cf = cf + fluid_term(dist, r, sr, ipress, Pj.w, idens, Vj.w, ivev, xyz(Vj), st);
-            }
-            if (w & 0x80000000u) {
-                fsum = fsum + cf;
-                cf = mk3(0.0f, 0.0f, 0.0f);
-            }
-        }
-    } else {
-        for (int c = 0; c < g.adj_cnt; c++) {
-            const uint32_t cell = gc + (uint32_t)g.adj[c];
-            const uint32_t n = d.CNT[cell];
-            f3 cf = mk3(0.0f, 0.0f, 0.0f);
-            if (n != 0) {
-                const uint32_t o = d.OFF[cell];
-                for (uint32_t j = o; j < o + n; j++) {
-                    const float4 Pj = d.P[j];
-                    f3 dist = ipos - xyz(Pj);
-                    float dsq = dist.x * dist.x + dist.y * dist.y + dist.z * dist.z;
-                    if (dsq < 1.0f && dsq > 0.0f) {
-                        float r = sqrtf(dsq);
-                        const float4 Vj = d.V[j];
-                        cf = cf + fluid_term(dist, r, sr, ipress, Pj.w, idens, Vj.w, ivev, xyz(Vj), st);
-                    }
+    for (int a32 = 0; a32 < B; a32 += 32) {
+        uint32_t live = 0;
+#pragma unroll
+        for (int k = 0; k < 32; k++)
+            if (a32 + k < B && d.BPARTNER[(size_t)(a32 + k) * N + h] != MG_NONE) live |= 1u << k;
+#pragma unroll
+        for (int q = 0; q < 32; q += 4) {
+            if (!((live >> q) & 0xFu)) continue;
+            uint32_t j[4];
+            float4 bp[4];
+            float2 in[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                j[k] = MG_NONE;
+                bp[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+                in[k] = make_float2(0.f, 0.f);
+                if (live & (1u << (q + k))) {
+                    const size_t bi = (size_t)(a32 + q + k) * N + h;
+                    j[k] = d.SLOT[d.BPARTNER[bi]];
+                    bp[k] = d.BPAR[bi];
+                    in[k] = d.BINT[bi];
                 }
             }
-            fsum = fsum + cf;
+            bool use[4];
+            float4 pj[4], vj[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                use[k] = j[k] < na && !((double)bp[k].y < 0.000000001);
+                pj[k] = vj[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (use[k]) {
+                    pj[k] = d.P[j[k]];
+                    vj[k] = d.V[j[k]];
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                if (!use[k]) continue;
+                const size_t bi = (size_t)(a32 + q + k) * N + h;
+                float strain, abs_dist;
+                f3 eterm = bond_term(ipos, ivev, xyz(pj[k]), xyz(vj[k]), bp[k], p.pmass, &strain, &abs_dist);
+                float2 v = in[k];
+                v.x = v.x + strain * strain;
+                v.y = v.y + strain;
+                d.BINT[bi] = v;
+                if (!has_nan3(eterm)) force = force - eterm;
+                if (abs_dist >= bp[k].x && !freeze) ((float*)&d.BPAR[bi])[1] = 0.0f;
+            }
         }
     }
+
+    /* fluid terms: per-cell sums added in gridAdj order */
+    const float sr = p.psmoothradius, st = p.psurface_t;
+    f3 fsum = mk3(0.0f, 0.0f, 0.0f), cf = mk3(0.0f, 0.0f, 0.0f);
+    sweep<8>(
+        d, g, gc, ipos, 1.0f, [&](uint32_t j) { return d.V[j]; },
+        [&](uint32_t, float4 Pj, f3 dist, float dsq, float4 Vj) {
+            float r = sqrtf(dsq);
+            cf = cf + fluid_term(dist, r, sr, ipress, Pj.w, idens, Vj.w, ivev, xyz(Vj), st);
+        },
+        [&]() {
+            fsum = fsum + cf;
+            cf = mk3(0.0f, 0.0f, 0.0f);
+        });
     force = force + fsum * p.pmass;
 
     f3 F = F0 + force;
-    const uint32_t e0 = d.INC_OFF[h], e1 = d.INC_OFF[h + 1];
-    for (uint32_t e = e0; e < e1; e++) {
-        const uint32_t packed = d.INC[e];
-        const uint32_t kh = packed / (uint32_t)B, b = packed - kh * (uint32_t)B;
-        const uint32_t k = d.SLOT[kh];
-        if (k >= na) continue;
-        const float4 bp = d.BPAR[(size_t)b * N + kh];
-        if ((double)bp.y < 0.000000001) continue;
-        float strain, abs_dist;
-        f3 eterm = bond_term(xyz(d.P[k]), xyz(d.V[k]), ipos, ivev, bp, p.pmass, &strain, &abs_dist);
-        if (!has_nan3(eterm)) F = F + eterm;
+    for (uint32_t e = e0; e < e1; e += 4) {
+        uint32_t kk[4];
+        float4 bp[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const uint32_t packed = d.INC[min(e + (uint32_t)k, e1 - 1u)];
+            const uint32_t kh = packed / (uint32_t)B, b = packed - kh * (uint32_t)B;
+            kk[k] = d.SLOT[kh];
+            bp[k] = d.BPAR[(size_t)b * N + kh];
+        }
+        bool use[4];
+        float4 pk[4], vk[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            use[k] = e + (uint32_t)k < e1 && kk[k] < na && !((double)bp[k].y < 0.000000001);
+            pk[k] = vk[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (use[k]) {
+                pk[k] = d.P[kk[k]];
+                vk[k] = d.V[kk[k]];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if (!use[k]) continue;
+            float strain, abs_dist;
+            f3 eterm = bond_term(xyz(pk[k]), xyz(vk[k]), ipos, ivev, bp[k], p.pmass, &strain, &abs_dist);
+            if (!has_nan3(eterm)) F = F + eterm;
+        }
     }
     return F;
 }
 
 /* ComputeForceCL: FFORCE[i] += force (accumulate) or = 0 + force. */
-__global__ LAUNCH_BOUNDS_256 void k_force(DS d, MgParams p, MgGrid g, int accumulate)
+__global__ LAUNCH_BOUNDS_SWEEP void k_force(DS d, MgParams p, MgGrid g, int accumulate)
 {
     const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
     const uint32_t na = d.ctr[MG_C_ACTIVE];
@@ -519,7 +641,7 @@ __global__ LAUNCH_BOUNDS_256 void k_force(DS d, MgParams p, MgGrid g, int accumu
     const uint32_t gc = d.GCELL[i];
     if (gc == MG_GRID_UNDEF) return;
     f3 F0 = accumulate ? xyz(d.F[i]) : mk3(0.0f, 0.0f, 0.0f);
-    f3 F = force_one(d, p, g, i, na, gc, F0, d.NBR != nullptr);
+    f3 F = force_one(d, p, g, i, na, gc, F0);
     d.F[i] = make_float4(F.x, F.y, F.z, 0.0f);
 }
 
@@ -610,35 +732,25 @@ __global__ LAUNCH_BOUNDS_256 void k_advance(DS d, MgParams p, float dt)
 /* Fused ComputeForceCL + AdvanceCL for a step that starts from a sort (FFORCE = 0): the force
  * stays in registers and the advanced state goes to the other half of the ping-pong
  * (P2/V2/VEL2) so neighbours keep reading the pre-step state. Inactive slots copy through. */
-__global__ LAUNCH_BOUNDS_256 void k_force_advance(DS d, MgParams p, MgGrid g, float dt)
+__global__ LAUNCH_BOUNDS_SWEEP void k_force_advance(DS d, MgParams p, MgGrid g, float dt)
 {
     const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
-    uint32_t bits = 0; /* |displacement| as float bits (positive floats order like uints) */
-    if (i < (uint32_t)d.N) {
-        const uint32_t na = d.ctr[MG_C_ACTIVE];
-        const float4 P_i = d.P[i], V_i = d.V[i], VEL_i = d.VEL[i];
-        const uint32_t gc = (i < na) ? d.GCELL[i] : MG_GRID_UNDEF;
-        if (gc == MG_GRID_UNDEF) {
-            d.P2[i] = P_i;
-            d.V2[i] = V_i;
-            d.VEL2[i] = VEL_i;
-        } else {
-            f3 F = force_one(d, p, g, i, na, gc, mk3(0.0f, 0.0f, 0.0f), d.NBR != nullptr);
-            f3 np, ve, vn;
-            advance_one(p, dt, xyz(P_i), xyz(V_i), xyz(VEL_i), F, &np, &ve, &vn);
-            d.P2[i] = make_float4(np.x, np.y, np.z, P_i.w);
-            d.V2[i] = make_float4(ve.x, ve.y, ve.z, V_i.w);
-            d.VEL2[i] = make_float4(vn.x, vn.y, vn.z, 0.0f);
-            f3 dx = np - xyz(P_i);
-            float disp = sqrtf(dx.x * dx.x + dx.y * dx.y + dx.z * dx.z);
-            /* NaN/inf map above every finite bound and disable the lists for diffusion */
-            bits = (disp == disp) ? __float_as_uint(disp) & 0x7FFFFFFFu : 0x7F800000u;
-        }
+    if (i >= (uint32_t)d.N) return;
+    const uint32_t na = d.ctr[MG_C_ACTIVE];
+    const float4 P_i = d.P[i], V_i = d.V[i], VEL_i = d.VEL[i];
+    const uint32_t gc = (i < na) ? d.GCELL[i] : MG_GRID_UNDEF;
+    if (gc == MG_GRID_UNDEF) {
+        d.P2[i] = P_i;
+        d.V2[i] = V_i;
+        d.VEL2[i] = VEL_i;
+        return;
     }
-    if (d.NBR) {
-        for (int o = 32; o > 0; o >>= 1) bits = max(bits, (uint32_t)__shfl_xor((int)bits, o));
-        if (__lane_id() == 0 && bits) atomicMax(&d.ctr[MG_C_MAXDISP], bits);
-    }
+    f3 F = force_one(d, p, g, i, na, gc, mk3(0.0f, 0.0f, 0.0f));
+    f3 np, ve, vn;
+    advance_one(p, dt, xyz(P_i), xyz(V_i), xyz(VEL_i), F, &np, &ve, &vn);
+    d.P2[i] = make_float4(np.x, np.y, np.z, P_i.w);
+    d.V2[i] = make_float4(ve.x, ve.y, ve.z, V_i.w);
+    d.VEL2[i] = make_float4(vn.x, vn.y, vn.z, 0.0f);
 }
 
 /* ----------------------------------------------------------- diffusion (a12)
@@ -663,64 +775,101 @@ __device__ __forceinline__ void store16(float* dst, const float v[16])
     d4[3] = make_float4(v[12], v[13], v[14], v[15]);
 }
 
-__device__ __forceinline__ void diffuse_one(const DS& d, const MgParams& p, const MgGrid& g,
-                                            const MgGenome& gen, uint32_t i, uint32_t gc,
-                                            const float cur[MG_NUM_TF], float out[MG_NUM_TF],
-                                            bool use_list)
+/* The diffusion sweep touches only the float4 groups of FCONC that hold a diffusing TF. The
+ * host prepares, from the genome and diff_scale:
+ *   ng        groups used (0 when no TF diffuses; 3 is padded to all 4),
+ *   grp[q]    the FCONC float4 group of slot q,
+ *   dm        bit 4q+e set when TF 4*grp[q]+e diffuses,
+ *   D[4q+e]   (float)tf_diffusability[4*grp[q]+e] * diff_scale. */
+struct DiffPlan {
+    int ng;
+    int grp[4];
+    uint32_t dm;
+    float D[MG_NUM_TF];
+};
+
+template <int NG>
+struct ConcGroups {
+    float4 q[NG];
+};
+
+/* nw[t] for the TFs of the plan's groups: sum over neighbours j within r2 (slot order) of
+ * D * ((r2 - dsq) * simscale^2) * (conc_j - conc_i). */
+template <int NG>
+__device__ __forceinline__ void diffuse_sweep(const DS& d, const MgParams& p, const MgGrid& g,
+                                              const DiffPlan& dp, uint32_t i, uint32_t gc,
+                                              float nw[4 * NG])
 {
     gc -= (uint32_t)((1 * g.res[2] + 1) * g.res[0] + 1);
     const float d2 = p.psimscale * p.psimscale;
     const float r2 = p.r2 / d2;
-    float D[MG_NUM_TF], nw[MG_NUM_TF];
-    uint32_t dmask = 0;
+    const float4* ci = (const float4*)(d.CONC + (size_t)i * MG_NUM_TF);
+    float cur[4 * NG];
 #pragma unroll
-    for (int t = 0; t < MG_NUM_TF; t++) {
-        D[t] = (float)gen.tf_diffusability[t] * p.diff_scale;
-        nw[t] = 0.0f;
-        if (gen.tf_diffusability[t]) dmask |= 1u << t;
+    for (int q = 0; q < NG; q++) {
+        const float4 v = ci[dp.grp[q]];
+        cur[4 * q] = v.x; cur[4 * q + 1] = v.y; cur[4 * q + 2] = v.z; cur[4 * q + 3] = v.w;
     }
-    const f3 pi = xyz(d.P[i]);
-    /* one neighbour candidate j (same order with or without the list) */
-    auto visit = [&](uint32_t j) {
-        f3 dist = pi - xyz(d.P[j]);
-        float dsq = dist.x * dist.x + dist.y * dist.y + dist.z * dist.z;
-        if (dsq < r2 && dsq > 0.0f) {
+#pragma unroll
+    for (int t = 0; t < 4 * NG; t++) nw[t] = 0.0f;
+    sweep<4>(
+        d, g, gc, xyz(d.P[i]), r2,
+        [&](uint32_t j) {
+            ConcGroups<NG> c;
+            const float4* src = (const float4*)(d.CONC + (size_t)j * MG_NUM_TF);
+#pragma unroll
+            for (int q = 0; q < NG; q++) c.q[q] = src[dp.grp[q]];
+            return c;
+        },
+        [&](uint32_t, float4, f3, float dsq, const ConcGroups<NG>& c) {
             const float cc = (r2 - dsq) * d2;
-            float cn[MG_NUM_TF];
-            load16(d.CONC + (size_t)j * MG_NUM_TF, cn);
+            const float* cn = (const float*)c.q;
+#pragma unroll
+            for (int t = 0; t < 4 * NG; t++)
+                if (dp.dm & (1u << t)) nw[t] += dp.D[t] * cc * (cn[t] - cur[t]);
+        },
+        []() {});
+}
+
+/* contributeDiffusion/computeDiffusion, src/fluid_system_cuda.cu:3026-3105, as a Jacobi update
+ * of particle i (cell gc, not GRID_UNDEF) from the concentrations CONC (read-only):
+ *   out[t] = tf_breakdown_rate[t] * (cur[t] + nw[t]),  nw[t] = +0 for TFs that do not diffuse. */
+template <int NG>
+__device__ __forceinline__ void diffuse_one(const DS& d, const MgParams& p, const MgGrid& g,
+                                            const MgGenome& gen, const DiffPlan& dp, uint32_t i,
+                                            uint32_t gc, float out[MG_NUM_TF])
+{
+    float nwf[MG_NUM_TF];
+#pragma unroll
+    for (int t = 0; t < MG_NUM_TF; t++) nwf[t] = 0.0f;
+    if (dp.ng > 0) {
+        float nw[4 * NG];
+        diffuse_sweep<NG>(d, p, g, dp, i, gc, nw);
+#pragma unroll
+        for (int q = 0; q < NG; q++)
 #pragma unroll
             for (int t = 0; t < MG_NUM_TF; t++)
-                if (dmask & (1u << t)) nw[t] += D[t] * cc * (cn[t] - cur[t]);
-        }
-    };
-    const uint32_t nl = use_list ? d.NBRN[i] : MG_NONE;
-    if (nl != MG_NONE) {
-        for (uint32_t e = 0; e < nl; e++) visit(d.NBR[(size_t)e * d.N + i] & 0x7FFFFFFFu);
-    } else {
-        for (int c = 0; c < g.adj_cnt; c++) {
-            const uint32_t cell = gc + (uint32_t)g.adj[c];
-            const uint32_t n = d.CNT[cell];
-            const uint32_t o = d.OFF[cell];
-            for (uint32_t j = o; j < o + n; j++) visit(j);
-        }
+                if ((t >> 2) == dp.grp[q]) nwf[t] = nw[4 * q + (t & 3)];
     }
+    float cur[MG_NUM_TF];
+    load16(d.CONC + (size_t)i * MG_NUM_TF, cur);
 #pragma unroll
-    for (int t = 0; t < MG_NUM_TF; t++) out[t] = (float)gen.tf_breakdown_rate[t] * (cur[t] + nw[t]);
+    for (int t = 0; t < MG_NUM_TF; t++) out[t] = (float)gen.tf_breakdown_rate[t] * (cur[t] + nwf[t]);
 }
 
 /* ComputeDiffusionCL: CONC -> CONC2 for every slot (slots that do not diffuse copy through). */
-__global__ LAUNCH_BOUNDS_256 void k_diffusion(DS d, MgParams p, MgGrid g, MgGenome gen)
+template <int NG>
+__global__ LAUNCH_BOUNDS_SWEEP void k_diffusion(DS d, MgParams p, MgGrid g, MgGenome gen, DiffPlan dp)
 {
     const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= (uint32_t)d.N) return;
-    float cur[MG_NUM_TF], out[MG_NUM_TF];
-    load16(d.CONC + (size_t)i * MG_NUM_TF, cur);
+    float out[MG_NUM_TF];
     const uint32_t gc = (i < d.ctr[MG_C_ACTIVE]) ? d.GCELL[i] : MG_GRID_UNDEF;
     if (gc == MG_GRID_UNDEF) {
-        store16(d.CONC2 + (size_t)i * MG_NUM_TF, cur);
-        return;
+        load16(d.CONC + (size_t)i * MG_NUM_TF, out);
+    } else {
+        diffuse_one<NG>(d, p, g, gen, dp, i, gc, out);
     }
-    diffuse_one(d, p, g, gen, i, gc, cur, out, false);
     store16(d.CONC2 + (size_t)i * MG_NUM_TF, out);
 }
 
@@ -795,25 +944,21 @@ __global__ LAUNCH_BOUNDS_256 void k_genes(DS d, MgParams p, MgGenome gen)
  * after the advance (d.P), the pre-step concentrations (d.CONC) and the pre-step grid
  * (d.CNT / d.OFF); writes CONC2, updates EPIGEN, records the post-gene membership in GMN for
  * the coming sort and bins the particle into the next grid (d.CNTN). */
-__global__ LAUNCH_BOUNDS_256 void k_diffuse_genes_insert(DS d, MgParams p, MgGrid g, MgGenome gen,
-                                                         int genes)
+template <int NG>
+__global__ LAUNCH_BOUNDS_SWEEP void k_diffuse_genes_insert(DS d, MgParams p, MgGrid g, MgGenome gen,
+                                                           DiffPlan dp, int genes)
 {
     const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
     const bool valid = i < (uint32_t)d.N;
     float4 P_i = make_float4(0.f, 0.f, 0.f, 0.f);
     if (valid) {
         P_i = d.P[i];
-        float cur[MG_NUM_TF], out[MG_NUM_TF];
-        load16(d.CONC + (size_t)i * MG_NUM_TF, cur);
+        float out[MG_NUM_TF];
         const uint32_t gc = (i < d.ctr[MG_C_ACTIVE]) ? d.GCELL[i] : MG_GRID_UNDEF;
         if (gc == MG_GRID_UNDEF || !genes) {
-#pragma unroll
-            for (int t = 0; t < MG_NUM_TF; t++) out[t] = cur[t];
+            load16(d.CONC + (size_t)i * MG_NUM_TF, out);
         } else {
-            /* the lists hold every j within sqrt(nbr_r2) of x_i before the advance; they cover
-             * all post-advance pairs within r2 while no particle moved more than nbr_slack */
-            const bool lists = d.NBR && __uint_as_float(d.ctr[MG_C_MAXDISP]) <= d.nbr_slack;
-            diffuse_one(d, p, g, gen, i, gc, cur, out, lists);
+            diffuse_one<NG>(d, p, g, gen, dp, i, gc, out);
         }
         uint32_t mnext = 0;
         if (i < d.ctr[MG_C_NUM_POINTS]) {
