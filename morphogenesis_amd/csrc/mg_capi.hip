@@ -612,6 +612,7 @@ int do_fused_step(mg_handle* h, float dt, unsigned flags)
         PhaseScope ps(h, MG_PH_DIFFUSION);
         DS d = make_ds(h, 1, 0);
         d.GMN = h->GMN;
+        d.cull = 0; /* positions moved since the binning */
         const DiffPlan dp = diff_plan(h);
         if (dp.ng <= 1)
             hipLaunchKernelGGL(k_diffuse_genes_insert<1>, dim3(nb), dim3(256), 0, h->st, d, h->p, h->g, h->gen, dp, genes);

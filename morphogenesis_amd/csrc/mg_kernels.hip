@@ -356,12 +356,13 @@ __global__ LAUNCH_BOUNDS_256 void k_gene_counts_sorted(DS d)
  * x fastest, so the cells come in srch^2 rows of srch consecutive cell indices; in the
  * sorted state (d.rows) cell k of a row starting at cell c0 holds the slots
  * [OFF[c0 + k], OFF[c0 + k + 1]), so one row needs srch + 1 offsets. A sweep calls
- *   visit(j, P[j], x_i - x_j, |x_i - x_j|^2, extra(j))  for candidates with 0 < dsq < lim,
- *   end_cell()                                          after each cell,
+ *   visit(j, P[j], x_i - x_j, |x_i - x_j|^2, x[])  for candidates with 0 < dsq < lim,
+ *   end_cell()                                     after each cell,
  * in exactly the reference's order, fetching up to K candidates of a cell at a time so their
- * loads are in flight together; extra(j) is issued for all hits of a batch before the first
- * visit. Callers use end_cell() to add a per-cell partial sum: for a cell without hits it
- * adds +0, a no-op for sums that start at +0, so cells may also be skipped.
+ * loads are in flight together; x[] holds the caller's NX extra float4 fields of slot j
+ * (velocity, concentrations), fetched for hits only. Callers use end_cell() to add a
+ * per-cell partial sum: for a cell without hits it adds +0, a no-op for sums that start at
+ * +0, so cells may also be skipped.
  *
  * With d.cull (slots sorted by cell and positions unchanged since the binning), cells whose
  * box lies farther than sqrt(lim) from x_i are skipped: no candidate in them can hit. The
@@ -403,18 +404,100 @@ __device__ __forceinline__ float axis_gap(float fr, int o, float inv_delta)
     return fmaxf(gap - 1e-3f, 0.0f) * inv_delta;
 }
 
-template <int K, class Extra, class Visit, class EndCell>
-__device__ __forceinline__ void sweep(const DS& d, const MgGrid& g, uint32_t gc0, f3 pi, float lim,
-                                      Extra extra, Visit visit, EndCell end_cell)
+/* Wave-staged bands. The 64 particles of a wavefront sit in a narrow range of cells
+ * [cmin, cmax] (the slots are cell-sorted), so for one row of the neighbourhood all their
+ * candidates lie in the slot band [OFF[cmin + a], OFF[cmax + a + srch]) (a = the row's
+ * gridAdj offset; OFF is non-decreasing in the sorted state). The wave loads the band once,
+ * coalesced, into LDS (positions and NX extra float4 fields per slot) and every lane then
+ * reads its candidates from LDS instead of issuing dependent global loads. The band of row
+ * s + 1 is fetched into registers while row s is swept. A band wider than BAND_CAP slots,
+ * or any sweep outside the sorted state, reads global memory directly. */
+#define BAND_CAP 128
+#define BAND_PAD 8 /* a batch of up to 8 candidates may read past the band's end */
+
+template <int NX>
+struct Band {
+    float4 p[BAND_CAP + BAND_PAD];
+    float4 x[NX > 0 ? NX : 1][BAND_CAP + BAND_PAD];
+};
+
+/* orders the wavefront's LDS writes before its later LDS reads (and the reverse) */
+__device__ __forceinline__ void wave_lds_sync()
 {
-    using X = decltype(extra(0u));
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ uint32_t wave_min(uint32_t v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, o));
+    return __builtin_amdgcn_readfirstlane(v);
+}
+
+__device__ __forceinline__ uint32_t wave_max(uint32_t v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o));
+    return __builtin_amdgcn_readfirstlane(v);
+}
+
+/* The candidates of one cell, slots [lo, hi), K at a time. getp(j) / getx(j, q) fetch the
+ * position / extra field q of slot j; extras are fetched for the hits of a batch only. */
+template <int K, int NX, class GetP, class GetX, class Visit>
+__device__ __forceinline__ void cell_candidates(uint32_t lo, uint32_t hi, f3 pi, float lim, GetP getp,
+                                                GetX getx, Visit visit)
+{
+    constexpr int NXX = NX > 0 ? NX : 1;
+    for (uint32_t j0 = lo; j0 < hi; j0 += K) {
+        float4 pj[K];
+#pragma unroll
+        for (int q = 0; q < K; q++) pj[q] = getp(j0 + (uint32_t)q);
+        bool hit[K];
+#pragma unroll
+        for (int q = 0; q < K; q++) {
+            const f3 dist = pi - xyz(pj[q]);
+            const float dsq = dist.x * dist.x + dist.y * dist.y + dist.z * dist.z;
+            hit[q] = (j0 + (uint32_t)q < hi) && dsq < lim && dsq > 0.0f;
+        }
+        float4 ex[K][NXX];
+#pragma unroll
+        for (int q = 0; q < K; q++)
+#pragma unroll
+            for (int x = 0; x < NX; x++)
+                if (hit[q]) ex[q][x] = getx(j0 + (uint32_t)q, x);
+#pragma unroll
+        for (int q = 0; q < K; q++) {
+            if (hit[q]) {
+                /* recomputed (same operations) rather than kept in registers */
+                const f3 dist = pi - xyz(pj[q]);
+                const float dsq = dist.x * dist.x + dist.y * dist.y + dist.z * dist.z;
+                visit(j0 + (uint32_t)q, pj[q], dist, dsq, ex[q]);
+            }
+        }
+    }
+}
+
+/* Must be called by all 64 lanes of the wavefront (`valid` selects the lanes that sweep);
+ * xload(j, q) is the global load of extra field q of slot j. */
+template <int K, int NX, class XLoad, class Visit, class EndCell>
+__device__ __forceinline__ void sweep(const DS& d, const MgGrid& g, Band<NX>& wb, bool valid,
+                                      uint32_t gc0, f3 pi, float lim, XLoad xload, Visit visit,
+                                      EndCell end_cell)
+{
+    constexpr int NXX = NX > 0 ? NX : 1;
+    const uint32_t lane = __lane_id();
     const bool rows = d.rows != 0;
     const int nseg = rows ? g.srch * g.srch : g.adj_cnt;
     const int ncell = rows ? g.srch : 1;
     const bool cull = rows && d.cull;
+    const uint32_t cmin = wave_min(valid ? gc0 : 0xFFFFFFFFu);
+    const uint32_t cmax = wave_max(valid ? gc0 : 0u);
+    if (cmin == 0xFFFFFFFFu) return; /* no lane sweeps */
+    const uint32_t N = (uint32_t)d.N;
+
     /* position of x_i inside its own cell (cell units) and cell sizes, for the culling */
     f3 fr = mk3(0.f, 0.f, 0.f), inv = mk3(0.f, 0.f, 0.f);
-    if (cull) {
+    if (cull && valid) {
         const uint32_t own = gc0 + (uint32_t)((1 * g.res[2] + 1) * g.res[0] + 1);
         const uint32_t cx = own % (uint32_t)g.res[0];
         const uint32_t cz = (own / (uint32_t)g.res[0]) % (uint32_t)g.res[2];
@@ -423,10 +506,62 @@ __device__ __forceinline__ void sweep(const DS& d, const MgGrid& g, uint32_t gc0
                  (pi.z - g.gmin[2]) * g.delta[2] - (float)cz);
         inv = mk3(1.0f / g.delta[0], 1.0f / g.delta[1], 1.0f / g.delta[2]);
     }
-    RowBounds nxt = row_bounds(d, g, gc0, 0);
+
+    /* band of segment s (wave-uniform); fits == false selects the global path */
+    auto band_of = [&](int s, uint32_t& blo, uint32_t& bhi) -> bool {
+        if (!rows) return false;
+        const uint32_t a = (uint32_t)g.adj[s * g.srch];
+        blo = min(d.OFF[cmin + a], N);
+        bhi = min(d.OFF[cmax + a + (uint32_t)g.srch], N);
+        return bhi >= blo && bhi - blo <= BAND_CAP;
+    };
+    /* the next band, two slots per lane (lane and lane + 64) */
+    float4 rp0 = make_float4(0.f, 0.f, 0.f, 0.f), rp1 = rp0, rx0[NXX], rx1[NXX];
+#pragma unroll
+    for (int x = 0; x < NXX; x++) rx0[x] = rx1[x] = rp0;
+    auto fetch = [&](bool fits, uint32_t blo, uint32_t bhi) {
+        if (!fits) return;
+        const uint32_t j0 = blo + lane, j1 = j0 + 64u;
+        if (j0 < bhi) {
+            rp0 = d.P[j0];
+#pragma unroll
+            for (int x = 0; x < NX; x++) rx0[x] = xload(j0, x);
+        }
+        if (j1 < bhi) {
+            rp1 = d.P[j1];
+#pragma unroll
+            for (int x = 0; x < NX; x++) rx1[x] = xload(j1, x);
+        }
+    };
+
+    uint32_t blo = 0, bhi = 0;
+    bool fits = band_of(0, blo, bhi);
+    fetch(fits, blo, bhi);
+    RowBounds nxt = row_bounds(d, g, valid ? gc0 : cmin, 0);
     for (int s = 0; s < nseg; s++) {
         const RowBounds rb = nxt;
-        if (s + 1 < nseg) nxt = row_bounds(d, g, gc0, s + 1);
+        const uint32_t cblo = blo, cbhi = bhi;
+        const bool cfits = fits;
+        if (cfits) {
+            wave_lds_sync(); /* the previous row's reads are done */
+            if (cblo + lane < cbhi) {
+                wb.p[lane] = rp0;
+#pragma unroll
+                for (int x = 0; x < NX; x++) wb.x[x][lane] = rx0[x];
+            }
+            if (cblo + lane + 64u < cbhi) {
+                wb.p[lane + 64u] = rp1;
+#pragma unroll
+                for (int x = 0; x < NX; x++) wb.x[x][lane + 64u] = rx1[x];
+            }
+            wave_lds_sync();
+        }
+        if (s + 1 < nseg) {
+            fits = band_of(s + 1, blo, bhi);
+            fetch(fits, blo, bhi);
+            nxt = row_bounds(d, g, valid ? gc0 : cmin, s + 1);
+        }
+        if (!valid) continue;
         float gyz = 0.0f;
         if (cull) {
             const float gy = axis_gap(fr.y, s / g.srch - 1, inv.y);
@@ -442,31 +577,14 @@ __device__ __forceinline__ void sweep(const DS& d, const MgGrid& g, uint32_t gc0
             }
             if (!live) continue;
             const uint32_t lo = rb.b[k], hi = max(rb.b[k + 1], lo);
-            for (uint32_t j0 = lo; j0 < hi; j0 += K) {
-                const float4* pb = d.P + j0;
-                float4 pj[K];
-#pragma unroll
-                for (int q = 0; q < K; q++) pj[q] = pb[q];
-                bool hit[K];
-#pragma unroll
-                for (int q = 0; q < K; q++) {
-                    const f3 dist = pi - xyz(pj[q]);
-                    const float dsq = dist.x * dist.x + dist.y * dist.y + dist.z * dist.z;
-                    hit[q] = (j0 + (uint32_t)q < hi) && dsq < lim && dsq > 0.0f;
-                }
-                X ex[K];
-#pragma unroll
-                for (int q = 0; q < K; q++)
-                    if (hit[q]) ex[q] = extra(j0 + (uint32_t)q);
-#pragma unroll
-                for (int q = 0; q < K; q++) {
-                    if (hit[q]) {
-                        /* recomputed (same operations) rather than kept in registers */
-                        const f3 dist = pi - xyz(pj[q]);
-                        const float dsq = dist.x * dist.x + dist.y * dist.y + dist.z * dist.z;
-                        visit(j0 + (uint32_t)q, pj[q], dist, dsq, ex[q]);
-                    }
-                }
+            if (cfits) {
+                cell_candidates<K, NX>(
+                    lo, hi, pi, lim, [&](uint32_t j) { return wb.p[j - cblo]; },
+                    [&](uint32_t j, int x) { return wb.x[x][j - cblo]; }, visit);
+            } else {
+                cell_candidates<K, NX>(
+                    lo, hi, pi, lim, [&](uint32_t j) { return d.P[j]; },
+                    [&](uint32_t j, int x) { return xload(j, x); }, visit);
             }
             end_cell();
         }
@@ -479,21 +597,22 @@ __device__ __forceinline__ void sweep(const DS& d, const MgGrid& g, uint32_t gc0
  * FDENSITY (= 1/density) into V.w. Also clears d.ZERO (the next grid's counters) when set. */
 __global__ LAUNCH_BOUNDS_SWEEP void k_pressure(DS d, MgParams p, MgGrid g)
 {
+    __shared__ Band<0> bands[4];
     if (d.ZERO) {
         for (uint32_t z = blockIdx.x * blockDim.x + threadIdx.x; z < d.zero_n; z += gridDim.x * blockDim.x)
             d.ZERO[z] = 0u;
     }
     const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
-    if (i >= d.ctr[MG_C_ACTIVE]) return;
-    uint32_t gc = d.GCELL[i];
-    if (gc == MG_GRID_UNDEF) return;
+    uint32_t gc = (i < d.ctr[MG_C_ACTIVE]) ? d.GCELL[i] : MG_GRID_UNDEF;
+    const bool valid = gc != MG_GRID_UNDEF;
     gc -= (uint32_t)((1 * g.res[2] + 1) * g.res[0] + 1);
-    const f3 pi = xyz(d.P[i]);
+    const f3 pi = valid ? xyz(d.P[i]) : mk3(0.f, 0.f, 0.f);
     const float sr = p.psmoothradius;
     float sum = 0.0f, csum = 0.0f;
-    sweep<8>(
-        d, g, gc, pi, p.r2, [](uint32_t) { return 0; },
-        [&](uint32_t, float4, f3, float dsq, int) {
+    sweep<8, 0>(
+        d, g, bands[threadIdx.x >> 6], valid, gc, pi, p.r2,
+        [](uint32_t, int) { return make_float4(0.f, 0.f, 0.f, 0.f); },
+        [&](uint32_t, float4, f3, float dsq, const float4*) {
             float r = sqrtf(dsq);
             float q = r / sr;
             float b = 1.0f - q;
@@ -505,6 +624,7 @@ __global__ LAUNCH_BOUNDS_SWEEP void k_pressure(DS d, MgParams p, MgGrid g)
             sum += csum;
             csum = 0.0f;
         });
+    if (!valid) return;
     sum = sum * p.pmass * p.wendlandC2kern;
     if (sum == 0.0f) sum = 1.0f;
     ((float*)&d.P[i])[3] = (sum - p.prest_dens) * p.pintstiff;
@@ -521,20 +641,26 @@ __global__ LAUNCH_BOUNDS_SWEEP void k_pressure(DS d, MgParams p, MgGrid g)
  * Updates the bond integrators and breaks bonds as the reference does. Bond slots are
  * scanned 32 at a time for live partners and the live ones fetched 4 at a time; incoming
  * bonds are fetched 4 at a time. */
-__device__ __forceinline__ f3 force_one(const DS& d, const MgParams& p, const MgGrid& g,
-                                        uint32_t i, uint32_t na, uint32_t gc, f3 F0)
+__device__ __forceinline__ f3 force_one(const DS& d, const MgParams& p, const MgGrid& g, Band<1>& wb,
+                                        bool valid, uint32_t i, uint32_t na, uint32_t gc, f3 F0)
 {
     gc -= (uint32_t)((1 * g.res[2] + 1) * g.res[0] + 1);
     const int N = d.N, B = d.B;
-    const uint32_t h = d.HOME[i];
-    const uint32_t e0 = d.INC_OFF[h], e1 = d.INC_OFF[h + 1];
-    const float4 P_i = d.P[i], V_i = d.V[i];
+    uint32_t h = 0, e0 = 0, e1 = 0;
+    float4 P_i = make_float4(0.f, 0.f, 0.f, 0.f), V_i = P_i;
+    if (valid) {
+        h = d.HOME[i];
+        e0 = d.INC_OFF[h];
+        e1 = d.INC_OFF[h + 1];
+        P_i = d.P[i];
+        V_i = d.V[i];
+    }
     const f3 ipos = xyz(P_i), ivev = xyz(V_i);
     const float ipress = P_i.w, idens = V_i.w;
     const bool freeze = p.freeze != 0;
 
     f3 force = mk3(0.0f, 0.0f, 0.0f);
-    for (int a32 = 0; a32 < B; a32 += 32) {
+    for (int a32 = 0; valid && a32 < B; a32 += 32) {
         uint32_t live = 0;
 #pragma unroll
         for (int k = 0; k < 32; k++)
@@ -587,9 +713,10 @@ __device__ __forceinline__ f3 force_one(const DS& d, const MgParams& p, const Mg
     /* fluid terms: per-cell sums added in gridAdj order */
     const float sr = p.psmoothradius, st = p.psurface_t;
     f3 fsum = mk3(0.0f, 0.0f, 0.0f), cf = mk3(0.0f, 0.0f, 0.0f);
-    sweep<8>(
-        d, g, gc, ipos, 1.0f, [&](uint32_t j) { return d.V[j]; },
-        [&](uint32_t, float4 Pj, f3 dist, float dsq, float4 Vj) {
+    sweep<4, 1>(
+        d, g, wb, valid, gc, ipos, 1.0f, [&](uint32_t j, int) { return d.V[j]; },
+        [&](uint32_t, float4 Pj, f3 dist, float dsq, const float4* x) {
+            const float4 Vj = x[0];
             float r = sqrtf(dsq);
             cf = cf + fluid_term(dist, r, sr, ipress, Pj.w, idens, Vj.w, ivev, xyz(Vj), st);
         },
@@ -600,7 +727,7 @@ __device__ __forceinline__ f3 force_one(const DS& d, const MgParams& p, const Mg
     force = force + fsum * p.pmass;
 
     f3 F = F0 + force;
-    for (uint32_t e = e0; e < e1; e += 4) {
+    for (uint32_t e = e0; valid && e < e1; e += 4) {
         uint32_t kk[4];
         float4 bp[4];
 #pragma unroll
@@ -635,14 +762,14 @@ __device__ __forceinline__ f3 force_one(const DS& d, const MgParams& p, const Mg
 /* ComputeForceCL: FFORCE[i] += force (accumulate) or = 0 + force. */
 __global__ LAUNCH_BOUNDS_SWEEP void k_force(DS d, MgParams p, MgGrid g, int accumulate)
 {
+    __shared__ Band<1> bands[4];
     const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
     const uint32_t na = d.ctr[MG_C_ACTIVE];
-    if (i >= na) return;
-    const uint32_t gc = d.GCELL[i];
-    if (gc == MG_GRID_UNDEF) return;
-    f3 F0 = accumulate ? xyz(d.F[i]) : mk3(0.0f, 0.0f, 0.0f);
-    f3 F = force_one(d, p, g, i, na, gc, F0);
-    d.F[i] = make_float4(F.x, F.y, F.z, 0.0f);
+    const uint32_t gc = (i < na) ? d.GCELL[i] : MG_GRID_UNDEF;
+    const bool valid = gc != MG_GRID_UNDEF;
+    const f3 F0 = (valid && accumulate) ? xyz(d.F[i]) : mk3(0.0f, 0.0f, 0.0f);
+    const f3 F = force_one(d, p, g, bands[threadIdx.x >> 6], valid, i, na, gc, F0);
+    if (valid) d.F[i] = make_float4(F.x, F.y, F.z, 0.0f);
 }
 
 /* ------------------------------------------------------------- advance (a11)
@@ -734,18 +861,20 @@ __global__ LAUNCH_BOUNDS_256 void k_advance(DS d, MgParams p, float dt)
  * (P2/V2/VEL2) so neighbours keep reading the pre-step state. Inactive slots copy through. */
 __global__ LAUNCH_BOUNDS_SWEEP void k_force_advance(DS d, MgParams p, MgGrid g, float dt)
 {
+    __shared__ Band<1> bands[4];
     const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
-    if (i >= (uint32_t)d.N) return;
     const uint32_t na = d.ctr[MG_C_ACTIVE];
-    const float4 P_i = d.P[i], V_i = d.V[i], VEL_i = d.VEL[i];
     const uint32_t gc = (i < na) ? d.GCELL[i] : MG_GRID_UNDEF;
-    if (gc == MG_GRID_UNDEF) {
+    const bool valid = gc != MG_GRID_UNDEF;
+    const f3 F = force_one(d, p, g, bands[threadIdx.x >> 6], valid, i, na, gc, mk3(0.0f, 0.0f, 0.0f));
+    if (i >= (uint32_t)d.N) return;
+    const float4 P_i = d.P[i], V_i = d.V[i], VEL_i = d.VEL[i];
+    if (!valid) {
         d.P2[i] = P_i;
         d.V2[i] = V_i;
         d.VEL2[i] = VEL_i;
         return;
     }
-    f3 F = force_one(d, p, g, i, na, gc, mk3(0.0f, 0.0f, 0.0f));
     f3 np, ve, vn;
     advance_one(p, dt, xyz(P_i), xyz(V_i), xyz(VEL_i), F, &np, &ve, &vn);
     d.P2[i] = make_float4(np.x, np.y, np.z, P_i.w);
@@ -777,7 +906,9 @@ __device__ __forceinline__ void store16(float* dst, const float v[16])
 
 /* The diffusion sweep touches only the float4 groups of FCONC that hold a diffusing TF. The
  * host prepares, from the genome and diff_scale:
- *   ng        groups used (0 when no TF diffuses; 3 is padded to all 4),
+ *   ng        groups used (0 when no TF diffuses; 3 is padded to all 4); a kernel
+ *             instantiated for NG >= ng groups also runs the plan (slots q >= ng hold no
+ *             diffusing TF and are ignored),
  *   grp[q]    the FCONC float4 group of slot q,
  *   dm        bit 4q+e set when TF 4*grp[q]+e diffuses,
  *   D[4q+e]   (float)tf_diffusability[4*grp[q]+e] * diff_scale. */
@@ -788,42 +919,38 @@ struct DiffPlan {
     float D[MG_NUM_TF];
 };
 
-template <int NG>
-struct ConcGroups {
-    float4 q[NG];
-};
-
 /* nw[t] for the TFs of the plan's groups: sum over neighbours j within r2 (slot order) of
- * D * ((r2 - dsq) * simscale^2) * (conc_j - conc_i). */
+ * D * ((r2 - dsq) * simscale^2) * (conc_j - conc_i). Called by all lanes of the wavefront;
+ * lanes with valid == false return zeros. */
 template <int NG>
 __device__ __forceinline__ void diffuse_sweep(const DS& d, const MgParams& p, const MgGrid& g,
-                                              const DiffPlan& dp, uint32_t i, uint32_t gc,
-                                              float nw[4 * NG])
+                                              const DiffPlan& dp, Band<NG>& wb, bool valid, uint32_t i,
+                                              uint32_t gc, float nw[4 * NG])
 {
     gc -= (uint32_t)((1 * g.res[2] + 1) * g.res[0] + 1);
     const float d2 = p.psimscale * p.psimscale;
     const float r2 = p.r2 / d2;
-    const float4* ci = (const float4*)(d.CONC + (size_t)i * MG_NUM_TF);
     float cur[4 * NG];
 #pragma unroll
-    for (int q = 0; q < NG; q++) {
-        const float4 v = ci[dp.grp[q]];
-        cur[4 * q] = v.x; cur[4 * q + 1] = v.y; cur[4 * q + 2] = v.z; cur[4 * q + 3] = v.w;
+    for (int t = 0; t < 4 * NG; t++) cur[t] = 0.0f;
+    f3 pi = mk3(0.f, 0.f, 0.f);
+    if (valid) {
+        const float4* ci = (const float4*)(d.CONC + (size_t)i * MG_NUM_TF);
+#pragma unroll
+        for (int q = 0; q < NG; q++) {
+            const float4 v = ci[dp.grp[q]];
+            cur[4 * q] = v.x; cur[4 * q + 1] = v.y; cur[4 * q + 2] = v.z; cur[4 * q + 3] = v.w;
+        }
+        pi = xyz(d.P[i]);
     }
 #pragma unroll
     for (int t = 0; t < 4 * NG; t++) nw[t] = 0.0f;
-    sweep<4>(
-        d, g, gc, xyz(d.P[i]), r2,
-        [&](uint32_t j) {
-            ConcGroups<NG> c;
-            const float4* src = (const float4*)(d.CONC + (size_t)j * MG_NUM_TF);
-#pragma unroll
-            for (int q = 0; q < NG; q++) c.q[q] = src[dp.grp[q]];
-            return c;
-        },
-        [&](uint32_t, float4, f3, float dsq, const ConcGroups<NG>& c) {
+    sweep<(NG >= 2 ? 2 : 4), NG>(
+        d, g, wb, valid, gc, pi, r2,
+        [&](uint32_t j, int q) { return ((const float4*)(d.CONC + (size_t)j * MG_NUM_TF))[dp.grp[q]]; },
+        [&](uint32_t, float4, f3, float dsq, const float4* x) {
             const float cc = (r2 - dsq) * d2;
-            const float* cn = (const float*)c.q;
+            const float* cn = (const float*)x;
 #pragma unroll
             for (int t = 0; t < 4 * NG; t++)
                 if (dp.dm & (1u << t)) nw[t] += dp.D[t] * cc * (cn[t] - cur[t]);
@@ -833,24 +960,26 @@ __device__ __forceinline__ void diffuse_sweep(const DS& d, const MgParams& p, co
 
 /* contributeDiffusion/computeDiffusion, src/fluid_system_cuda.cu:3026-3105, as a Jacobi update
  * of particle i (cell gc, not GRID_UNDEF) from the concentrations CONC (read-only):
- *   out[t] = tf_breakdown_rate[t] * (cur[t] + nw[t]),  nw[t] = +0 for TFs that do not diffuse. */
+ *   out[t] = tf_breakdown_rate[t] * (cur[t] + nw[t]),  nw[t] = +0 for TFs that do not diffuse.
+ * Called by all lanes of the wavefront; out is written for valid lanes only. */
 template <int NG>
 __device__ __forceinline__ void diffuse_one(const DS& d, const MgParams& p, const MgGrid& g,
-                                            const MgGenome& gen, const DiffPlan& dp, uint32_t i,
-                                            uint32_t gc, float out[MG_NUM_TF])
+                                            const MgGenome& gen, const DiffPlan& dp, Band<NG>& wb,
+                                            bool valid, uint32_t i, uint32_t gc, float out[MG_NUM_TF])
 {
     float nwf[MG_NUM_TF];
 #pragma unroll
     for (int t = 0; t < MG_NUM_TF; t++) nwf[t] = 0.0f;
     if (dp.ng > 0) {
         float nw[4 * NG];
-        diffuse_sweep<NG>(d, p, g, dp, i, gc, nw);
+        diffuse_sweep<NG>(d, p, g, dp, wb, valid, i, gc, nw);
 #pragma unroll
         for (int q = 0; q < NG; q++)
 #pragma unroll
             for (int t = 0; t < MG_NUM_TF; t++)
-                if ((t >> 2) == dp.grp[q]) nwf[t] = nw[4 * q + (t & 3)];
+                if (q < dp.ng && (t >> 2) == dp.grp[q]) nwf[t] = nw[4 * q + (t & 3)];
     }
+    if (!valid) return;
     float cur[MG_NUM_TF];
     load16(d.CONC + (size_t)i * MG_NUM_TF, cur);
 #pragma unroll
@@ -861,15 +990,14 @@ __device__ __forceinline__ void diffuse_one(const DS& d, const MgParams& p, cons
 template <int NG>
 __global__ LAUNCH_BOUNDS_SWEEP void k_diffusion(DS d, MgParams p, MgGrid g, MgGenome gen, DiffPlan dp)
 {
+    __shared__ Band<NG> bands[4];
     const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
-    if (i >= (uint32_t)d.N) return;
-    float out[MG_NUM_TF];
     const uint32_t gc = (i < d.ctr[MG_C_ACTIVE]) ? d.GCELL[i] : MG_GRID_UNDEF;
-    if (gc == MG_GRID_UNDEF) {
-        load16(d.CONC + (size_t)i * MG_NUM_TF, out);
-    } else {
-        diffuse_one<NG>(d, p, g, gen, dp, i, gc, out);
-    }
+    const bool valid = gc != MG_GRID_UNDEF;
+    float out[MG_NUM_TF];
+    diffuse_one<NG>(d, p, g, gen, dp, bands[threadIdx.x >> 6], valid, i, gc, out);
+    if (i >= (uint32_t)d.N) return;
+    if (!valid) load16(d.CONC + (size_t)i * MG_NUM_TF, out);
     store16(d.CONC2 + (size_t)i * MG_NUM_TF, out);
 }
 
@@ -948,18 +1076,17 @@ template <int NG>
 __global__ LAUNCH_BOUNDS_SWEEP void k_diffuse_genes_insert(DS d, MgParams p, MgGrid g, MgGenome gen,
                                                            DiffPlan dp, int genes)
 {
+    __shared__ Band<NG> bands[4];
     const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
     const bool valid = i < (uint32_t)d.N;
+    const uint32_t gc = (i < d.ctr[MG_C_ACTIVE]) ? d.GCELL[i] : MG_GRID_UNDEF;
+    const bool diffuses = genes && gc != MG_GRID_UNDEF;
+    float out[MG_NUM_TF];
+    if (genes) diffuse_one<NG>(d, p, g, gen, dp, bands[threadIdx.x >> 6], diffuses, i, gc, out);
     float4 P_i = make_float4(0.f, 0.f, 0.f, 0.f);
     if (valid) {
         P_i = d.P[i];
-        float out[MG_NUM_TF];
-        const uint32_t gc = (i < d.ctr[MG_C_ACTIVE]) ? d.GCELL[i] : MG_GRID_UNDEF;
-        if (gc == MG_GRID_UNDEF || !genes) {
-            load16(d.CONC + (size_t)i * MG_NUM_TF, out);
-        } else {
-            diffuse_one<NG>(d, p, g, gen, dp, i, gc, out);
-        }
+        if (!diffuses) load16(d.CONC + (size_t)i * MG_NUM_TF, out);
         uint32_t mnext = 0;
         if (i < d.ctr[MG_C_NUM_POINTS]) {
             const uint32_t h = d.HOME[i];

@@ -250,3 +250,52 @@ def test_graph_replay_matches_eager(built):
         graph.sync()
     for b in COMPARE:
         assert_bits_equal(graph.download(b), eager.download(b), f"graph {F.BUFFER_NAMES[b]}")
+
+
+@pytest.mark.parametrize("mask", [0x0000, 0x0006, 0x8008, 0xF0F0, 0x1111, 0xFFFF])
+def test_diffusion_group_plans(built, mask):
+    """Diffusing-TF sets that select each FCONC group layout of the diffusion sweep (no group,
+    one group, two apart, two adjacent, all four with one TF each, all sixteen), through
+    the fused step and through the phase path."""
+    sc = scenes.tissue(n_side=8, b=6, seed=31)
+    for t in range(F.NUM_TF):
+        sc.genome.tf_diffusability[t] = (t + 2) if (mask >> t) & 1 else 0
+        sc.genome.tf_breakdown_rate[t] = 1
+    say("case: diffusion plan mask", hex(mask))
+    g, o = pair(sc)
+    for sim in (g, o):
+        sim.Run2PhysicalSort()
+        sim.step(12, sc.dt, sc.flags)
+    compare_all(g, o, f"fused, mask {mask:#06x}", [F.FPOS, F.FCONC, F.FEPIGEN, F.FGCELL])
+    for sim in (g, o):
+        sim.Run2InnerPhysicalLoop(sc.dt)
+        sim.Run2GeneAction()
+        sim.Run2PhysicalSort()
+    compare_all(g, o, f"phases, mask {mask:#06x}", [F.FPOS, F.FCONC, F.FEPIGEN, F.FGCELL])
+
+
+def test_dense_cells_band_fallback(built):
+    """Spacing 0.25 puts 64 particles in a cell: a wavefront's neighbour band outgrows the LDS
+    stage and the sweeps read global memory; results still match bit for bit."""
+    sc = scenes.tissue(n_side=16, b=6, seed=4, spacing=0.25, vmax=12.0)
+    say("case: dense cells (band fallback)", sc.n)
+    g, o = pair(sc)
+    for sim in (g, o):
+        sim.Run2PhysicalSort()
+        sim.step(10, sc.dt, sc.flags)
+    compare_all(g, o, "dense cells", [F.FPOS, F.FVEL, F.FPRESS, F.FDENSITY, F.FCONC, F.FGCELL])
+
+
+def test_phase_calls_after_advance(built):
+    """Pressure and force called on positions that moved since the last sort (no culling may
+    apply) and diffusion called right after a sort (culling applies)."""
+    sc = scenes.tissue(n_side=9, b=8, seed=12)
+    say("case: phases after advance")
+    g, o = pair(sc)
+    for sim in (g, o):
+        sim.Run2PhysicalSort()
+        sim.ComputeDiffusionCL()
+        sim.Run2InnerPhysicalLoop(sc.dt)
+        sim.ComputePressureCL()
+        sim.ComputeForceCL()
+    compare_all(g, o, "after advance", [F.FPOS, F.FFORCE, F.FPRESS, F.FDENSITY, F.FCONC])
