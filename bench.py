@@ -63,7 +63,7 @@ def algorithmic_bytes(phase: str, scene, stats: dict) -> float:
     return 0.0
 
 
-def scene_stats(scene) -> dict:
+def scene_stats(scene, neighbours: bool = True) -> dict:
     import numpy as np
     from scipy.spatial import cKDTree
 
@@ -76,6 +76,8 @@ def scene_stats(scene) -> dict:
         valid = rec[:, :, 0] != F.NONE
         bo = float(valid.sum()) / scene.n
         bi = bo
+    if not neighbours:
+        return {"bonds_out": bo, "bonds_in": bi, "neighbours": None}
     pos = scene.buffers[F.FPOS][:, :3].astype(np.float64)
     rng = np.random.default_rng(0)
     idx = rng.choice(pos.shape[0], size=min(4096, pos.shape[0]), replace=False)
@@ -118,6 +120,112 @@ def cpu_baseline(scene, budget_s: float = 15.0) -> dict:
                       f"{el:.1f} s, FP32 oracle (CPU restatement), {threads} threads on {model}"}
 
 
+def run_batch(args, count: int = 256):
+    """The batched workload: `count` bonded tissue blocks of log-uniform size in
+    [2048, 65536] particles, each with its own domain, grid and genome, stepped together."""
+    import torch
+
+    from morphogenesis_amd import _ffi as F
+    from morphogenesis_amd import scenes
+    from morphogenesis_amd.batch import Batch
+    from morphogenesis_amd.fluid_system import FluidSystem
+
+    def progress(msg):
+        print(f"[bench batch] {msg}", file=sys.stderr, flush=True)
+
+    scs = scenes.tissue_batch(count, 32, seed=3)
+    total = sum(sc.n for sc in scs)
+    progress(f"{count} scenes built, {total} particles")
+    systems = []
+    for sc in scs:
+        sim = FluidSystem(sc.n, sc.b)
+        sim.load_scene(sc)
+        sim.Run2PhysicalSort()
+        systems.append(sim)
+    progress("uploaded and sorted")
+    dt, flags = scs[0].dt, scs[0].flags
+    bat = Batch(systems)
+    chosen = bat.autotune(2, dt, flags)
+    progress(f"autotuned: {chosen}")
+    bat.step(args.warmup, dt, flags | F.STEP_GRAPH)
+    bat.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    bat.step(args.steps, dt, flags | F.STEP_GRAPH)
+    bat.sync()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms = (C.c_double * 11)()
+    calls = (C.c_int * 11)()
+    bat._call("batch_step_timed", bat.h, args.timed_steps, dt, flags, ms, calls)
+    per_step = {ph: ms[i] / args.timed_steps for i, ph in enumerate(PHASES) if calls[i]}
+    kern = {"pressure": "kb_pressure", "force": "kb_force_advance",
+            "diffusion": "kb_diffuse_genes_insert", "scan": "kb_scan", "place": "kb_place+kb_cellsort",
+            "sort": "kb_sort_gather"}
+    # algorithmic bytes of a launch group = sum over the instances
+    stats = []
+    for sc, sim in zip(scs, systems):
+        st = scene_stats(sc, neighbours=False)
+        st["n_active"] = sim.counts()[1]
+        stats.append(st)
+    kernels = {}
+    for p, v in per_step.items():
+        bts = sum(algorithmic_bytes(p, sc, st) for sc, st in zip(scs, stats))
+        if p in kern and bts > 0:
+            kernels[kern[p]] = {"ms": round(v, 5), "alg_bytes": int(bts), "GB/s": round(bts / (v * 1e-3) / 1e9, 1)}
+    dom = max((p for p in per_step if p in kern and p != "place"), key=lambda p: per_step[p])
+    alg = sum(algorithmic_bytes(dom, sc, st) for sc, st in zip(scs, stats))
+    achieved = alg / (per_step[dom] * 1e-3) / 1e9
+    classes = bat.classes()
+    out = {
+        "metric": METRIC, "value": total * args.steps / elapsed, "unit": "particle-steps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": args.workload, "instances": count, "particles": total,
+                   "bonds_per_particle": 32, "dt": dt, "genes": bool(flags & F.STEP_GENES),
+                   "hipgraph": True, "size_classes": [{"instances": m, "threads": t} for m, t in classes],
+                   "autotuned_threads": chosen,
+                   "step": "pressure+force+advance+diffusion+genes+insert+scan+sort, one launch per phase and size class"},
+        "roofline": {"bound": "hbm", "kernel": kern[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": load_traffic(kern[dom], args.workload),
+                     "algorithmic_bytes_per_launch": alg, "avg_launch_ms": per_step[dom]},
+        "step_breakdown_ms": {p: round(v, 5) for p, v in per_step.items()},
+        "kernels": kernels,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline_batch(scs)
+    bat.close()
+    print(json.dumps(out))
+
+
+def cpu_baseline_batch(scs, budget_s: float = 15.0) -> dict:
+    """The FP32 oracle stepping the instances one after another (as a CPU loop over the
+    batch would) until the time budget is spent; rate = particle-steps / time."""
+    import multiprocessing
+
+    from oracle.mg_oracle import Oracle
+
+    threads = int(os.environ.get("MG_CPU_THREADS", "0")) or multiprocessing.cpu_count()
+    done, spent, used = 0, 0.0, 0
+    for sc in scs:
+        o = Oracle(sc.n, sc.b, precision=32, threads=threads)
+        o.load_scene(sc)
+        o.Run2PhysicalSort()
+        t = time.perf_counter()
+        o.step(2, sc.dt, sc.flags)
+        spent += time.perf_counter() - t
+        done += 2 * sc.n
+        used += 1
+        o.close()
+        if spent > budget_s:
+            break
+    return {"value": done / spent, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "sample": f"2 steps of each of the first {used} of {len(scs)} instances, {spent:.1f} s, "
+                      f"FP32 oracle (CPU restatement), {threads} threads"}
+
+
 def load_traffic(kernel: str, workload: str):
     p = os.path.join(REPO, "profiles", f"traffic_{kernel}_{workload}.json")
     if not os.path.exists(p):
@@ -153,6 +261,9 @@ def main():
     from morphogenesis_amd.fluid_system import FluidSystem
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
+    if args.workload == "tissue_batch_256":
+        run_batch(args)
+        return
     builders = {"tissue_262k": lambda: scenes.tissue(64, 32, seed=1),
                 "fluid_block_32k": lambda: scenes.fluid_block(32, seed=1)}
     scene = builders[args.workload]()

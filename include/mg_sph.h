@@ -109,6 +109,35 @@ enum {
 int mg_step_timed(mg_handle* h, int nsteps, float dt, unsigned flags, double* ms_out,
                   int* calls_out);
 
+/* ---- batched instances (no counterpart in the reference, which steps one FluidSystem) ----
+ * A batch steps many independent handles (each with its own buffers, grid, parameters and
+ * genome) with one launch per phase and size class instead of one per handle; every
+ * instance evolves bit for bit as under mg_step. While a handle belongs to a batch its work
+ * is queued on the batch's stream (uploads, downloads and mg_step keep working); destroy the
+ * batch before any of its handles. Handles must be in the sorted state (after
+ * mg_physical_sort or a step) when mg_batch_step is called. */
+typedef struct mg_batch mg_batch;
+int mg_batch_create(mg_batch** out, mg_handle* const* handles, int n);
+int mg_batch_destroy(mg_batch* b);
+/* nsteps fused steps of every instance; flags as mg_step (MG_STEP_GRAPH replays two
+ * captured steps at a time). */
+int mg_batch_step(mg_batch* b, int nsteps, float dt, unsigned flags);
+/* Wait for the batch's stream; fails if a kernel of any instance gave up a bounded wait. */
+int mg_batch_sync(mg_batch* b);
+/* Size classes (by particle count: < 8192, < 24576, larger): *nclass, and per class the
+ * number of instances and the block size of its particle and cell launches (arrays of at
+ * least *nclass entries, either may be null). */
+int mg_batch_classes(mg_batch* b, int* nclass, int* members, unsigned* threads);
+/* Measures `steps` steps at each block size (256, 128, 64) and keeps, per size class, the one
+ * with the least kernel time; threads_out (one entry per class, may be null) receives the
+ * choice. The steps are real steps of the batch. */
+int mg_batch_autotune(mg_batch* b, int steps, float dt, unsigned flags, unsigned* threads_out);
+/* Like mg_batch_step without graph replay, with HIP events around every launch group:
+ * ms_out[phase] accumulates milliseconds summed over the size classes and calls_out[phase]
+ * the launch groups, phase ids MG_PH_* (MG_PH_COUNT entries each; either may be null). */
+int mg_batch_step_timed(mg_batch* b, int nsteps, float dt, unsigned flags, double* ms_out,
+                        int* calls_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -241,6 +241,14 @@ def declare(lib, prefix: str):
         sig["sync"] = [h]
         sig["fluid_setup"] = [h, C.c_int, C.c_int, P(C.c_int32), P(C.c_float), P(C.c_float),
                               P(C.c_float), P(C.c_float), C.c_int, C.c_int]
+        sig["step_timed"] = [h, C.c_int, C.c_float, C.c_uint, P(C.c_double), P(C.c_int)]
+        sig["batch_create"] = [P(h), P(h), C.c_int]
+        sig["batch_destroy"] = [h]
+        sig["batch_step"] = [h, C.c_int, C.c_float, C.c_uint]
+        sig["batch_sync"] = [h]
+        sig["batch_classes"] = [h, P(C.c_int), P(C.c_int), P(C.c_uint)]
+        sig["batch_autotune"] = [h, C.c_int, C.c_float, C.c_uint, P(C.c_uint)]
+        sig["batch_step_timed"] = [h, C.c_int, C.c_float, C.c_uint, P(C.c_double), P(C.c_int)]
     else:
         sig["create"] = [P(h), C.c_int, C.c_int, C.c_int]
         sig["insert_particles"] = [h]

@@ -1,0 +1,58 @@
+"""Batched instances: many FluidSystem handles stepped together (mg_batch_* in
+include/mg_sph.h). No counterpart in the reference, which steps one FluidSystem at a time."""
+from __future__ import annotations
+
+import ctypes as C
+
+from morphogenesis_amd import _ffi as F
+
+
+class Batch:
+    """Steps the given FluidSystem objects together. While the batch exists their work runs on
+    the batch's stream; close() (or garbage collection) releases them."""
+
+    def __init__(self, systems):
+        self.systems = list(systems)
+        if not self.systems:
+            raise ValueError("Batch needs at least one FluidSystem")
+        self.lib = self.systems[0].lib
+        arr = (C.c_void_p * len(self.systems))(*[s.h.value for s in self.systems])
+        self.h = C.c_void_p()
+        self._call("batch_create", C.byref(self.h), arr, len(self.systems))
+
+    def _call(self, name, *args):
+        r = getattr(self.lib, f"mg_{name}")(*args)
+        if r != F.MG_OK:
+            raise F.MgError(f"mg_{name} -> {F.ERRORS.get(r, r)}: {self.lib.mg_last_error().decode()}")
+        return r
+
+    def step(self, nsteps: int, dt: float, flags: int):
+        self._call("batch_step", self.h, int(nsteps), float(dt), int(flags))
+
+    def sync(self):
+        self._call("batch_sync", self.h)
+
+    def classes(self):
+        n = C.c_int()
+        self._call("batch_classes", self.h, C.byref(n), None, None)
+        members = (C.c_int * n.value)()
+        threads = (C.c_uint * n.value)()
+        self._call("batch_classes", self.h, C.byref(n), members, threads)
+        return [(members[i], threads[i]) for i in range(n.value)]
+
+    def autotune(self, steps: int, dt: float, flags: int):
+        n = len(self.classes())
+        out = (C.c_uint * n)()
+        self._call("batch_autotune", self.h, int(steps), float(dt), int(flags), out)
+        return list(out)
+
+    def close(self):
+        if self.h:
+            self.lib.mg_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

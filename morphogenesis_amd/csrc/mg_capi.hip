@@ -208,7 +208,9 @@ struct mg_handle {
     MgGrid g;
     MgGenome gen;
     bool have_params = false, have_grid = false;
-    hipStream_t st = nullptr;
+    hipStream_t st = nullptr;     /* the stream work is queued on (a batch's while in one) */
+    hipStream_t own_st = nullptr; /* the handle's own stream */
+    struct mg_batch* batch = nullptr;
     /* sorted ping-pong halves; current-half indices: kcur for P/V/VEL, icur for
      * HOME/GCELL/GNDX/GMASK, ccur for CONC (each group flips on its own) */
     float4 *P[2] = {}, *V[2] = {}, *VEL[2] = {};
@@ -361,6 +363,7 @@ DS make_ds(mg_handle* h, int kin = 0, int conc = 0)
     d.zero_n = 0;
     d.rows = (h->grid_scanned && h->adj_rows) ? 1 : 0;
     d.cull = (d.rows && h->cells_fresh) ? 1 : 0;
+    d.DVc = nullptr;
     d.ctr = h->ctr;
     d.N = h->N; d.B = h->B; d.C = h->C;
     return d;
@@ -471,6 +474,9 @@ int normalize(mg_handle* h)
     h->epoch++;
     return rebuild_inc(h);
 }
+
+/* dynamic LDS of a sweep kernel: one Band<NX> per wavefront of the block */
+template <int NX> constexpr size_t band_lds(unsigned threads) { return (threads / 64) * sizeof(Band<NX>); }
 
 /* The FCONC float4 groups the diffusion sweep reads and the per-TF coefficients
  * tf_diffusability * diff_scale (the same float operations as the reference's loop). */
@@ -602,11 +608,11 @@ int do_fused_step(mg_handle* h, float dt, unsigned flags)
         DS d = make_ds(h);
         d.ZERO = h->CNT[1 - h->gcur];
         d.zero_n = (uint32_t)h->C;
-        hipLaunchKernelGGL(k_pressure, dim3(nb), dim3(256), 0, h->st, d, h->p, h->g);
+        hipLaunchKernelGGL(k_pressure, dim3(nb), dim3(256), band_lds<0>(256), h->st, d, h->p, h->g);
     }
     {
         PhaseScope ps(h, MG_PH_FORCE);
-        hipLaunchKernelGGL(k_force_advance, dim3(nb), dim3(256), 0, h->st, make_ds(h), h->p, h->g, dt);
+        hipLaunchKernelGGL(k_force_advance, dim3(nb), dim3(256), band_lds<1>(256), h->st, make_ds(h), h->p, h->g, dt);
     }
     {
         PhaseScope ps(h, MG_PH_DIFFUSION);
@@ -615,11 +621,11 @@ int do_fused_step(mg_handle* h, float dt, unsigned flags)
         d.cull = 0; /* positions moved since the binning */
         const DiffPlan dp = diff_plan(h);
         if (dp.ng <= 1)
-            hipLaunchKernelGGL(k_diffuse_genes_insert<1>, dim3(nb), dim3(256), 0, h->st, d, h->p, h->g, h->gen, dp, genes);
+            hipLaunchKernelGGL(k_diffuse_genes_insert<1>, dim3(nb), dim3(256), band_lds<1>(256), h->st, d, h->p, h->g, h->gen, dp, genes);
         else if (dp.ng == 2)
-            hipLaunchKernelGGL(k_diffuse_genes_insert<2>, dim3(nb), dim3(256), 0, h->st, d, h->p, h->g, h->gen, dp, genes);
+            hipLaunchKernelGGL(k_diffuse_genes_insert<2>, dim3(nb), dim3(256), band_lds<2>(256), h->st, d, h->p, h->g, h->gen, dp, genes);
         else
-            hipLaunchKernelGGL(k_diffuse_genes_insert<4>, dim3(nb), dim3(256), 0, h->st, d, h->p, h->g, h->gen, dp, genes);
+            hipLaunchKernelGGL(k_diffuse_genes_insert<4>, dim3(nb), dim3(256), band_lds<4>(256), h->st, d, h->p, h->g, h->gen, dp, genes);
     }
     {
         PhaseScope ps(h, MG_PH_SCAN);
@@ -714,7 +720,7 @@ int ensure_force(mg_handle* h)
 int do_pressure(mg_handle* h)
 {
     PhaseScope ps(h, MG_PH_PRESSURE);
-    hipLaunchKernelGGL(k_pressure, dim3(nblk(h->N)), dim3(256), 0, h->st, make_ds(h), h->p, h->g);
+    hipLaunchKernelGGL(k_pressure, dim3(nblk(h->N)), dim3(256), band_lds<0>(256), h->st, make_ds(h), h->p, h->g);
     HIPCHK(hipGetLastError());
     return MG_OK;
 }
@@ -724,7 +730,7 @@ int do_force(mg_handle* h)
     CHK(ensure_force(h));
     PhaseScope ps(h, MG_PH_FORCE);
     DS d = make_ds(h);
-    hipLaunchKernelGGL(k_force, dim3(nblk(h->N)), dim3(256), 0, h->st, d, h->p, h->g, 1);
+    hipLaunchKernelGGL(k_force, dim3(nblk(h->N)), dim3(256), band_lds<1>(256), h->st, d, h->p, h->g, 1);
     HIPCHK(hipGetLastError());
     return MG_OK;
 }
@@ -746,11 +752,11 @@ int do_diffusion(mg_handle* h)
     const DiffPlan dp = diff_plan(h);
     const DS d = make_ds(h);
     if (dp.ng <= 1)
-        hipLaunchKernelGGL(k_diffusion<1>, dim3(nblk(h->N)), dim3(256), 0, h->st, d, h->p, h->g, h->gen, dp);
+        hipLaunchKernelGGL(k_diffusion<1>, dim3(nblk(h->N)), dim3(256), band_lds<1>(256), h->st, d, h->p, h->g, h->gen, dp);
     else if (dp.ng == 2)
-        hipLaunchKernelGGL(k_diffusion<2>, dim3(nblk(h->N)), dim3(256), 0, h->st, d, h->p, h->g, h->gen, dp);
+        hipLaunchKernelGGL(k_diffusion<2>, dim3(nblk(h->N)), dim3(256), band_lds<2>(256), h->st, d, h->p, h->g, h->gen, dp);
     else
-        hipLaunchKernelGGL(k_diffusion<4>, dim3(nblk(h->N)), dim3(256), 0, h->st, d, h->p, h->g, h->gen, dp);
+        hipLaunchKernelGGL(k_diffusion<4>, dim3(nblk(h->N)), dim3(256), band_lds<4>(256), h->st, d, h->p, h->g, h->gen, dp);
     HIPCHK(hipGetLastError());
     h->ccur = 1 - h->ccur;
     return MG_OK;
@@ -798,6 +804,7 @@ int mg_create(mg_handle** out, int max_points, int bonds_per_particle)
         delete h;
         return MG_ERR_HIP;
     }
+    h->own_st = h->st;
     const size_t N = (size_t)max_points, B = (size_t)bonds_per_particle;
     int r = MG_OK;
     for (int k = 0; k < 2 && r == MG_OK; k++) {
@@ -877,6 +884,10 @@ int mg_create(mg_handle** out, int max_points, int bonds_per_particle)
 int mg_destroy(mg_handle* h)
 {
     if (!h) return MG_ERR_ARG;
+    if (h->batch) {
+        set_err("mg_destroy: the handle belongs to a batch; destroy the batch first");
+        return MG_ERR_STATE;
+    }
     if (h->st) (void)hipStreamSynchronize(h->st);
     for (void* q : h->allocs) (void)hipFree(q);
     h->allocs.clear();
@@ -1295,3 +1306,5 @@ int mg_step_timed(mg_handle* h, int nsteps, float dt, unsigned flags, double* ms
 }
 
 } // extern "C"
+
+#include "mg_batch.hip"

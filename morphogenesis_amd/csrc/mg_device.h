@@ -65,6 +65,9 @@ struct DS {
     /* 1 when the slots are sorted by cell (CNT / OFF / GCELL of the last sort) and the
      * positions have not changed since: sweeps may skip cells out of reach */
     int cull;
+    /* learned coarse step: per-slot velocity correction (xyz) and use flag (w != 0) added
+     * by the advance; null on plain steps */
+    const float4* DVc;
     uint32_t* ctr;
     int N, B, C;
 };

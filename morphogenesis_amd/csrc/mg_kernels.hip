@@ -144,17 +144,16 @@ __global__ __launch_bounds__(1024) void k_scan(const uint32_t* in, uint32_t* out
  * "not ready" without any reset (also under hipGraph replay).
  * status word: [63:34] launch epoch, [33:32] flag (1 aggregate, 2 inclusive), [31:0] value. */
 #define SCAN_TILE 4096
-__global__ __launch_bounds__(256) void k_scan_lookback(const uint32_t* __restrict__ in,
-                                                       uint32_t* __restrict__ out, int n,
-                                                       unsigned long long* status,
-                                                       uint32_t* ticket, uint32_t* ctr, int gsm)
+__device__ __forceinline__ void scan_lookback_body(const uint32_t* __restrict__ in,
+                                                   uint32_t* __restrict__ out, int n,
+                                                   unsigned long long* status, uint32_t* ticket,
+                                                   uint32_t* ctr, int gsm, uint32_t tiles)
 {
     __shared__ uint32_t s_ticket, s_excl;
     __shared__ uint32_t wsum[4];
     const int t = threadIdx.x;
     if (t == 0) s_ticket = atomicAdd(ticket, 1u);
     __syncthreads();
-    const uint32_t tiles = gridDim.x;
     const uint32_t tile = s_ticket % tiles;
     const unsigned long long epoch = (unsigned long long)(s_ticket / tiles + 1u) & 0x3FFFFFFFull;
     const int base = (int)tile * SCAN_TILE + t * 16;
@@ -232,21 +231,31 @@ __global__ __launch_bounds__(256) void k_scan_lookback(const uint32_t* __restric
     }
 }
 
+__global__ __launch_bounds__(256) void k_scan_lookback(const uint32_t* __restrict__ in,
+                                                       uint32_t* __restrict__ out, int n,
+                                                       unsigned long long* status,
+                                                       uint32_t* ticket, uint32_t* ctr, int gsm)
+{
+    scan_lookback_body(in, out, n, status, ticket, ctr, gsm, gridDim.x);
+}
+
 /* ------------------------------------------------------ stable placement
  * PERM[OFF[cell] + slot] = s, then each cell's run of PERM is sorted ascending so the
  * intra-cell order is the pre-sort index order; FGNDX becomes that rank. */
-__global__ LAUNCH_BOUNDS_256 void k_place(DS d)
+__device__ __forceinline__ void place_body(const DS& d, uint32_t bid)
 {
-    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t s = bid * blockDim.x + threadIdx.x;
     if (s >= (uint32_t)d.N) return;
     const uint32_t c = d.GCELL[s];
     if (c == MG_GRID_UNDEF) return;
     d.PERM[d.OFF[c] + d.GNDX[s]] = s;
 }
 
-__global__ LAUNCH_BOUNDS_256 void k_cellsort(DS d)
+__global__ LAUNCH_BOUNDS_256 void k_place(DS d) { place_body(d, blockIdx.x); }
+
+__device__ __forceinline__ void cellsort_body(const DS& d, uint32_t bid)
 {
-    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t c = bid * blockDim.x + threadIdx.x;
     if (c >= (uint32_t)d.C) return;
     const uint32_t n = d.CNT[c];
     if (n == 0) return;
@@ -260,6 +269,8 @@ __global__ LAUNCH_BOUNDS_256 void k_cellsort(DS d)
     for (uint32_t r = 0; r < n; r++) d.GNDX[run[r]] = r;
 }
 
+__global__ LAUNCH_BOUNDS_256 void k_cellsort(DS d) { cellsort_body(d, blockIdx.x); }
+
 /* ------------------------------------------------------- counting sort (a7)
  * countingSortFull (src/fluid_system_cuda.cu:196-296) with the host resets of
  * src/host_CL.cpp:1156-1206, written as a gather over destination slots: slot dd <
@@ -269,11 +280,11 @@ __global__ LAUNCH_BOUNDS_256 void k_cellsort(DS d)
  * and the partner indices the reference rewrites here are resolved through SLOT.
  * Reads P/V/VEL/CONC/HOME/GCELL/GNDX, writes their "2" halves. The gene membership comes
  * from GMN (precomputed) when given, else from EPIGEN. Block 0 commits the counters. */
-__global__ LAUNCH_BOUNDS_256 void k_sort_gather(DS d, float vmax)
+__device__ __forceinline__ void sort_gather_body(const DS& d, float vmax, uint32_t bid)
 {
-    const uint32_t dd = xcd_block() * blockDim.x + threadIdx.x;
+    const uint32_t dd = bid * blockDim.x + threadIdx.x;
     const uint32_t nump = d.ctr[MG_C_SCAN_NUM];
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (bid == 0 && threadIdx.x == 0) {
         d.ctr[MG_C_NUM_POINTS] = nump;
         d.ctr[MG_C_ACTIVE] = d.ctr[MG_C_SCAN_ACTIVE];
         d.ctr[MG_C_TAIL] = nump;
@@ -315,6 +326,8 @@ __global__ LAUNCH_BOUNDS_256 void k_sort_gather(DS d, float vmax)
         d.GMASK2[dd] = 0u;
     }
 }
+
+__global__ LAUNCH_BOUNDS_256 void k_sort_gather(DS d, float vmax) { sort_gather_body(d, vmax, xcd_block()); }
 
 /* countingSortDenseLists (src/fluid_system_cuda.cu:329-404): per bin, particles in order,
  * genes with a nonzero epigenetic state append the particle to the gene's list. */
@@ -595,14 +608,14 @@ __device__ __forceinline__ void sweep(const DS& d, const MgGrid& g, Band<NX>& wb
  * contributePressure/computePressure, src/fluid_system_cuda.cu:620-712. Per cell partial
  * sums are added to the running total in gridAdj order. Writes FPRESS into P.w and
  * FDENSITY (= 1/density) into V.w. Also clears d.ZERO (the next grid's counters) when set. */
-__global__ LAUNCH_BOUNDS_SWEEP void k_pressure(DS d, MgParams p, MgGrid g)
+__device__ __forceinline__ void pressure_body(const DS& d, const MgParams& p, const MgGrid& g,
+                                              Band<0>* bands, uint32_t bid, uint32_t nblk)
 {
-    __shared__ Band<0> bands[4];
     if (d.ZERO) {
-        for (uint32_t z = blockIdx.x * blockDim.x + threadIdx.x; z < d.zero_n; z += gridDim.x * blockDim.x)
+        for (uint32_t z = bid * blockDim.x + threadIdx.x; z < d.zero_n; z += nblk * blockDim.x)
             d.ZERO[z] = 0u;
     }
-    const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
+    const uint32_t i = bid * blockDim.x + threadIdx.x;
     uint32_t gc = (i < d.ctr[MG_C_ACTIVE]) ? d.GCELL[i] : MG_GRID_UNDEF;
     const bool valid = gc != MG_GRID_UNDEF;
     gc -= (uint32_t)((1 * g.res[2] + 1) * g.res[0] + 1);
@@ -629,6 +642,13 @@ __global__ LAUNCH_BOUNDS_SWEEP void k_pressure(DS d, MgParams p, MgGrid g)
     if (sum == 0.0f) sum = 1.0f;
     ((float*)&d.P[i])[3] = (sum - p.prest_dens) * p.pintstiff;
     ((float*)&d.V[i])[3] = 1.0f / sum;
+}
+
+/* the sweep kernels take (blockDim.x / 64) Band<NX> of dynamic LDS, one per wavefront */
+__global__ LAUNCH_BOUNDS_SWEEP void k_pressure(DS d, MgParams p, MgGrid g)
+{
+    extern __shared__ float4 mg_lds[];
+    pressure_body(d, p, g, (Band<0>*)mg_lds, xcd_block(), gridDim.x);
 }
 
 /* --------------------------------------------------------------- force (a10)
@@ -762,7 +782,8 @@ __device__ __forceinline__ f3 force_one(const DS& d, const MgParams& p, const Mg
 /* ComputeForceCL: FFORCE[i] += force (accumulate) or = 0 + force. */
 __global__ LAUNCH_BOUNDS_SWEEP void k_force(DS d, MgParams p, MgGrid g, int accumulate)
 {
-    __shared__ Band<1> bands[4];
+    extern __shared__ float4 mg_lds[];
+    Band<1>* bands = (Band<1>*)mg_lds;
     const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
     const uint32_t na = d.ctr[MG_C_ACTIVE];
     const uint32_t gc = (i < na) ? d.GCELL[i] : MG_GRID_UNDEF;
@@ -776,7 +797,8 @@ __global__ LAUNCH_BOUNDS_SWEEP void k_force(DS d, MgParams p, MgGrid g, int accu
  * advanceParticles, src/fluid_system_cuda.cu:3470-3614, for one particle with acceleration
  * `accel` (FFORCE). Returns the new position, veval and velocity. */
 __device__ __forceinline__ void advance_one(const MgParams& p, float dt, f3 pos, f3 vev, f3 v,
-                                            f3 accel, f3* np_out, f3* ve_out, f3* vnext_out)
+                                            f3 accel, f3* np_out, f3* ve_out, f3* vnext_out,
+                                            const float4* corr = nullptr)
 {
     const float eps = MG_EPSILON, prad = p.pradius, slope = p.pground_slope;
     const float estiff = p.pextstiff, damp = p.pdamp;
@@ -836,6 +858,7 @@ __device__ __forceinline__ void advance_one(const MgParams& p, float dt, f3 pos,
         v = v * (p.VL / sqrtf(speed)); /* no-op limit, src/fluid_system_cuda.cu:3577-3579 */
     }
     f3 vnext = accel * dt + v;
+    if (corr && corr->w != 0.0f) vnext = vnext + mk3(corr->x, corr->y, corr->z); /* learned step */
     *ve_out = (v + vnext) * 0.5f;
     *np_out = pos + vnext * dt;
     *vnext_out = vnext;
@@ -859,10 +882,10 @@ __global__ LAUNCH_BOUNDS_256 void k_advance(DS d, MgParams p, float dt)
 /* Fused ComputeForceCL + AdvanceCL for a step that starts from a sort (FFORCE = 0): the force
  * stays in registers and the advanced state goes to the other half of the ping-pong
  * (P2/V2/VEL2) so neighbours keep reading the pre-step state. Inactive slots copy through. */
-__global__ LAUNCH_BOUNDS_SWEEP void k_force_advance(DS d, MgParams p, MgGrid g, float dt)
+__device__ __forceinline__ void force_advance_body(const DS& d, const MgParams& p, const MgGrid& g,
+                                                   Band<1>* bands, float dt, uint32_t bid)
 {
-    __shared__ Band<1> bands[4];
-    const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
+    const uint32_t i = bid * blockDim.x + threadIdx.x;
     const uint32_t na = d.ctr[MG_C_ACTIVE];
     const uint32_t gc = (i < na) ? d.GCELL[i] : MG_GRID_UNDEF;
     const bool valid = gc != MG_GRID_UNDEF;
@@ -876,10 +899,22 @@ __global__ LAUNCH_BOUNDS_SWEEP void k_force_advance(DS d, MgParams p, MgGrid g, 
         return;
     }
     f3 np, ve, vn;
-    advance_one(p, dt, xyz(P_i), xyz(V_i), xyz(VEL_i), F, &np, &ve, &vn);
+    if (d.DVc) {
+        /* learned coarse step: the correction joins v_next before veval and the position */
+        const float4 c = d.DVc[i];
+        advance_one(p, dt, xyz(P_i), xyz(V_i), xyz(VEL_i), F, &np, &ve, &vn, &c);
+    } else {
+        advance_one(p, dt, xyz(P_i), xyz(V_i), xyz(VEL_i), F, &np, &ve, &vn);
+    }
     d.P2[i] = make_float4(np.x, np.y, np.z, P_i.w);
     d.V2[i] = make_float4(ve.x, ve.y, ve.z, V_i.w);
     d.VEL2[i] = make_float4(vn.x, vn.y, vn.z, 0.0f);
+}
+
+__global__ LAUNCH_BOUNDS_SWEEP void k_force_advance(DS d, MgParams p, MgGrid g, float dt)
+{
+    extern __shared__ float4 mg_lds[];
+    force_advance_body(d, p, g, (Band<1>*)mg_lds, dt, xcd_block());
 }
 
 /* ----------------------------------------------------------- diffusion (a12)
@@ -945,7 +980,7 @@ __device__ __forceinline__ void diffuse_sweep(const DS& d, const MgParams& p, co
     }
 #pragma unroll
     for (int t = 0; t < 4 * NG; t++) nw[t] = 0.0f;
-    sweep<(NG >= 2 ? 2 : 4), NG>(
+    sweep<(NG >= 4 ? 1 : (NG >= 2 ? 2 : 4)), NG>(
         d, g, wb, valid, gc, pi, r2,
         [&](uint32_t j, int q) { return ((const float4*)(d.CONC + (size_t)j * MG_NUM_TF))[dp.grp[q]]; },
         [&](uint32_t, float4, f3, float dsq, const float4* x) {
@@ -990,7 +1025,8 @@ __device__ __forceinline__ void diffuse_one(const DS& d, const MgParams& p, cons
 template <int NG>
 __global__ LAUNCH_BOUNDS_SWEEP void k_diffusion(DS d, MgParams p, MgGrid g, MgGenome gen, DiffPlan dp)
 {
-    __shared__ Band<NG> bands[4];
+    extern __shared__ float4 mg_lds[];
+    Band<NG>* bands = (Band<NG>*)mg_lds;
     const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
     const uint32_t gc = (i < d.ctr[MG_C_ACTIVE]) ? d.GCELL[i] : MG_GRID_UNDEF;
     const bool valid = gc != MG_GRID_UNDEF;
@@ -1073,11 +1109,11 @@ __global__ LAUNCH_BOUNDS_256 void k_genes(DS d, MgParams p, MgGenome gen)
  * (d.CNT / d.OFF); writes CONC2, updates EPIGEN, records the post-gene membership in GMN for
  * the coming sort and bins the particle into the next grid (d.CNTN). */
 template <int NG>
-__global__ LAUNCH_BOUNDS_SWEEP void k_diffuse_genes_insert(DS d, MgParams p, MgGrid g, MgGenome gen,
-                                                           DiffPlan dp, int genes)
+__device__ __forceinline__ void diffuse_genes_insert_body(const DS& d, const MgParams& p, const MgGrid& g,
+                                                          const MgGenome& gen, const DiffPlan& dp,
+                                                          int genes, Band<NG>* bands, uint32_t bid)
 {
-    __shared__ Band<NG> bands[4];
-    const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
+    const uint32_t i = bid * blockDim.x + threadIdx.x;
     const bool valid = i < (uint32_t)d.N;
     const uint32_t gc = (i < d.ctr[MG_C_ACTIVE]) ? d.GCELL[i] : MG_GRID_UNDEF;
     const bool diffuses = genes && gc != MG_GRID_UNDEF;
@@ -1110,6 +1146,14 @@ __global__ LAUNCH_BOUNDS_SWEEP void k_diffuse_genes_insert(DS d, MgParams p, MgG
     insert_one(d, g, i, valid, P_i, d.CNTN, nullptr);
 }
 
+template <int NG>
+__global__ LAUNCH_BOUNDS_SWEEP void k_diffuse_genes_insert(DS d, MgParams p, MgGrid g, MgGenome gen,
+                                                           DiffPlan dp, int genes)
+{
+    extern __shared__ float4 mg_lds[];
+    diffuse_genes_insert_body<NG>(d, p, g, gen, dp, genes, (Band<NG>*)mg_lds, xcd_block());
+}
+
 /* ZeroVelCL (src/host_CL.cpp:2175-2180) over the whole FVEL / FVEVAL buffers. */
 __global__ LAUNCH_BOUNDS_256 void k_zero_vel(DS d)
 {
@@ -1119,3 +1163,82 @@ __global__ LAUNCH_BOUNDS_256 void k_zero_vel(DS d)
     float4 v = d.V[i];
     d.V[i] = make_float4(0.f, 0.f, 0.f, v.w);
 }
+
+/* -------------------------------------------------------------- batched step
+ * One launch per phase steps many independent instances (own buffers, grid, parameters and
+ * genome). Block b of a launch belongs to instance map[b], which owns the launch's blocks
+ * [b0, b0 + nb) for that phase; each instance then computes exactly what its own fused step
+ * computes (the same bodies with the same block-local indices), so a batched instance
+ * matches the instance stepped alone bit for bit. */
+struct BatchInst {
+    DS pressure, force, diffuse, place, sort; /* the views of the fused step's launches */
+    MgParams p;
+    MgGrid g;
+    DiffPlan dp;
+    const MgGenome* gen;
+    const uint32_t* scan_in;
+    uint32_t* scan_out;
+    unsigned long long* scan_status;
+    uint32_t* scan_ticket;
+    uint32_t* ctr;
+    int C, gsm, genes;
+    float vmax, dt;
+    uint32_t pb0, pnb; /* particle launches */
+    uint32_t cb0, cnb; /* per-cell launch */
+    uint32_t tb0, tnb; /* scan tiles */
+};
+
+__global__ LAUNCH_BOUNDS_SWEEP void kb_pressure(const BatchInst* __restrict__ I, const uint16_t* __restrict__ map)
+{
+    extern __shared__ float4 mg_lds[];
+    const uint32_t b = xcd_block();
+    const BatchInst& in = I[map[b]];
+    pressure_body(in.pressure, in.p, in.g, (Band<0>*)mg_lds, b - in.pb0, in.pnb);
+}
+
+__global__ LAUNCH_BOUNDS_SWEEP void kb_force_advance(const BatchInst* __restrict__ I,
+                                                     const uint16_t* __restrict__ map)
+{
+    extern __shared__ float4 mg_lds[];
+    const uint32_t b = xcd_block();
+    const BatchInst& in = I[map[b]];
+    force_advance_body(in.force, in.p, in.g, (Band<1>*)mg_lds, in.dt, b - in.pb0);
+}
+
+template <int NG>
+__global__ LAUNCH_BOUNDS_SWEEP void kb_diffuse_genes_insert(const BatchInst* __restrict__ I,
+                                                            const uint16_t* __restrict__ map)
+{
+    extern __shared__ float4 mg_lds[];
+    const uint32_t b = xcd_block();
+    const BatchInst& in = I[map[b]];
+    diffuse_genes_insert_body<NG>(in.diffuse, in.p, in.g, *in.gen, in.dp, in.genes, (Band<NG>*)mg_lds,
+                                  b - in.pb0);
+}
+
+__global__ __launch_bounds__(256) void kb_scan(const BatchInst* __restrict__ I, const uint16_t* __restrict__ map)
+{
+    const BatchInst& in = I[map[blockIdx.x]];
+    scan_lookback_body(in.scan_in, in.scan_out, in.C, in.scan_status, in.scan_ticket, in.ctr, in.gsm, in.tnb);
+}
+
+__global__ LAUNCH_BOUNDS_256 void kb_place(const BatchInst* __restrict__ I, const uint16_t* __restrict__ map)
+{
+    const BatchInst& in = I[map[blockIdx.x]];
+    place_body(in.place, blockIdx.x - in.pb0);
+}
+
+__global__ LAUNCH_BOUNDS_256 void kb_cellsort(const BatchInst* __restrict__ I, const uint16_t* __restrict__ map)
+{
+    const BatchInst& in = I[map[blockIdx.x]];
+    cellsort_body(in.place, blockIdx.x - in.cb0);
+}
+
+__global__ LAUNCH_BOUNDS_256 void kb_sort_gather(const BatchInst* __restrict__ I, const uint16_t* __restrict__ map)
+{
+    const uint32_t b = xcd_block();
+    const BatchInst& in = I[map[b]];
+    sort_gather_body(in.sort, in.vmax, b - in.pb0);
+}
+
+#include "mg_mlp.hip"

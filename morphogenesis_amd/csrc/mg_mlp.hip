@@ -1,0 +1,430 @@
+/*
+ * mg_mlp.hip — the learned coarse-step correction (no counterpart in the reference).
+ *
+ * A per-particle MLP 64 -> 128 -> 128 -> 16 (SiLU hidden layers; 50 real inputs, 3 real
+ * outputs, the rest zero padding to MFMA tile sizes) predicts the velocity correction
+ * dv_i = v_fine - v_coarse that four fine steps of dt would add to one coarse step of 4 dt.
+ * Inputs, from the particle's sorted state after the pressure pass:
+ *   8 bonded neighbours (live own bond slots in slot order, then incoming bonds in CSR
+ *   order): (x_j - x_i) and (v_j - v_i), 48 values; density and pressure; zeros to 64.
+ * Each input is normalised as (x - mu) * inv_sigma; the output is y * out_scale.
+ *
+ * Weights are bf16, accumulation fp32, on gfx950 MFMA (v_mfma_f32_16x16x32_bf16): for a
+ * 16 x 32 A tile lane l holds A[l % 16][8 (l / 16) + e], for a 32 x 16 B tile lane l holds
+ * B[8 (l / 16) + e][l % 16] (e = 0..7), and the 16 x 16 fp32 result lane l holds
+ * D[4 (l / 16) + r][l % 16] (r = 0..3).
+ *
+ * Training uses a generic bf16 GEMM with split-K partials reduced in a fixed order, and
+ * fp32 Adam on fp32 master weights; inference is one fused kernel per step.
+ */
+
+#define MLP_IN 64
+#define MLP_IN_REAL 50
+#define MLP_H 128
+#define MLP_OUT 16
+#define MLP_OUT_REAL 3
+#define MLP_NBR 8
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8_t;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4_t;
+
+__device__ __forceinline__ f32x4_t mfma16(bf16x8_t a, bf16x8_t b, f32x4_t c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ __bf16 to_bf16(float x) { return (__bf16)x; }
+
+__device__ __forceinline__ float silu(float z) { return z / (1.0f + expf(-z)); }
+
+/* d silu / dz = s (1 + z (1 - s)), s = sigmoid(z) */
+__device__ __forceinline__ float silu_grad(float z)
+{
+    const float s = 1.0f / (1.0f + expf(-z));
+    return s * (1.0f + z * (1.0f - s));
+}
+
+/* Everything the inference kernel reads, device resident. Weight fragments are laid out so
+ * that each lane's 8 bf16 of a B operand are contiguous:
+ *   W?f[ntile][kchunk][col 0..15][k 0..31] = W[kchunk * 32 + k][ntile * 16 + col]. */
+struct MlpDev {
+    __bf16 w1f[(MLP_IN / 32) * (MLP_H / 16) * 16 * 32];
+    __bf16 w2f[(MLP_H / 32) * (MLP_H / 16) * 16 * 32];
+    __bf16 w3f[(MLP_H / 32) * (MLP_OUT / 16) * 16 * 32];
+    float b1[MLP_H], b2[MLP_H], b3[MLP_OUT];
+    float mu[MLP_IN], inv_sigma[MLP_IN];
+    float out_scale;
+    float in_limit;  /* |normalised input| above this -> fallback (plain coarse step) */
+    float out_limit; /* |y| above this -> fallback */
+};
+
+/* ------------------------------------------------------------------ features
+ * The 64 raw inputs of active particle i (sorted state after pressure). */
+__device__ __forceinline__ void mlp_features(const DS& d, uint32_t i, uint32_t na, float x[MLP_IN])
+{
+    const int N = d.N, B = d.B;
+#pragma unroll
+    for (int k = 0; k < MLP_IN; k++) x[k] = 0.0f;
+    const uint32_t h = d.HOME[i];
+    const float4 Pi = d.P[i], Vi = d.VEL[i];
+    int m = 0;
+    auto add = [&](uint32_t j) {
+        const float4 Pj = d.P[j], Vj = d.VEL[j];
+        const float f[6] = {Pj.x - Pi.x, Pj.y - Pi.y, Pj.z - Pi.z, Vj.x - Vi.x, Vj.y - Vi.y, Vj.z - Vi.z};
+#pragma unroll
+        for (int q = 0; q < MLP_NBR; q++)
+            if (q == m)
+#pragma unroll
+                for (int c = 0; c < 6; c++) x[6 * q + c] = f[c];
+        m++;
+    };
+    for (int a = 0; a < B && m < MLP_NBR; a++) {
+        const size_t bi = (size_t)a * N + h;
+        const uint32_t jh = d.BPARTNER[bi];
+        if (jh == MG_NONE) continue;
+        const uint32_t j = d.SLOT[jh];
+        if (j >= na || (double)d.BPAR[bi].y < 0.000000001) continue;
+        add(j);
+    }
+    const uint32_t e0 = d.INC_OFF[h], e1 = d.INC_OFF[h + 1];
+    for (uint32_t e = e0; e < e1 && m < MLP_NBR; e++) {
+        const uint32_t packed = d.INC[e];
+        const uint32_t kh = packed / (uint32_t)B, b = packed - kh * (uint32_t)B;
+        const uint32_t k = d.SLOT[kh];
+        if (k >= na || (double)d.BPAR[(size_t)b * N + kh].y < 0.000000001) continue;
+        add(k);
+    }
+    x[48] = 1.0f / d.V[i].w; /* density */
+    x[49] = Pi.w;            /* pressure */
+}
+
+/* raw features of every active slot (rows of inactive slots are zero), for training data */
+__global__ LAUNCH_BOUNDS_256 void k_mlp_features(DS d, float* __restrict__ X)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (uint32_t)d.N) return;
+    float x[MLP_IN];
+    const uint32_t na = d.ctr[MG_C_ACTIVE];
+    if (i < na && d.GCELL[i] != MG_GRID_UNDEF) {
+        mlp_features(d, i, na, x);
+    } else {
+#pragma unroll
+        for (int k = 0; k < MLP_IN; k++) x[k] = 0.0f;
+    }
+    float4* o = (float4*)(X + (size_t)i * MLP_IN);
+#pragma unroll
+    for (int k = 0; k < MLP_IN / 4; k++) o[k] = make_float4(x[4 * k], x[4 * k + 1], x[4 * k + 2], x[4 * k + 3]);
+}
+
+/* --------------------------------------------------------------- inference
+ * One wavefront per 64 particles: each lane gathers and normalises its particle's inputs
+ * into an LDS tile, then the wave runs the three layers as 16-row MFMA tiles through LDS.
+ * DV[i] = (dv, 1) for corrected particles, (0, 0, 0, 0) for inactive or fallback ones;
+ * fallbacks are counted in *fallbacks. Weights are staged in LDS once per block. */
+#define MLP_WAVES 4
+struct MlpLds {
+    __bf16 w1f[(MLP_IN / 32) * (MLP_H / 16) * 16 * 32];
+    __bf16 w2f[(MLP_H / 32) * (MLP_H / 16) * 16 * 32];
+    __bf16 w3f[(MLP_H / 32) * (MLP_OUT / 16) * 16 * 32];
+    __bf16 x[MLP_WAVES][64][MLP_IN + 8];   /* +8: row pitch off the bank period */
+    __bf16 h[MLP_WAVES][16][MLP_H + 8];
+    __bf16 h2[MLP_WAVES][16][MLP_H + 8];
+    int ok[MLP_WAVES][64];
+};
+
+__device__ __forceinline__ bf16x8_t lds8(const __bf16* p) { return *(const bf16x8_t*)p; }
+
+__global__ __launch_bounds__(256) void k_mlp_infer(DS d, const MlpDev* __restrict__ net, float4* __restrict__ DV,
+                                                   uint32_t* __restrict__ fallbacks)
+{
+    extern __shared__ float4 mg_lds[];
+    MlpLds& L = *(MlpLds*)mg_lds;
+    const int t = threadIdx.x, w = t >> 6, l = t & 63;
+    {
+        const uint4* s1 = (const uint4*)net->w1f;
+        uint4* d1 = (uint4*)L.w1f;
+        for (int k = t; k < (int)(sizeof(L.w1f) / 16); k += 256) d1[k] = s1[k];
+        const uint4* s2 = (const uint4*)net->w2f;
+        uint4* d2 = (uint4*)L.w2f;
+        for (int k = t; k < (int)(sizeof(L.w2f) / 16); k += 256) d2[k] = s2[k];
+        const uint4* s3 = (const uint4*)net->w3f;
+        uint4* d3 = (uint4*)L.w3f;
+        for (int k = t; k < (int)(sizeof(L.w3f) / 16); k += 256) d3[k] = s3[k];
+    }
+    __syncthreads();
+    const uint32_t na = d.ctr[MG_C_ACTIVE];
+    const uint32_t ntile = ((uint32_t)d.N + 63u) / 64u;
+    uint32_t nfall = 0;
+    for (uint32_t tile = blockIdx.x * MLP_WAVES + w; tile < ntile; tile += gridDim.x * MLP_WAVES) {
+        const uint32_t i = tile * 64u + (uint32_t)l;
+        /* gather + normalise: lane l owns row l */
+        bool act = i < (uint32_t)d.N && i < na && d.GCELL[i] != MG_GRID_UNDEF;
+        bool ok = act;
+        {
+            float x[MLP_IN];
+            if (act) {
+                mlp_features(d, i, na, x);
+            } else {
+#pragma unroll
+                for (int k = 0; k < MLP_IN; k++) x[k] = 0.0f;
+            }
+#pragma unroll
+            for (int k = 0; k < MLP_IN; k++) {
+                float v = act ? (x[k] - net->mu[k]) * net->inv_sigma[k] : 0.0f;
+                if (!(fabsf(v) <= net->in_limit)) ok = false;
+                L.x[w][l][k] = to_bf16(act ? v : 0.0f);
+            }
+        }
+        L.ok[w][l] = ok ? 1 : 0;
+        wave_lds_sync();
+        for (int rt = 0; rt < 4; rt++) {
+            const int row = l & 15, kq = l >> 4;
+            /* layer 1: [16 x 64] x [64 x 128] */
+            for (int nt = 0; nt < MLP_H / 16; nt++) {
+                f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int kc = 0; kc < MLP_IN / 32; kc++) {
+                    const bf16x8_t a = lds8(&L.x[w][rt * 16 + row][kc * 32 + 8 * kq]);
+                    const bf16x8_t b = lds8(&L.w1f[((nt * (MLP_IN / 32) + kc) * 16 + row) * 32 + 8 * kq]);
+                    acc = mfma16(a, b, acc);
+                }
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int col = nt * 16 + (l & 15);
+                    L.h[w][4 * kq + r][col] = to_bf16(silu(acc[r] + net->b1[col]));
+                }
+            }
+            wave_lds_sync();
+            /* layer 2: [16 x 128] x [128 x 128] */
+            for (int nt = 0; nt < MLP_H / 16; nt++) {
+                f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int kc = 0; kc < MLP_H / 32; kc++) {
+                    const bf16x8_t a = lds8(&L.h[w][row][kc * 32 + 8 * kq]);
+                    const bf16x8_t b = lds8(&L.w2f[((nt * (MLP_H / 32) + kc) * 16 + row) * 32 + 8 * kq]);
+                    acc = mfma16(a, b, acc);
+                }
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int col = nt * 16 + (l & 15);
+                    L.h2[w][4 * kq + r][col] = to_bf16(silu(acc[r] + net->b2[col]));
+                }
+            }
+            wave_lds_sync();
+            /* layer 3: [16 x 128] x [128 x 16] */
+            f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int kc = 0; kc < MLP_H / 32; kc++) {
+                const bf16x8_t a = lds8(&L.h2[w][row][kc * 32 + 8 * kq]);
+                const bf16x8_t b = lds8(&L.w3f[(kc * 16 + row) * 32 + 8 * kq]);
+                acc = mfma16(a, b, acc);
+            }
+            /* lanes with column < 3 hold outputs of rows 4 kq + r; gather them per row */
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int col = l & 15;
+                const float y = acc[r] + net->b3[col < MLP_OUT ? col : 0];
+                const int prow = rt * 16 + 4 * kq + r; /* particle row within the 64-tile */
+                if (col < MLP_OUT_REAL) {
+                    if (!(fabsf(y) <= net->out_limit)) L.ok[w][prow] = 0;
+                    ((float*)&L.h[w][0][0])[prow * 4 + col] = y; /* layer-1 tile is free now */
+                }
+            }
+            wave_lds_sync();
+            if ((l >> 4) == rt) {
+                /* lanes 16 rt .. 16 rt + 15 write their rows */
+                const int prow = l;
+                const uint32_t ii = tile * 64u + (uint32_t)prow;
+                if (ii < (uint32_t)d.N) {
+                    const float* y = &((float*)&L.h[w][0][0])[prow * 4];
+                    const bool use = L.ok[w][prow] != 0;
+                    if (use)
+                        DV[ii] = make_float4(y[0] * net->out_scale, y[1] * net->out_scale, y[2] * net->out_scale, 1.0f);
+                    else
+                        DV[ii] = make_float4(0.f, 0.f, 0.f, 0.f);
+                    const bool wasact = ii < na && d.GCELL[ii] != MG_GRID_UNDEF;
+                    if (wasact && !use) nfall++;
+                }
+            }
+            wave_lds_sync();
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) nfall += (uint32_t)__shfl_xor((int)nfall, o);
+    if (l == 0 && nfall) atomicAdd(fallbacks, nfall);
+}
+
+/* ------------------------------------------------------------ training GEMM
+ * C[s][m][n] = sum over k in split s of A(m, k) * B(k, n), A(m, k) = A[m * sam + k * sak],
+ * B(k, n) = B[k * sbk + n * sbn] (bf16), fp32 partials per split; k_gemm_reduce adds the
+ * splits in order (deterministic). 64 x 64 block tile, four 32 x 32 wave tiles, K step 32. */
+__global__ __launch_bounds__(256) void k_gemm_bf16(int M, int N, int K, const __bf16* __restrict__ A, long sam,
+                                                   long sak, const __bf16* __restrict__ Bm, long sbk, long sbn,
+                                                   float* __restrict__ C, int kper)
+{
+    __shared__ __bf16 As[64][32 + 8];
+    __shared__ __bf16 Bs[64][32 + 8]; /* transposed: Bs[n][k] */
+    const int t = threadIdx.x, w = t >> 6, l = t & 63;
+    const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+    const int k_lo = blockIdx.z * kper, k_hi = min(K, k_lo + kper);
+    const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
+    f32x4_t acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) acc[a][b] = (f32x4_t){0.f, 0.f, 0.f, 0.f};
+    for (int k0 = k_lo; k0 < k_hi; k0 += 32) {
+        /* 64 x 32 elements each of A and B: 8 per thread */
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const int idx = t * 8 + e;
+            const int r = idx >> 5, c = idx & 31;
+            const int m = m0 + r, k = k0 + c;
+            As[r][c] = (m < M && k < k_hi) ? A[(long)m * sam + (long)k * sak] : to_bf16(0.0f);
+            const int n = n0 + r;
+            Bs[r][c] = (n < N && k < k_hi) ? Bm[(long)k * sbk + (long)n * sbn] : to_bf16(0.0f);
+        }
+        __syncthreads();
+        const int row = l & 15, kq = l >> 4;
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+            for (int b = 0; b < 2; b++) {
+                const bf16x8_t av = lds8(&As[wm + a * 16 + row][8 * kq]);
+                const bf16x8_t bv = lds8(&Bs[wn + b * 16 + row][8 * kq]);
+                acc[a][b] = mfma16(av, bv, acc[a][b]);
+            }
+        __syncthreads();
+    }
+    float* Cs = C + (size_t)blockIdx.z * M * N;
+    const int kq = l >> 4;
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int m = m0 + wm + a * 16 + 4 * kq + r, n = n0 + wn + b * 16 + (l & 15);
+                if (m < M && n < N) Cs[(size_t)m * N + n] = acc[a][b][r];
+            }
+}
+
+__global__ LAUNCH_BOUNDS_256 void k_gemm_reduce(const float* __restrict__ part, int splits, size_t mn,
+                                                float* __restrict__ C)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= mn) return;
+    float s = part[i];
+    for (int k = 1; k < splits; k++) s += part[(size_t)k * mn + i];
+    C[i] = s;
+}
+
+/* ---------------------------------------------------------- elementwise steps */
+__global__ LAUNCH_BOUNDS_256 void k_to_bf16(const float* __restrict__ x, __bf16* __restrict__ y, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = to_bf16(x[i]);
+}
+
+/* rows of X (raw features) normalised and converted to bf16 */
+__global__ LAUNCH_BOUNDS_256 void k_normalize_rows(const float* __restrict__ X, const float* __restrict__ mu,
+                                                   const float* __restrict__ inv_sigma, __bf16* __restrict__ Xn,
+                                                   size_t rows)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= rows * MLP_IN) return;
+    const int k = (int)(i % MLP_IN);
+    Xn[i] = to_bf16((X[i] - mu[k]) * inv_sigma[k]);
+}
+
+/* Z += bias (per column); A = silu(Z) in fp32 and bf16 */
+__global__ LAUNCH_BOUNDS_256 void k_bias_silu(float* __restrict__ Z, const float* __restrict__ bias, int cols,
+                                              size_t rows, __bf16* __restrict__ Ab, int apply_silu)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= rows * cols) return;
+    const float z = Z[i] + bias[i % cols];
+    Z[i] = z;
+    if (Ab) Ab[i] = to_bf16(apply_silu ? silu(z) : z);
+}
+
+/* loss = sum over rows with mask of |Y[:, :3] - T|^2 / count; dY = 2 (Y - T) / count */
+__global__ LAUNCH_BOUNDS_256 void k_mse(const float* __restrict__ Y, const float* __restrict__ T,
+                                        const float* __restrict__ mask, size_t rows, float inv_count,
+                                        float* __restrict__ dY, __bf16* __restrict__ dYb, float* __restrict__ lpart)
+{
+    __shared__ float red[256];
+    const size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    float l = 0.f;
+    if (r < rows) {
+        const float mk = mask[r];
+        for (int c = 0; c < MLP_OUT; c++) {
+            float g = 0.f;
+            if (c < MLP_OUT_REAL && mk != 0.f) {
+                const float e = Y[r * MLP_OUT + c] - T[r * MLP_OUT_REAL + c];
+                l += e * e;
+                g = 2.0f * e * inv_count;
+            }
+            dY[r * MLP_OUT + c] = g;
+            dYb[r * MLP_OUT + c] = to_bf16(g);
+        }
+    }
+    red[threadIdx.x] = l;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) lpart[blockIdx.x] = red[0] * inv_count;
+}
+
+/* dZ = dA * silu'(Z) (fp32 and bf16) */
+__global__ LAUNCH_BOUNDS_256 void k_silu_back(const float* __restrict__ dA, const float* __restrict__ Z, size_t n,
+                                              float* __restrict__ dZ, __bf16* __restrict__ dZb)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float g = dA[i] * silu_grad(Z[i]);
+    dZ[i] = g;
+    dZb[i] = to_bf16(g);
+}
+
+/* column sums of G [rows x cols], one block per column, fixed-order tree: deterministic */
+__global__ LAUNCH_BOUNDS_256 void k_colsum(const float* __restrict__ G, size_t rows, int cols, float* __restrict__ out)
+{
+    __shared__ float red[256];
+    const int c = blockIdx.x;
+    float s = 0.f;
+    for (size_t r = threadIdx.x; r < rows; r += 256) s += G[r * cols + c];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[c] = red[0];
+}
+
+/* Adam (Kingma & Ba, 2015) on fp32 master weights */
+__global__ LAUNCH_BOUNDS_256 void k_adam(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m,
+                                         float* __restrict__ v, size_t n, float lr, float b1, float b2, float eps,
+                                         float c1, float c2)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+/* W [K x N] fp32 row-major -> fragment layout (ntile, kchunk, col, k) in bf16 */
+__global__ LAUNCH_BOUNDS_256 void k_pack_frag(const float* __restrict__ W, int K, int N, __bf16* __restrict__ F)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= K * N) return;
+    const int k = i / N, n = i % N;
+    const int nt = n / 16, col = n % 16, kc = k / 32, kk = k % 32;
+    F[(((size_t)nt * (K / 32) + kc) * 16 + col) * 32 + kk] = to_bf16(W[i]);
+}

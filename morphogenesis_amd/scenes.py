@@ -240,10 +240,11 @@ def fluid_block(n_side: int = 32, seed: int = 1, spacing: float = 0.5, jitter: f
     vmin3, vmax3 = (0.0, 0.0, 0.0), (vmax, vmax, vmax)
     params = make_params(vmin3, vmax3, spacing=spacing)
     grid = make_grid(vmin3, vmax3)
-    extent = (n_side - 1) * spacing
-    lo = [(p + q) / 2 - extent / 2 for p, q in zip(params.pboundmin, params.pboundmax)]
+    nx = ny = nz = n_side
+    extent = [(m - 1) * spacing for m in (nx, ny, nz)]
+    lo = [(p + q) / 2 - e / 2 for p, q, e in zip(params.pboundmin, params.pboundmax, extent)]
     lo[1] = params.pboundmin[1] + 0.25
-    pos = lattice(n_side, n_side, n_side, spacing, lo, jitter, rng)
+    pos = lattice(nx, ny, nz, spacing, lo, jitter, rng)
     n = pos.shape[0]
     bufs = base_buffers(n, 0, pos)
     return Scene(f"fluid_block_{n_side}", n, 0, params, grid, empty_genome(), bufs, dt, steps, 0,
@@ -253,22 +254,24 @@ def fluid_block(n_side: int = 32, seed: int = 1, spacing: float = 0.5, jitter: f
 def tissue(n_side: int = 64, b: int = 32, seed: int = 1, spacing: float = 0.5,
            jitter: float = 0.05, vmax: float | None = None, k0: float = 10.0, zeta: float = 0.1,
            limit: float = 2.0, genes: bool = True, steps: int = 1000, dt: float = 1e-4,
-           diff_scale: float = 0.01, secrete_scale: float = 0.001, reserve: int = 0) -> Scene:
-    """Bonded elastic tissue block (the headline workload at n_side = 64, B = 32).
-    `reserve` extra null particles (AddNullPoints, src/fluid_system.cpp:988-1034) are parked
-    at PBOUNDMAX, outside the scan window."""
+           diff_scale: float = 0.01, secrete_scale: float = 0.001, reserve: int = 0,
+           dims: tuple | None = None) -> Scene:
+    """Bonded elastic tissue block (the headline workload at n_side = 64, B = 32), or an
+    nx x ny x nz block when `dims` is given. `reserve` extra null particles (AddNullPoints,
+    src/fluid_system.cpp:988-1034) are parked at PBOUNDMAX, outside the scan window."""
     rng = np.random.default_rng(seed)
+    nx, ny, nz = dims if dims is not None else (n_side, n_side, n_side)
     if vmax is None:
-        vmax = float(np.ceil((n_side - 1) * spacing + 4.5))
+        vmax = float(np.ceil((max(nx, ny, nz) - 1) * spacing + 4.5))
     vmin3, vmax3 = (0.0, 0.0, 0.0), (vmax, vmax, vmax)
     params = make_params(vmin3, vmax3, spacing=spacing)
     params.diff_scale = diff_scale
     params.secrete_scale = secrete_scale
     grid = make_grid(vmin3, vmax3)
-    extent = (n_side - 1) * spacing
-    lo = [(p + q) / 2 - extent / 2 for p, q in zip(params.pboundmin, params.pboundmax)]
+    extent = [(m - 1) * spacing for m in (nx, ny, nz)]
+    lo = [(p + q) / 2 - e / 2 for p, q, e in zip(params.pboundmin, params.pboundmax, extent)]
     lo[1] = params.pboundmin[1] + 0.25
-    pos = lattice(n_side, n_side, n_side, spacing, lo, jitter, rng)
+    pos = lattice(nx, ny, nz, spacing, lo, jitter, rng)
     n_live = pos.shape[0]
     if reserve:
         park = np.tile(np.asarray(params.pboundmax, np.float32), (reserve, 1))
@@ -287,7 +290,26 @@ def tissue(n_side: int = 64, b: int = 32, seed: int = 1, spacing: float = 0.5,
         flags = F.STEP_GENES
     info.update({"spacing": spacing, "jitter": jitter, "seed": seed, "k0": k0, "zeta": zeta,
                  "limit": limit, "reserve": reserve})
-    return Scene(f"tissue_{n_side}_B{b}", n, b, params, grid, genome, bufs, dt, steps, flags, info)
+    name = f"tissue_{n_side}_B{b}" if dims is None else f"tissue_{nx}x{ny}x{nz}_B{b}"
+    return Scene(name, n, b, params, grid, genome, bufs, dt, steps, flags, info)
+
+
+def batch_sizes(count: int = 256, lo: int = 2048, hi: int = 65536, seed: int = 3) -> list:
+    """Particle counts drawn log-uniformly in [lo, hi] (seeded), each rounded to a block
+    nx x ny x nz of the tissue lattice (nx = ny, nz chosen to land closest)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for v in np.exp(rng.uniform(np.log(lo), np.log(hi), count)):
+        nx = max(2, int(round(v ** (1.0 / 3.0))))
+        nz = max(2, int(round(v / (nx * nx))))
+        out.append((nx, nx, nz))
+    return out
+
+
+def tissue_batch(count: int = 256, b: int = 32, seed: int = 3, lo: int = 2048, hi: int = 65536) -> list:
+    """The batched workload: `count` bonded tissue blocks of log-uniform size, each with its
+    own domain, grid and seeded genome."""
+    return [tissue(b=b, seed=seed * 1000 + k, dims=d) for k, d in enumerate(batch_sizes(count, lo, hi, seed))]
 
 
 WORKLOADS = {

@@ -299,3 +299,50 @@ def test_phase_calls_after_advance(built):
         sim.ComputePressureCL()
         sim.ComputeForceCL()
     compare_all(g, o, "after advance", [F.FPOS, F.FFORCE, F.FPRESS, F.FDENSITY, F.FCONC])
+
+
+def test_batch_matches_single(built):
+    """Instances of three size classes (different genomes, bond counts and grids) stepped as
+    one batch match the same instances stepped alone, bit for bit, through graph replay, odd
+    step counts, an autotuning pass and a second batch call; one instance is also checked
+    against the oracle."""
+    from morphogenesis_amd.batch import Batch
+    from morphogenesis_amd.fluid_system import FluidSystem
+    from oracle.mg_oracle import Oracle
+
+    specs = [(9, 6, 41), (12, 32, 42), (21, 8, 43), (30, 8, 44), (13, 4, 45)]
+    scs = [scenes.tissue(n_side=n, b=b, seed=sd) for n, b, sd in specs]
+    say("case: batch of", [sc.n for sc in scs])
+    solo, inb = [], []
+    for sc in scs:
+        for lst in (solo, inb):
+            s = FluidSystem(sc.n, sc.b)
+            s.load_scene(sc)
+            s.Run2PhysicalSort()
+            lst.append(s)
+    bat = Batch(inb)
+    assert [m for m, _ in bat.classes()] == [3, 1, 1]
+    dt, fl = scs[0].dt, scs[0].flags
+    bat.step(7, dt, fl | F.STEP_GRAPH)
+    for s in solo:
+        s.step(7, dt, fl)
+    chosen = bat.autotune(2, dt, fl)
+    assert all(t in (64, 128, 256) for t in chosen)
+    for s in solo:
+        s.step(6, dt, fl)          # the autotune pass took 3 x 2 steps
+    bat.step(5, dt, fl | F.STEP_GRAPH)
+    bat.sync()
+    for s in solo:
+        s.step(5, dt, fl)
+    bufs = [F.FPOS, F.FVEL, F.FVEVAL, F.FPRESS, F.FDENSITY, F.FCONC, F.FEPIGEN, F.FELASTIDX,
+            F.FGCELL, F.FGNDX, F.FBIN_COUNT, F.FBIN_OFFSET, F.FPARTICLE_ID]
+    for k, (a, b) in enumerate(zip(solo, inb)):
+        assert a.counts() == b.counts(), k
+        for buf in bufs:
+            assert_bits_equal(b.download(buf), a.download(buf), f"instance {k}: {F.BUFFER_NAMES[buf]}")
+    o = Oracle(scs[1].n, scs[1].b)
+    o.load_scene(scs[1])
+    o.Run2PhysicalSort()
+    o.step(18, dt, fl)
+    compare_all(inb[1], o, "batched instance vs oracle", [F.FPOS, F.FVEL, F.FCONC, F.FGCELL])
+    bat.close()
