@@ -23,7 +23,7 @@ sys.path.insert(0, REPO)
 METRIC = ("particle-steps/sec + % HBM peak, 262k bonded tissue; learned 4×dt vs plain at 1e-3")
 HBM_PEAK_GBS = 8000.0
 PHASES = ["pressure", "force", "advance", "diffusion", "genes", "insert", "scan", "place",
-          "sort", "lists", "fill"]
+          "sort", "lists", "fill", "learn"]
 # kernels of the fused step (mg_step) behind each timed phase
 KERNEL_OF_PHASE = {"pressure": "k_pressure", "force": "k_force_advance",
                    "diffusion": "k_diffuse_genes_insert", "scan": "k_scan_lookback",
@@ -155,8 +155,8 @@ def run_batch(args, count: int = 256):
     bat.sync()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
-    ms = (C.c_double * 11)()
-    calls = (C.c_int * 11)()
+    ms = (C.c_double * len(PHASES))()
+    calls = (C.c_int * len(PHASES))()
     bat._call("batch_step_timed", bat.h, args.timed_steps, dt, flags, ms, calls)
     per_step = {ph: ms[i] / args.timed_steps for i, ph in enumerate(PHASES) if calls[i]}
     kern = {"pressure": "kb_pressure", "force": "kb_force_advance",
@@ -294,8 +294,8 @@ def main():
     stats["n_active"] = na
 
     # per-kernel durations with HIP events on the library's own stream
-    ms = (C.c_double * 11)()
-    calls = (C.c_int * 11)()
+    ms = (C.c_double * len(PHASES))()
+    calls = (C.c_int * len(PHASES))()
     lib = sim.lib
     lib.mg_step_timed.argtypes = [C.c_void_p, C.c_int, C.c_float, C.c_uint,
                                   C.POINTER(C.c_double), C.POINTER(C.c_int)]

@@ -219,6 +219,9 @@ struct mg_handle {
     int kcur = 0, icur = 0, ccur = 0, gcur = 0;
     float4* F = nullptr;
     bool force_zero = true;
+    /* learned coarse steps: per-slot velocity correction and the fallback counter */
+    float4* DV = nullptr;
+    uint32_t* fallbacks = nullptr;
     /* home-indexed */
     uint32_t *SLOT = nullptr, *EPIGEN = nullptr, *BPARTNER = nullptr;
     float4* BPAR = nullptr;
@@ -265,6 +268,8 @@ struct mg_handle {
     float graph_dt = 0.f;
     unsigned graph_flags = 0;
     int graph_parity = -1;
+    const void* graph_net = nullptr; /* the net of a learned-step graph, with its version */
+    uint64_t graph_net_version = 0;
 };
 
 namespace {
@@ -599,7 +604,16 @@ int materialize_gene_grid(mg_handle* h)
 
 /* One step of the hot loop with the fused kernels. Starts and ends in the sorted state with
  * FFORCE = 0 (as after CountingSortFull). */
-int do_fused_step(mg_handle* h, float dt, unsigned flags)
+} // namespace
+
+/* learned steps (mg_learn.hip) */
+struct mg_net;
+int launch_mlp_infer(mg_handle* h, const mg_net* net);
+uint64_t net_version(const mg_net* net);
+
+namespace {
+
+int do_fused_step(mg_handle* h, float dt, unsigned flags, const mg_net* net = nullptr)
 {
     const int genes = (flags & MG_STEP_GENES) ? 1 : 0;
     const unsigned nb = nblk(h->N);
@@ -610,9 +624,15 @@ int do_fused_step(mg_handle* h, float dt, unsigned flags)
         d.zero_n = (uint32_t)h->C;
         hipLaunchKernelGGL(k_pressure, dim3(nb), dim3(256), band_lds<0>(256), h->st, d, h->p, h->g);
     }
+    if (net) {
+        PhaseScope ps(h, MG_PH_LEARN);
+        CHK(launch_mlp_infer(h, net));
+    }
     {
         PhaseScope ps(h, MG_PH_FORCE);
-        hipLaunchKernelGGL(k_force_advance, dim3(nb), dim3(256), band_lds<1>(256), h->st, make_ds(h), h->p, h->g, dt);
+        DS d = make_ds(h);
+        d.DVc = net ? h->DV : nullptr;
+        hipLaunchKernelGGL(k_force_advance, dim3(nb), dim3(256), band_lds<1>(256), h->st, d, h->p, h->g, dt);
     }
     {
         PhaseScope ps(h, MG_PH_DIFFUSION);
@@ -665,20 +685,21 @@ int do_fused_step(mg_handle* h, float dt, unsigned flags)
  * period in its buffer roles (icur, gcur flip every step), so a two-step graph leaves the
  * handle's indices where they started. Recaptured when dt, flags, the parities or the
  * handle's epoch change. */
-int graph_steps(mg_handle* h, int pairs, float dt, unsigned flags)
+int graph_steps(mg_handle* h, int pairs, float dt, unsigned flags, const mg_net* net = nullptr)
 {
     const int parity = ((h->cells_fresh && h->grid_scanned) << 4) | (h->kcur << 3) | (h->icur << 2) |
                        (h->ccur << 1) | h->gcur;
+    const uint64_t nv = net ? net_version(net) : 0;
     if (!h->graph || h->graph_epoch != h->epoch || h->graph_dt != dt || h->graph_flags != flags ||
-        h->graph_parity != parity) {
+        h->graph_parity != parity || h->graph_net != (const void*)net || h->graph_net_version != nv) {
         if (h->graph) {
             (void)hipGraphExecDestroy(h->graph);
             h->graph = nullptr;
         }
         hipGraph_t g = nullptr;
         HIPCHK(hipStreamBeginCapture(h->st, hipStreamCaptureModeThreadLocal));
-        int r1 = do_fused_step(h, dt, flags);
-        int r2 = (r1 == MG_OK) ? do_fused_step(h, dt, flags) : r1;
+        int r1 = do_fused_step(h, dt, flags, net);
+        int r2 = (r1 == MG_OK) ? do_fused_step(h, dt, flags, net) : r1;
         hipError_t ec = hipStreamEndCapture(h->st, &g);
         if (r1 != MG_OK || r2 != MG_OK) return r1 != MG_OK ? r1 : r2;
         if (ec != hipSuccess) {
@@ -696,6 +717,8 @@ int graph_steps(mg_handle* h, int pairs, float dt, unsigned flags)
         h->graph_dt = dt;
         h->graph_flags = flags;
         h->graph_parity = parity;
+        h->graph_net = net;
+        h->graph_net_version = nv;
     }
     for (int k = 0; k < pairs; k++) HIPCHK(hipGraphLaunch(h->graph, h->st));
     h->force_zero = true;
@@ -1308,3 +1331,4 @@ int mg_step_timed(mg_handle* h, int nsteps, float dt, unsigned flags, double* ms
 } // extern "C"
 
 #include "mg_batch.hip"
+#include "mg_learn.hip"

@@ -145,6 +145,23 @@ class FluidSystem:
     def Run2GeneAction(self):
         self._call("gene_action", self.h)
 
+    def step_learned(self, net, nsteps: int, dt: float, flags: int = 0):
+        """nsteps learned coarse steps of dt (mg_step_learned): the fused step with the
+        net's velocity correction applied in the advance."""
+        from .learned import _declare
+
+        _declare(self.lib)
+        self._call("step_learned", self.h, net.h, int(nsteps), float(dt), int(flags))
+        self.m_Time += nsteps * dt
+
+    def learned_fallbacks(self) -> int:
+        from .learned import _declare
+
+        _declare(self.lib)
+        c = C.c_uint64()
+        self._call("learned_fallbacks", self.h, C.byref(c))
+        return int(c.value)
+
     def step(self, nsteps: int, dt: float, flags: int = 0):
         self._call("step", self.h, int(nsteps), float(dt), int(flags))
         self.m_Time += nsteps * dt

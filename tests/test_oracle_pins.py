@@ -432,3 +432,41 @@ def test_oracle_deterministic_and_fp64_drift_small(built):
     np.testing.assert_array_equal(runs[0], runs[1])
     rel = np.abs(runs[0].astype(np.float64) - runs[2]).max() / np.abs(runs[2]).max()
     assert rel < 1e-5, rel
+
+
+# ------------------------------------------------------------------ learned-step MLP (FP64 reference)
+def test_mlp_reference_gradient_matches_finite_differences():
+    from oracle import mlp_ref as R
+
+    rng = np.random.default_rng(5)
+    p = rng.normal(0, 0.2, R.NPARAMS)
+    x = rng.normal(0, 1, (7, R.IN))
+    x[:, 50:] = 0.0
+    t = rng.normal(0, 1, (7, R.OUT_REAL))
+    mask = np.array([1, 1, 0, 1, 1, 1, 1], float)
+    loss, g = R.loss_and_grad(p, x, t, mask)
+    idx = rng.choice(R.NPARAMS, 60, replace=False)
+    names = np.repeat([n for n, _ in R.SHAPES], [int(np.prod(s)) for _, s in R.SHAPES])
+    for k in idx:
+        h = 1e-6
+        pp, pm = p.copy(), p.copy()
+        pp[k] += h
+        pm[k] -= h
+        fd = (R.loss_and_grad(pp, x, t, mask)[0] - R.loss_and_grad(pm, x, t, mask)[0]) / (2 * h)
+        assert abs(fd - g[k]) <= 1e-6 + 1e-5 * abs(fd), (names[k], k, fd, g[k])
+    # padding inputs carry no gradient into W1 rows 50.. and masked rows change nothing
+    w = R.unpack(g)
+    assert np.all(w["W1"][50:] == 0)
+    t2 = t.copy()
+    t2[2] += 100.0
+    assert R.loss_and_grad(p, x, t2, mask)[0] == loss
+
+
+def test_bf16_rounding_reference():
+    from oracle import mlp_ref as R
+
+    # exactly representable, halfway ties to even, and the largest finite rounding
+    x = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -8, -2.5, 3.0e38], np.float32)
+    y = R.bf16_round(x)
+    assert y[0] == 1.0 and y[1] == 1.0 and y[2] == 1.0 + 2 ** -6 and y[3] == -2.5
+    assert abs(y[4] - 3.0e38) / 3.0e38 < 2 ** -8
