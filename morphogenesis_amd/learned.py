@@ -221,3 +221,51 @@ def positions_by_id(sim) -> tuple:
     x = sim.download(F.FPOS)[:na, :3].astype(np.float64)
     order = np.argsort(ids)
     return ids[order], x[order]
+
+
+def rollout_errors(scene, net, coarse_steps: int, ratio: int = 4, dt: float | None = None, lib=None,
+                   graph: bool = True) -> dict:
+    """From the scene's sorted start state: `ratio * coarse_steps` fine steps of dt, and
+    `coarse_steps` steps of ratio * dt with and without the network. Positions are compared by
+    particle ID: rel_disp = |x - x_fine| / |x_fine - x_0| (2-norms over all particles) and
+    rel_pos = |x - x_fine| / |x_fine|. Wall times are measured around the stepping calls."""
+    import time
+
+    from .fluid_system import FluidSystem
+
+    dt = scene.dt if dt is None else dt
+    flags = scene.flags | (F.STEP_GRAPH if graph else 0)
+    out = {}
+    sims = {}
+    for name in ("fine", "learned", "coarse"):
+        s = FluidSystem(scene.n, scene.b, lib=lib)
+        s.load_scene(scene)
+        s.Run2PhysicalSort()
+        sims[name] = s
+    ids0, x0 = positions_by_id(sims["fine"])
+    runs = {"fine": lambda s: s.step(ratio * coarse_steps, dt, flags),
+            "learned": lambda s: s.step_learned(net, coarse_steps, ratio * dt, flags),
+            "coarse": lambda s: s.step(coarse_steps, ratio * dt, flags)}
+    for name, run in runs.items():
+        s = sims[name]
+        s.sync()
+        t = time.perf_counter()
+        run(s)
+        s.sync()
+        out[f"{name}_s"] = time.perf_counter() - t
+    ids_f, xf = positions_by_id(sims["fine"])
+    disp = np.linalg.norm(xf - x0[np.searchsorted(ids0, ids_f)])
+    for name in ("learned", "coarse"):
+        ids, x = positions_by_id(sims[name])
+        common, ia, ib = np.intersect1d(ids_f, ids, return_indices=True)
+        d = np.linalg.norm(x[ib] - xf[ia])
+        out[f"{name}_rel_disp"] = float(d / max(disp, 1e-30))
+        out[f"{name}_rel_pos"] = float(d / max(np.linalg.norm(xf[ia]), 1e-30))
+        out[f"{name}_lost"] = int(len(ids_f) - len(common))
+    out["fallbacks"] = sims["learned"].learned_fallbacks()
+    out["speedup_vs_fine"] = out["fine_s"] / out["learned_s"]
+    out["coarse_steps"] = coarse_steps
+    out["ratio"] = ratio
+    for s in sims.values():
+        s.close()
+    return out
