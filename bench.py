@@ -226,6 +226,55 @@ def cpu_baseline_batch(scs, budget_s: float = 15.0) -> dict:
                       f"FP32 oracle (CPU restatement), {threads} threads"}
 
 
+def run_learned(args):
+    """The learned 4 x dt step on the 262,144-particle tissue: train the MLP on the GPU from
+    fine/coarse pairs along a fine trajectory of a second tissue (other seed), then from the
+    same start run 4K fine steps of dt, K learned steps of 4 dt and K plain steps of 4 dt
+    (the ablation) and compare positions by particle ID. `value` counts a learned step as the
+    4 fine steps it replaces (fine-equivalent particle-steps/s)."""
+    from morphogenesis_amd import scenes
+    from morphogenesis_amd.learned import Net, make_training_data, rollout_errors
+
+    def progress(msg):
+        print(f"[bench learned] {msg}", file=sys.stderr, flush=True)
+
+    K = args.steps
+    t0 = time.perf_counter()
+    train = scenes.tissue(64, 32, seed=21)
+    data = make_training_data(train, samples=50, ratio=4, gap=80, warm=0)
+    progress(f"training data {data.X.shape} in {time.perf_counter() - t0:.1f} s")
+    net = Net(seed=5)
+    t1 = time.perf_counter()
+    loss = net.fit(data, steps=3000, batch=16384, lr=1e-3, in_limit=12.0, out_limit=12.0)
+    train_s = time.perf_counter() - t1
+    progress(f"trained in {train_s:.1f} s, loss {loss[:5].mean():.3f} -> {loss[-100:].mean():.3f}")
+    sc = scenes.tissue(64, 32, seed=1)
+    r = rollout_errors(sc, net, K, ratio=4)
+    out = {
+        "metric": METRIC, "value": sc.n * 4 * K / r["learned_s"], "unit": "particle-steps/s",
+        "n_gpus": 1, "steps": K, "warmup": 0, "ms_per_step": r["learned_s"] / K * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16+fp32",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "particles": sc.n, "bonds_per_particle": 32,
+                   "dt_fine": sc.dt, "dt_coarse": 4 * sc.dt, "coarse_steps": K, "hipgraph": True,
+                   "mlp": "50(64)-128-128-3(16) SiLU, bf16 MFMA, fp32 accumulate",
+                   "value_counts": "each learned step as 4 fine-equivalent steps"},
+        "learned": {"speedup_vs_fine": r["speedup_vs_fine"],
+                    "speedup_plain_coarse_vs_fine": r["fine_s"] / r["coarse_s"],
+                    "rel_error_vs_fine_pos": r["learned_rel_pos"],
+                    "rel_error_vs_fine_disp": r["learned_rel_disp"],
+                    "ablation_plain_coarse_rel_error_pos": r["coarse_rel_pos"],
+                    "ablation_plain_coarse_rel_error_disp": r["coarse_rel_disp"],
+                    "fallback_particle_steps": r["fallbacks"], "fine_s": r["fine_s"],
+                    "learned_s": r["learned_s"], "coarse_s": r["coarse_s"],
+                    "train_rows": int(data.X.shape[0]), "train_s": train_s,
+                    "train_loss_first": float(loss[:5].mean()), "train_loss_last": float(loss[-100:].mean()),
+                    "zero_prediction_loss": 3.0},
+    }
+    net.close()
+    print(json.dumps(out))
+
+
 def load_traffic(kernel: str, workload: str):
     p = os.path.join(REPO, "profiles", f"traffic_{kernel}_{workload}.json")
     if not os.path.exists(p):
@@ -264,8 +313,12 @@ def main():
     if args.workload == "tissue_batch_256":
         run_batch(args)
         return
+    if args.workload == "tissue_262k_learned":
+        run_learned(args)
+        return
     builders = {"tissue_262k": lambda: scenes.tissue(64, 32, seed=1),
-                "fluid_block_32k": lambda: scenes.fluid_block(32, seed=1)}
+                "fluid_block_32k": lambda: scenes.fluid_block(32, seed=1),
+                "mixed_4m": lambda: scenes.mixed(4194304, 32, seed=2)}
     scene = builders[args.workload]()
     stats = scene_stats(scene)
 

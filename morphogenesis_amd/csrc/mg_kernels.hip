@@ -24,6 +24,14 @@
  * 262k-particle launch resident at once) */
 #define LAUNCH_BOUNDS_SWEEP __launch_bounds__(256, 4)
 
+/* orders the wavefront's LDS writes before its later LDS reads (and the reverse) */
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 /* ------------------------------------------------------------------ fills */
 __global__ LAUNCH_BOUNDS_256 void k_fill_u32(uint32_t* p, uint32_t v, size_t n)
 {
@@ -253,20 +261,50 @@ __device__ __forceinline__ void place_body(const DS& d, uint32_t bid)
 
 __global__ LAUNCH_BOUNDS_256 void k_place(DS d) { place_body(d, blockIdx.x); }
 
+/* One lane per cell. Cells of up to CELLSORT_SMALL particles are insertion-sorted by their
+ * lane; a larger cell is ranked by the whole wavefront through LDS (each element's rank is
+ * the number of smaller slot indices in the run; the values are distinct), so a crowded
+ * cell costs O(n^2 / 64) per lane instead of O(n^2) on one lane. Runs longer than
+ * CELLSORT_CAP fall back to the owning lane's insertion sort. */
+#define CELLSORT_SMALL 16
+#define CELLSORT_CAP 512
 __device__ __forceinline__ void cellsort_body(const DS& d, uint32_t bid)
 {
+    __shared__ uint32_t buf[4][CELLSORT_CAP];
     const uint32_t c = bid * blockDim.x + threadIdx.x;
-    if (c >= (uint32_t)d.C) return;
-    const uint32_t n = d.CNT[c];
-    if (n == 0) return;
-    uint32_t* run = d.PERM + d.OFF[c];
-    for (uint32_t i = 1; i < n; i++) {
-        uint32_t v = run[i];
-        uint32_t j = i;
-        while (j > 0 && run[j - 1] > v) { run[j] = run[j - 1]; j--; }
-        run[j] = v;
+    const uint32_t n = (c < (uint32_t)d.C) ? d.CNT[c] : 0u;
+    const uint32_t lane = __lane_id(), w = threadIdx.x >> 6;
+    const bool big = n > CELLSORT_SMALL && n <= CELLSORT_CAP;
+    if (n > 1 && !big) {
+        uint32_t* run = d.PERM + d.OFF[c];
+        for (uint32_t i = 1; i < n; i++) {
+            uint32_t v = run[i];
+            uint32_t j = i;
+            while (j > 0 && run[j - 1] > v) { run[j] = run[j - 1]; j--; }
+            run[j] = v;
+        }
+        for (uint32_t r = 0; r < n; r++) d.GNDX[run[r]] = r;
+    } else if (n == 1) {
+        d.GNDX[d.PERM[d.OFF[c]]] = 0u;
     }
-    for (uint32_t r = 0; r < n; r++) d.GNDX[run[r]] = r;
+    uint64_t todo = __ballot(big);
+    while (todo) {
+        const int src = __ffsll((long long)todo) - 1;
+        todo &= todo - 1ull;
+        const uint32_t cb = (uint32_t)__shfl((int)c, src);
+        const uint32_t nb = (uint32_t)__shfl((int)n, src);
+        uint32_t* run = d.PERM + d.OFF[cb];
+        for (uint32_t e = lane; e < nb; e += 64) buf[w][e] = run[e];
+        wave_lds_sync();
+        for (uint32_t e = lane; e < nb; e += 64) {
+            const uint32_t v = buf[w][e];
+            uint32_t rank = 0;
+            for (uint32_t k = 0; k < nb; k++) rank += buf[w][k] < v ? 1u : 0u;
+            run[rank] = v;
+            d.GNDX[v] = rank;
+        }
+        wave_lds_sync();
+    }
 }
 
 __global__ LAUNCH_BOUNDS_256 void k_cellsort(DS d) { cellsort_body(d, blockIdx.x); }
@@ -434,13 +472,6 @@ struct Band {
     float4 x[NX > 0 ? NX : 1][BAND_CAP + BAND_PAD];
 };
 
-/* orders the wavefront's LDS writes before its later LDS reads (and the reverse) */
-__device__ __forceinline__ void wave_lds_sync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 __device__ __forceinline__ uint32_t wave_min(uint32_t v)
 {

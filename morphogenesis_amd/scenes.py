@@ -294,6 +294,71 @@ def tissue(n_side: int = 64, b: int = 32, seed: int = 1, spacing: float = 0.5,
     return Scene(name, n, b, params, grid, genome, bufs, dt, steps, flags, info)
 
 
+def mixed(n_total: int = 4194304, b: int = 32, seed: int = 2, slab_frac: float = 0.7,
+          spacing: float = 0.5, droplet_frac: float = 0.05, k0: float = 10.0, zeta: float = 0.1,
+          limit: float = 2.0, steps: int = 200, dt: float = 1e-4) -> Scene:
+    """A bonded slab (slab_frac of the particles, a square nx x ny x nx jittered lattice
+    resting on the floor, B bond slots) with a fluid column dropped onto it: a cylinder above
+    the slab centre at the slab's spacing falling at 1 unit/s, with droplet_frac of the fluid
+    packed into dense droplets at half the spacing (about 64 particles per cell), so cell
+    occupancy runs from 1 to about 64. Genes and concentrations on every particle. Particle
+    counts are exact: the fluid takes n_total - n_slab."""
+    rng = np.random.default_rng(seed)
+    n_slab_target = int(round(slab_frac * n_total))
+    nx = int(np.sqrt(n_slab_target / 45.0))
+    ny = max(2, int(round(n_slab_target / (nx * nx))))
+    n_slab = nx * nx * ny
+    n_fluid = n_total - n_slab
+    n_drop = int(round(droplet_frac * n_fluid))
+    n_col = n_fluid - n_drop
+    side = (nx - 1) * spacing
+    radius = 0.35 * side
+    col_h = n_col / ((np.pi * radius * radius) / spacing ** 3)
+    vxz = float(np.ceil(side + 5.5))
+    vy = float(np.ceil(2.5 + (ny - 1) * spacing + 2.0 + col_h + 6.0))
+    vmin3, vmax3 = (0.0, 0.0, 0.0), (vxz, vy, vxz)
+    params = make_params(vmin3, vmax3, spacing=spacing)
+    params.diff_scale = 0.01
+    params.secrete_scale = 0.001
+    grid = make_grid(vmin3, vmax3)
+    cx = (params.pboundmin[0] + params.pboundmax[0]) / 2
+    cz = (params.pboundmin[2] + params.pboundmax[2]) / 2
+    y0 = params.pboundmin[1] + 0.25
+    slab = lattice(nx, ny, nx, spacing, (cx - side / 2, y0, cz - side / 2), 0.05, rng)
+    top = y0 + (ny - 1) * spacing + 2.0
+    # column: jittered lattice points inside the cylinder, lowest first, exactly n_col of them
+    m = int(np.ceil(2 * radius / spacing)) + 1
+    k = int(np.ceil(col_h / spacing)) + 3
+    g = lattice(m, k, m, spacing, (cx - radius, top, cz - radius), 0.05, rng)
+    inside = (g[:, 0] - cx) ** 2 + (g[:, 2] - cz) ** 2 <= radius * radius
+    g = g[inside]
+    g = g[np.argsort(g[:, 1], kind="stable")][:n_col]
+    # droplets: dense balls at half the spacing near the top of the column
+    drops = []
+    left = n_drop
+    while left > 0:
+        r = 2.0
+        c = np.array([cx + rng.uniform(-0.5, 0.5) * radius, top + col_h + rng.uniform(1.0, 3.0),
+                      cz + rng.uniform(-0.5, 0.5) * radius])
+        q = int(np.ceil(2 * r / (spacing / 2))) + 1
+        ball = lattice(q, q, q, spacing / 2, c - r, 0.01, rng)
+        ball = ball[((ball - c) ** 2).sum(1) <= r * r][:left]
+        drops.append(ball)
+        left -= ball.shape[0]
+    fluid = np.concatenate([g] + drops).astype(np.float32)
+    pos = np.concatenate([slab, fluid]).astype(np.float32)
+    assert pos.shape[0] == n_total, (pos.shape, n_total)
+    bufs = base_buffers(n_total, b, pos)
+    bufs[F.FVEL][n_slab:, 1] = -1.0
+    bufs[F.FVEVAL][n_slab:, 1] = -1.0
+    info = add_bonds(bufs, slab, b, 1.5 * spacing, k0, zeta, limit, params.pmass, rng)
+    add_genes(bufs, n_total, 0.3, rng)
+    info.update({"n_slab": n_slab, "n_fluid": n_fluid, "n_droplets": n_drop, "slab": (nx, ny, nx),
+                 "column_radius": radius, "column_height": col_h, "seed": seed})
+    return Scene(f"mixed_{n_total}", n_total, b, params, grid, tissue_genome(seed), bufs, dt, steps,
+                 F.STEP_GENES, info)
+
+
 def batch_sizes(count: int = 256, lo: int = 2048, hi: int = 65536, seed: int = 3) -> list:
     """Particle counts drawn log-uniformly in [lo, hi] (seeded), each rounded to a block
     nx x ny x nz of the tissue lattice (nx = ny, nz chosen to land closest)."""

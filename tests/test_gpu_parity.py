@@ -346,3 +346,18 @@ def test_batch_matches_single(built):
     o.step(18, dt, fl)
     compare_all(inb[1], o, "batched instance vs oracle", [F.FPOS, F.FVEL, F.FCONC, F.FGCELL])
     bat.close()
+
+
+def test_mixed_slab_and_fluid_column(built):
+    """The mixed scene (bonded slab + falling fluid column with dense droplets: 1 to ~60
+    particles per cell) at 65,536 particles, 12 steps, bit for bit."""
+    sc = scenes.mixed(65536, 32, seed=2)
+    say("case: mixed slab + column", sc.n, sc.info["n_slab"], sc.info["n_droplets"])
+    g, o = pair(sc)
+    for sim in (g, o):
+        sim.Run2PhysicalSort()
+    compare_all(g, o, "mixed initial sort", [F.FBIN_COUNT, F.FBIN_OFFSET, F.FGCELL, F.FPARTICLE_ID])
+    for sim in (g, o):
+        sim.step(12, sc.dt, sc.flags)
+    compare_all(g, o, "mixed 12 steps", [F.FPOS, F.FVEL, F.FPRESS, F.FDENSITY, F.FCONC, F.FEPIGEN,
+                                         F.FELASTIDX, F.FGCELL, F.FBIN_OFFSET])
