@@ -231,6 +231,7 @@ struct mg_handle {
     uint32_t* FBIN = nullptr;
     uint32_t *INC_OFF = nullptr, *INC = nullptr;
     size_t inc_cap = 0;
+    uint64_t* LIVE = nullptr; /* per home: bit a set when bond slot a holds a partner */
     /* grid */
     uint32_t *CNT[2] = {}, *OFF[2] = {};  /* gcur is current; the fused step bins into 1-gcur */
     uint32_t *GCNT = nullptr, *GOFF = nullptr, *LENS = nullptr;
@@ -358,7 +359,7 @@ DS make_ds(mg_handle* h, int kin = 0, int conc = 0)
     d.HOME2 = h->HOME[io]; d.GCELL2 = h->GCELL[io]; d.GNDX2 = h->GNDX[io]; d.GMASK2 = h->GMASK[io];
     d.SLOT = h->SLOT; d.EPIGEN = h->EPIGEN; d.BPARTNER = h->BPARTNER;
     d.BPAR = h->BPAR; d.BINT = h->BINT;
-    d.INC_OFF = h->INC_OFF; d.INC = h->INC;
+    d.INC_OFF = h->INC_OFF; d.INC = h->INC; d.LIVE = h->LIVE;
     d.CNT = h->CNT[h->gcur]; d.OFF = h->OFF[h->gcur];
     d.CNTN = h->CNT[1 - h->gcur]; d.OFFN = h->OFF[1 - h->gcur];
     d.GCNT = h->GCNT; d.GOFF = h->GOFF;
@@ -395,10 +396,14 @@ int rebuild_inc(mg_handle* h)
         HIPCHK(hipStreamSynchronize(h->st));
     }
     std::vector<uint32_t> off((size_t)N + 1, 0);
+    std::vector<uint64_t> live((size_t)N, 0);
     for (int a = 0; a < B; a++)
         for (int k = 0; k < N; k++) {
             uint32_t j = partner[(size_t)a * N + k];
-            if (j < (uint32_t)N) off[j + 1]++;
+            if (j < (uint32_t)N) {
+                off[j + 1]++;
+                live[k] |= 1ull << a;
+            }
         }
     for (int j = 0; j < N; j++) off[j + 1] += off[j];
     std::vector<uint32_t> ent(std::max<uint32_t>(off[N], 1));
@@ -417,6 +422,7 @@ int rebuild_inc(mg_handle* h)
     h->epoch++;
     HIPCHK(hipMemcpyAsync(h->INC_OFF, off.data(), off.size() * 4, hipMemcpyHostToDevice, h->st));
     HIPCHK(hipMemcpyAsync(h->INC, ent.data(), ent.size() * 4, hipMemcpyHostToDevice, h->st));
+    HIPCHK(hipMemcpyAsync(h->LIVE, live.data(), live.size() * 8, hipMemcpyHostToDevice, h->st));
     HIPCHK(hipStreamSynchronize(h->st));
     return MG_OK;
 }
@@ -855,6 +861,7 @@ int mg_create(mg_handle** out, int max_points, int bonds_per_particle)
     if (r == MG_OK) r = dalloc(h, &h->NERVE, N);
     if (r == MG_OK) r = dalloc(h, &h->FBIN, N);
     if (r == MG_OK) r = dalloc(h, &h->INC_OFF, N + 1);
+    if (r == MG_OK) r = dalloc(h, &h->LIVE, N);
     if (r == MG_OK) r = dalloc(h, &h->PERM, N);
     if (r == MG_OK) r = dalloc(h, &h->GMN, N);
     if (r == MG_OK) r = dalloc(h, &h->LISTS, N * MG_NUM_GENES);
@@ -887,6 +894,7 @@ int mg_create(mg_handle** out, int max_points, int bonds_per_particle)
     fill_u32(h, h->NERVE, 0, N);
     fill_u32(h, h->FBIN, 0, N);
     fill_u32(h, h->INC_OFF, 0, N + 1);
+    fill_u32(h, h->LIVE, 0, N * 2);
     fill_u32(h, h->LISTS, MG_NONE, N * MG_NUM_GENES);
     fill_u32(h, h->LENS, 0, MG_NUM_GENES);
     hipLaunchKernelGGL(k_iota, dim3(nblk(n)), dim3(256), 0, h->st, h->HOME[0], n);

@@ -50,6 +50,7 @@ struct DS {
     float4* BPAR;
     float2* BINT;
     uint32_t *INC_OFF, *INC;
+    uint64_t* LIVE; /* per home: bit a set when bond slot a holds a partner (B <= 64) */
     /* grid (current) and the grid the fused step bins into */
     uint32_t *CNT, *OFF, *GCNT, *GOFF, *PERM, *LISTS, *LENS;
     uint32_t *CNTN, *OFFN;

@@ -689,9 +689,9 @@ __global__ LAUNCH_BOUNDS_SWEEP void k_pressure(DS d, MgParams p, MgGrid g)
  * partners is replaced by each particle gathering the terms of its incoming bonds (same
  * expression, same operands, hence the same bits) in (owner home, slot) order:
  *   FFORCE[i] = ((F0 + own) + in_1) + in_2 ...
- * Updates the bond integrators and breaks bonds as the reference does. Bond slots are
- * scanned 32 at a time for live partners and the live ones fetched 4 at a time; incoming
- * bonds are fetched 4 at a time. */
+ * Updates the bond integrators and breaks bonds as the reference does. The live bond slots
+ * (LIVE, kept with the bond records) are fetched 4 at a time in slot order; incoming bonds
+ * are fetched 4 at a time. */
 __device__ __forceinline__ f3 force_one(const DS& d, const MgParams& p, const MgGrid& g, Band<1>& wb,
                                         bool valid, uint32_t i, uint32_t na, uint32_t gc, f3 F0)
 {
@@ -711,53 +711,50 @@ __device__ __forceinline__ f3 force_one(const DS& d, const MgParams& p, const Mg
     const bool freeze = p.freeze != 0;
 
     f3 force = mk3(0.0f, 0.0f, 0.0f);
-    for (int a32 = 0; valid && a32 < B; a32 += 32) {
-        uint32_t live = 0;
+    /* live slots in groups of 4, ascending */
+    uint64_t live = valid ? d.LIVE[h] : 0ull;
+    while (live) {
+        const int q = (__ffsll((long long)live) - 1) & ~3;
+        const uint32_t g4 = (uint32_t)(live >> q) & 0xFu;
+        live &= ~(0xFull << q);
+        uint32_t j[4];
+        float4 bp[4];
+        float2 in[4];
 #pragma unroll
-        for (int k = 0; k < 32; k++)
-            if (a32 + k < B && d.BPARTNER[(size_t)(a32 + k) * N + h] != MG_NONE) live |= 1u << k;
-#pragma unroll
-        for (int q = 0; q < 32; q += 4) {
-            if (!((live >> q) & 0xFu)) continue;
-            uint32_t j[4];
-            float4 bp[4];
-            float2 in[4];
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                j[k] = MG_NONE;
-                bp[k] = make_float4(0.f, 0.f, 0.f, 0.f);
-                in[k] = make_float2(0.f, 0.f);
-                if (live & (1u << (q + k))) {
-                    const size_t bi = (size_t)(a32 + q + k) * N + h;
-                    j[k] = d.SLOT[d.BPARTNER[bi]];
-                    bp[k] = d.BPAR[bi];
-                    in[k] = d.BINT[bi];
-                }
+        for (int k = 0; k < 4; k++) {
+            j[k] = MG_NONE;
+            bp[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+            in[k] = make_float2(0.f, 0.f);
+            if (g4 & (1u << k)) {
+                const size_t bi = (size_t)(q + k) * N + h;
+                j[k] = d.SLOT[d.BPARTNER[bi]];
+                bp[k] = d.BPAR[bi];
+                in[k] = d.BINT[bi];
             }
-            bool use[4];
-            float4 pj[4], vj[4];
+        }
+        bool use[4];
+        float4 pj[4], vj[4];
 #pragma unroll
-            for (int k = 0; k < 4; k++) {
-                use[k] = j[k] < na && !((double)bp[k].y < 0.000000001);
-                pj[k] = vj[k] = make_float4(0.f, 0.f, 0.f, 0.f);
-                if (use[k]) {
-                    pj[k] = d.P[j[k]];
-                    vj[k] = d.V[j[k]];
-                }
+        for (int k = 0; k < 4; k++) {
+            use[k] = j[k] < na && !((double)bp[k].y < 0.000000001);
+            pj[k] = vj[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (use[k]) {
+                pj[k] = d.P[j[k]];
+                vj[k] = d.V[j[k]];
             }
+        }
 #pragma unroll
-            for (int k = 0; k < 4; k++) {
-                if (!use[k]) continue;
-                const size_t bi = (size_t)(a32 + q + k) * N + h;
-                float strain, abs_dist;
-                f3 eterm = bond_term(ipos, ivev, xyz(pj[k]), xyz(vj[k]), bp[k], p.pmass, &strain, &abs_dist);
-                float2 v = in[k];
-                v.x = v.x + strain * strain;
-                v.y = v.y + strain;
-                d.BINT[bi] = v;
-                if (!has_nan3(eterm)) force = force - eterm;
-                if (abs_dist >= bp[k].x && !freeze) ((float*)&d.BPAR[bi])[1] = 0.0f;
-            }
+        for (int k = 0; k < 4; k++) {
+            if (!use[k]) continue;
+            const size_t bi = (size_t)(q + k) * N + h;
+            float strain, abs_dist;
+            f3 eterm = bond_term(ipos, ivev, xyz(pj[k]), xyz(vj[k]), bp[k], p.pmass, &strain, &abs_dist);
+            float2 v = in[k];
+            v.x = v.x + strain * strain;
+            v.y = v.y + strain;
+            d.BINT[bi] = v;
+            if (!has_nan3(eterm)) force = force - eterm;
+            if (abs_dist >= bp[k].x && !freeze) ((float*)&d.BPAR[bi])[1] = 0.0f;
         }
     }
 
