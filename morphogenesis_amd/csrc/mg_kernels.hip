@@ -653,21 +653,41 @@ __device__ __forceinline__ void pressure_body(const DS& d, const MgParams& p, co
     const f3 pi = valid ? xyz(d.P[i]) : mk3(0.f, 0.f, 0.f);
     const float sr = p.psmoothradius;
     float sum = 0.0f, csum = 0.0f;
-    sweep<8, 0>(
-        d, g, bands[threadIdx.x >> 6], valid, gc, pi, p.r2,
-        [](uint32_t, int) { return make_float4(0.f, 0.f, 0.f, 0.f); },
-        [&](uint32_t, float4, f3, float dsq, const float4*) {
-            float r = sqrtf(dsq);
-            float q = r / sr;
-            float b = 1.0f - q;
-            b *= b;
-            b *= b;
-            csum += b * (4.0f * q + 1.0f);
-        },
-        [&]() {
-            sum += csum;
-            csum = 0.0f;
-        });
+    auto flush = [&]() {
+        sum += csum;
+        csum = 0.0f;
+    };
+    auto none = [](uint32_t, int) { return make_float4(0.f, 0.f, 0.f, 0.f); };
+    /* r / sr for a power-of-two sr is r * (1 / sr) exactly (a scaling by 2^-k), so that case
+     * skips the correctly rounded division; the sums are the same bits either way */
+    const uint32_t srb = __float_as_uint(sr);
+    const bool sr_pow2 = (srb & 0x007FFFFFu) == 0u && ((srb >> 23) & 0xFFu) != 0u && ((srb >> 23) & 0xFFu) != 0xFFu;
+    if (sr_pow2) {
+        const float inv_sr = 1.0f / sr;
+        sweep<8, 0>(
+            d, g, bands[threadIdx.x >> 6], valid, gc, pi, p.r2, none,
+            [&](uint32_t, float4, f3, float dsq, const float4*) {
+                float r = sqrtf(dsq);
+                float q = r * inv_sr;
+                float b = 1.0f - q;
+                b *= b;
+                b *= b;
+                csum += b * (4.0f * q + 1.0f);
+            },
+            flush);
+    } else {
+        sweep<8, 0>(
+            d, g, bands[threadIdx.x >> 6], valid, gc, pi, p.r2, none,
+            [&](uint32_t, float4, f3, float dsq, const float4*) {
+                float r = sqrtf(dsq);
+                float q = r / sr;
+                float b = 1.0f - q;
+                b *= b;
+                b *= b;
+                csum += b * (4.0f * q + 1.0f);
+            },
+            flush);
+    }
     if (!valid) return;
     sum = sum * p.pmass * p.wendlandC2kern;
     if (sum == 0.0f) sum = 1.0f;

@@ -361,3 +361,17 @@ def test_mixed_slab_and_fluid_column(built):
         sim.step(12, sc.dt, sc.flags)
     compare_all(g, o, "mixed 12 steps", [F.FPOS, F.FVEL, F.FPRESS, F.FDENSITY, F.FCONC, F.FEPIGEN,
                                          F.FELASTIDX, F.FGCELL, F.FBIN_OFFSET])
+
+
+def test_smoothing_radius_not_a_power_of_two(built):
+    """psmoothradius = 0.75 takes the correctly rounded r / sr path of the pressure sweep (a
+    power-of-two radius scales exactly); both must match the oracle bit for bit."""
+    sc = scenes.tissue(n_side=8, b=8, seed=17)
+    sc.params.psmoothradius = 0.75
+    sc.params.r2 = 0.75 * 0.75
+    say("case: smoothing radius 0.75")
+    g, o = pair(sc)
+    for sim in (g, o):
+        sim.Run2PhysicalSort()
+        sim.step(10, sc.dt, sc.flags)
+    compare_all(g, o, "sr 0.75", [F.FPOS, F.FVEL, F.FPRESS, F.FDENSITY, F.FCONC])
