@@ -316,6 +316,10 @@ int batch_check(mg_batch* b)
 int batch_steps(mg_batch* b, int nsteps, float dt, unsigned flags, bool use_graph)
 {
     CHK(batch_check(b));
+    for (mg_handle* h : b->hs) {
+        CHK(maybe_rehome(h));
+        h->steps_since_rehome += nsteps;
+    }
     flags &= MG_STEP_GENES;
     int q = batch_parity(b, dt, flags);
     if (q < 0) {

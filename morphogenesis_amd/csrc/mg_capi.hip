@@ -9,6 +9,7 @@
 #include "mg_kernels.hip"
 
 #include <algorithm>
+#include <type_traits>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -80,6 +81,36 @@ __global__ void k_iota(uint32_t* p, int n)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = (uint32_t)i;
+}
+/* ---------------------------------------------------------- re-homing
+ * A permutation sigma of the home indices: every home-indexed record moves from h to
+ * sigma[h] and every stored home value v becomes sigma[v]. */
+template <typename T>
+__global__ void k_scatter_home(T* dst, const T* src, const uint32_t* sigma, int n, int rows)
+{
+    int h = blockIdx.x * blockDim.x + threadIdx.x;
+    int r = blockIdx.y;
+    if (h >= n || r >= rows) return;
+    dst[(size_t)r * n + sigma[h]] = src[(size_t)r * n + h];
+}
+__global__ void k_rehome_bonds(uint32_t* nPartner, uint2* nPidx, const uint32_t* Partner, const uint2* Pidx,
+                               const uint32_t* sigma, int n)
+{
+    int h = blockIdx.x * blockDim.x + threadIdx.x;
+    int a = blockIdx.y;
+    if (h >= n) return;
+    const size_t o = (size_t)a * n + h, no = (size_t)a * n + sigma[h];
+    const uint32_t jh = Partner[o];
+    nPartner[no] = (jh < (uint32_t)n) ? sigma[jh] : jh;
+    const uint2 pi = Pidx[o];
+    nPidx[no] = make_uint2(pi.x < (uint32_t)n ? sigma[pi.x] : pi.x, pi.y);
+}
+__global__ void k_map_values(uint32_t* v, const uint32_t* sigma, int n)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const uint32_t x = v[s];
+    if (x < (uint32_t)n) v[s] = sigma[x];
 }
 /* home-indexed u32 field -> sorted order; slots >= tail get `reset` unless keep_stale */
 __global__ void k_gather_home(uint32_t* dst, const uint32_t* src, const uint32_t* HOME,
@@ -251,6 +282,10 @@ struct mg_handle {
     void* staging = nullptr;
     size_t staging_bytes = 0;
     bool normalized = true;
+    /* re-homing: due after a sort outside the fused step (the homes no longer follow the
+     * slots) and every kRehomeSteps fused steps */
+    bool rehome_due = false;
+    int steps_since_rehome = 0;
     double time = 0.0;
     std::vector<void*> allocs;
     /* phase timing (mg_step_timed) */
@@ -486,6 +521,116 @@ int normalize(mg_handle* h)
     return rebuild_inc(h);
 }
 
+/* Re-homing: after a sort the particles' records stay at the homes they had at upload, so a
+ * wavefront's bond-record reads scatter. rehome() renumbers the homes to the current sorted
+ * slots (the particles outside the sorted region take the remaining numbers in order), moves
+ * every home-indexed record, maps every stored home value, and relocates each particle's
+ * incoming-bond row keeping its entries in their order, so the force gathers the same terms
+ * in the same order: no result changes. */
+const int kRehomeSteps = 2048;
+
+int rehome(mg_handle* h)
+{
+    const int N = h->N, B = h->B;
+    std::vector<uint32_t> slot(N);
+    HIPCHK(hipMemcpyAsync(slot.data(), h->SLOT, (size_t)N * 4, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    std::vector<uint32_t> sigma(N, MG_NONE);
+    std::vector<char> used(N, 0);
+    for (int k = 0; k < N; k++) {
+        const uint32_t sl = slot[k];
+        if (sl >= (uint32_t)N) continue;
+        if (used[sl]) return MG_OK; /* not a permutation: leave the homes as they are */
+        used[sl] = 1;
+        sigma[k] = sl;
+    }
+    uint32_t next = 0;
+    bool ident = true;
+    for (int k = 0; k < N; k++) {
+        if (sigma[k] == MG_NONE) {
+            while (used[next]) next++;
+            sigma[k] = next;
+            used[next] = 1;
+        }
+        if (sigma[k] != (uint32_t)k) ident = false;
+    }
+    h->rehome_due = false;
+    h->steps_since_rehome = 0;
+    if (ident) return MG_OK;
+    uint32_t* dsig = nullptr;
+    CHK(dalloc(h, &dsig, N));
+    HIPCHK(hipMemcpyAsync(dsig, sigma.data(), (size_t)N * 4, hipMemcpyHostToDevice, h->st));
+    const unsigned nb = nblk(N);
+    auto move = [&](auto** field, int rows) -> int {
+        using T = std::remove_pointer_t<std::remove_reference_t<decltype(*field)>>;
+        T* nf = nullptr;
+        CHK(dalloc(h, &nf, (size_t)rows * N));
+        hipLaunchKernelGGL(k_scatter_home<T>, dim3(nb, rows), dim3(256), 0, h->st, nf, (const T*)*field, dsig, N, rows);
+        HIPCHK(hipGetLastError());
+        dfree(h, *field);
+        *field = nf;
+        return MG_OK;
+    };
+    CHK(move(&h->SLOT, 1));
+    CHK(move(&h->AGE, 1));
+    CHK(move(&h->COLOR, 1));
+    CHK(move(&h->PID, 1));
+    CHK(move(&h->MRAD, 1));
+    CHK(move(&h->NERVE, 1));
+    CHK(move(&h->LIVE, 1));
+    CHK(move(&h->EPIGEN, MG_NUM_GENES));
+    if (B > 0) {
+        CHK(move(&h->BPAR, B));
+        CHK(move(&h->BINT, B));
+        CHK(move(&h->BCOLD, B));
+        uint32_t* nP = nullptr;
+        uint2* nPidx = nullptr;
+        CHK(dalloc(h, &nP, (size_t)B * N));
+        CHK(dalloc(h, &nPidx, (size_t)B * N));
+        hipLaunchKernelGGL(k_rehome_bonds, dim3(nb, B), dim3(256), 0, h->st, nP, nPidx, h->BPARTNER, h->PIDX, dsig, N);
+        HIPCHK(hipGetLastError());
+        dfree(h, h->BPARTNER);
+        dfree(h, h->PIDX);
+        h->BPARTNER = nP;
+        h->PIDX = nPidx;
+    }
+    for (int k = 0; k < 2; k++)
+        hipLaunchKernelGGL(k_map_values, dim3(nb), dim3(256), 0, h->st, h->HOME[k], dsig, N);
+    HIPCHK(hipGetLastError());
+    /* incoming-bond rows follow their particle, entries in the same order */
+    std::vector<uint32_t> off((size_t)N + 1);
+    HIPCHK(hipMemcpyAsync(off.data(), h->INC_OFF, off.size() * 4, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    std::vector<uint32_t> ent(std::max<uint32_t>(off[N], 1));
+    if (off[N]) HIPCHK(hipMemcpyAsync(ent.data(), h->INC, (size_t)off[N] * 4, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    std::vector<uint32_t> inv(N);
+    for (int k = 0; k < N; k++) inv[sigma[k]] = (uint32_t)k;
+    std::vector<uint32_t> noff((size_t)N + 1, 0), nent(ent.size());
+    for (int q = 0; q < N; q++) {
+        const uint32_t k = inv[q];
+        uint32_t o = noff[q];
+        for (uint32_t e = off[k]; e < off[k + 1]; e++) {
+            const uint32_t packed = ent[e];
+            const uint32_t kh = packed / (uint32_t)B, b = packed - kh * (uint32_t)B;
+            nent[o++] = sigma[kh] * (uint32_t)B + b;
+        }
+        noff[q + 1] = o;
+    }
+    HIPCHK(hipMemcpyAsync(h->INC_OFF, noff.data(), noff.size() * 4, hipMemcpyHostToDevice, h->st));
+    if (off[N]) HIPCHK(hipMemcpyAsync(h->INC, nent.data(), (size_t)off[N] * 4, hipMemcpyHostToDevice, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    dfree(h, dsig);
+    h->epoch++;
+    return MG_OK;
+}
+
+int maybe_rehome(mg_handle* h)
+{
+    if (h->force_zero && (h->rehome_due || h->steps_since_rehome >= kRehomeSteps)) return rehome(h);
+    return MG_OK;
+}
+
 /* dynamic LDS of a sweep kernel: one Band<NX> per wavefront of the block */
 template <int NX> constexpr size_t band_lds(unsigned threads) { return (threads / 64) * sizeof(Band<NX>); }
 
@@ -577,6 +722,7 @@ int do_sort(mg_handle* h, int lists)
     h->icur = 1 - h->icur;
     h->ccur = 1 - h->ccur;
     h->cells_fresh = h->pos_binned;
+    h->rehome_due = true;
     h->force_zero = true;
     h->normalized = false;
     if (lists) {
@@ -1286,6 +1432,8 @@ int mg_step(mg_handle* h, int nsteps, float dt, unsigned flags)
 {
     CHK(ready(h));
     if (nsteps < 0) { set_err("mg_step: nsteps < 0"); return MG_ERR_ARG; }
+    CHK(maybe_rehome(h));
+    h->steps_since_rehome += nsteps;
     int s = 0;
     if ((flags & MG_STEP_GRAPH) && h->force_zero && nsteps >= 2) {
         const int pairs = nsteps / 2;

@@ -415,6 +415,8 @@ int mg_step_learned(mg_handle* h, mg_net* n, int nsteps, float dt, unsigned flag
     }
     HIPCHK(hipStreamSynchronize(n->st));
     CHK(ensure_learn_buffers(h));
+    CHK(maybe_rehome(h));
+    h->steps_since_rehome += nsteps;
     int s = 0;
     if ((flags & MG_STEP_GRAPH) && nsteps >= 2) {
         const int pairs = nsteps / 2;
