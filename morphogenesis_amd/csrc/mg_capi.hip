@@ -421,14 +421,23 @@ int ready(mg_handle* h)
 }
 
 /* Rebuild the incoming-bond CSR from BPARTNER (home space): entries of particle j are the
- * (owner home, slot) pairs whose partner is j, ascending. */
+ * (owner home, slot) pairs whose partner is j, ordered by the owner's particle ID, then its
+ * home, then the slot. The ID order makes the force's summation order a property of the
+ * state itself: homes are renumbered by uploads and re-homing, IDs travel with particles. */
 int rebuild_inc(mg_handle* h)
 {
     const int N = h->N, B = h->B;
     std::vector<uint32_t> partner((size_t)N * B);
+    std::vector<uint32_t> owners((size_t)N);
+    for (int k = 0; k < N; k++) owners[k] = (uint32_t)k;
     if (B > 0) {
+        std::vector<uint32_t> pid((size_t)N);
         HIPCHK(hipMemcpyAsync(partner.data(), h->BPARTNER, partner.size() * 4, hipMemcpyDeviceToHost, h->st));
+        HIPCHK(hipMemcpyAsync(pid.data(), h->PID, pid.size() * 4, hipMemcpyDeviceToHost, h->st));
         HIPCHK(hipStreamSynchronize(h->st));
+        if (!std::is_sorted(pid.begin(), pid.end()))
+            std::stable_sort(owners.begin(), owners.end(),
+                             [&](uint32_t x, uint32_t y) { return pid[x] < pid[y]; });
     }
     std::vector<uint32_t> off((size_t)N + 1, 0);
     std::vector<uint64_t> live((size_t)N, 0);
@@ -443,7 +452,7 @@ int rebuild_inc(mg_handle* h)
     for (int j = 0; j < N; j++) off[j + 1] += off[j];
     std::vector<uint32_t> ent(std::max<uint32_t>(off[N], 1));
     std::vector<uint32_t> fillp(off.begin(), off.end() - 1);
-    for (int k = 0; k < N; k++)
+    for (uint32_t k : owners)
         for (int a = 0; a < B; a++) {
             uint32_t j = partner[(size_t)a * N + k];
             if (j < (uint32_t)N) ent[fillp[j]++] = (uint32_t)k * (uint32_t)B + (uint32_t)a;
@@ -1229,7 +1238,10 @@ int mg_upload(mg_handle* h, int buf_id, const void* src, size_t bytes)
     case MG_FGCELL: HIPCHK(hipMemcpyAsync(h->GCELL[ic], stg, bytes, hipMemcpyDeviceToDevice, h->st)); break;
     case MG_FGNDX: HIPCHK(hipMemcpyAsync(h->GNDX[ic], stg, bytes, hipMemcpyDeviceToDevice, h->st)); break;
     case MG_FNERVEIDX: HIPCHK(hipMemcpyAsync(h->NERVE, stg, bytes, hipMemcpyDeviceToDevice, h->st)); break;
-    case MG_FPARTICLE_ID: HIPCHK(hipMemcpyAsync(h->PID, stg, bytes, hipMemcpyDeviceToDevice, h->st)); break;
+    case MG_FPARTICLE_ID:
+        HIPCHK(hipMemcpyAsync(h->PID, stg, bytes, hipMemcpyDeviceToDevice, h->st));
+        if (B > 0) CHK(rebuild_inc(h)); /* the incoming-bond order follows the owners' IDs */
+        break;
     case MG_FMASS_RADIUS: HIPCHK(hipMemcpyAsync(h->MRAD, stg, bytes, hipMemcpyDeviceToDevice, h->st)); break;
     case MG_FBIN: {
         HIPCHK(hipMemcpyAsync(h->FBIN, stg, bytes, hipMemcpyDeviceToDevice, h->st));

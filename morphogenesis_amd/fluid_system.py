@@ -29,6 +29,8 @@ class FluidSystem:
         self.h = C.c_void_p()
         self._call("create", C.byref(self.h), self.n, self.b)
         self.m_Time = 0.0
+        self.params = self.grid = self.genome = None
+        self.sim_params = None
 
     # ------------------------------------------------------------------ plumbing
     def _call(self, name, *args):
@@ -55,12 +57,15 @@ class FluidSystem:
     def set_grid(self, grid: F.MgGrid):
         self._call("set_grid", self.h, C.byref(grid))
         self.c = int(grid.total)
+        self.grid = grid
 
     def set_params(self, params: F.MgParams):
         self._call("set_params", self.h, C.byref(params))
+        self.params = params
 
     def set_genome(self, genome: F.MgGenome):
         self._call("set_genome", self.h, C.byref(genome))
+        self.genome = genome
 
     def setFreeze(self, freeze: bool):
         self._call("set_freeze", self.h, int(bool(freeze)))
@@ -165,3 +170,70 @@ class FluidSystem:
     def step(self, nsteps: int, dt: float, flags: int = 0):
         self._call("step", self.h, int(nsteps), float(dt), int(flags))
         self.m_Time += nsteps * dt
+
+    # ------------------------------------------------------------------ reference files
+    @classmethod
+    def from_reference_files(cls, sim_params_path: str, points_path: str,
+                             genome_path: str | None = None, lib=None,
+                             check_bounds: bool = True) -> "FluidSystem":
+        """ReadSimParams + ReadGenome + ReadPointsCSV2 (src/fileCL_IO.cpp:71-157,336-530):
+        a system sized to the point file (mMaxPoints = its rows, B from its header). With
+        check_bounds, a particle outside PBOUNDMIN/MAX or faster than PVEL_LIMIT is an
+        error, as in the reference's reader."""
+        from . import fileio
+
+        sc = fileio.load_scene(sim_params_path, points_path, genome_path,
+                               check_bounds=check_bounds)
+        sim = cls(sc.n, sc.b, lib=lib)
+        sim.load_scene(sc)
+        sim.m_Time = sc.info["time"]
+        sim.m_DT = sc.dt
+        sim.sim_params = sc.info["sim_params"]
+        return sim
+
+    def SavePointsCSV2(self, directory: str, frame: int) -> str:
+        from . import fileio
+
+        return fileio.save_points_csv2(self.TransferFromCL(), directory, frame)
+
+    def WriteGenome(self, directory: str) -> str:
+        from . import fileio
+
+        if self.genome is None:
+            raise F.MgError("WriteGenome: no genome set")
+        return fileio.write_genome(self.genome, directory)
+
+    def WriteSimParams(self, directory: str) -> str:
+        from . import fileio
+
+        if self.sim_params is None:
+            raise F.MgError("WriteSimParams: the system was not set up from SimParams")
+        sp = self.sim_params.copy()
+        sp.time = self.m_Time
+        return fileio.write_sim_params(sp, directory)
+
+    def save_checkpoint(self, path: str) -> str:
+        from . import fileio
+
+        return fileio.save_checkpoint(self, path, genome=self.genome, params=self.params,
+                                      grid=self.grid)
+
+    @classmethod
+    def from_checkpoint(cls, path: str, lib=None) -> "FluidSystem":
+        """A system in the state save_checkpoint wrote; stepping it continues the saved run
+        bit for bit."""
+        from . import fileio
+
+        n, b, bufs, params, grid, genome, time = fileio.load_checkpoint(path)
+        if params is None or grid is None or genome is None:
+            raise F.MgError(f"{path}: checkpoint lacks parameters, grid or genome")
+        sim = cls(n, b, lib=lib)
+        sim.set_grid(grid)
+        sim.set_params(params)
+        sim.set_genome(genome)
+        sim.TransferToCL(bufs)
+        # the reference's run loop starts with a physical sort; on a sorted state it moves
+        # nothing and restores the point counts
+        sim.Run2PhysicalSort()
+        sim.m_Time = time
+        return sim

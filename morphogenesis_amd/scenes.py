@@ -91,7 +91,7 @@ def empty_genome() -> F.MgGenome:
 
 
 def example_genome() -> F.MgGenome:
-    """SetupExampleGenome (src/fluid_system.cpp:2415-2436)."""
+    """SetupExampleGenome (src/fluid_system.cpp:2415-2519)."""
     g = F.MgGenome()
     for i in range(F.NUM_GENES):
         g.delay[i] = 1
@@ -105,6 +105,20 @@ def example_genome() -> F.MgGenome:
     g.secrete[0][2], g.secrete[0][3] = 3, 4
     g.activate[0][2 * F.NUM_GENES] = 1
     g.activate[0][0], g.activate[0][1] = 5, 6
+    # remodelling rows elastin, collagen, apatite (src/fluid_system.cpp:2441-2486), columns in
+    # FGenome::params order: elongation_threshold, elongation_factor, strength_threshold,
+    # strengthening_factor, max/min_rest_length, max/min_modulus, elastLim,
+    # default_rest_length, default_modulus, default_damping
+    rows = [(0.5, 0.002, 1.0, 0.002, 1.0, 0.3, 1e5, 10, 8, 0.5, 1000, 10),
+            (4.0, 0.01, 4.1, 0.01, 1.0, 0.3, 1e9, 1e6, 0.55, 0.5, 1e7, 100),
+            (1.0, 0.001, 1.0, 0.001, 1.0, 0.3, 1e9, 1e6, 0.05, 0.5, 1e7, 1000)]
+    for r, vals in enumerate(rows):
+        for k, v in enumerate(vals):
+            g.param[r][k] = v
+    # tanh remodelling curve of elastin's strengthening (src/fluid_system.cpp:2489-2497);
+    # collagen and apatite rows stay zero
+    for k, v in enumerate((0.0, 0.0, 0.0, 0.0, 1.008, 0.01, 10.0, 7.0)):
+        g.tanh_param[0][k] = v
     return g
 
 

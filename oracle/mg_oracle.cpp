@@ -14,8 +14,10 @@
  *    src/fluid_system_cuda.cu:81 becomes the stable rank of the particle in its cell;
  *  - the bond-force scatter (atomicAdd into FFORCE[j], src/fluid_system_cuda.cu:3311-3313,
  *    3354-3356) is applied as an ordered sum: FFORCE[i] += own total, then += each
- *    incoming bond's term in (owner home index, slot) order, where "home index" is the
- *    slot a particle occupied when the oracle was created;
+ *    incoming bond's term in (owner FPARTICLE_ID, owner home index, slot) order, where
+ *    "home index" is the slot a particle occupied when the oracle was created and only
+ *    breaks ties between equal IDs. With unique IDs the order is a function of the state
+ *    buffers alone, so a state downloaded and uploaded again continues bit for bit;
  *  - diffusion is a Jacobi update (reads the pre-step concentrations) instead of the
  *    in-place race of src/fluid_system_cuda.cu:3102-3104;
  *  - gene action uses float FCONC in AoS layout and applies the tally once per living
@@ -526,22 +528,21 @@ template <typename R> struct Oracle : Base {
             own[3 * (size_t)i] = force.x; own[3 * (size_t)i + 1] = force.y; own[3 * (size_t)i + 2] = force.z;
         }
 
-        /* canonical incoming order: owners by home index, then slot */
-        std::vector<uint32_t> slot_of_home(N, MG_NONE);
-        for (int s = 0; s < na; s++) slot_of_home[home[s]] = (uint32_t)s;
+        /* canonical incoming order: owners by (particle ID, home index), then slot */
+        std::vector<uint32_t> owners((size_t)na);
+        for (int s = 0; s < na; s++) owners[s] = (uint32_t)s;
+        std::sort(owners.begin(), owners.end(), [&](uint32_t x, uint32_t y) {
+            return pid[x] != pid[y] ? pid[x] < pid[y] : home[x] < home[y];
+        });
         std::vector<uint32_t> icnt(na + 1, 0);
-        for (int h = 0; h < N; h++) {
-            uint32_t k = slot_of_home[h];
-            if (k == MG_NONE) continue;
+        for (uint32_t k : owners) {
             for (int a = 0; a < B; a++)
                 if (ok[(size_t)k * B + a]) icnt[bu[brec(k, a) + MG_B_INDEX] + 1]++;
         }
         for (int j = 0; j < na; j++) icnt[j + 1] += icnt[j];
         std::vector<uint32_t> ient(icnt[na]);
         std::vector<uint32_t> fill(icnt.begin(), icnt.end() - 1);
-        for (int h = 0; h < N; h++) {
-            uint32_t k = slot_of_home[h];
-            if (k == MG_NONE) continue;
+        for (uint32_t k : owners) {
             for (int a = 0; a < B; a++)
                 if (ok[(size_t)k * B + a]) ient[fill[bu[brec(k, a) + MG_B_INDEX]]++] = k * B + a;
         }
