@@ -707,7 +707,8 @@ __global__ LAUNCH_BOUNDS_SWEEP void k_pressure(DS d, MgParams p, MgGrid g)
  * cell gc (not GRID_UNDEF). The owner subtracts its bond terms in slot order and adds pmass *
  * (sum over cells of the per-cell fluid sums); the atomicAdd scatter of bond terms to
  * partners is replaced by each particle gathering the terms of its incoming bonds (same
- * expression, same operands, hence the same bits) in (owner home, slot) order:
+ * expression, same operands, hence the same bits) in the incoming CSR's order (owner particle
+ * ID, owner home, slot):
  *   FFORCE[i] = ((F0 + own) + in_1) + in_2 ...
  * Updates the bond integrators and breaks bonds as the reference does. The live bond slots
  * (LIVE, kept with the bond records) are fetched 4 at a time in slot order; incoming bonds
