@@ -88,7 +88,9 @@ void fill_inst(BatchInst& t, mg_handle* h, const MgGenome* dgen, float dt, unsig
     t.pressure = make_ds(h);
     t.pressure.ZERO = h->CNT[1 - h->gcur];
     t.pressure.zero_n = (uint32_t)h->C;
+    t.pressure.HMW = hit_masks(h);
     t.force = make_ds(h);
+    t.force.HMR = t.pressure.HMW;
     t.diffuse = make_ds(h, 1, 0);
     t.diffuse.GMN = h->GMN;
     t.diffuse.cull = 0;
@@ -319,6 +321,7 @@ int batch_steps(mg_batch* b, int nsteps, float dt, unsigned flags, bool use_grap
     for (mg_handle* h : b->hs) {
         CHK(maybe_rehome(h));
         h->steps_since_rehome += nsteps;
+        ensure_hit_masks(h);
     }
     flags &= MG_STEP_GENES;
     int q = batch_parity(b, dt, flags);

@@ -263,6 +263,8 @@ struct mg_handle {
     uint32_t *INC_OFF = nullptr, *INC = nullptr;
     size_t inc_cap = 0;
     uint64_t* LIVE = nullptr; /* per home: bit a set when bond slot a holds a partner */
+    uint64_t* HM = nullptr;   /* hit masks, pressure -> force of a fused step: [row][slot] */
+    int hm_rows = 0;          /* rows HM holds per slot */
     /* grid */
     uint32_t *CNT[2] = {}, *OFF[2] = {};  /* gcur is current; the fused step bins into 1-gcur */
     uint32_t *GCNT = nullptr, *GOFF = nullptr, *LENS = nullptr;
@@ -405,6 +407,8 @@ DS make_ds(mg_handle* h, int kin = 0, int conc = 0)
     d.rows = (h->grid_scanned && h->adj_rows) ? 1 : 0;
     d.cull = (d.rows && h->cells_fresh) ? 1 : 0;
     d.DVc = nullptr;
+    d.HMW = nullptr;
+    d.HMR = nullptr;
     d.ctr = h->ctr;
     d.N = h->N; d.B = h->B; d.C = h->C;
     return d;
@@ -774,15 +778,44 @@ uint64_t net_version(const mg_net* net);
 
 namespace {
 
+/* Hit masks for the fused step: the force sweep reuses the pressure sweep's neighbour sets
+ * when both sweeps walk rows of the sorted grid (rows of srch <= 4 cells) and test the same
+ * cut-off (pressure: dsq < r2, force: dsq < 1). ensure_hit_masks allocates the buffer outside
+ * any stream capture (a new buffer bumps the epoch, so captured graphs are rebuilt);
+ * hit_masks returns it when the current state may use it. */
+bool hm_eligible(const mg_handle* h) { return h->adj_rows && h->p.r2 == 1.0f && h->g.srch <= 4; }
+
+void ensure_hit_masks(mg_handle* h)
+{
+    const int nseg = h->g.srch * h->g.srch;
+    if (!hm_eligible(h) || h->hm_rows >= nseg) return;
+    if (h->HM) dfree(h, h->HM);
+    h->HM = nullptr;
+    h->hm_rows = 0;
+    h->epoch++;
+    if (dalloc(h, &h->HM, (size_t)nseg * h->N) != MG_OK) {
+        h->HM = nullptr; /* no masks: the force sweep tests its candidates itself */
+        return;
+    }
+    h->hm_rows = nseg;
+}
+
+uint64_t* hit_masks(const mg_handle* h)
+{
+    return (h->grid_scanned && hm_eligible(h) && h->hm_rows >= h->g.srch * h->g.srch) ? h->HM : nullptr;
+}
+
 int do_fused_step(mg_handle* h, float dt, unsigned flags, const mg_net* net = nullptr)
 {
     const int genes = (flags & MG_STEP_GENES) ? 1 : 0;
     const unsigned nb = nblk(h->N);
+    uint64_t* hm = hit_masks(h);
     {
         PhaseScope ps(h, MG_PH_PRESSURE);
         DS d = make_ds(h);
         d.ZERO = h->CNT[1 - h->gcur];
         d.zero_n = (uint32_t)h->C;
+        d.HMW = hm;
         hipLaunchKernelGGL(k_pressure, dim3(nb), dim3(256), band_lds<0>(256), h->st, d, h->p, h->g);
     }
     if (net) {
@@ -793,6 +826,7 @@ int do_fused_step(mg_handle* h, float dt, unsigned flags, const mg_net* net = nu
         PhaseScope ps(h, MG_PH_FORCE);
         DS d = make_ds(h);
         d.DVc = net ? h->DV : nullptr;
+        d.HMR = hm;
         hipLaunchKernelGGL(k_force_advance, dim3(nb), dim3(256), band_lds<1>(256), h->st, d, h->p, h->g, dt);
     }
     {
@@ -848,6 +882,7 @@ int do_fused_step(mg_handle* h, float dt, unsigned flags, const mg_net* net = nu
  * handle's epoch change. */
 int graph_steps(mg_handle* h, int pairs, float dt, unsigned flags, const mg_net* net = nullptr)
 {
+    ensure_hit_masks(h); /* allocates (and bumps the epoch) before any capture, not inside it */
     const int parity = ((h->cells_fresh && h->grid_scanned) << 4) | (h->kcur << 3) | (h->icur << 2) |
                        (h->ccur << 1) | h->gcur;
     const uint64_t nv = net ? net_version(net) : 0;
@@ -1446,6 +1481,7 @@ int mg_step(mg_handle* h, int nsteps, float dt, unsigned flags)
     if (nsteps < 0) { set_err("mg_step: nsteps < 0"); return MG_ERR_ARG; }
     CHK(maybe_rehome(h));
     h->steps_since_rehome += nsteps;
+    ensure_hit_masks(h);
     int s = 0;
     if ((flags & MG_STEP_GRAPH) && h->force_zero && nsteps >= 2) {
         const int pairs = nsteps / 2;

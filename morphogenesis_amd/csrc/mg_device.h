@@ -69,6 +69,12 @@ struct DS {
     /* learned coarse step: per-slot velocity correction (xyz) and use flag (w != 0) added
      * by the advance; null on plain steps */
     const float4* DVc;
+    /* Hit masks handed from the pressure sweep to the force sweep of the same step (same
+     * positions, same cut-off): row s of slot i at [s * N + i], bit k = slot OFF[c0] + k is
+     * a neighbour; ~0 = the row's window exceeds 64 slots (the force tests that row). Null
+     * when not in use. */
+    uint64_t* HMW;
+    const uint64_t* HMR;
     uint32_t* ctr;
     int N, B, C;
 };

@@ -523,10 +523,10 @@ __device__ __forceinline__ void cell_candidates(uint32_t lo, uint32_t hi, f3 pi,
 
 /* Must be called by all 64 lanes of the wavefront (`valid` selects the lanes that sweep);
  * xload(j, q) is the global load of extra field q of slot j. */
-template <int K, int NX, class XLoad, class Visit, class EndCell>
+template <int K, int NX, int HM = 0, class XLoad, class Visit, class EndCell>
 __device__ __forceinline__ void sweep(const DS& d, const MgGrid& g, Band<NX>& wb, bool valid,
                                       uint32_t gc0, f3 pi, float lim, XLoad xload, Visit visit,
-                                      EndCell end_cell)
+                                      EndCell end_cell, uint32_t self = 0u)
 {
     constexpr int NXX = NX > 0 ? NX : 1;
     const uint32_t lane = __lane_id();
@@ -582,8 +582,16 @@ __device__ __forceinline__ void sweep(const DS& d, const MgGrid& g, Band<NX>& wb
     bool fits = band_of(0, blo, bhi);
     fetch(fits, blo, bhi);
     RowBounds nxt = row_bounds(d, g, valid ? gc0 : cmin, 0);
+    /* HM == 1: record the hits of each row (pressure); HM == 2: visit the recorded hits
+     * instead of testing the candidates (force) */
+    const bool hm_w = HM == 1 && rows && d.HMW != nullptr;
+    const bool hm_r = HM == 2 && rows && d.HMR != nullptr;
+    uint64_t hm_next = ~0ull;
+    if (hm_r && valid) hm_next = d.HMR[self];
     for (int s = 0; s < nseg; s++) {
         const RowBounds rb = nxt;
+        const uint64_t hm_row = hm_next;
+        if (hm_r && valid && s + 1 < nseg) hm_next = d.HMR[(size_t)(s + 1) * N + self];
         const uint32_t cblo = blo, cbhi = bhi;
         const bool cfits = fits;
         if (cfits) {
@@ -606,6 +614,52 @@ __device__ __forceinline__ void sweep(const DS& d, const MgGrid& g, Band<NX>& wb
             nxt = row_bounds(d, g, valid ? gc0 : cmin, s + 1);
         }
         if (!valid) continue;
+        if (hm_r && hm_row != ~0ull) {
+            /* the pressure sweep's hits of this row, ascending; a cell's partial sum is
+             * flushed where the first hit past its end is visited and after the row (the
+             * skipped flushes add +0 to sums that start at +0) */
+            uint64_t m = hm_row;
+            const uint32_t b0 = rb.b[0];
+            auto next_bound = [&](uint32_t j) -> uint32_t {
+                uint32_t c = 0xFFFFFFFFu;
+#pragma unroll
+                for (int k = 3; k >= 1; k--)
+                    if (k < ncell && j < rb.b[k]) c = rb.b[k];
+                return c;
+            };
+            uint32_t cnext = next_bound(b0);
+            while (m != 0ull) {
+                const uint32_t j = b0 + (uint32_t)__builtin_ctzll(m);
+                m &= m - 1ull;
+                if (j >= cnext) {
+                    end_cell();
+                    cnext = next_bound(j);
+                }
+                float4 pj, ex[NXX];
+                if (cfits) {
+                    pj = wb.p[j - cblo];
+#pragma unroll
+                    for (int x = 0; x < NX; x++) ex[x] = wb.x[x][j - cblo];
+                } else {
+                    pj = d.P[j];
+#pragma unroll
+                    for (int x = 0; x < NX; x++) ex[x] = xload(j, x);
+                }
+                const f3 dist = pi - xyz(pj);
+                const float dsq = dist.x * dist.x + dist.y * dist.y + dist.z * dist.z;
+                visit(j, pj, dist, dsq, ex);
+            }
+            end_cell();
+            continue;
+        }
+        /* window of the row for the hit record: [b[0], b[ncell]) */
+        const uint32_t wlo = rb.b[0];
+        const bool wide = !hm_w || (ncell >= 4 ? rb.b[4] : ncell == 3 ? rb.b[3] : ncell == 2 ? rb.b[2] : rb.b[1]) - wlo > 64u;
+        uint64_t rec = 0ull;
+        auto visit_rec = [&](uint32_t j, float4 pj, f3 dist, float dsq, const float4* x) {
+            if (!wide) rec |= 1ull << ((j - wlo) & 63u);
+            visit(j, pj, dist, dsq, x);
+        };
         float gyz = 0.0f;
         if (cull) {
             const float gy = axis_gap(fr.y, s / g.srch - 1, inv.y);
@@ -624,14 +678,15 @@ __device__ __forceinline__ void sweep(const DS& d, const MgGrid& g, Band<NX>& wb
             if (cfits) {
                 cell_candidates<K, NX>(
                     lo, hi, pi, lim, [&](uint32_t j) { return wb.p[j - cblo]; },
-                    [&](uint32_t j, int x) { return wb.x[x][j - cblo]; }, visit);
+                    [&](uint32_t j, int x) { return wb.x[x][j - cblo]; }, visit_rec);
             } else {
                 cell_candidates<K, NX>(
                     lo, hi, pi, lim, [&](uint32_t j) { return d.P[j]; },
-                    [&](uint32_t j, int x) { return xload(j, x); }, visit);
+                    [&](uint32_t j, int x) { return xload(j, x); }, visit_rec);
             }
             end_cell();
         }
+        if (hm_w) d.HMW[(size_t)s * N + self] = wide ? ~0ull : rec;
     }
 }
 
@@ -664,7 +719,7 @@ __device__ __forceinline__ void pressure_body(const DS& d, const MgParams& p, co
     const bool sr_pow2 = (srb & 0x007FFFFFu) == 0u && ((srb >> 23) & 0xFFu) != 0u && ((srb >> 23) & 0xFFu) != 0xFFu;
     if (sr_pow2) {
         const float inv_sr = 1.0f / sr;
-        sweep<8, 0>(
+        sweep<8, 0, 1>(
             d, g, bands[threadIdx.x >> 6], valid, gc, pi, p.r2, none,
             [&](uint32_t, float4, f3, float dsq, const float4*) {
                 float r = sqrtf(dsq);
@@ -674,9 +729,9 @@ __device__ __forceinline__ void pressure_body(const DS& d, const MgParams& p, co
                 b *= b;
                 csum += b * (4.0f * q + 1.0f);
             },
-            flush);
+            flush, i);
     } else {
-        sweep<8, 0>(
+        sweep<8, 0, 1>(
             d, g, bands[threadIdx.x >> 6], valid, gc, pi, p.r2, none,
             [&](uint32_t, float4, f3, float dsq, const float4*) {
                 float r = sqrtf(dsq);
@@ -686,7 +741,7 @@ __device__ __forceinline__ void pressure_body(const DS& d, const MgParams& p, co
                 b *= b;
                 csum += b * (4.0f * q + 1.0f);
             },
-            flush);
+            flush, i);
     }
     if (!valid) return;
     sum = sum * p.pmass * p.wendlandC2kern;
@@ -782,7 +837,7 @@ __device__ __forceinline__ f3 force_one(const DS& d, const MgParams& p, const Mg
     /* fluid terms: per-cell sums added in gridAdj order */
     const float sr = p.psmoothradius, st = p.psurface_t;
     f3 fsum = mk3(0.0f, 0.0f, 0.0f), cf = mk3(0.0f, 0.0f, 0.0f);
-    sweep<4, 1>(
+    sweep<4, 1, 2>(
         d, g, wb, valid, gc, ipos, 1.0f, [&](uint32_t j, int) { return d.V[j]; },
         [&](uint32_t, float4 Pj, f3 dist, float dsq, const float4* x) {
             const float4 Vj = x[0];
@@ -792,7 +847,8 @@ __device__ __forceinline__ f3 force_one(const DS& d, const MgParams& p, const Mg
         [&]() {
             fsum = fsum + cf;
             cf = mk3(0.0f, 0.0f, 0.0f);
-        });
+        },
+        i);
     force = force + fsum * p.pmass;
 
     f3 F = F0 + force;

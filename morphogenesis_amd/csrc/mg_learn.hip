@@ -417,6 +417,7 @@ int mg_step_learned(mg_handle* h, mg_net* n, int nsteps, float dt, unsigned flag
     CHK(ensure_learn_buffers(h));
     CHK(maybe_rehome(h));
     h->steps_since_rehome += nsteps;
+    ensure_hit_masks(h);
     int s = 0;
     if ((flags & MG_STEP_GRAPH) && nsteps >= 2) {
         const int pairs = nsteps / 2;
