@@ -39,15 +39,18 @@ def algorithmic_bytes(phase: str, scene, stats: dict) -> float:
     n, c, b = stats["n_active"], scene.c, scene.b
     bo, bi = stats["bonds_out"], stats["bonds_in"]
     genes = bool(scene.flags & 1)
+    srch = int(scene.grid.srch)
+    # hit masks handed from the pressure to the force sweep: 8 bytes per row of cells
+    hm = 8 * srch * srch if (float(scene.params.r2) == 1.0 and srch <= 4) else 0
     if phase == "pressure":
-        # GCELL 4, P 16 read; press, 1/density 8 written; OFF read once, next grid's CNT
-        # cleared
-        return n * (4 + 16 + 8) + 8 * c
+        # GCELL 4, P 16 read; press, 1/density 8 written; hit masks written; OFF read once,
+        # next grid's CNT cleared
+        return n * (4 + 16 + 8 + hm) + 8 * c
     if phase == "force":
-        # force: GCELL 4, HOME 4, P 16, V 16 (+VEL 16 for the advance); BPARTNER 4 per slot;
-        # per bond BPAR 16 + SLOT 4 + BINT 8 read + 8 written; INC_OFF 8, INC 4 per incoming
-        # bond; advance writes P, V, VEL: 48
-        return n * (56 + 4 * b + bo * 36 + 8 + 4 * bi + 48)
+        # force: GCELL 4, HOME 4, P 16, V 16 (+VEL 16 for the advance); hit masks read;
+        # BPARTNER 4 per slot; per bond BPAR 16 + SLOT 4 + BINT 8 read + 8 written; INC_OFF 8,
+        # INC 4 per incoming bond; advance writes P, V, VEL: 48
+        return n * (56 + hm + 4 * b + bo * 36 + 8 + 4 * bi + 48)
     if phase == "diffusion":
         # P 16, CONC 64 read, CONC 64 written, GCELL 4 read/written, GNDX 4, GMASK 4, HOME 4,
         # EPIGEN 64 read (+64 written with genes), GMN 4
