@@ -89,6 +89,11 @@ int mg_compute_genes(mg_handle* h);
 /* ZeroVelCL (src/host_CL.cpp:2175-2180). */
 int mg_zero_vel(mg_handle* h);
 
+/* InitializeBondsCL (src/host_CUDA.cpp:650-660, src/host_CL.cpp:1884-1916): the
+ * initialize_bonds kernel (src/fluid_system_cuda.cu:1938-2044) over gene 1's dense list. Needs
+ * the sorted state (after a physical sort or a step). */
+int mg_initialize_bonds(mg_handle* h);
+
 /* ---- composites (src/fluid_system.cpp:1919-2014) ---- */
 int mg_physical_sort(mg_handle* h);                          /* Run2PhysicalSort      */
 int mg_inner_physical_loop(mg_handle* h, float time, float dt); /* Run2InnerPhysicalLoop */

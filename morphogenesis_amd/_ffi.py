@@ -229,6 +229,7 @@ def declare(lib, prefix: str):
         "compute_diffusion": [h],
         "compute_genes": [h],
         "zero_vel": [h],
+        "initialize_bonds": [h],
         "physical_sort": [h],
         "inner_physical_loop": [h, C.c_float, C.c_float],
         "gene_action": [h],

@@ -120,6 +120,7 @@ void FluidSystem::AdvanceCL(float time, float dt, float ss)
     check(mg_advance(h_, time, dt, ss), "AdvanceCL");
 }
 void FluidSystem::ZeroVelCL() { check(mg_zero_vel(h_), "ZeroVelCL"); }
+void FluidSystem::InitializeBondsCL() { check(mg_initialize_bonds(h_), "InitializeBondsCL"); }
 
 void FluidSystem::Run2PhysicalSort() { check(mg_physical_sort(h_), "Run2PhysicalSort"); }
 void FluidSystem::Run2InnerPhysicalLoop()

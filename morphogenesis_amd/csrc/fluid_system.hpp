@@ -73,6 +73,7 @@ public:
     void AdvanceCL(float time, float dt, float ss);
     void TransferPosVelVevalFromTemp() {}  /* folded into AdvanceCL */
     void ZeroVelCL();
+    void InitializeBondsCL(); /* src/host_CUDA.cpp:650-660 */
     void AdvanceTime() { m_Time += m_DT; }
 
     /* src/fluid_system.cpp:1919-2014 */

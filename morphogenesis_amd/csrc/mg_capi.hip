@@ -1450,6 +1450,21 @@ int mg_zero_vel(mg_handle* h)
     return MG_OK;
 }
 
+int mg_initialize_bonds(mg_handle* h)
+{
+    CHK(ready(h));
+    if (h->B == 0) return MG_OK;
+    if (!h->force_zero) {
+        set_err("mg_initialize_bonds: the handle must be in the sorted state (after a physical sort or a step)");
+        return MG_ERR_STATE;
+    }
+    CHK(materialize_gene_grid(h));
+    hipLaunchKernelGGL(k_initialize_bonds, dim3(nblk(h->N)), dim3(256), 0, h->st, make_ds(h), h->p, h->g,
+                       h->gen, h->PID, h->BCOLD);
+    HIPCHK(hipGetLastError());
+    return rebuild_inc(h); /* new partners: incoming-bond CSR and live-slot masks */
+}
+
 int mg_physical_sort(mg_handle* h)
 {
     CHK(ready(h));

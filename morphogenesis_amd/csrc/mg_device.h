@@ -139,6 +139,31 @@ __device__ __forceinline__ f3 fluid_term(f3 dist, float r, float sr, float ipres
     return pterm + vterm;
 }
 
+/* acos in float by the Cephes single-precision method (S. L. Moshier, Cephes Math Library,
+ * asinf.c / acosf.c): |x| <= 0.5 -> pi/2 - asin(x); |x| > 0.5 -> 2 asin(sqrt((1 - |x|) / 2)),
+ * mirrored for x < 0; asin(a) for a <= 0.5 by its degree-11 odd minimax polynomial
+ * (published coefficients). Relative error a few 1e-7. NaN outside [-1, 1]. The oracle evaluates
+ * the same operations (oracle/mg_oracle.cpp, mg_acosf), so both sides produce the same bits. */
+__host__ __device__ __forceinline__ float mg_asinf_half(float a) /* 0 <= a <= 0.5 */
+{
+    if (a < 1.0e-4f) return a;
+    const float z = a * a;
+    float p = 4.2163199048e-2f;
+    p = p * z + 2.4181311049e-2f;
+    p = p * z + 4.5470025998e-2f;
+    p = p * z + 7.4953002686e-2f;
+    p = p * z + 1.6666752422e-1f;
+    return p * z * a + a;
+}
+__host__ __device__ __forceinline__ float mg_acosf(float x)
+{
+    const float pio2 = 1.5707963267948966f, pi = 3.141592653589793f;
+    if (!(x >= -1.0f && x <= 1.0f)) return __builtin_nanf("");
+    if (x < -0.5f) return pi - 2.0f * mg_asinf_half(sqrtf(0.5f * (1.0f + x)));
+    if (x > 0.5f) return 2.0f * mg_asinf_half(sqrtf(0.5f * (1.0f - x)));
+    return x < 0.0f ? pio2 + mg_asinf_half(-x) : pio2 - mg_asinf_half(x);
+}
+
 /* XCD-aware block index: the dispatcher places consecutive workgroups on different XCDs
  * (round robin over 8); remap so each XCD walks one contiguous range of particles and
  * neighbour data stays in that XCD's L2. */

@@ -1259,6 +1259,89 @@ __global__ LAUNCH_BOUNDS_SWEEP void k_diffuse_genes_insert(DS d, MgParams p, MgG
     diffuse_genes_insert_body<NG>(d, p, g, gen, dp, genes, (Band<NG>*)mg_lds, xcd_block());
 }
 
+/* -------------------------------------------------------- bond formation
+ * initialize_bonds + contribFindBonds (src/fluid_system_cuda.cu:1938-2044) for the particles of
+ * gene 1's dense list ("solid") below mActivePoints. For each bond slot in turn, every
+ * candidate j of the 27 cells gc + gridAdj[c] (the reference does not subtract the (1,1,1)
+ * offset here, so the block is the one above and right of the particle's cell) with a
+ * non-zero particle ID and 0 < |x_i - x_j|^2 < rd2 is scored by the sum over the particle's
+ * current bonds k of |1.91 - acos(u.v / (|u| |v|))| (u = x_i - x_j, v = x_i - x_k; FLT_MAX when
+ * j is already bonded), or by |u|^2 for slot 0; the first strictly best candidate is written
+ * into the slot with the genome's default parameters of the slot's bond type (slots 0-3 by
+ * tissue gene: bone 9 -> apatite, tendon 6 / muscle 7 / elastic ligament 10 -> collagen in
+ * slot 0; elastin otherwise). Each particle writes only its own records. */
+__global__ LAUNCH_BOUNDS_256 void k_initialize_bonds(DS d, MgParams p, MgGrid g, MgGenome gen,
+                                                     const uint32_t* __restrict__ PID, uint2* BCOLD)
+{
+    const uint32_t li = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t N = (uint32_t)d.N;
+    const int B = d.B;
+    if (li >= N || li >= d.LENS[1]) return;
+    const uint32_t i = d.LISTS[(size_t)1 * N + li];
+    const uint32_t na = d.ctr[MG_C_ACTIVE];
+    if (i >= na) return;
+    const uint32_t gc = d.GCELL[i];
+    if (gc == MG_GRID_UNDEF) return;
+    const uint32_t h = d.HOME[i];
+    const f3 ipos = xyz(d.P[i]);
+    int bt4[4] = {0, 0, 0, 0}; /* bond type of slots 0-3: 0 elastin, 1 collagen, 2 apatite */
+    auto gene_on = [&](int gidx) { return d.EPIGEN[(size_t)gidx * N + h] > 0u; };
+    if (gene_on(9)) {
+        bt4[0] = bt4[1] = bt4[2] = bt4[3] = 2;
+    } else if (gene_on(6) || gene_on(7) || gene_on(10)) {
+        bt4[0] = 1;
+    }
+    const float rd2 = p.rd2;
+    const uint32_t C = (uint32_t)d.C;
+    for (int bond = 0; bond < B; bond++) {
+        float best = FLT_MAX;
+        uint32_t to = MG_NONE;
+        for (int c = 0; c < g.adj_cnt; c++) {
+            const uint32_t cell = gc + (uint32_t)g.adj[c];
+            if (cell >= C) continue;
+            const uint32_t cn = d.CNT[cell];
+            if (cn == 0u) continue;
+            const uint32_t c0 = d.OFF[cell];
+            for (uint32_t j = c0; j < c0 + cn && j < N; j++) {
+                if (PID[d.HOME[j]] == 0u) continue;
+                const f3 u = ipos - xyz(d.P[j]);
+                const float dsq = u.x * u.x + u.y * u.y + u.z * u.z;
+                if (!(dsq < rd2 && dsq > 0.0f)) continue;
+                float theta = 0.0f;
+                for (int b = 0; b < B; b++) {
+                    const uint32_t kh = d.BPARTNER[(size_t)b * N + h];
+                    const uint32_t k = kh == MG_NONE ? MG_NONE : d.SLOT[kh];
+                    if (j == k) {
+                        theta = FLT_MAX;
+                        break;
+                    }
+                    if (k < N) {
+                        const f3 v = ipos - xyz(d.P[k]);
+                        const float lu = sqrtf(u.x * u.x + u.y * u.y + u.z * u.z);
+                        const float lv = sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
+                        const float cosang = (u.x * v.x + u.y * v.y + u.z * v.z) / (lu * lv);
+                        theta = (float)((double)theta + fabs(1.91 - (double)mg_acosf(cosang)));
+                    }
+                }
+                if (bond == 0) theta = dsq;
+                if (theta < best) {
+                    best = theta;
+                    to = j;
+                }
+            }
+        }
+        if (to < na) {
+            const int bt = bond < 4 ? bt4[bond] : 0;
+            const size_t bi = (size_t)bond * N + h;
+            const uint32_t th = d.HOME[to];
+            d.BPARTNER[bi] = th;
+            d.BPAR[bi] = make_float4(gen.param[bt][8], gen.param[bt][9], gen.param[bt][10], gen.param[bt][11]);
+            d.BINT[bi] = make_float2(0.0f, 0.0f);
+            BCOLD[bi].x = PID[th];
+        }
+    }
+}
+
 /* ZeroVelCL (src/host_CL.cpp:2175-2180) over the whole FVEL / FVEVAL buffers. */
 __global__ LAUNCH_BOUNDS_256 void k_zero_vel(DS d)
 {

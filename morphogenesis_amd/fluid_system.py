@@ -137,6 +137,11 @@ class FluidSystem:
     def ComputeGenesCL(self):
         self._call("compute_genes", self.h)
 
+    def InitializeBondsCL(self):
+        """initialize_bonds over gene 1's dense list (src/host_CUDA.cpp:650-660); needs the
+        sorted state."""
+        self._call("initialize_bonds", self.h)
+
     def ZeroVelCL(self):
         self._call("zero_vel", self.h)
 

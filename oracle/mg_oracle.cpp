@@ -55,6 +55,33 @@ template <typename R> inline V3<R> divs(V3<R> a, R s) { R inv = R(1) / s; return
 template <typename R> inline R dot(V3<R> a, V3<R> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 template <typename R> inline bool has_nan(V3<R> a) { return a.x != a.x || a.y != a.y || a.z != a.z; }
 
+/* acos in float by the Cephes single-precision method (S. L. Moshier, Cephes Math Library,
+ * asinf.c / acosf.c), the operations of mg_acosf in morphogenesis_amd/csrc/mg_device.h one
+ * for one: |x| <= 0.5 -> pi/2 - asin(x); |x| > 0.5 -> 2 asin(sqrt((1 - |x|) / 2)), mirrored for
+ * x < 0; asin(a), a <= 0.5, by the published odd polynomial. NaN outside [-1, 1].
+ * Used for the bond-angle score of initialize_bonds (the reference calls CUDA's acos there). */
+inline float asinf_half(float a)
+{
+    if (a < 1.0e-4f) return a;
+    const float z = a * a;
+    float p = 4.2163199048e-2f;
+    p = p * z + 2.4181311049e-2f;
+    p = p * z + 4.5470025998e-2f;
+    p = p * z + 7.4953002686e-2f;
+    p = p * z + 1.6666752422e-1f;
+    return p * z * a + a;
+}
+inline float acos_f(float x)
+{
+    const float pio2 = 1.5707963267948966f, pi = 3.141592653589793f;
+    if (!(x >= -1.0f && x <= 1.0f)) return std::nanf("");
+    if (x < -0.5f) return pi - 2.0f * asinf_half(std::sqrt(0.5f * (1.0f + x)));
+    if (x > 0.5f) return 2.0f * asinf_half(std::sqrt(0.5f * (1.0f - x)));
+    return x < 0.0f ? pio2 + asinf_half(-x) : pio2 - asinf_half(x);
+}
+inline double acos_r(double x) { return std::acos(x); }
+inline float acos_r(float x) { return acos_f(x); }
+
 inline float u2f(uint32_t u) { float f; std::memcpy(&f, &u, 4); return f; }
 inline uint32_t f2u(float f) { uint32_t u; std::memcpy(&u, &f, 4); return u; }
 
@@ -85,6 +112,7 @@ struct Base {
     virtual void diffusion() = 0;
     virtual void genes() = 0;
     virtual void zero_vel() = 0;
+    virtual void initialize_bonds() = 0;
     MgParams p;
     MgGrid g;
     MgGenome gen;
@@ -757,6 +785,80 @@ template <typename R> struct Oracle : Base {
         std::fill(veval.begin(), veval.end(), R(0));
     }
 
+    /* ------------------------------------------------ bond formation
+     * initialize_bonds + contribFindBonds, src/fluid_system_cuda.cu:1938-2044, over gene 1's
+     * dense list below mActivePoints. Per bond slot: candidates j of the cells gc + gridAdj[c]
+     * (no (1,1,1) offset is subtracted in the reference here) with ID != 0 and
+     * 0 < dsq < rd2, scored by sum_k |1.91 - acos(u.v / (|u| |v|))| over the particle's current
+     * bonds k (FLT_MAX when j is one of them; the sum runs in double as the reference's
+     * double literal 1.91 makes it), or by dsq for slot 0; the first strictly best candidate
+     * below mActivePoints gets the slot, with the genome's default elastLim, rest length,
+     * modulus and damping of the slot's bond type (src/fluid_system_cuda.cu:1975-2043).
+     * FPARTICLEIDX is not updated, as in the reference. The candidate index is the slot
+     * itself (FBIN is the identity after a sort, src/fluid_system_cuda.cu:221). */
+    void initialize_bonds() override
+    {
+        const uint32_t na = active_points, len = lens[1];
+        const R rd2 = (R)p.rd2;
+        for (uint32_t li = 0; li < len && li < (uint32_t)N; li++) {
+            const uint32_t i = lists[(size_t)N + li];
+            if (i >= na) continue;
+            const uint32_t gc = gcell[i];
+            if (gc == MG_GRID_UNDEF) continue;
+            int bt4[4] = {0, 0, 0, 0};
+            auto on = [&](int gi) { return epigen[(size_t)gi * N + i] > 0u; };
+            if (on(9)) bt4[0] = bt4[1] = bt4[2] = bt4[3] = 2;       /* bone: apatite */
+            else if (on(6) || on(7) || on(10)) bt4[0] = 1;         /* collagen in slot 0 */
+            const V3<R> ipos = P(i);
+            for (int bond = 0; bond < B; bond++) {
+                R best = (R)FLT_MAX;
+                uint32_t to = MG_NONE;
+                for (int c = 0; c < g.adj_cnt; c++) {
+                    const uint32_t cell = gc + (uint32_t)g.adj[c];
+                    if (cell >= (uint32_t)C || cnt[cell] == 0) continue;
+                    const uint32_t c0 = off[cell];
+                    for (uint32_t j = c0; j < c0 + cnt[cell] && j < (uint32_t)N; j++) {
+                        if (pid[j] == 0u) continue;
+                        const V3<R> u = ipos - P(j);
+                        const R dsq = dot(u, u);
+                        if (!(dsq < rd2 && dsq > R(0))) continue;
+                        R theta = R(0);
+                        for (int b = 0; b < B; b++) {
+                            const uint32_t k = bu[brec(i, b) + MG_B_INDEX];
+                            if (j == k) {
+                                theta = (R)FLT_MAX;
+                                break;
+                            }
+                            if (k < (uint32_t)N) {
+                                const V3<R> v = ipos - P(k);
+                                const R lu = std::sqrt(dot(u, u)), lv = std::sqrt(dot(v, v));
+                                const R cosang = dot(u, v) / (lu * lv);
+                                theta = (R)((double)theta + std::fabs(1.91 - (double)acos_r(cosang)));
+                            }
+                        }
+                        if (bond == 0) theta = dsq;
+                        if (theta < best) {
+                            best = theta;
+                            to = j;
+                        }
+                    }
+                }
+                if (to < na) {
+                    const int bt = bond < 4 ? bt4[bond] : 0;
+                    const size_t r = brec(i, bond);
+                    bu[r + MG_B_INDEX] = to;
+                    sbf(r + MG_B_LIMIT, (R)gen.param[bt][8]);
+                    sbf(r + MG_B_REST, (R)gen.param[bt][9]);
+                    sbf(r + MG_B_MODULUS, (R)gen.param[bt][10]);
+                    sbf(r + MG_B_DAMP, (R)gen.param[bt][11]);
+                    bu[r + 5] = pid[to];
+                    sbf(r + MG_B_STRAIN2, R(0));
+                    sbf(r + MG_B_STRAIN, R(0));
+                }
+            }
+        }
+    }
+
     int nth() const
     {
 #ifdef _OPENMP
@@ -877,6 +979,8 @@ int mgo_advance(mgo_handle* h, float time, float dt, float ss)
 int mgo_compute_diffusion(mgo_handle* h) { return with_grid(h, [&] { h->o->diffusion(); }); }
 int mgo_compute_genes(mgo_handle* h) { return with_grid(h, [&] { h->o->genes(); }); }
 int mgo_zero_vel(mgo_handle* h) { return with_grid(h, [&] { h->o->zero_vel(); }); }
+int mgo_initialize_bonds(mgo_handle* h) { return with_grid(h, [&] { h->o->initialize_bonds(); }); }
+float mgo_acosf(float x) { return acos_f(x); }
 
 int mgo_physical_sort(mgo_handle* h)
 {

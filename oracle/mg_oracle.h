@@ -42,6 +42,8 @@ int mgo_advance(mgo_handle* h, float time, float dt, float ss); /* AdvanceCL  */
 int mgo_compute_diffusion(mgo_handle* h);             /* ComputeDiffusionCL  */
 int mgo_compute_genes(mgo_handle* h);                 /* ComputeGenesCL      */
 int mgo_zero_vel(mgo_handle* h);                      /* ZeroVelCL           */
+int mgo_initialize_bonds(mgo_handle* h);              /* InitializeBondsCL   */
+float mgo_acosf(float x);                             /* the bond-angle acos */
 
 /* composites */
 int mgo_physical_sort(mgo_handle* h);                 /* Run2PhysicalSort    */

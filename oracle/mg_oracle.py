@@ -26,7 +26,14 @@ def load():
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} missing; run __graft_entry__.build() first")
         _lib = F.declare(C.CDLL(LIB_PATH), "mgo")
+        _lib.mgo_acosf.argtypes = [C.c_float]
+        _lib.mgo_acosf.restype = C.c_float
     return _lib
+
+
+def acosf(x: float) -> float:
+    """The float acos the bond-angle score uses (Cephes method, shared with the kernels)."""
+    return float(load().mgo_acosf(float(x)))
 
 
 class Oracle:
@@ -133,6 +140,9 @@ class Oracle:
 
     def ComputeGenesCL(self):
         self._call("compute_genes", self.h)
+
+    def InitializeBondsCL(self):
+        self._call("initialize_bonds", self.h)
 
     def ZeroVelCL(self):
         self._call("zero_vel", self.h)
