@@ -89,6 +89,11 @@ enum {
 /* ---- step flags for mg_step ---- */
 #define MG_STEP_GENES 1u   /* run Run2GeneAction (diffusion + genes) each step  */
 #define MG_STEP_GRAPH 2u   /* replay the step as a captured hipGraph             */
+/* SpecialParticlesCL (src/host_CUDA.cpp:869-898) after each step's advance: particles of
+ * gene 12's list keep their pre-advance state with pos.z += actuation_factor *
+ * sin(time / (actuation_period * dt)) (externalActuation), those of gene 11's list keep their
+ * pre-advance state (fixedParticles, applied last). */
+#define MG_STEP_SPECIAL 4u
 
 /*
  * Genome: same field order and sizes as the reference FGenome (src/fluid.h:398-422),

@@ -44,6 +44,7 @@ NONE = 0xFFFFFFFF
 GRID_UNDEF = 0xFFFFFFFF
 STEP_GENES = 1
 STEP_GRAPH = 2
+STEP_SPECIAL = 4
 
 MG_OK = 0
 ERRORS = {-1: "MG_ERR_ARG", -2: "MG_ERR_HIP", -3: "MG_ERR_STATE", -4: "MG_ERR_UNSUPPORTED",

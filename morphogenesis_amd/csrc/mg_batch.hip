@@ -317,6 +317,10 @@ int batch_check(mg_batch* b)
 /* nsteps fused steps; with use_graph, pairs of steps replay a captured graph */
 int batch_steps(mg_batch* b, int nsteps, float dt, unsigned flags, bool use_graph)
 {
+    if (flags & MG_STEP_SPECIAL) {
+        set_err("mg_batch_step: MG_STEP_SPECIAL is not supported for batches");
+        return MG_ERR_UNSUPPORTED;
+    }
     CHK(batch_check(b));
     for (mg_handle* h : b->hs) {
         CHK(maybe_rehome(h));

@@ -778,6 +778,22 @@ uint64_t net_version(const mg_net* net);
 
 namespace {
 
+/* The z displacement of externally actuated particles for a step starting at the handle's
+ * time: actuation_factor * sin(time / (actuation_period * dt)) in float
+ * (src/fluid_system_cuda.cu:3682), evaluated once on the host. */
+float actuation_offset(const mg_handle* h, float dt)
+{
+    const float t = (float)h->time;
+    return h->p.actuation_factor * sinf(t / (h->p.actuation_period * dt));
+}
+
+/* Graph replay bakes kernel arguments in: a step whose actuation offset changes with time
+ * runs eagerly. */
+bool graph_ok(const mg_handle* h, unsigned flags)
+{
+    return !(flags & MG_STEP_SPECIAL) || h->p.actuation_factor == 0.0f;
+}
+
 /* Hit masks for the fused step: the force sweep reuses the pressure sweep's neighbour sets
  * when both sweeps walk rows of the sorted grid (rows of srch <= 4 cells) and test the same
  * cut-off (pressure: dsq < r2, force: dsq < 1). ensure_hit_masks allocates the buffer outside
@@ -827,7 +843,10 @@ int do_fused_step(mg_handle* h, float dt, unsigned flags, const mg_net* net = nu
         DS d = make_ds(h);
         d.DVc = net ? h->DV : nullptr;
         d.HMR = hm;
-        hipLaunchKernelGGL(k_force_advance, dim3(nb), dim3(256), band_lds<1>(256), h->st, d, h->p, h->g, dt);
+        const int special = (flags & MG_STEP_SPECIAL) ? 1 : 0;
+        const float zoff = special ? actuation_offset(h, dt) : 0.0f;
+        hipLaunchKernelGGL(k_force_advance, dim3(nb), dim3(256), band_lds<1>(256), h->st, d, h->p, h->g, dt,
+                           special, zoff);
     }
     {
         PhaseScope ps(h, MG_PH_DIFFUSION);
@@ -954,11 +973,13 @@ int do_force(mg_handle* h)
     return MG_OK;
 }
 
-int do_advance(mg_handle* h, float dt)
+int do_advance(mg_handle* h, float dt, unsigned flags = 0)
 {
     CHK(ensure_force(h));
     PhaseScope ps(h, MG_PH_ADVANCE);
-    hipLaunchKernelGGL(k_advance, dim3(nblk(h->N)), dim3(256), 0, h->st, make_ds(h), h->p, dt);
+    const int special = (flags & MG_STEP_SPECIAL) ? 1 : 0;
+    const float zoff = special ? actuation_offset(h, dt) : 0.0f;
+    hipLaunchKernelGGL(k_advance, dim3(nblk(h->N)), dim3(256), 0, h->st, make_ds(h), h->p, dt, special, zoff);
     HIPCHK(hipGetLastError());
     h->cells_fresh = false;
     h->pos_binned = false;
@@ -1498,9 +1519,9 @@ int mg_step(mg_handle* h, int nsteps, float dt, unsigned flags)
     h->steps_since_rehome += nsteps;
     ensure_hit_masks(h);
     int s = 0;
-    if ((flags & MG_STEP_GRAPH) && h->force_zero && nsteps >= 2) {
+    if ((flags & MG_STEP_GRAPH) && h->force_zero && nsteps >= 2 && graph_ok(h, flags)) {
         const int pairs = nsteps / 2;
-        CHK(graph_steps(h, pairs, dt, flags & MG_STEP_GENES));
+        CHK(graph_steps(h, pairs, dt, flags & (MG_STEP_GENES | MG_STEP_SPECIAL)));
         h->time += 2.0 * pairs * dt;
         s = 2 * pairs;
     }
@@ -1514,7 +1535,7 @@ int mg_step(mg_handle* h, int nsteps, float dt, unsigned flags)
         CHK(materialize_gene_grid(h));
         CHK(do_pressure(h));
         CHK(do_force(h));
-        CHK(do_advance(h, dt));
+        CHK(do_advance(h, dt, flags));
         h->time += dt;
         if (flags & MG_STEP_GENES) {
             CHK(do_diffusion(h));

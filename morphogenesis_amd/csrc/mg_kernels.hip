@@ -971,7 +971,26 @@ __device__ __forceinline__ void advance_one(const MgParams& p, float dt, f3 pos,
 
 /* AdvanceCL in place. With the copies of src/host_CL.cpp:199-226 around it, a GRID_UNDEF
  * particle ends unchanged. */
-__global__ LAUNCH_BOUNDS_256 void k_advance(DS d, MgParams p, float dt)
+/* externalActuation + fixedParticles (src/fluid_system_cuda.cu:3617-3705) on one advanced
+ * particle: gene 12's list members keep their pre-advance state with z moved by zoff, gene
+ * 11's (applied last, so they win) keep it unchanged. */
+__device__ __forceinline__ void special_one(const DS& d, uint32_t i, float zoff, f3 pos, f3 vev, f3 v,
+                                            f3* np, f3* ve, f3* vn)
+{
+    const uint32_t m = d.GMASK[i];
+    if (m & (1u << 12)) {
+        *np = mk3(pos.x, pos.y, pos.z + zoff);
+        *ve = vev;
+        *vn = v;
+    }
+    if (m & (1u << 11)) {
+        *np = pos;
+        *ve = vev;
+        *vn = v;
+    }
+}
+
+__global__ LAUNCH_BOUNDS_256 void k_advance(DS d, MgParams p, float dt, int special, float zoff)
 {
     const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= d.ctr[MG_C_ACTIVE]) return;
@@ -979,6 +998,7 @@ __global__ LAUNCH_BOUNDS_256 void k_advance(DS d, MgParams p, float dt)
     const float4 P_i = d.P[i], V_i = d.V[i];
     f3 np, ve, vn;
     advance_one(p, dt, xyz(P_i), xyz(V_i), xyz(d.VEL[i]), xyz(d.F[i]), &np, &ve, &vn);
+    if (special) special_one(d, i, zoff, xyz(P_i), xyz(V_i), xyz(d.VEL[i]), &np, &ve, &vn);
     d.P[i] = make_float4(np.x, np.y, np.z, P_i.w);
     d.V[i] = make_float4(ve.x, ve.y, ve.z, V_i.w);
     d.VEL[i] = make_float4(vn.x, vn.y, vn.z, 0.0f);
@@ -988,7 +1008,8 @@ __global__ LAUNCH_BOUNDS_256 void k_advance(DS d, MgParams p, float dt)
  * stays in registers and the advanced state goes to the other half of the ping-pong
  * (P2/V2/VEL2) so neighbours keep reading the pre-step state. Inactive slots copy through. */
 __device__ __forceinline__ void force_advance_body(const DS& d, const MgParams& p, const MgGrid& g,
-                                                   Band<1>* bands, float dt, uint32_t bid)
+                                                   Band<1>* bands, float dt, uint32_t bid,
+                                                   int special = 0, float zoff = 0.0f)
 {
     const uint32_t i = bid * blockDim.x + threadIdx.x;
     const uint32_t na = d.ctr[MG_C_ACTIVE];
@@ -1011,15 +1032,17 @@ __device__ __forceinline__ void force_advance_body(const DS& d, const MgParams& 
     } else {
         advance_one(p, dt, xyz(P_i), xyz(V_i), xyz(VEL_i), F, &np, &ve, &vn);
     }
+    if (special) special_one(d, i, zoff, xyz(P_i), xyz(V_i), xyz(VEL_i), &np, &ve, &vn);
     d.P2[i] = make_float4(np.x, np.y, np.z, P_i.w);
     d.V2[i] = make_float4(ve.x, ve.y, ve.z, V_i.w);
     d.VEL2[i] = make_float4(vn.x, vn.y, vn.z, 0.0f);
 }
 
-__global__ LAUNCH_BOUNDS_SWEEP void k_force_advance(DS d, MgParams p, MgGrid g, float dt)
+__global__ LAUNCH_BOUNDS_SWEEP void k_force_advance(DS d, MgParams p, MgGrid g, float dt, int special,
+                                                    float zoff)
 {
     extern __shared__ float4 mg_lds[];
-    force_advance_body(d, p, g, (Band<1>*)mg_lds, dt, xcd_block());
+    force_advance_body(d, p, g, (Band<1>*)mg_lds, dt, xcd_block(), special, zoff);
 }
 
 /* ----------------------------------------------------------- diffusion (a12)

@@ -419,13 +419,16 @@ int mg_step_learned(mg_handle* h, mg_net* n, int nsteps, float dt, unsigned flag
     h->steps_since_rehome += nsteps;
     ensure_hit_masks(h);
     int s = 0;
-    if ((flags & MG_STEP_GRAPH) && nsteps >= 2) {
+    if ((flags & MG_STEP_GRAPH) && nsteps >= 2 && graph_ok(h, flags)) {
         const int pairs = nsteps / 2;
-        CHK(graph_steps(h, pairs, dt, flags & MG_STEP_GENES, n));
+        CHK(graph_steps(h, pairs, dt, flags & (MG_STEP_GENES | MG_STEP_SPECIAL), n));
+        h->time += 2.0 * pairs * dt;
         s = 2 * pairs;
     }
-    for (; s < nsteps; s++) CHK(do_fused_step(h, dt, flags, n));
-    h->time += (double)nsteps * dt;
+    for (; s < nsteps; s++) {
+        CHK(do_fused_step(h, dt, flags, n));
+        h->time += dt;
+    }
     return MG_OK;
 }
 

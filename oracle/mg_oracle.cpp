@@ -113,6 +113,7 @@ struct Base {
     virtual void genes() = 0;
     virtual void zero_vel() = 0;
     virtual void initialize_bonds() = 0;
+    virtual void advance_special(float time, float dt, float ss, float zoff) = 0;
     MgParams p;
     MgGrid g;
     MgGenome gen;
@@ -673,6 +674,34 @@ template <typename R> struct Oracle : Base {
         }
     }
 
+    /* advance followed by SpecialParticlesCL (src/host_CUDA.cpp:869-898): externalActuation
+     * (src/fluid_system_cuda.cu:3617-3692) gives gene 12's list members below mActivePoints
+     * their pre-advance state with z + zoff, then fixedParticles (:3694-3705) gives gene 11's
+     * members below mNumPoints their pre-advance state. */
+    void advance_special(float time_, float dt_, float ss, float zoff) override
+    {
+        const std::vector<R> p0 = pos, v0 = vel, e0 = veval;
+        advance(time_, dt_, ss);
+        auto restore = [&](uint32_t i) {
+            for (int k = 0; k < 3; k++) {
+                pos[3 * (size_t)i + k] = p0[3 * (size_t)i + k];
+                vel[3 * (size_t)i + k] = v0[3 * (size_t)i + k];
+                veval[3 * (size_t)i + k] = e0[3 * (size_t)i + k];
+            }
+        };
+        for (uint32_t li = 0; li < lens[12] && li < (uint32_t)N; li++) {
+            const uint32_t i = lists[(size_t)12 * N + li];
+            if (i >= active_points || gcell[i] == MG_GRID_UNDEF) continue;
+            restore(i);
+            pos[3 * (size_t)i + 2] = pos[3 * (size_t)i + 2] + (R)zoff;
+        }
+        for (uint32_t li = 0; li < lens[11] && li < (uint32_t)N; li++) {
+            const uint32_t i = lists[(size_t)11 * N + li];
+            if (i >= num_points) continue;
+            restore(i);
+        }
+    }
+
     /* -------------------------------------------------------- diffusion (a12)
      * contributeDiffusion/computeDiffusion, src/fluid_system_cuda.cu:3026-3105, as a Jacobi
      * update. The uint diffusability is scaled by diff_scale (1.0 = reference). */
@@ -1015,7 +1044,15 @@ int mgo_step(mgo_handle* h, int nsteps, float dt, unsigned flags)
         for (int s = 0; s < nsteps; s++) {
             h->o->pressure();
             h->o->force();
-            h->o->advance((float)h->o->time, dt, h->o->p.psimscale);
+            if (flags & MG_STEP_SPECIAL) {
+                /* actuation offset for a step starting at this time, evaluated as the library's
+                 * host does (src/fluid_system_cuda.cu:3682) */
+                const float t = (float)h->o->time;
+                const float zoff = h->o->p.actuation_factor * sinf(t / (h->o->p.actuation_period * dt));
+                h->o->advance_special(t, dt, h->o->p.psimscale, zoff);
+            } else {
+                h->o->advance((float)h->o->time, dt, h->o->p.psimscale);
+            }
             h->o->time += dt;
             if (flags & MG_STEP_GENES) {
                 h->o->diffusion();
