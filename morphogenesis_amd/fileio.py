@@ -558,6 +558,104 @@ def read_points_csv2(path: str, bounds=None, vel_limit: float | None = None, emp
     return Points(n, b, bufs)
 
 
+# ---------------------------------------------------------------------------- VTP
+def _vtk_array(name, vtype, ncomp, values, fmt) -> str:
+    body = " ".join(fmt(v) for v in np.ravel(values))
+    comp = f' NumberOfComponents="{ncomp}"' if ncomp > 1 else ""
+    nm = f' Name="{name}"' if name else ""
+    return f'        <DataArray type="{vtype}"{nm}{comp} format="ascii">{body}</DataArray>\n'
+
+
+def save_points_vtp(buffers: dict, directory: str, frame: int, vol_min, vol_max) -> str:
+    """SavePointsVTP2 (src/fileCL_IO.cpp:1223-1684): a VTK XML PolyData file
+    <directory>/particles_pos_vel_color%04d.vtp (frame + 100000) with the particles up to the
+    first one whose ID exceeds the particle count, the 8 corners of [vol_min, vol_max], and the
+    reference's point arrays (FAGE, FCLR, FPARTICLE_ID, FMASS_RADIUS, FNERVEIDX, FVEL, FVEVAL,
+    FFORCE, FPRESS, FDENSITY as (1/density, density), FCONC_t, FEPIGEN_g, F_TISSUE_TYPE; zeros for
+    the corners) and per-bond arrays. Unlike the reference's writer, whose line insertion is
+    commented out and which sets no vertices, every point gets a vertex cell and every bond slot a
+    line cell (to the partner, or to the particle itself when the slot is empty or broken), so the
+    per-bond arrays are cell data of a well-formed file (zeros on the vertex cells)."""
+    pos = np.asarray(buffers[F.FPOS], np.float32)
+    n = pos.shape[0]
+    rec = np.asarray(buffers[F.FELASTIDX], np.uint32)
+    b = rec.shape[1] if rec.ndim == 3 else 0
+    rec = rec.reshape(n, b, F.DATA_PER_BOND)
+    pid = np.asarray(buffers[F.FPARTICLE_ID], np.uint32)
+    na = 0
+    while na < n and pid[na] <= n:
+        na += 1
+    corners = np.array([[vol_max[0] if c & 1 else vol_min[0], vol_max[1] if c & 2 else vol_min[1],
+                         vol_max[2] if c & 4 else vol_min[2]] for c in range(8)], np.float32)
+    npts = na + 8
+
+    def pad(a, ncomp=1):
+        a = np.asarray(a)[:na].reshape(na, ncomp) if ncomp > 1 else np.asarray(a)[:na]
+        z = np.zeros((8, ncomp) if ncomp > 1 else 8, a.dtype)
+        return np.concatenate([a, z])
+
+    fl = lambda v: str(np.float32(v))        # shortest string that reads back as the float32
+    ui = lambda v: str(int(v))
+    epi = np.asarray(buffers[F.FEPIGEN], np.uint32).reshape(F.NUM_GENES, n)
+    conc = np.asarray(buffers[F.FCONC], np.float32).reshape(n, F.NUM_TF)
+    inv = np.asarray(buffers[F.FDENSITY], np.float32)[:na]
+    with np.errstate(divide="ignore"):
+        dens = np.where(inv == 0, np.float32(0), np.float32(1) / np.where(inv == 0, np.float32(1), inv))
+    mr = np.asarray(buffers[F.FMASS_RADIUS], np.uint32)
+    tissue = np.select([epi[9] > 0, epi[6] > 0, epi[7] > 0, epi[10] > 0, epi[8] > 0],
+                       [9, 6, 7, 10, 8], 0).astype(np.uint32)
+    point_arrays = [
+        ("FAGE", "UInt32", 1, pad(buffers[F.FAGE]), ui),
+        ("FCLR", "UInt32", 1, pad(buffers[F.FCOLOR]), ui),
+        ("FPARTICLE_ID", "UInt32", 1, pad(pid), ui),
+        ("FMASS_RADIUS", "UInt32", 2, pad(np.stack([mr & 0xFFFF, mr >> 16], 1), 2), ui),
+        ("FNERVEIDX", "UInt32", 1, pad(buffers[F.FNERVEIDX]), ui),
+        ("FVEL", "Float32", 3, pad(np.asarray(buffers[F.FVEL])[:, :3], 3), fl),
+        ("FVEVAL", "Float32", 3, pad(np.asarray(buffers[F.FVEVAL])[:, :3], 3), fl),
+        ("FFORCE", "Float32", 3, pad(np.asarray(buffers[F.FFORCE])[:, :3], 3), fl),
+        ("FPRESS", "Float32", 1, pad(buffers[F.FPRESS]), fl),
+        ("FDENSITY,(1/density,density)", "Float32", 2, pad(np.stack([inv, dens], 1), 2), fl),
+    ]
+    point_arrays += [(f"FCONC_{t}", "Float32", 1, pad(conc[:, t]), fl) for t in range(F.NUM_TF)]
+    point_arrays += [(f"FEPIGEN_{g}", "UInt32", 1, pad(epi[g]), ui) for g in range(F.NUM_GENES)]
+    point_arrays.append(("F_TISSUE_TYPE", "UInt32", 1, pad(tissue), ui))
+    # bond lines: (i, partner), self when empty / broken (rest length < 1e-6)
+    recf = rec.view(np.float32)
+    src = np.repeat(np.arange(na, dtype=np.int64), b)
+    dst = rec[:na, :, 0].astype(np.int64).ravel()
+    rest = recf[:na, :, 2].ravel()
+    bad = (rest < 1e-6) | (dst >= na) | np.isnan(rest)
+    dst = np.where(bad, src, dst)
+    nl = na * b
+    cz = lambda a: np.concatenate([np.zeros((npts,) + np.shape(a)[1:], np.asarray(a).dtype), a])
+    uint6 = np.stack([src, np.repeat(pid[:na], b), rec[:na, :, 0].ravel(), rec[:na, :, 5].ravel(),
+                      np.tile(np.arange(b), na), np.zeros(nl, np.int64)], 1).astype(np.uint32)
+    cell_arrays = [("parent_idx, parent_ID, curr_idx, particle ID, bond index", "UInt32", 6, cz(uint6), ui)]
+    for name, w in (("elastic_limit", 1), ("restlength", 2), ("modulus", 3), ("damping_coeff", 4),
+                    ("strain_integrator", 7), ("strain_sq_integrator", 6)):
+        cell_arrays.append((name, "Float32", 1, cz(recf[:na, :, w].ravel()), fl))
+    out = ['<?xml version="1.0"?>\n',
+           '<VTKFile type="PolyData" version="0.1" byte_order="LittleEndian">\n', "  <PolyData>\n",
+           f'    <Piece NumberOfPoints="{npts}" NumberOfVerts="{npts}" NumberOfLines="{nl}" '
+           'NumberOfStrips="0" NumberOfPolys="0">\n', "      <PointData>\n"]
+    out += [_vtk_array(*a) for a in point_arrays]
+    out.append("      </PointData>\n      <CellData>\n")
+    out += [_vtk_array(*a) for a in cell_arrays]
+    out.append("      </CellData>\n      <Points>\n")
+    out.append(_vtk_array("", "Float32", 3, np.concatenate([pos[:na, :3], corners]), fl))
+    out.append("      </Points>\n      <Verts>\n")
+    out.append(_vtk_array("connectivity", "Int64", 1, np.arange(npts), ui))
+    out.append(_vtk_array("offsets", "Int64", 1, np.arange(1, npts + 1), ui))
+    out.append("      </Verts>\n      <Lines>\n")
+    out.append(_vtk_array("connectivity", "Int64", 1, np.stack([src, dst], 1), ui))
+    out.append(_vtk_array("offsets", "Int64", 1, 2 * np.arange(1, nl + 1), ui))
+    out.append("      </Lines>\n    </Piece>\n  </PolyData>\n</VTKFile>\n")
+    path = os.path.join(directory, "particles_pos_vel_color%04d.vtp" % (frame + 100000))
+    with open(path, "w") as fh:
+        fh.write("".join(out))
+    return path
+
+
 # ---------------------------------------------------------------------------- scenes
 def load_scene(sim_params_path: str, points_path: str, genome_path: str | None = None,
                flags: int = 0, name: str = "reference_files", check_bounds: bool = True):

@@ -201,6 +201,15 @@ class FluidSystem:
 
         return fileio.save_points_csv2(self.TransferFromCL(), directory, frame)
 
+    def SavePointsVTP2(self, directory: str, frame: int) -> str:
+        """SavePointsVTP2 (src/fileCL_IO.cpp:1223-1684): VTK XML PolyData of the state."""
+        from . import fileio
+
+        g = self.grid
+        vmin = g.gmin[:] if g is not None else (0.0, 0.0, 0.0)
+        vmax = g.gmax[:] if g is not None else (0.0, 0.0, 0.0)
+        return fileio.save_points_vtp(self.TransferFromCL(), directory, frame, vmin, vmax)
+
     def WriteGenome(self, directory: str) -> str:
         from . import fileio
 

@@ -285,3 +285,53 @@ def test_checkpoint_resumes_the_oracle_exactly(built, tmp_path, shuffle):
         x, y = a.download(k), o.download(k)
         assert np.array_equal(x.view(np.uint32) if x.dtype == np.float32 else x,
                               y.view(np.uint32) if y.dtype == np.float32 else y), F.BUFFER_NAMES[k]
+
+
+# ---------------------------------------------------------------- VTP
+def test_vtp_holds_the_reference_arrays(tmp_path):
+    import xml.etree.ElementTree as ET
+
+    sc = scenes.tissue(n_side=4, b=6, seed=2)
+    bufs = sc.buffers
+    bufs[F.FEPIGEN][6:11] = 0
+    bufs[F.FEPIGEN][9, :5] = 1
+    bufs[F.FEPIGEN][8, 5:9] = 1
+    bufs[F.FDENSITY][:] = np.linspace(0.0, 2.0, sc.n).astype(np.float32)
+    bufs[F.FELASTIDX][3, 1, 2] = np.float32(0.0).view(np.uint32)       # a broken bond
+    p = fio.save_points_vtp(bufs, str(tmp_path), 12, (0, 0, 0), (7, 8, 9))
+    assert p.endswith("particles_pos_vel_color100012.vtp")
+    piece = ET.parse(p).getroot().find("PolyData/Piece")
+    n, b = sc.n, sc.b
+    assert int(piece.get("NumberOfPoints")) == n + 8 and int(piece.get("NumberOfLines")) == n * b
+
+    def arr(section, name=None):
+        for a in piece.find(section).findall("DataArray"):
+            if name is None or a.get("Name") == name:
+                return np.array(a.text.split(), dtype=np.float64)
+        raise KeyError(name)
+
+    pts = arr("Points").reshape(-1, 3)
+    assert np.array_equal(pts[:n].astype(np.float32), bufs[F.FPOS][:, :3])
+    assert pts[n + 7].tolist() == [7, 8, 9] and pts[n].tolist() == [0, 0, 0]
+    names = [a.get("Name") for a in piece.find("PointData").findall("DataArray")]
+    assert names[:10] == ["FAGE", "FCLR", "FPARTICLE_ID", "FMASS_RADIUS", "FNERVEIDX", "FVEL", "FVEVAL",
+                          "FFORCE", "FPRESS", "FDENSITY,(1/density,density)"]
+    assert names[10:26] == [f"FCONC_{t}" for t in range(16)] and names[-1] == "F_TISSUE_TYPE"
+    mr = arr("PointData", "FMASS_RADIUS").reshape(-1, 2)
+    assert mr[0].tolist() == [int(bufs[F.FMASS_RADIUS][0]) & 0xFFFF, int(bufs[F.FMASS_RADIUS][0]) >> 16]
+    dens = arr("PointData", "FDENSITY,(1/density,density)").reshape(-1, 2)
+    assert dens[0].tolist() == [0.0, 0.0]
+    assert np.float32(dens[10, 1]) == np.float32(1) / bufs[F.FDENSITY][10]
+    assert np.array_equal(arr("PointData", "FCONC_3")[:n].astype(np.float32), bufs[F.FCONC][:, 3])
+    tt = arr("PointData", "F_TISSUE_TYPE")
+    assert tt[:5].tolist() == [9] * 5 and tt[5:9].tolist() == [8] * 4 and tt[n:].tolist() == [0] * 8
+    conn = arr("Lines", "connectivity").reshape(-1, 2).astype(np.int64)
+    partner = bufs[F.FELASTIDX][:, :, 0].astype(np.int64).ravel()
+    own = np.repeat(np.arange(n), b)
+    rest = bufs[F.FELASTIDX][:, :, 2].view(np.float32).ravel()
+    expect = np.where((partner == F.NONE) | (rest < 1e-6), own, partner)
+    assert np.array_equal(conn[:, 0], own) and np.array_equal(conn[:, 1], expect)
+    assert conn[3 * b + 1, 1] == 3                                       # broken bond drawn to itself
+    rl = arr("CellData", "restlength")
+    assert len(rl) == (n + 8) + n * b and np.all(rl[:n + 8] == 0)
+    assert np.array_equal(rl[n + 8:].astype(np.float32), rest)

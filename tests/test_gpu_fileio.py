@@ -115,3 +115,20 @@ def test_point_file_round_trip_through_the_gpu(built, tmp_path):
     assert np.abs(pg - ph).max() <= 5e-7 * max(1.0, np.abs(pg).max())
     h.Run2PhysicalSort()
     h.step(10, h.m_DT, 0)
+
+
+def test_vtp_of_a_gpu_state(built, tmp_path):
+    import xml.etree.ElementTree as ET
+
+    from morphogenesis_amd.fluid_system import FluidSystem
+
+    say("case: SavePointsVTP2 of the demo scene after 20 steps")
+    g = FluidSystem.from_reference_files(PARAMS, POINTS, GENOME)
+    g.Run2PhysicalSort()
+    g.step(20, g.m_DT, 0)
+    p = g.SavePointsVTP2(str(tmp_path), 20)
+    piece = ET.parse(p).getroot().find("PolyData/Piece")
+    assert int(piece.get("NumberOfPoints")) == g.counts()[0] + 8
+    pts = np.array(piece.find("Points/DataArray").text.split(), np.float32).reshape(-1, 3)
+    assert np.array_equal(pts[:20], g.download(F.FPOS)[:20, :3])
+    assert pts[-1].tolist() == [10.0, 20.0, 50.0]
