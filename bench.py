@@ -288,6 +288,58 @@ def load_traffic(kernel: str, workload: str):
         return None
 
 
+def run_bonds(args):
+    """Bond formation (mg_initialize_bonds) on the 262,144-particle tissue with gene 1 ("solid")
+    on 30% of the particles: each call searches every bond slot of every listed particle.
+    `value` = slot searches per second over whole calls (kernel + host rebuild of the
+    incoming-bond lists); the CPU baseline is the oracle's initialize_bonds on a 13,824-particle
+    sample of the same construction."""
+    import numpy as np
+
+    from morphogenesis_amd import _ffi as F
+    from morphogenesis_amd import scenes
+    from morphogenesis_amd.fluid_system import FluidSystem
+
+    def scene(n_side):
+        sc = scenes.tissue(n_side, 32, seed=1)
+        sc.genome = scenes.example_genome()
+        return sc
+
+    sc = scene(64)
+    sim = FluidSystem(sc.n, sc.b)
+    sim.load_scene(sc)
+    sim.Run2PhysicalSort()
+    listed = int(sim.download(F.FDENSE_LIST_LENGTHS)[1])
+    sim.InitializeBondsCL()
+    sim.sync()
+    reps = max(args.steps // 40, 3)
+    t = time.perf_counter()
+    for _ in range(reps):
+        sim.InitializeBondsCL()
+    sim.sync()
+    dt = (time.perf_counter() - t) / reps
+    out = {"metric": "bond-slot searches/s (initialize_bonds), 262k bonded tissue", "value": listed * sc.b / dt,
+           "unit": "slot-searches/s", "n_gpus": 1, "steps": reps, "warmup": 1, "ms_per_step": dt * 1e3,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32 (fp64 angle sum)",
+           "data": "synthetic",
+           "config": {"workload": args.workload, "particles": sc.n, "listed": listed, "bonds_per_particle": sc.b}}
+    if not args.no_cpu:
+        from oracle.mg_oracle import Oracle
+
+        small = scene(24)
+        o = Oracle(small.n, small.b)
+        o.load_scene(small)
+        o.Run2PhysicalSort()
+        ls = int(o.download(F.FDENSE_LIST_LENGTHS)[1])
+        t = time.perf_counter()
+        o.InitializeBondsCL()
+        ct = time.perf_counter() - t
+        out["cpu_baseline"] = {"value": ls * small.b / ct, "unit": "slot-searches/s", "cores": 1,
+                               "kind": "port", "sample": f"one initialize_bonds over {ls} listed particles "
+                               f"of a {small.n}-particle tissue, {ct:.2f} s, FP32 oracle, 1 thread"}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -318,6 +370,9 @@ def main():
         return
     if args.workload == "tissue_262k_learned":
         run_learned(args)
+        return
+    if args.workload == "tissue_262k_bonds":
+        run_bonds(args)
         return
     builders = {"tissue_262k": lambda: scenes.tissue(64, 32, seed=1),
                 "fluid_block_32k": lambda: scenes.fluid_block(32, seed=1),
