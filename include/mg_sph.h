@@ -94,6 +94,32 @@ int mg_zero_vel(mg_handle* h);
  * the sorted state (after a physical sort or a step). */
 int mg_initialize_bonds(mg_handle* h);
 
+/* ---- remodelling: Run2Remodelling (src/fluid_system.cpp:2037-2071) is the four calls below
+ * in order (its AssembleFibresCL launches nothing). They work on the slots, grid and gene lists
+ * of the last sort, as the reference does between its inner loops and the next sort. ---- */
+/* ComputeBondChangesCL(steps_per_InnerPhysicalLoop) (src/host_CL.cpp:2057-2086,
+ * src/host_CUDA.cpp:744-761): clears the change counts and offsets, then computeBondChanges
+ * (src/fluid_system_cuda.cu:1065-1330) over gene 2's dense list: strain integrators averaged
+ * over steps_per_inner_loop (> 0) and cleared, rest length and modulus remodelled by the
+ * genome's tanh parameters, change words cleared, and each particle's first broken or empty
+ * slot counted into its cell's heal count. */
+int mg_compute_bond_changes(mg_handle* h, uint32_t steps_per_inner_loop);
+/* PrefixSumChangesCL(zero_offsets) (src/host_CL.cpp:865-1062) with tally_denselist_lengths
+ * (src/fluid_system_cuda.cu:185-194): exclusive per-cell offsets of the nine change lists and
+ * their lengths. The reference always passes 1; the offsets are exclusive for any value. */
+int mg_prefix_sum_changes(mg_handle* h, int zero_offsets);
+/* CountingSortChangesCL (src/host_CL.cpp:1415-1519, src/host_CUDA.cpp:604-648):
+ * countingSortChanges (src/fluid_system_cuda.cu:406-618) writes the (slot, bond) pairs of each
+ * change list in cell order, slot order within a cell. */
+int mg_counting_sort_changes(mg_handle* h);
+/* ComputeParticleChangesCL (src/host_CL.cpp:2088-2173, src/host_CUDA.cpp:763-841): heal
+ * (src/fluid_system_cuda.cu:2047-2107) over list 0, its random shift drawn from (seed, particle
+ * ID) instead of the reference's curand states. computeBondChanges fills no other list (its code
+ * for them is commented out in the reference), so their kernels have nothing to do; a
+ * non-empty one (possible only by uploading counts) is rejected with MG_ERR_UNSUPPORTED before
+ * anything runs. Returns with the incoming-bond lists rebuilt. */
+int mg_compute_particle_changes(mg_handle* h, uint64_t seed);
+
 /* ---- composites (src/fluid_system.cpp:1919-2014) ---- */
 int mg_physical_sort(mg_handle* h);                          /* Run2PhysicalSort      */
 int mg_inner_physical_loop(mg_handle* h, float time, float dt); /* Run2InnerPhysicalLoop */

@@ -46,11 +46,20 @@ enum {
     MG_FBIN_OFFSET_ACTIVE_GENES = 33, /* uint[16][C]                               */
     MG_FDENSE_LIST_LENGTHS = 34,      /* uint[16]                                  */
     MG_FDENSE_LISTS = 35,             /* uint[16][N], entries past length = MG_NONE */
+    /* remodelling change lists (src/fluid.h:197-201) */
+    MG_FBIN_OFFSET_CHANGES = 37,         /* uint[9][C]                             */
+    MG_FBIN_COUNT_CHANGES = 38,          /* uint[9][C]                             */
+    MG_FDENSE_LIST_LENGTHS_CHANGES = 39, /* uint[9]                                */
+    MG_FDENSE_LISTS_CHANGES = 40,        /* uint[9][2][N]: slots, then bond indices;
+                                            entries past length = MG_NONE          */
     MG_MAX_BUF = 44
 };
 
 #define MG_NUM_TF 16
 #define MG_NUM_GENES 16
+/* change lists: heal, lengthen/shorten/strengthen/weaken x (muscle, tissue)
+ * (src/fluid.h:103) */
+#define MG_NUM_CHANGES 9
 #define MG_DATA_PER_BOND 9
 #define MG_MAX_ADJ 64
 
@@ -280,7 +289,8 @@ static inline int mg_buffer_known(int buf)
     case MG_FNERVEIDX: case MG_FCONC: case MG_FEPIGEN: case MG_FBIN: case MG_FBIN_COUNT:
     case MG_FBIN_OFFSET: case MG_FPARTICLEIDX: case MG_FPARTICLE_ID: case MG_FMASS_RADIUS:
     case MG_FBIN_COUNT_ACTIVE_GENES: case MG_FBIN_OFFSET_ACTIVE_GENES: case MG_FDENSE_LIST_LENGTHS:
-    case MG_FDENSE_LISTS: return 1;
+    case MG_FDENSE_LISTS: case MG_FBIN_OFFSET_CHANGES: case MG_FBIN_COUNT_CHANGES:
+    case MG_FDENSE_LIST_LENGTHS_CHANGES: case MG_FDENSE_LISTS_CHANGES: return 1;
     default: return 0;
     }
 }
@@ -297,6 +307,9 @@ static inline size_t mg_buffer_bytes(int buf, int N, int C, int B)
         return (size_t)MG_NUM_GENES * C * 4;
     case MG_FDENSE_LIST_LENGTHS: return MG_NUM_GENES * 4;
     case MG_FDENSE_LISTS: return (size_t)MG_NUM_GENES * N * 4;
+    case MG_FBIN_OFFSET_CHANGES: case MG_FBIN_COUNT_CHANGES: return (size_t)MG_NUM_CHANGES * C * 4;
+    case MG_FDENSE_LIST_LENGTHS_CHANGES: return MG_NUM_CHANGES * 4;
+    case MG_FDENSE_LISTS_CHANGES: return (size_t)MG_NUM_CHANGES * 2 * N * 4;
     default: return 0;
     }
 }

@@ -20,6 +20,8 @@ FPOS, FVEL, FVEVAL, FFORCE, FPRESS, FDENSITY, FAGE, FCOLOR, FGCELL, FGNDX = rang
 FELASTIDX, FNERVEIDX, FCONC, FEPIGEN, FBIN, FBIN_COUNT, FBIN_OFFSET = 14, 15, 16, 17, 18, 19, 20
 FPARTICLEIDX, FPARTICLE_ID, FMASS_RADIUS = 29, 30, 31
 FBIN_COUNT_ACTIVE_GENES, FBIN_OFFSET_ACTIVE_GENES, FDENSE_LIST_LENGTHS, FDENSE_LISTS = 32, 33, 34, 35
+# remodelling change lists (src/fluid.h:197-201)
+FBIN_OFFSET_CHANGES, FBIN_COUNT_CHANGES, FDENSE_LIST_LENGTHS_CHANGES, FDENSE_LISTS_CHANGES = 37, 38, 39, 40
 
 PARTICLE_BUFFERS = [FPOS, FVEL, FVEVAL, FFORCE, FPRESS, FDENSITY, FAGE, FCOLOR, FGCELL, FGNDX,
                     FELASTIDX, FNERVEIDX, FCONC, FEPIGEN, FBIN, FPARTICLEIDX, FPARTICLE_ID,
@@ -35,10 +37,16 @@ BUFFER_NAMES = {
     FBIN_COUNT_ACTIVE_GENES: "FBIN_COUNT_ACTIVE_GENES",
     FBIN_OFFSET_ACTIVE_GENES: "FBIN_OFFSET_ACTIVE_GENES",
     FDENSE_LIST_LENGTHS: "FDENSE_LIST_LENGTHS", FDENSE_LISTS: "FDENSE_LISTS",
+    FBIN_OFFSET_CHANGES: "FBIN_OFFSET_CHANGES", FBIN_COUNT_CHANGES: "FBIN_COUNT_CHANGES",
+    FDENSE_LIST_LENGTHS_CHANGES: "FDENSE_LIST_LENGTHS_CHANGES",
+    FDENSE_LISTS_CHANGES: "FDENSE_LISTS_CHANGES",
 }
+CHANGE_BUFFERS = [FBIN_OFFSET_CHANGES, FBIN_COUNT_CHANGES, FDENSE_LIST_LENGTHS_CHANGES,
+                  FDENSE_LISTS_CHANGES]
 
 NUM_TF = 16
 NUM_GENES = 16
+NUM_CHANGES = 9
 DATA_PER_BOND = 9
 NONE = 0xFFFFFFFF
 GRID_UNDEF = 0xFFFFFFFF
@@ -181,6 +189,9 @@ def buffer_shape(buf: int, n: int, c: int, b: int):
         FBIN_COUNT: ((c,), u), FBIN_OFFSET: ((c,), u),
         FBIN_COUNT_ACTIVE_GENES: ((NUM_GENES, c), u), FBIN_OFFSET_ACTIVE_GENES: ((NUM_GENES, c), u),
         FDENSE_LIST_LENGTHS: ((NUM_GENES,), u), FDENSE_LISTS: ((NUM_GENES, n), u),
+        FBIN_OFFSET_CHANGES: ((NUM_CHANGES, c), u), FBIN_COUNT_CHANGES: ((NUM_CHANGES, c), u),
+        FDENSE_LIST_LENGTHS_CHANGES: ((NUM_CHANGES,), u),
+        FDENSE_LISTS_CHANGES: ((NUM_CHANGES, 2, n), u),
     }
     return table[buf]
 
@@ -231,6 +242,10 @@ def declare(lib, prefix: str):
         "compute_genes": [h],
         "zero_vel": [h],
         "initialize_bonds": [h],
+        "compute_bond_changes": [h, C.c_uint32],
+        "prefix_sum_changes": [h, C.c_int],
+        "counting_sort_changes": [h],
+        "compute_particle_changes": [h, C.c_uint64],
         "physical_sort": [h],
         "inner_physical_loop": [h, C.c_float, C.c_float],
         "gene_action": [h],

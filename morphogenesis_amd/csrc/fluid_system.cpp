@@ -121,6 +121,26 @@ void FluidSystem::AdvanceCL(float time, float dt, float ss)
 }
 void FluidSystem::ZeroVelCL() { check(mg_zero_vel(h_), "ZeroVelCL"); }
 void FluidSystem::InitializeBondsCL() { check(mg_initialize_bonds(h_), "InitializeBondsCL"); }
+void FluidSystem::ComputeBondChangesCL(uint32_t steps)
+{
+    check(mg_compute_bond_changes(h_, steps), "ComputeBondChangesCL");
+}
+void FluidSystem::PrefixSumChangesCL(int zero_offsets)
+{
+    check(mg_prefix_sum_changes(h_, zero_offsets), "PrefixSumChangesCL");
+}
+void FluidSystem::CountingSortChangesCL() { check(mg_counting_sort_changes(h_), "CountingSortChangesCL"); }
+void FluidSystem::ComputeParticleChangesCL(uint64_t seed)
+{
+    check(mg_compute_particle_changes(h_, seed), "ComputeParticleChangesCL");
+}
+void FluidSystem::Run2Remodelling(uint32_t steps)
+{
+    ComputeBondChangesCL(steps);
+    PrefixSumChangesCL(1);
+    CountingSortChangesCL();
+    ComputeParticleChangesCL(heal_calls_++);
+}
 
 void FluidSystem::Run2PhysicalSort() { check(mg_physical_sort(h_), "Run2PhysicalSort"); }
 void FluidSystem::Run2InnerPhysicalLoop()

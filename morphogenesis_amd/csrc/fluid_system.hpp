@@ -74,6 +74,13 @@ public:
     void TransferPosVelVevalFromTemp() {}  /* folded into AdvanceCL */
     void ZeroVelCL();
     void InitializeBondsCL(); /* src/host_CUDA.cpp:650-660 */
+    /* remodelling (src/host_CL.cpp:865-1062, 1415-1519, 2057-2173) */
+    void ComputeBondChangesCL(uint32_t steps_per_InnerPhysicalLoop);
+    void PrefixSumChangesCL(int zero_offsets);
+    void CountingSortChangesCL();
+    void ComputeParticleChangesCL(uint64_t seed); /* heal's random shifts from (seed, ID) */
+    /* src/fluid_system.cpp:2037-2071; seed = the number of earlier calls */
+    void Run2Remodelling(uint32_t steps_per_InnerPhysicalLoop);
     void AdvanceTime() { m_Time += m_DT; }
 
     /* src/fluid_system.cpp:1919-2014 */
@@ -96,6 +103,7 @@ public:
 private:
     void check(int r, const char* what);
     mg_handle* h_ = nullptr;
+    uint64_t heal_calls_ = 0; /* Run2Remodelling's heal seed */
     int mMaxPoints = 0, mBonds = 0, mGridTotal = 0;
     double m_Time = 0.0, m_DT = 0.003;
     MgGenome m_FGenome{};

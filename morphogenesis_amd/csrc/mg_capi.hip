@@ -270,6 +270,9 @@ struct mg_handle {
     uint32_t *GCNT = nullptr, *GOFF = nullptr, *LENS = nullptr;
     uint32_t *LISTS = nullptr, *PERM = nullptr, *GMN = nullptr;
     bool gene_grid_stale = false;         /* gene counts/offsets/lists not rebuilt since a fused step */
+    /* remodelling change lists, allocated on first use: counts and offsets [9][C], lengths [9],
+     * (slot, bond) pairs [9][2][N], and per slot the bond computeBondChanges counted for heal */
+    uint32_t *CCNT = nullptr, *COFF = nullptr, *CLEN = nullptr, *CL = nullptr, *HS = nullptr;
     /* OFF[gcur] is the exclusive prefix sum of CNT[gcur] (set by a scan, cleared by an insert
      * or an upload of either); with a row-structured gridAdj the sweeps walk rows */
     bool grid_scanned = false;
@@ -751,6 +754,19 @@ int do_sort(mg_handle* h, int lists)
 /* Rebuild FBIN_COUNT_ACTIVE_GENES / FBIN_OFFSET_ACTIVE_GENES / FDENSE_LIST_LENGTHS /
  * FDENSE_LISTS from the sorted state after fused steps (which skip them): the values the
  * reference's insert + scan + dense-list kernels leave after a sort. */
+/* The change-list buffers, allocated and cleared (pairs to MG_NONE) on first use. */
+int ensure_changes(mg_handle* h)
+{
+    const size_t nc = (size_t)MG_NUM_CHANGES * h->C;
+    if (!h->CCNT) { CHK(dalloc(h, &h->CCNT, nc)); CHK(fill_u32(h, h->CCNT, 0, nc)); }
+    if (!h->COFF) { CHK(dalloc(h, &h->COFF, nc)); CHK(fill_u32(h, h->COFF, 0, nc)); }
+    if (!h->CLEN) { CHK(dalloc(h, &h->CLEN, MG_NUM_CHANGES)); CHK(fill_u32(h, h->CLEN, 0, MG_NUM_CHANGES)); }
+    const size_t np = (size_t)MG_NUM_CHANGES * 2 * h->N;
+    if (!h->CL) { CHK(dalloc(h, &h->CL, np)); CHK(fill_u32(h, h->CL, MG_NONE, np)); }
+    if (!h->HS) { CHK(dalloc(h, &h->HS, h->N)); CHK(fill_u32(h, h->HS, MG_NONE, h->N)); }
+    return MG_OK;
+}
+
 int materialize_gene_grid(mg_handle* h)
 {
     if (!h->gene_grid_stale) return MG_OK;
@@ -1169,6 +1185,8 @@ int mg_set_grid(mg_handle* h, const MgGrid* g)
     if (h->have_grid && g->total != h->C) {
         for (int k = 0; k < 2; k++) { dfree(h, h->CNT[k]); dfree(h, h->OFF[k]); }
         dfree(h, h->GCNT); dfree(h, h->GOFF);
+        dfree(h, h->CCNT); dfree(h, h->COFF);
+        h->CCNT = h->COFF = nullptr;
         h->have_grid = false;
     }
     if (!h->have_grid) {
@@ -1274,6 +1292,7 @@ int mg_upload(mg_handle* h, int buf_id, const void* src, size_t bytes)
     h->cells_fresh = false;
     h->pos_binned = false;
     if (buf_id >= MG_FBIN_COUNT_ACTIVE_GENES && buf_id <= MG_FDENSE_LISTS) CHK(materialize_gene_grid(h));
+    if (buf_id >= MG_FBIN_OFFSET_CHANGES && buf_id <= MG_FDENSE_LISTS_CHANGES) CHK(ensure_changes(h));
     CHK(staging(h, bytes));
     HIPCHK(hipMemcpyAsync(h->staging, src, bytes, hipMemcpyHostToDevice, h->st));
     const int c = h->kcur, ic = h->icur;
@@ -1332,6 +1351,10 @@ int mg_upload(mg_handle* h, int buf_id, const void* src, size_t bytes)
     case MG_FBIN_OFFSET_ACTIVE_GENES: HIPCHK(hipMemcpyAsync(h->GOFF, stg, bytes, hipMemcpyDeviceToDevice, h->st)); break;
     case MG_FDENSE_LIST_LENGTHS: HIPCHK(hipMemcpyAsync(h->LENS, stg, bytes, hipMemcpyDeviceToDevice, h->st)); break;
     case MG_FDENSE_LISTS: HIPCHK(hipMemcpyAsync(h->LISTS, stg, bytes, hipMemcpyDeviceToDevice, h->st)); break;
+    case MG_FBIN_COUNT_CHANGES: HIPCHK(hipMemcpyAsync(h->CCNT, stg, bytes, hipMemcpyDeviceToDevice, h->st)); break;
+    case MG_FBIN_OFFSET_CHANGES: HIPCHK(hipMemcpyAsync(h->COFF, stg, bytes, hipMemcpyDeviceToDevice, h->st)); break;
+    case MG_FDENSE_LIST_LENGTHS_CHANGES: HIPCHK(hipMemcpyAsync(h->CLEN, stg, bytes, hipMemcpyDeviceToDevice, h->st)); break;
+    case MG_FDENSE_LISTS_CHANGES: HIPCHK(hipMemcpyAsync(h->CL, stg, bytes, hipMemcpyDeviceToDevice, h->st)); break;
     default: set_err("mg_upload: unsupported buffer id %d", buf_id); return MG_ERR_UNSUPPORTED;
     }
     HIPCHK(hipGetLastError());
@@ -1352,6 +1375,7 @@ int mg_download(mg_handle* h, int buf_id, void* dst, size_t bytes)
     if (bytes != need) { set_err("mg_download: buffer %d needs %zu bytes, got %zu", buf_id, need, bytes); return MG_ERR_SIZE; }
     if (need == 0) return MG_OK;
     if (buf_id >= MG_FBIN_COUNT_ACTIVE_GENES && buf_id <= MG_FDENSE_LISTS) CHK(materialize_gene_grid(h));
+    if (buf_id >= MG_FBIN_OFFSET_CHANGES && buf_id <= MG_FDENSE_LISTS_CHANGES) CHK(ensure_changes(h));
     CHK(staging(h, bytes));
     const int c = h->kcur, ic = h->icur;
     const unsigned nb = nblk(N);
@@ -1393,6 +1417,10 @@ int mg_download(mg_handle* h, int buf_id, void* dst, size_t bytes)
     case MG_FBIN_OFFSET_ACTIVE_GENES: direct = h->GOFF; break;
     case MG_FDENSE_LIST_LENGTHS: direct = h->LENS; break;
     case MG_FDENSE_LISTS: direct = h->LISTS; break;
+    case MG_FBIN_COUNT_CHANGES: direct = h->CCNT; break;
+    case MG_FBIN_OFFSET_CHANGES: direct = h->COFF; break;
+    case MG_FDENSE_LIST_LENGTHS_CHANGES: direct = h->CLEN; break;
+    case MG_FDENSE_LISTS_CHANGES: direct = h->CL; break;
     default: set_err("mg_download: unsupported buffer id %d", buf_id); return MG_ERR_UNSUPPORTED;
     }
     HIPCHK(hipGetLastError());
@@ -1482,6 +1510,72 @@ int mg_initialize_bonds(mg_handle* h)
     CHK(materialize_gene_grid(h));
     hipLaunchKernelGGL(k_initialize_bonds, dim3(nblk(h->N)), dim3(256), 0, h->st, make_ds(h), h->p, h->g,
                        h->gen, h->PID, h->BCOLD);
+    HIPCHK(hipGetLastError());
+    return rebuild_inc(h); /* new partners: incoming-bond CSR and live-slot masks */
+}
+
+int mg_compute_bond_changes(mg_handle* h, uint32_t steps_per_inner_loop)
+{
+    CHK(ready(h));
+    if (steps_per_inner_loop == 0) {
+        set_err("mg_compute_bond_changes: steps_per_inner_loop must be > 0");
+        return MG_ERR_ARG;
+    }
+    CHK(ensure_changes(h));
+    CHK(materialize_gene_grid(h));
+    CHK(fill_u32(h, h->CCNT, 0, (size_t)MG_NUM_CHANGES * h->C));
+    CHK(fill_u32(h, h->COFF, 0, (size_t)MG_NUM_CHANGES * h->C));
+    CHK(fill_u32(h, h->HS, MG_NONE, h->N));
+    if (h->B == 0) return MG_OK;
+    TanhTab tt;
+    std::memcpy(tt.t, h->gen.tanh_param, sizeof(tt.t));
+    hipLaunchKernelGGL(k_bond_changes, dim3(nblk(h->N)), dim3(256), 0, h->st, make_ds(h), tt,
+                       steps_per_inner_loop, h->CCNT, h->HS, h->BCOLD);
+    HIPCHK(hipGetLastError());
+    return MG_OK;
+}
+
+int mg_prefix_sum_changes(mg_handle* h, int zero_offsets)
+{
+    (void)zero_offsets;
+    CHK(ready(h));
+    CHK(ensure_changes(h));
+    hipLaunchKernelGGL(k_scan, dim3(MG_NUM_CHANGES), dim3(1024), 0, h->st, h->CCNT, h->COFF, h->C, h->ctr, 0, 0,
+                       h->CLEN);
+    HIPCHK(hipGetLastError());
+    return MG_OK;
+}
+
+int mg_counting_sort_changes(mg_handle* h)
+{
+    CHK(ready(h));
+    CHK(ensure_changes(h));
+    CHK(fill_u32(h, h->CL, MG_NONE, (size_t)MG_NUM_CHANGES * 2 * h->N));
+    if (h->B == 0) return MG_OK;
+    hipLaunchKernelGGL(k_sort_changes, dim3(nblk(h->C)), dim3(256), 0, h->st, make_ds(h), h->CCNT, h->COFF, h->HS,
+                       h->CL, h->BCOLD);
+    HIPCHK(hipGetLastError());
+    return MG_OK;
+}
+
+int mg_compute_particle_changes(mg_handle* h, uint64_t seed)
+{
+    CHK(ready(h));
+    CHK(ensure_changes(h));
+    uint32_t len[MG_NUM_CHANGES];
+    HIPCHK(hipMemcpyAsync(len, h->CLEN, sizeof(len), hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    for (int l = 1; l < MG_NUM_CHANGES; l++)
+        if (len[l] != 0) {
+            set_err("mg_compute_particle_changes: change list %d holds %u entries; only heal (list 0) is built", l,
+                    len[l]);
+            return MG_ERR_UNSUPPORTED;
+        }
+    if (h->B == 0 || len[0] == 0) return MG_OK;
+    const float4 elastin = make_float4(h->gen.param[0][8], h->gen.param[0][9], h->gen.param[0][10], h->gen.param[0][11]);
+    const uint32_t n = std::min<uint32_t>(len[0], (uint32_t)h->N);
+    hipLaunchKernelGGL(k_heal, dim3(nblk(n)), dim3(256), 0, h->st, make_ds(h), h->p, h->g, elastin, h->CL, h->CLEN,
+                       h->PID, h->BCOLD, (unsigned long long)seed);
     HIPCHK(hipGetLastError());
     return rebuild_inc(h); /* new partners: incoming-bond CSR and live-slot masks */
 }

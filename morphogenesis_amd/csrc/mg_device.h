@@ -164,6 +164,64 @@ __host__ __device__ __forceinline__ float mg_acosf(float x)
     return x < 0.0f ? pio2 + mg_asinf_half(-x) : pio2 - mg_asinf_half(x);
 }
 
+/* exp and tanh in float by the Cephes single-precision method (S. L. Moshier, Cephes Math
+ * Library, expf.c / tanhf.c), every operation in float:
+ *   exp(x) = 2^n exp(r), n = floor(x log2(e) + 1/2), r = x - n ln2 with ln2 split in two
+ *   parts (0.693359375 and -2.12194440e-4), exp(r) by the published degree-5 polynomial;
+ *   tanh(x) = sign(x) (1 - 2 / (exp(2|x|) + 1)) for |x| >= 0.625, +-1 above 44.36, and the
+ *   published odd polynomial below 0.625; evaluated on |x| with the sign copied back, so
+ *   tanh(-x) = -tanh(x) exactly, signed zeros included.
+ * Max error about 1.2 ulp against the exact function. Used by the bond remodelling
+ * (the reference calls CUDA's tanh there); the oracle evaluates the same operations. */
+__host__ __device__ __forceinline__ float mg_expf(float x)
+{
+    if (x > 88.72283905206835f) return __builtin_inff();
+    if (x < -103.278929903431851103f) return 0.0f;
+    const float z = floorf(1.44269504088896341f * x + 0.5f);
+    x = x - z * 0.693359375f;
+    x = x - z * -2.12194440e-4f;
+    const int n = (int)z;
+    const float x2 = x * x;
+    float p = 1.9875691500e-4f;
+    p = p * x + 1.3981999507e-3f;
+    p = p * x + 8.3334519073e-3f;
+    p = p * x + 4.1665795894e-2f;
+    p = p * x + 1.6666665459e-1f;
+    p = p * x + 5.0000001201e-1f;
+    return ldexpf(p * x2 + x + 1.0f, n);
+}
+__host__ __device__ __forceinline__ float mg_tanhf(float x)
+{
+    const float a = fabsf(x);
+    float r;
+    if (a > 44.361419526034175f) {
+        r = 1.0f;
+    } else if (a >= 0.625f) {
+        r = 1.0f - 2.0f / (mg_expf(a + a) + 1.0f);
+    } else {
+        const float s = a * a;
+        float p = -5.70498872745e-3f;
+        p = p * s + 2.06390887954e-2f;
+        p = p * s + -5.37397155531e-2f;
+        p = p * s + 1.33314422036e-1f;
+        p = p * s + -3.33332819422e-1f;
+        r = p * s * a + a;
+    }
+    return copysignf(r, x);
+}
+
+/* The random word heal draws for a particle: the SplitMix64 output function (G. Steele,
+ * D. Lea, C. Flood, "Fast splittable pseudorandom number generators", OOPSLA 2014) of
+ * seed + golden * (particle ID + 1), low 32 bits. The reference draws from per-particle
+ * curand states instead; this word depends only on (seed, ID), not on the launch. */
+__host__ __device__ __forceinline__ uint32_t mg_heal_word(uint64_t seed, uint32_t pid)
+{
+    uint64_t z = seed + 0x9E3779B97F4A7C15ull * ((uint64_t)pid + 1ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return (uint32_t)(z ^ (z >> 31));
+}
+
 /* XCD-aware block index: the dispatcher places consecutive workgroups on different XCDs
  * (round robin over 8); remap so each XCD walks one contiguous range of particles and
  * neighbour data stays in that XCD's L2. */

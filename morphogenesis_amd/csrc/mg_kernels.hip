@@ -1283,16 +1283,77 @@ __global__ LAUNCH_BOUNDS_SWEEP void k_diffuse_genes_insert(DS d, MgParams p, MgG
 }
 
 /* -------------------------------------------------------- bond formation
- * initialize_bonds + contribFindBonds (src/fluid_system_cuda.cu:1938-2044) for the particles of
- * gene 1's dense list ("solid") below mActivePoints. For each bond slot in turn, every
- * candidate j of the 27 cells gc + gridAdj[c] (the reference does not subtract the (1,1,1)
- * offset here, so the block is the one above and right of the particle's cell) with a
- * non-zero particle ID and 0 < |x_i - x_j|^2 < rd2 is scored by the sum over the particle's
- * current bonds k of |1.91 - acos(u.v / (|u| |v|))| (u = x_i - x_j, v = x_i - x_k; FLT_MAX when
- * j is already bonded), or by |u|^2 for slot 0; the first strictly best candidate is written
- * into the slot with the genome's default parameters of the slot's bond type (slots 0-3 by
- * tissue gene: bone 9 -> apatite, tendon 6 / muscle 7 / elastic ligament 10 -> collagen in
- * slot 0; elastin otherwise). Each particle writes only its own records. */
+ * contribFindBonds (src/fluid_system_cuda.cu:1938-1976) over the cells gc + gridAdj[c] for one
+ * bond slot of the particle at slot i (home h), seen from the point x (the particle's position
+ * for initialize_bonds, a shifted point for heal). The reference does not subtract the (1,1,1)
+ * offset here, so the block is the one above and right of the particle's cell. Every candidate
+ * j with a non-zero particle ID and 0 < |x - x_j|^2 < rd2 is scored by the sum over the
+ * particle's current bonds k of |1.91 - acos(u.v / (|u| |v|))| (u = x - x_j, v = x - x_k;
+ * FLT_MAX when j is already bonded), or by |u|^2 for slot 0; returns the first strictly best
+ * candidate, or MG_NONE. */
+__device__ __forceinline__ uint32_t find_bond(const DS& d, const MgGrid& g, const uint32_t* __restrict__ PID,
+                                              uint32_t gc, uint32_t h, f3 x, int bond, float rd2)
+{
+    const uint32_t N = (uint32_t)d.N, C = (uint32_t)d.C;
+    const int B = d.B;
+    float best = FLT_MAX;
+    uint32_t to = MG_NONE;
+    for (int c = 0; c < g.adj_cnt; c++) {
+        const uint32_t cell = gc + (uint32_t)g.adj[c];
+        if (cell >= C) continue;
+        const uint32_t cn = d.CNT[cell];
+        if (cn == 0u) continue;
+        const uint32_t c0 = d.OFF[cell];
+        for (uint32_t j = c0; j < c0 + cn && j < N; j++) {
+            if (PID[d.HOME[j]] == 0u) continue;
+            const f3 u = x - xyz(d.P[j]);
+            const float dsq = u.x * u.x + u.y * u.y + u.z * u.z;
+            if (!(dsq < rd2 && dsq > 0.0f)) continue;
+            float theta = 0.0f;
+            for (int b = 0; b < B; b++) {
+                const uint32_t kh = d.BPARTNER[(size_t)b * N + h];
+                const uint32_t k = kh == MG_NONE ? MG_NONE : d.SLOT[kh];
+                if (j == k) {
+                    theta = FLT_MAX;
+                    break;
+                }
+                if (k < N) {
+                    const f3 v = x - xyz(d.P[k]);
+                    const float lu = sqrtf(u.x * u.x + u.y * u.y + u.z * u.z);
+                    const float lv = sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
+                    const float cosang = (u.x * v.x + u.y * v.y + u.z * v.z) / (lu * lv);
+                    theta = (float)((double)theta + fabs(1.91 - (double)mg_acosf(cosang)));
+                }
+            }
+            if (bond == 0) theta = dsq;
+            if (theta < best) {
+                best = theta;
+                to = j;
+            }
+        }
+    }
+    return to;
+}
+
+/* Bond types of slots 0-3 by tissue gene (src/fluid_system_cuda.cu:1133-1139, :2003-2009):
+ * bone 9 -> apatite (2) in all four; tendon 6 / muscle 7 / elastic ligament 10 -> collagen (1)
+ * in slot 0; elastin (0) otherwise. The reference's cartilage branch tests gene 6 again after
+ * the tendon branch and is never taken. */
+__device__ __forceinline__ void bond_types(const DS& d, uint32_t h, int bt4[4])
+{
+    auto gene_on = [&](int gidx) { return d.EPIGEN[(size_t)gidx * d.N + h] > 0u; };
+    bt4[0] = bt4[1] = bt4[2] = bt4[3] = 0;
+    if (gene_on(9)) {
+        bt4[0] = bt4[1] = bt4[2] = bt4[3] = 2;
+    } else if (gene_on(6) || gene_on(7) || gene_on(10)) {
+        bt4[0] = 1;
+    }
+}
+
+/* initialize_bonds (src/fluid_system_cuda.cu:1978-2044) for the particles of gene 1's dense
+ * list ("solid") below mActivePoints: for each bond slot in turn the best candidate of
+ * find_bond is written into the slot with the genome's default parameters of the slot's bond
+ * type. Each particle writes only its own records. */
 __global__ LAUNCH_BOUNDS_256 void k_initialize_bonds(DS d, MgParams p, MgGrid g, MgGenome gen,
                                                      const uint32_t* __restrict__ PID, uint2* BCOLD)
 {
@@ -1307,52 +1368,10 @@ __global__ LAUNCH_BOUNDS_256 void k_initialize_bonds(DS d, MgParams p, MgGrid g,
     if (gc == MG_GRID_UNDEF) return;
     const uint32_t h = d.HOME[i];
     const f3 ipos = xyz(d.P[i]);
-    int bt4[4] = {0, 0, 0, 0}; /* bond type of slots 0-3: 0 elastin, 1 collagen, 2 apatite */
-    auto gene_on = [&](int gidx) { return d.EPIGEN[(size_t)gidx * N + h] > 0u; };
-    if (gene_on(9)) {
-        bt4[0] = bt4[1] = bt4[2] = bt4[3] = 2;
-    } else if (gene_on(6) || gene_on(7) || gene_on(10)) {
-        bt4[0] = 1;
-    }
-    const float rd2 = p.rd2;
-    const uint32_t C = (uint32_t)d.C;
+    int bt4[4];
+    bond_types(d, h, bt4);
     for (int bond = 0; bond < B; bond++) {
-        float best = FLT_MAX;
-        uint32_t to = MG_NONE;
-        for (int c = 0; c < g.adj_cnt; c++) {
-            const uint32_t cell = gc + (uint32_t)g.adj[c];
-            if (cell >= C) continue;
-            const uint32_t cn = d.CNT[cell];
-            if (cn == 0u) continue;
-            const uint32_t c0 = d.OFF[cell];
-            for (uint32_t j = c0; j < c0 + cn && j < N; j++) {
-                if (PID[d.HOME[j]] == 0u) continue;
-                const f3 u = ipos - xyz(d.P[j]);
-                const float dsq = u.x * u.x + u.y * u.y + u.z * u.z;
-                if (!(dsq < rd2 && dsq > 0.0f)) continue;
-                float theta = 0.0f;
-                for (int b = 0; b < B; b++) {
-                    const uint32_t kh = d.BPARTNER[(size_t)b * N + h];
-                    const uint32_t k = kh == MG_NONE ? MG_NONE : d.SLOT[kh];
-                    if (j == k) {
-                        theta = FLT_MAX;
-                        break;
-                    }
-                    if (k < N) {
-                        const f3 v = ipos - xyz(d.P[k]);
-                        const float lu = sqrtf(u.x * u.x + u.y * u.y + u.z * u.z);
-                        const float lv = sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
-                        const float cosang = (u.x * v.x + u.y * v.y + u.z * v.z) / (lu * lv);
-                        theta = (float)((double)theta + fabs(1.91 - (double)mg_acosf(cosang)));
-                    }
-                }
-                if (bond == 0) theta = dsq;
-                if (theta < best) {
-                    best = theta;
-                    to = j;
-                }
-            }
-        }
+        const uint32_t to = find_bond(d, g, PID, gc, h, ipos, bond, p.rd2);
         if (to < na) {
             const int bt = bond < 4 ? bt4[bond] : 0;
             const size_t bi = (size_t)bond * N + h;
@@ -1362,6 +1381,146 @@ __global__ LAUNCH_BOUNDS_256 void k_initialize_bonds(DS d, MgParams p, MgGrid g,
             d.BINT[bi] = make_float2(0.0f, 0.0f);
             BCOLD[bi].x = PID[th];
         }
+    }
+}
+
+/* ------------------------------------------------------------- remodelling
+ * Change lists (src/fluid.h:103): list l of cell c holds the (slot, bond) pairs of the cell's
+ * particles whose change word (bond record word 8) has bit l set; list 0 is heal. */
+
+/* computeBondChanges (src/fluid_system_cuda.cu:1065-1330) for the particles of gene 2's dense
+ * list ("living") below mActivePoints. Per bond slot in order: a slot with rest length > 0 has
+ * its strain integrators averaged over `steps` and cleared, its rest length multiplied by
+ * l_a + l_b tanh(l_c <strain> - l_d) and its modulus by s_a + s_b tanh(s_c <strain^2> - s_d)
+ * (tanh_param row of the slot's bond type); the change word is cleared; the first slot that
+ * is broken (rest length 0) or empty is counted into the cell's heal count, gets change bit 0,
+ * is recorded in HS[slot] (MG_NONE otherwise) and ends the particle's loop. Slots after it
+ * keep their change words. */
+struct TanhTab {
+    float t[3][8];
+};
+__global__ LAUNCH_BOUNDS_256 void k_bond_changes(DS d, TanhTab tt, uint32_t steps, uint32_t* __restrict__ CCNT,
+                                                 uint32_t* __restrict__ HS, uint2* BCOLD)
+{
+    const uint32_t li = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t N = (uint32_t)d.N;
+    const int B = d.B;
+    if (li >= N || li >= d.LENS[2]) return;
+    const uint32_t i = d.LISTS[(size_t)2 * N + li];
+    if (i >= d.ctr[MG_C_ACTIVE]) return;
+    const uint32_t h = d.HOME[i];
+    int bt4[4];
+    bond_types(d, h, bt4);
+    const float fs = (float)steps;
+    for (int a = 0; a < B; a++) {
+        const size_t bi = (size_t)a * N + h;
+        float4 bp = d.BPAR[bi];
+        if (bp.y > 0.0f) {
+            const float2 in = d.BINT[bi];
+            const float si = in.y / fs, ssi = in.x / fs;
+            d.BINT[bi] = make_float2(0.0f, 0.0f);
+            const float* t = tt.t[a < 4 ? bt4[a] : 0];
+            bp.y = bp.y * (t[0] + t[1] * mg_tanhf(t[2] * si - t[3]));
+            bp.z = bp.z * (t[4] + t[5] * mg_tanhf(t[6] * ssi - t[7]));
+            d.BPAR[bi] = bp;
+        }
+        BCOLD[bi].y = 0u;
+        if (bp.y == 0.0f || d.BPARTNER[bi] == MG_NONE) {
+            const uint32_t gc = d.GCELL[i];
+            if (gc < (uint32_t)d.C) {
+                atomicAdd(&CCNT[gc], 1u);
+                HS[i] = (uint32_t)a;
+            }
+            BCOLD[bi].y = 1u;
+            break;
+        }
+    }
+}
+
+/* countingSortChanges (src/fluid_system_cuda.cu:406-618), one thread per cell: the cell's
+ * particles in slot order, their bond slots in order, and for each set change bit l the pair
+ * (slot, bond) goes to list l at COFF[l][c] + the cell's running count. List 0 (heal) takes
+ * exactly the pairs computeBondChanges counted (HS): the reference re-reads change bit 0,
+ * which slots after a particle's counted one can still carry from earlier calls, and then
+ * writes past the cell's range into the next cell's. Every list is also capped at CCNT[l][c]
+ * pairs per cell. A cell whose range ends beyond mActivePoints is skipped, as in the
+ * reference. CL holds list l's slots at [l][0][*] and bond indices at [l][1][*]. */
+__global__ LAUNCH_BOUNDS_256 void k_sort_changes(DS d, const uint32_t* __restrict__ CCNT,
+                                                 const uint32_t* __restrict__ COFF, const uint32_t* __restrict__ HS,
+                                                 uint32_t* __restrict__ CL, const uint2* __restrict__ BCOLD)
+{
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t C = (uint32_t)d.C, N = (uint32_t)d.N;
+    if (c >= C) return;
+    uint32_t cap[MG_NUM_CHANGES], any = 0;
+    for (int l = 0; l < MG_NUM_CHANGES; l++) {
+        cap[l] = CCNT[(size_t)l * C + c];
+        any |= cap[l];
+    }
+    if (any == 0u) return; /* nothing can be written */
+    const uint32_t count = d.CNT[c];
+    if (count == 0u) return;
+    const uint32_t off = d.OFF[c];
+    if (off + count > d.ctr[MG_C_ACTIVE]) return;
+    uint32_t base[MG_NUM_CHANGES], n[MG_NUM_CHANGES];
+    for (int l = 0; l < MG_NUM_CHANGES; l++) {
+        base[l] = COFF[(size_t)l * C + c];
+        n[l] = 0;
+    }
+    auto put = [&](int l, uint32_t s, uint32_t a) {
+        if (n[l] >= cap[l]) return;
+        const uint32_t at = base[l] + n[l]++;
+        if (at >= N) return;
+        CL[(size_t)l * 2 * N + at] = s;
+        CL[(size_t)l * 2 * N + N + at] = a;
+    };
+    for (uint32_t s = off; s < off + count; s++) {
+        const uint32_t hs = HS[s];
+        const uint32_t h = d.HOME[s];
+        for (int a = 0; a < d.B; a++) {
+            if (hs == (uint32_t)a) put(0, s, (uint32_t)a);
+            const uint32_t ch = BCOLD[(size_t)a * N + h].y;
+            for (int l = 1; l < MG_NUM_CHANGES; l++)
+                if ((ch >> l) & 1u) put(l, s, (uint32_t)a);
+        }
+    }
+}
+
+/* heal (src/fluid_system_cuda.cu:2047-2107) over list 0: the particle's position is shifted
+ * by -sqrt(rd2) / (4 + r) per axis where the axis's sign bit of the random word is set
+ * (r = 3 bits of the word; x: bits 0-2 and 3, y: 4-6 and 7, z: 8-10 and 11), find_bond runs
+ * from the shifted point for the listed slot, and a candidate found is bonded with elastin's
+ * default parameters (makeBond, :1573-1596; the partner's incoming list is derived from the
+ * owners afterwards). Each listed particle appears once, so each thread writes only its own
+ * record. */
+__global__ LAUNCH_BOUNDS_256 void k_heal(DS d, MgParams p, MgGrid g, float4 elastin, const uint32_t* __restrict__ CL,
+                                         const uint32_t* __restrict__ CLEN, const uint32_t* __restrict__ PID,
+                                         uint2* BCOLD, unsigned long long seed)
+{
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t N = (uint32_t)d.N;
+    if (k >= N || k >= CLEN[0]) return;
+    const uint32_t i = CL[k];
+    if (i >= d.ctr[MG_C_ACTIVE]) return;
+    const uint32_t bond = CL[N + k];
+    if (bond >= (uint32_t)d.B) return;
+    const uint32_t gc = d.GCELL[i];
+    if (gc == MG_GRID_UNDEF) return;
+    const uint32_t h = d.HOME[i];
+    const uint32_t r = mg_heal_word(seed, PID[h]);
+    const float max_len = sqrtf(p.rd2);
+    f3 x = xyz(d.P[i]);
+    x.x = x.x + (max_len / (float)(4u + (r & 7u))) * (-1.0f * (float)(1u & (r >> 3)));
+    x.y = x.y + (max_len / (float)(4u + ((r >> 4) & 7u))) * (-1.0f * (float)(1u & (r >> 7)));
+    x.z = x.z + (max_len / (float)(4u + ((r >> 8) & 7u))) * (-1.0f * (float)(1u & (r >> 11)));
+    const uint32_t to = find_bond(d, g, PID, gc, h, x, (int)bond, p.rd2);
+    if (to < N) {
+        const size_t bi = (size_t)bond * N + h;
+        const uint32_t th = d.HOME[to];
+        d.BPARTNER[bi] = th;
+        d.BPAR[bi] = elastin;
+        d.BINT[bi] = make_float2(0.0f, 0.0f);
+        BCOLD[bi] = make_uint2(PID[th], 0u);
     }
 }
 

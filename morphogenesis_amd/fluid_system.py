@@ -142,6 +142,38 @@ class FluidSystem:
         sorted state."""
         self._call("initialize_bonds", self.h)
 
+    # ------------------------------------------------------------------ remodelling
+    def ComputeBondChangesCL(self, steps_per_inner_loop: int):
+        """computeBondChanges over gene 2's dense list (src/host_CL.cpp:2057-2086): remodels
+        rest length and modulus from the averaged strain integrators and counts each particle's
+        first broken or empty bond slot for heal."""
+        self._call("compute_bond_changes", self.h, int(steps_per_inner_loop))
+
+    def PrefixSumChangesCL(self, zero_offsets: int = 1):
+        """Per-cell offsets and lengths of the nine change lists (src/host_CL.cpp:865-1062)."""
+        self._call("prefix_sum_changes", self.h, int(zero_offsets))
+
+    def CountingSortChangesCL(self):
+        """The (slot, bond) pairs of each change list (src/host_CL.cpp:1415-1519)."""
+        self._call("counting_sort_changes", self.h)
+
+    def ComputeParticleChangesCL(self, seed: int | None = None):
+        """heal over list 0 (src/host_CL.cpp:2088-2173). The random shifts come from
+        (seed, particle ID); without a seed the handle's remodelling counter is used."""
+        if seed is None:
+            seed = self._heal_seed
+            self._heal_seed += 1
+        self._call("compute_particle_changes", self.h, int(seed) & 0xFFFFFFFFFFFFFFFF)
+
+    _heal_seed = 0
+
+    def Run2Remodelling(self, steps_per_inner_loop: int, seed: int | None = None):
+        """src/fluid_system.cpp:2037-2071: bond changes, their lists, then heal."""
+        self.ComputeBondChangesCL(steps_per_inner_loop)
+        self.PrefixSumChangesCL(1)
+        self.CountingSortChangesCL()
+        self.ComputeParticleChangesCL(seed)
+
     def ZeroVelCL(self):
         self._call("zero_vel", self.h)
 

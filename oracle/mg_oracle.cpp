@@ -82,6 +82,62 @@ inline float acos_f(float x)
 inline double acos_r(double x) { return std::acos(x); }
 inline float acos_r(float x) { return acos_f(x); }
 
+/* exp and tanh in float by the Cephes single-precision method (S. L. Moshier, Cephes Math
+ * Library, expf.c / tanhf.c), the operations of mg_expf / mg_tanhf in
+ * morphogenesis_amd/csrc/mg_device.h one for one: exp(x) = 2^n exp(r) with n = floor(x log2(e)
+ * + 1/2), ln2 split in two parts and the published degree-5 polynomial; tanh(x) =
+ * sign(x) (1 - 2 / (exp(2|x|) + 1)) for |x| >= 0.625, +-1 above 44.36, the published odd
+ * polynomial below; evaluated on |x| with the sign copied back. Used by the bond remodelling
+ * (the reference calls CUDA's tanh there). */
+inline float exp_f(float x)
+{
+    if (x > 88.72283905206835f) return HUGE_VALF;
+    if (x < -103.278929903431851103f) return 0.0f;
+    const float z = std::floor(1.44269504088896341f * x + 0.5f);
+    x = x - z * 0.693359375f;
+    x = x - z * -2.12194440e-4f;
+    const int n = (int)z;
+    const float x2 = x * x;
+    float p = 1.9875691500e-4f;
+    p = p * x + 1.3981999507e-3f;
+    p = p * x + 8.3334519073e-3f;
+    p = p * x + 4.1665795894e-2f;
+    p = p * x + 1.6666665459e-1f;
+    p = p * x + 5.0000001201e-1f;
+    return std::ldexp(p * x2 + x + 1.0f, n);
+}
+inline float tanh_f(float x)
+{
+    const float a = std::fabs(x);
+    float r;
+    if (a > 44.361419526034175f) {
+        r = 1.0f;
+    } else if (a >= 0.625f) {
+        r = 1.0f - 2.0f / (exp_f(a + a) + 1.0f);
+    } else {
+        const float s = a * a;
+        float p = -5.70498872745e-3f;
+        p = p * s + 2.06390887954e-2f;
+        p = p * s + -5.37397155531e-2f;
+        p = p * s + 1.33314422036e-1f;
+        p = p * s + -3.33332819422e-1f;
+        r = p * s * a + a;
+    }
+    return std::copysign(r, x);
+}
+inline double tanh_r(double x) { return std::tanh(x); }
+inline float tanh_r(float x) { return tanh_f(x); }
+
+/* heal's random word: the SplitMix64 output function (Steele, Lea, Flood, OOPSLA 2014) of
+ * seed + golden * (particle ID + 1), low 32 bits; mg_heal_word in mg_device.h. */
+inline uint32_t heal_word(uint64_t seed, uint32_t pid)
+{
+    uint64_t z = seed + 0x9E3779B97F4A7C15ull * ((uint64_t)pid + 1ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return (uint32_t)(z ^ (z >> 31));
+}
+
 inline float u2f(uint32_t u) { float f; std::memcpy(&f, &u, 4); return f; }
 inline uint32_t f2u(float f) { uint32_t u; std::memcpy(&u, &f, 4); return u; }
 
@@ -113,6 +169,11 @@ struct Base {
     virtual void genes() = 0;
     virtual void zero_vel() = 0;
     virtual void initialize_bonds() = 0;
+    virtual void compute_bond_changes(uint32_t steps) = 0;
+    virtual void prefix_sum_changes() = 0;
+    virtual void counting_sort_changes() = 0;
+    virtual void heal(uint64_t seed) = 0;
+    uint32_t clen[MG_NUM_CHANGES] = {};
     virtual void advance_special(float time, float dt, float ss, float zoff) = 0;
     MgParams p;
     MgGrid g;
@@ -138,6 +199,8 @@ template <typename R> struct Oracle : Base {
     std::vector<uint32_t> home;              /* [N] creation slot of the occupant      */
     std::vector<uint32_t> cnt, off, gcnt, goff, lists;
     uint32_t lens[MG_NUM_GENES];
+    /* change lists: counts / offsets [9][C], pairs [9][2][N], heal slot per sorted slot */
+    std::vector<uint32_t> ccnt, coff, clists, hs;
 
     Oracle(int n, int b) : N(n), B(b)
     {
@@ -154,6 +217,8 @@ template <typename R> struct Oracle : Base {
         for (int i = 0; i < N; i++) home[i] = (uint32_t)i;
         lists.assign((size_t)MG_NUM_GENES * N, MG_NONE);
         std::memset(lens, 0, sizeof(lens));
+        clists.assign((size_t)MG_NUM_CHANGES * 2 * N, MG_NONE);
+        hs.assign(N, MG_NONE);
         num_points = active_points = (uint32_t)N;
     }
 
@@ -162,6 +227,7 @@ template <typename R> struct Oracle : Base {
         C = g.total;
         cnt.assign(C, 0); off.assign(C, 0);
         gcnt.assign((size_t)MG_NUM_GENES * C, 0); goff = gcnt;
+        ccnt.assign((size_t)MG_NUM_CHANGES * C, 0); coff = ccnt;
     }
 
     /* ---- bond record float-slot access ---- */
@@ -222,6 +288,10 @@ template <typename R> struct Oracle : Base {
         case MG_FBIN_OFFSET_ACTIVE_GENES: std::memcpy(goff.data(), u, bytes); break;
         case MG_FDENSE_LIST_LENGTHS: std::memcpy(lens, u, bytes); break;
         case MG_FDENSE_LISTS: std::memcpy(lists.data(), u, bytes); break;
+        case MG_FBIN_COUNT_CHANGES: std::memcpy(ccnt.data(), u, bytes); break;
+        case MG_FBIN_OFFSET_CHANGES: std::memcpy(coff.data(), u, bytes); break;
+        case MG_FDENSE_LIST_LENGTHS_CHANGES: std::memcpy(clen, u, bytes); break;
+        case MG_FDENSE_LISTS_CHANGES: std::memcpy(clists.data(), u, bytes); break;
         default: return MG_ERR_UNSUPPORTED;
         }
         return MG_OK;
@@ -272,6 +342,10 @@ template <typename R> struct Oracle : Base {
         case MG_FBIN_OFFSET_ACTIVE_GENES: std::memcpy(u, goff.data(), bytes); break;
         case MG_FDENSE_LIST_LENGTHS: std::memcpy(u, lens, bytes); break;
         case MG_FDENSE_LISTS: std::memcpy(u, lists.data(), bytes); break;
+        case MG_FBIN_COUNT_CHANGES: std::memcpy(u, ccnt.data(), bytes); break;
+        case MG_FBIN_OFFSET_CHANGES: std::memcpy(u, coff.data(), bytes); break;
+        case MG_FDENSE_LIST_LENGTHS_CHANGES: std::memcpy(u, clen, bytes); break;
+        case MG_FDENSE_LISTS_CHANGES: std::memcpy(u, clists.data(), bytes); break;
         default: return MG_ERR_UNSUPPORTED;
         }
         return MG_OK;
@@ -825,65 +899,211 @@ template <typename R> struct Oracle : Base {
      * modulus and damping of the slot's bond type (src/fluid_system_cuda.cu:1975-2043).
      * FPARTICLEIDX is not updated, as in the reference. The candidate index is the slot
      * itself (FBIN is the identity after a sort, src/fluid_system_cuda.cu:221). */
+    /* contribFindBonds (src/fluid_system_cuda.cu:1938-1976) over the cells gc + gridAdj[c]
+     * (no (1,1,1) offset, as in the reference) for bond slot `bond` of particle i, seen from x:
+     * candidates with a non-zero ID and 0 < |x - x_j|^2 < rd2, scored by the sum over i's
+     * current bonds k of |1.91 - acos(u.v / (|u| |v|))| (FLT_MAX when j is already bonded) or by
+     * |u|^2 for slot 0; the first strictly best, or MG_NONE. */
+    uint32_t find_bond(uint32_t i, uint32_t gc, V3<R> x, int bond) const
+    {
+        const R rd2 = (R)p.rd2;
+        R best = (R)FLT_MAX;
+        uint32_t to = MG_NONE;
+        for (int c = 0; c < g.adj_cnt; c++) {
+            const uint32_t cell = gc + (uint32_t)g.adj[c];
+            if (cell >= (uint32_t)C || cnt[cell] == 0) continue;
+            const uint32_t c0 = off[cell];
+            for (uint32_t j = c0; j < c0 + cnt[cell] && j < (uint32_t)N; j++) {
+                if (pid[j] == 0u) continue;
+                const V3<R> u = x - P(j);
+                const R dsq = dot(u, u);
+                if (!(dsq < rd2 && dsq > R(0))) continue;
+                R theta = R(0);
+                for (int b = 0; b < B; b++) {
+                    const uint32_t k = bu[brec(i, b) + MG_B_INDEX];
+                    if (j == k) {
+                        theta = (R)FLT_MAX;
+                        break;
+                    }
+                    if (k < (uint32_t)N) {
+                        const V3<R> v = x - P(k);
+                        const R lu = std::sqrt(dot(u, u)), lv = std::sqrt(dot(v, v));
+                        const R cosang = dot(u, v) / (lu * lv);
+                        theta = (R)((double)theta + std::fabs(1.91 - (double)acos_r(cosang)));
+                    }
+                }
+                if (bond == 0) theta = dsq;
+                if (theta < best) {
+                    best = theta;
+                    to = j;
+                }
+            }
+        }
+        return to;
+    }
+
+    /* bond types of slots 0-3 by tissue gene (src/fluid_system_cuda.cu:1133-1139, :2003-2009):
+     * bone 9 -> apatite in all four, tendon 6 / muscle 7 / elastic ligament 10 -> collagen in
+     * slot 0, elastin otherwise (the cartilage branch re-tests gene 6 and is never taken) */
+    void bond_types(uint32_t i, int bt4[4]) const
+    {
+        auto on = [&](int gi) { return epigen[(size_t)gi * N + i] > 0u; };
+        bt4[0] = bt4[1] = bt4[2] = bt4[3] = 0;
+        if (on(9)) bt4[0] = bt4[1] = bt4[2] = bt4[3] = 2;
+        else if (on(6) || on(7) || on(10)) bt4[0] = 1;
+    }
+
+    /* initialize_bonds (src/fluid_system_cuda.cu:1978-2044) over gene 1's dense list below
+     * mActivePoints: each slot in turn takes find_bond's candidate with the slot type's
+     * default parameters. */
     void initialize_bonds() override
     {
         const uint32_t na = active_points, len = lens[1];
-        const R rd2 = (R)p.rd2;
         for (uint32_t li = 0; li < len && li < (uint32_t)N; li++) {
             const uint32_t i = lists[(size_t)N + li];
             if (i >= na) continue;
             const uint32_t gc = gcell[i];
             if (gc == MG_GRID_UNDEF) continue;
-            int bt4[4] = {0, 0, 0, 0};
-            auto on = [&](int gi) { return epigen[(size_t)gi * N + i] > 0u; };
-            if (on(9)) bt4[0] = bt4[1] = bt4[2] = bt4[3] = 2;       /* bone: apatite */
-            else if (on(6) || on(7) || on(10)) bt4[0] = 1;         /* collagen in slot 0 */
-            const V3<R> ipos = P(i);
+            int bt4[4];
+            bond_types(i, bt4);
             for (int bond = 0; bond < B; bond++) {
-                R best = (R)FLT_MAX;
-                uint32_t to = MG_NONE;
-                for (int c = 0; c < g.adj_cnt; c++) {
-                    const uint32_t cell = gc + (uint32_t)g.adj[c];
-                    if (cell >= (uint32_t)C || cnt[cell] == 0) continue;
-                    const uint32_t c0 = off[cell];
-                    for (uint32_t j = c0; j < c0 + cnt[cell] && j < (uint32_t)N; j++) {
-                        if (pid[j] == 0u) continue;
-                        const V3<R> u = ipos - P(j);
-                        const R dsq = dot(u, u);
-                        if (!(dsq < rd2 && dsq > R(0))) continue;
-                        R theta = R(0);
-                        for (int b = 0; b < B; b++) {
-                            const uint32_t k = bu[brec(i, b) + MG_B_INDEX];
-                            if (j == k) {
-                                theta = (R)FLT_MAX;
-                                break;
-                            }
-                            if (k < (uint32_t)N) {
-                                const V3<R> v = ipos - P(k);
-                                const R lu = std::sqrt(dot(u, u)), lv = std::sqrt(dot(v, v));
-                                const R cosang = dot(u, v) / (lu * lv);
-                                theta = (R)((double)theta + std::fabs(1.91 - (double)acos_r(cosang)));
-                            }
-                        }
-                        if (bond == 0) theta = dsq;
-                        if (theta < best) {
-                            best = theta;
-                            to = j;
-                        }
-                    }
-                }
-                if (to < na) {
-                    const int bt = bond < 4 ? bt4[bond] : 0;
-                    const size_t r = brec(i, bond);
-                    bu[r + MG_B_INDEX] = to;
-                    sbf(r + MG_B_LIMIT, (R)gen.param[bt][8]);
-                    sbf(r + MG_B_REST, (R)gen.param[bt][9]);
-                    sbf(r + MG_B_MODULUS, (R)gen.param[bt][10]);
-                    sbf(r + MG_B_DAMP, (R)gen.param[bt][11]);
-                    bu[r + 5] = pid[to];
-                    sbf(r + MG_B_STRAIN2, R(0));
+                const uint32_t to = find_bond(i, gc, P(i), bond);
+                if (to < na) make_bond(i, bond, to, bond < 4 ? bt4[bond] : 0);
+            }
+        }
+    }
+
+    /* the record words 0-7 written by initialize_bonds and makeBond
+     * (src/fluid_system_cuda.cu:1573-1596): partner, the bond type's default limit / rest
+     * length / modulus / damping, partner ID, words 6-7 zero (makeBond also clears word 8) */
+    void make_bond(uint32_t i, int a, uint32_t to, int bt)
+    {
+        const size_t r = brec(i, a);
+        bu[r + MG_B_INDEX] = to;
+        sbf(r + MG_B_LIMIT, (R)gen.param[bt][8]);
+        sbf(r + MG_B_REST, (R)gen.param[bt][9]);
+        sbf(r + MG_B_MODULUS, (R)gen.param[bt][10]);
+        sbf(r + MG_B_DAMP, (R)gen.param[bt][11]);
+        bu[r + MG_B_ID] = pid[to];
+        sbf(r + MG_B_STRAIN2, R(0));
+        sbf(r + MG_B_STRAIN, R(0));
+    }
+
+    /* ----------------------------------------------------------- remodelling
+     * computeBondChanges (src/fluid_system_cuda.cu:1065-1330) with the host clears of
+     * ComputeBondChangesCL (src/host_CL.cpp:2057-2086), over gene 2's dense list below
+     * mActivePoints: per slot in order, rest length > 0 -> integrators averaged over `steps`
+     * and cleared, rest *= l_a + l_b tanh(l_c <strain> - l_d), modulus *= s_a + s_b
+     * tanh(s_c <strain^2> - s_d); change word cleared; the first broken (rest 0) or empty slot
+     * is counted for heal in the particle's cell, gets change bit 0 and ends the loop. */
+    void compute_bond_changes(uint32_t steps) override
+    {
+        std::fill(ccnt.begin(), ccnt.end(), 0u);
+        std::fill(coff.begin(), coff.end(), 0u);
+        std::fill(hs.begin(), hs.end(), MG_NONE);
+        const uint32_t na = active_points, len = lens[2];
+        const R fs = (R)steps;
+        for (uint32_t li = 0; li < len && li < (uint32_t)N; li++) {
+            const uint32_t i = lists[(size_t)2 * N + li];
+            if (i >= na) continue;
+            int bt4[4];
+            bond_types(i, bt4);
+            for (int a = 0; a < B; a++) {
+                const size_t r = brec(i, a);
+                R rest = bf(r + MG_B_REST);
+                if (rest > R(0)) {
+                    const R si = bf(r + MG_B_STRAIN) / fs, ssi = bf(r + MG_B_STRAIN2) / fs;
                     sbf(r + MG_B_STRAIN, R(0));
+                    sbf(r + MG_B_STRAIN2, R(0));
+                    const float* t = gen.tanh_param[a < 4 ? bt4[a] : 0];
+                    rest = rest * ((R)t[0] + (R)t[1] * tanh_r((R)t[2] * si - (R)t[3]));
+                    sbf(r + MG_B_REST, rest);
+                    sbf(r + MG_B_MODULUS,
+                        bf(r + MG_B_MODULUS) * ((R)t[4] + (R)t[5] * tanh_r((R)t[6] * ssi - (R)t[7])));
                 }
+                bu[r + MG_B_CHANGE] = 0;
+                if (rest == R(0) || bu[r + MG_B_INDEX] == MG_NONE) {
+                    const uint32_t gc = gcell[i];
+                    if (gc < (uint32_t)C) {
+                        ccnt[gc]++;
+                        hs[i] = (uint32_t)a;
+                    }
+                    bu[r + MG_B_CHANGE] = 1;
+                    break;
+                }
+            }
+        }
+    }
+
+    /* PrefixSumChangesCL + tally_denselist_lengths (src/host_CL.cpp:865-1062,
+     * src/fluid_system_cuda.cu:185-194): exclusive offsets per list, length = last count +
+     * last offset */
+    void prefix_sum_changes() override
+    {
+        for (int l = 0; l < MG_NUM_CHANGES; l++) {
+            uint32_t run = 0;
+            for (int c = 0; c < C; c++) {
+                coff[(size_t)l * C + c] = run;
+                run += ccnt[(size_t)l * C + c];
+            }
+            clen[l] = C > 0 ? ccnt[(size_t)l * C + C - 1] + coff[(size_t)l * C + C - 1] : 0u;
+        }
+    }
+
+    /* countingSortChanges (src/fluid_system_cuda.cu:406-618): per cell, its slots in order and
+     * their bond slots in order; list 0 takes the pairs compute_bond_changes counted, lists
+     * 1-8 the slots whose change word has the list's bit; at most the cell's count per list;
+     * cells whose range ends beyond mActivePoints are skipped */
+    void counting_sort_changes() override
+    {
+        std::fill(clists.begin(), clists.end(), MG_NONE);
+        const uint32_t na = active_points;
+        for (int c = 0; c < C; c++) {
+            const uint32_t count = cnt[c];
+            if (count == 0) continue;
+            const uint32_t o = off[c];
+            if (o + count > na) continue;
+            uint32_t n[MG_NUM_CHANGES] = {};
+            auto put = [&](int l, uint32_t s, uint32_t a) {
+                if (n[l] >= ccnt[(size_t)l * C + c]) return;
+                const uint32_t at = coff[(size_t)l * C + c] + n[l]++;
+                if (at >= (uint32_t)N) return;
+                clists[(size_t)l * 2 * N + at] = s;
+                clists[(size_t)l * 2 * N + N + at] = a;
+            };
+            for (uint32_t s = o; s < o + count; s++)
+                for (int a = 0; a < B; a++) {
+                    if (hs[s] == (uint32_t)a) put(0, s, (uint32_t)a);
+                    const uint32_t ch = bu[brec(s, a) + MG_B_CHANGE];
+                    for (int l = 1; l < MG_NUM_CHANGES; l++)
+                        if ((ch >> l) & 1u) put(l, s, (uint32_t)a);
+                }
+        }
+    }
+
+    /* heal (src/fluid_system_cuda.cu:2047-2107) over list 0: the position shifted by
+     * -sqrt(rd2) / (4 + r) on each axis whose sign bit of the random word is set, find_bond
+     * from there for the listed slot, and a found candidate bonded as elastin (makeBond). */
+    void heal(uint64_t seed) override
+    {
+        const uint32_t na = active_points, len = std::min<uint32_t>(clen[0], (uint32_t)N);
+        const R max_len = std::sqrt((R)p.rd2);
+        for (uint32_t k = 0; k < len; k++) {
+            const uint32_t i = clists[k];
+            if (i >= na) continue;
+            const uint32_t bond = clists[(size_t)N + k];
+            if (bond >= (uint32_t)B) continue;
+            const uint32_t gc = gcell[i];
+            if (gc == MG_GRID_UNDEF) continue;
+            const uint32_t r = heal_word(seed, pid[i]);
+            V3<R> x = P(i);
+            x.x = x.x + (max_len / (R)(4u + (r & 7u))) * (R(-1) * (R)(1u & (r >> 3)));
+            x.y = x.y + (max_len / (R)(4u + ((r >> 4) & 7u))) * (R(-1) * (R)(1u & (r >> 7)));
+            x.z = x.z + (max_len / (R)(4u + ((r >> 8) & 7u))) * (R(-1) * (R)(1u & (r >> 11)));
+            const uint32_t to = find_bond(i, gc, x, (int)bond);
+            if (to < (uint32_t)N) {
+                make_bond(i, (int)bond, to, 0);
+                bu[brec(i, (int)bond) + MG_B_CHANGE] = 0;
             }
         }
     }
@@ -1009,6 +1229,27 @@ int mgo_compute_diffusion(mgo_handle* h) { return with_grid(h, [&] { h->o->diffu
 int mgo_compute_genes(mgo_handle* h) { return with_grid(h, [&] { h->o->genes(); }); }
 int mgo_zero_vel(mgo_handle* h) { return with_grid(h, [&] { h->o->zero_vel(); }); }
 int mgo_initialize_bonds(mgo_handle* h) { return with_grid(h, [&] { h->o->initialize_bonds(); }); }
+int mgo_compute_bond_changes(mgo_handle* h, uint32_t steps)
+{
+    if (steps == 0) return MG_ERR_ARG;
+    return with_grid(h, [&] { h->o->compute_bond_changes(steps); });
+}
+int mgo_prefix_sum_changes(mgo_handle* h, int zero_offsets)
+{
+    (void)zero_offsets;
+    return with_grid(h, [&] { h->o->prefix_sum_changes(); });
+}
+int mgo_counting_sort_changes(mgo_handle* h) { return with_grid(h, [&] { h->o->counting_sort_changes(); }); }
+int mgo_compute_particle_changes(mgo_handle* h, uint64_t seed)
+{
+    if (h && h->o)
+        for (int l = 1; l < MG_NUM_CHANGES; l++)
+            if (h->o->clen[l] != 0) return MG_ERR_UNSUPPORTED;
+    return with_grid(h, [&] { h->o->heal(seed); });
+}
+float mgo_tanhf(float x) { return tanh_f(x); }
+float mgo_expf(float x) { return exp_f(x); }
+uint32_t mgo_heal_word(uint64_t seed, uint32_t pid) { return heal_word(seed, pid); }
 float mgo_acosf(float x) { return acos_f(x); }
 
 int mgo_physical_sort(mgo_handle* h)

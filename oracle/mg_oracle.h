@@ -44,6 +44,13 @@ int mgo_compute_genes(mgo_handle* h);                 /* ComputeGenesCL      */
 int mgo_zero_vel(mgo_handle* h);                      /* ZeroVelCL           */
 int mgo_initialize_bonds(mgo_handle* h);              /* InitializeBondsCL   */
 float mgo_acosf(float x);                             /* the bond-angle acos */
+int mgo_compute_bond_changes(mgo_handle* h, uint32_t steps); /* ComputeBondChangesCL */
+int mgo_prefix_sum_changes(mgo_handle* h, int zero_offsets); /* PrefixSumChangesCL   */
+int mgo_counting_sort_changes(mgo_handle* h);         /* CountingSortChangesCL */
+int mgo_compute_particle_changes(mgo_handle* h, uint64_t seed); /* ComputeParticleChangesCL (heal) */
+float mgo_tanhf(float x);                             /* the remodelling tanh   */
+float mgo_expf(float x);                              /* its exp                */
+uint32_t mgo_heal_word(uint64_t seed, uint32_t pid);  /* heal's random word     */
 
 /* composites */
 int mgo_physical_sort(mgo_handle* h);                 /* Run2PhysicalSort    */

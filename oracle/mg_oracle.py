@@ -28,12 +28,32 @@ def load():
         _lib = F.declare(C.CDLL(LIB_PATH), "mgo")
         _lib.mgo_acosf.argtypes = [C.c_float]
         _lib.mgo_acosf.restype = C.c_float
+        for name in ("mgo_tanhf", "mgo_expf"):
+            getattr(_lib, name).argtypes = [C.c_float]
+            getattr(_lib, name).restype = C.c_float
+        _lib.mgo_heal_word.argtypes = [C.c_uint64, C.c_uint32]
+        _lib.mgo_heal_word.restype = C.c_uint32
     return _lib
 
 
 def acosf(x: float) -> float:
     """The float acos the bond-angle score uses (Cephes method, shared with the kernels)."""
     return float(load().mgo_acosf(float(x)))
+
+
+def tanhf(x: float) -> float:
+    """The float tanh of the bond remodelling (Cephes method, shared with the kernels)."""
+    return float(load().mgo_tanhf(float(x)))
+
+
+def expf(x: float) -> float:
+    """The float exp inside tanhf."""
+    return float(load().mgo_expf(float(x)))
+
+
+def heal_word(seed: int, pid: int) -> int:
+    """heal's 32-bit random word for a particle ID (SplitMix64 output function)."""
+    return int(load().mgo_heal_word(int(seed) & 0xFFFFFFFFFFFFFFFF, int(pid)))
 
 
 class Oracle:
@@ -143,6 +163,24 @@ class Oracle:
 
     def InitializeBondsCL(self):
         self._call("initialize_bonds", self.h)
+
+    def ComputeBondChangesCL(self, steps_per_inner_loop):
+        self._call("compute_bond_changes", self.h, int(steps_per_inner_loop))
+
+    def PrefixSumChangesCL(self, zero_offsets=1):
+        self._call("prefix_sum_changes", self.h, int(zero_offsets))
+
+    def CountingSortChangesCL(self):
+        self._call("counting_sort_changes", self.h)
+
+    def ComputeParticleChangesCL(self, seed):
+        self._call("compute_particle_changes", self.h, int(seed) & 0xFFFFFFFFFFFFFFFF)
+
+    def Run2Remodelling(self, steps_per_inner_loop, seed):
+        self.ComputeBondChangesCL(steps_per_inner_loop)
+        self.PrefixSumChangesCL(1)
+        self.CountingSortChangesCL()
+        self.ComputeParticleChangesCL(seed)
 
     def ZeroVelCL(self):
         self._call("zero_vel", self.h)

@@ -1,0 +1,90 @@
+"""GPU remodelling (mg_compute_bond_changes, mg_prefix_sum_changes, mg_counting_sort_changes,
+mg_compute_particle_changes; the reference's Run2Remodelling) against the oracle: bond records,
+change counts, offsets, lengths, lists and healed bonds bit-equal, and the remodelled state
+stepping on bit-equal."""
+import sys
+
+import numpy as np
+import pytest
+
+from morphogenesis_amd import _ffi as F
+from tests.test_gpu_parity import COMPARE, compare_all, pair
+from tests.test_remodel import remodel_scene
+
+pytestmark = pytest.mark.gpu
+
+ALL = COMPARE + F.CHANGE_BUFFERS
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("seed,b", [(21, 12), (22, 32)])
+def test_remodelling_cycles_match_oracle(built, seed, b):
+    """The reference's loop order: inner loops without a sort, gene action, remodelling, sort."""
+    sc = remodel_scene(seed, n_side=8, b=b)
+    say("case: remodelling cycles", sc.name, "seed", seed, "B", b)
+    g, o = pair(sc)
+    for sim in (g, o):
+        sim.Run2PhysicalSort()
+    healed = 0
+    for cycle in range(3):
+        for sim in (g, o):
+            for _ in range(4):
+                sim.Run2InnerPhysicalLoop(sc.dt)
+            sim.Run2GeneAction()
+            sim.ComputeBondChangesCL(4)
+        compare_all(g, o, f"cycle {cycle}: bond changes", ALL)
+        for sim in (g, o):
+            sim.PrefixSumChangesCL(1)
+            sim.CountingSortChangesCL()
+        compare_all(g, o, f"cycle {cycle}: change lists", ALL)
+        healed += int(o.download(F.FDENSE_LIST_LENGTHS_CHANGES)[0])
+        for sim in (g, o):
+            sim.ComputeParticleChangesCL(1000 + cycle)
+        compare_all(g, o, f"cycle {cycle}: heal", ALL)
+        for sim in (g, o):
+            sim.Run2PhysicalSort()
+        compare_all(g, o, f"cycle {cycle}: sort", ALL)
+    assert healed > 100
+    say("phase: 20 fused steps")
+    for sim in (g, o):
+        sim.step(20, sc.dt, sc.flags | (F.STEP_GRAPH if sim is g else 0))
+    compare_all(g, o, "20 fused steps after remodelling", ALL)
+
+
+def test_remodelling_after_fused_steps_32k(built):
+    """32,768 particles, B = 32: fused steps, then Run2Remodelling in the sorted state."""
+    sc = remodel_scene(23, n_side=32, b=32)
+    say("case: remodelling after fused steps", sc.name, sc.n)
+    g, o = pair(sc)
+    for sim in (g, o):
+        sim.Run2PhysicalSort()
+        sim.step(6, sc.dt, sc.flags | (F.STEP_GRAPH if sim is g else 0))
+        sim.Run2Remodelling(6, seed=77)
+    compare_all(g, o, "remodelled", ALL)
+    assert o.download(F.FDENSE_LIST_LENGTHS_CHANGES)[0] > 1000
+    for sim in (g, o):
+        sim.step(10, sc.dt, sc.flags | (F.STEP_GRAPH if sim is g else 0))
+    compare_all(g, o, "10 fused steps after remodelling", ALL)
+
+
+def test_remodelling_errors(built):
+    sc = remodel_scene(24, n_side=5, b=4)
+    say("case: remodelling argument and list checks")
+    g, o = pair(sc)
+    g.Run2PhysicalSort()
+    with pytest.raises(F.MgError):
+        g.ComputeBondChangesCL(0)
+    g.ComputeBondChangesCL(1)
+    cc = g.download(F.FBIN_COUNT_CHANGES)
+    cc[3, 5] = 2
+    g.upload(F.FBIN_COUNT_CHANGES, cc)
+    g.PrefixSumChangesCL(1)
+    assert g.download(F.FDENSE_LIST_LENGTHS_CHANGES)[3] == 2
+    before = g.download(F.FELASTIDX)
+    with pytest.raises(F.MgError):
+        g.ComputeParticleChangesCL(1)
+    assert np.array_equal(g.download(F.FELASTIDX), before)

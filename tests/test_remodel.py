@@ -1,0 +1,363 @@
+"""Remodelling (computeBondChanges, PrefixSumChanges, countingSortChanges, heal;
+src/fluid_system_cuda.cu:406-618,1065-1330,2047-2107) in the oracle, pinned by closed forms of
+the tanh remodel, float64 evaluation, brute-force change lists and a brute-force heal search;
+the Cephes tanh/exp against float64 and the heal word against SplitMix64's known outputs."""
+import numpy as np
+import pytest
+
+from morphogenesis_amd import _ffi as F
+from morphogenesis_amd import scenes
+
+INT_MAX = 2147483647
+F32 = np.float32
+
+
+def f2u(x):
+    return np.asarray(x, F32).view(np.uint32)
+
+
+def u2f(x):
+    return np.asarray(x, np.uint32).view(F32)
+
+
+# ------------------------------------------------------------------ the math it uses
+def test_tanhf_within_two_ulp_and_exactly_odd():
+    from oracle import mg_oracle as O
+
+    xs = np.concatenate([np.linspace(-12, 12, 24001), np.linspace(-0.7, 0.7, 7001),
+                         [0.625, np.nextafter(F32(0.625), F32(0)), 1e-30, 44.0, 44.4, 88.0]]).astype(F32)
+    got = np.array([O.tanhf(x) for x in xs], np.float64)
+    ref = np.tanh(xs.astype(np.float64))
+    ulp = np.spacing(np.abs(ref).astype(F32)).astype(np.float64)
+    assert np.max(np.abs(got - ref) / np.maximum(ulp, np.spacing(F32(0)))) <= 2.0
+    neg = np.array([O.tanhf(-x) for x in xs], np.float64)
+    assert np.array_equal(neg, -got)
+    assert O.tanhf(50.0) == 1.0 and O.tanhf(-50.0) == -1.0 and O.tanhf(30.0) == 1.0
+    assert np.signbit(O.tanhf(-0.0)) and not np.signbit(O.tanhf(0.0))
+    assert np.isnan(O.tanhf(float("nan")))
+
+
+def test_expf_relative_error():
+    from oracle import mg_oracle as O
+
+    xs = np.concatenate([np.linspace(-87, 88.5, 40001), [0.0, 1.0, -1.0, 0.5]]).astype(F32)
+    got = np.array([O.expf(x) for x in xs], np.float64)
+    ref = np.exp(xs.astype(np.float64))
+    assert np.max(np.abs(got - ref) / ref) < 2.5e-7
+    assert O.expf(0.0) == 1.0
+    assert O.expf(89.0) == float("inf") and O.expf(-104.0) == 0.0
+
+
+def splitmix64_outputs(state, count):
+    """SplitMix64 (Steele, Lea, Flood 2014): state += golden; output = mix(state)."""
+    m = (1 << 64) - 1
+    out = []
+    for _ in range(count):
+        state = (state + 0x9E3779B97F4A7C15) & m
+        z = state
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & m
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & m
+        out.append(z ^ (z >> 31))
+    return out
+
+
+def test_heal_word_is_splitmix64():
+    from oracle import mg_oracle as O
+
+    # the published first outputs of SplitMix64 from state 0
+    assert [O.heal_word(0, k) for k in range(3)] == [0x7B1DCDAF, 0xA1B965F4, 0x8009454F]
+    assert splitmix64_outputs(0, 3) == [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
+    for seed in (1, 12345, 2 ** 63 + 7):
+        outs = splitmix64_outputs(seed, 40)
+        assert [O.heal_word(seed, k) for k in range(40)] == [o & 0xFFFFFFFF for o in outs]
+
+
+# ------------------------------------------------------------------ scenes
+def remodel_scene(seed, n_side=8, b=12, tanh=None, p_living=0.7, p_broken=0.1, p_empty=0.05,
+                  integrators=True):
+    """Bonded tissue with gene 2 ("living") on a random subset, the tissue genes on others, some
+    bonds broken (rest length 0) or emptied, and random strain integrators."""
+    sc = scenes.tissue(n_side=n_side, b=b, seed=seed)
+    g = scenes.example_genome()
+    rows = tanh if tanh is not None else [(1.0, 0.05, 10.0, 0.5, 1.0, 0.03, 40.0, 0.2),
+                                          (1.01, 0.04, 6.0, 1.0, 0.99, 0.02, 20.0, 0.1),
+                                          (0.995, 0.02, 3.0, 0.3, 1.02, 0.05, 8.0, 0.4)]
+    for r, row in enumerate(rows):
+        for k, v in enumerate(row):
+            g.tanh_param[r][k] = v
+    sc.genome = g
+    rng = np.random.default_rng(seed + 300)
+    epi = sc.buffers[F.FEPIGEN]
+    for gi, frac in ((1, 0.5), (2, p_living), (6, 0.1), (7, 0.1), (9, 0.15), (10, 0.1)):
+        epi[gi] = np.where(rng.random(sc.n) < frac, INT_MAX, 0).astype(np.uint32)
+    rec = sc.buffers[F.FELASTIDX]
+    used = rec[:, :, 0] != F.NONE
+    broken = used & (rng.random(used.shape) < p_broken)
+    rec[:, :, 2][broken] = f2u(0.0)
+    empty = used & ~broken & (rng.random(used.shape) < p_empty)
+    rec[empty] = 0
+    rec[:, :, 0][empty] = F.NONE
+    rec[:, :, 5][empty] = F.NONE
+    if integrators:
+        live = rec[:, :, 0] != F.NONE
+        rec[:, :, 7][live] = f2u(rng.uniform(0.0, 1.0, live.sum()))   # strain integrator
+        rec[:, :, 6][live] = f2u(rng.uniform(0.0, 0.1, live.sum()))   # strain^2 integrator
+    return sc
+
+
+def sorted_oracle(sc):
+    from oracle.mg_oracle import Oracle
+
+    o = Oracle(sc.n, sc.b)
+    o.load_scene(sc)
+    o.Run2PhysicalSort()
+    o.scene = sc
+    return o
+
+
+def living_slots(o):
+    """Gene 2's dense list of the last sort."""
+    lens = o.download(F.FDENSE_LIST_LENGTHS)
+    return o.download(F.FDENSE_LISTS)[2][: lens[2]]
+
+
+def active(o):
+    return o.counts()[1]
+
+
+def bond_type_rows(epi, s):
+    bt = [0, 0, 0, 0]
+    if epi[9, s] > 0:
+        bt = [2, 2, 2, 2]
+    elif epi[6, s] > 0 or epi[7, s] > 0 or epi[10, s] > 0:
+        bt[0] = 1
+    return bt
+
+
+# ------------------------------------------------------------------ computeBondChanges
+def test_remodel_closed_forms():
+    """l_b = s_b = 0: rest *= l_a and modulus *= s_a exactly; with l_c = s_c = 0 and large
+    shifts tanh saturates to +-1: rest *= l_a + l_b, modulus *= s_a - s_b exactly."""
+    for rows, lmul, smul in (
+        ([(1.25, 0.0, 10.0, 0.5, 0.75, 0.0, 40.0, 0.2)] * 3, F32(1.25), F32(0.75)),
+        ([(1.25, 0.5, 0.0, -30.0, 0.75, 0.125, 0.0, 30.0)] * 3, F32(1.25) + F32(0.5), F32(0.75) - F32(0.125)),
+    ):
+        o = sorted_oracle(remodel_scene(11, tanh=rows))
+        before = o.download(F.FELASTIDX)
+        epi = o.download(F.FEPIGEN)
+        o.ComputeBondChangesCL(4)
+        after = o.download(F.FELASTIDX)
+        na = active(o)
+        liv = [s for s in living_slots(o) if s < na]
+        assert len(liv) > 100
+        checked = 0
+        for s in liv:
+            for a in range(o.b):
+                r0, r1 = before[s, a], after[s, a]
+                rest0 = u2f(r0[2])
+                if rest0 > 0:
+                    assert u2f(r1[2]) == rest0 * lmul
+                    assert u2f(r1[3]) == u2f(r0[3]) * smul
+                    assert r1[6] == 0 and r1[7] == 0
+                    checked += 1
+                if u2f(r1[2]) == 0 or r1[0] == F.NONE:
+                    break
+        assert checked > 500
+
+
+def test_remodel_against_float64():
+    """The general remodel against float64 with numpy's tanh; the L and S rows differ per bond
+    type and read different integrators, so a swapped row, column or integrator fails."""
+    o = sorted_oracle(remodel_scene(12))
+    before = o.download(F.FELASTIDX)
+    epi = o.download(F.FEPIGEN)
+    g = np.array(o.scene.genome.tanh_param, np.float64)
+    steps = 4
+    o.ComputeBondChangesCL(steps)
+    after = o.download(F.FELASTIDX)
+    na = active(o)
+    worst, n = 0.0, 0
+    for s in [s for s in living_slots(o) if s < na]:
+        bt = bond_type_rows(epi, s)
+        for a in range(o.b):
+            r0, r1 = before[s, a], after[s, a]
+            rest0 = float(u2f(r0[2]))
+            if rest0 > 0:
+                t = g[bt[a] if a < 4 else 0]
+                si = float(u2f(r0[7])) / steps
+                ssi = float(u2f(r0[6])) / steps
+                rest = rest0 * (t[0] + t[1] * np.tanh(t[2] * si - t[3]))
+                mod = float(u2f(r0[3])) * (t[4] + t[5] * np.tanh(t[6] * ssi - t[7]))
+                worst = max(worst, abs(float(u2f(r1[2])) - rest) / rest, abs(float(u2f(r1[3])) - mod) / mod)
+                n += 1
+            if u2f(r1[2]) == 0 or r1[0] == F.NONE:
+                break
+    assert n > 500
+    assert worst < 1e-6
+
+
+def expected_heal(before, after, lists2, na, gcell, b):
+    """Per counted particle (slot s) the slot a computeBondChanges must flag: the first slot
+    that is broken after the remodel or empty."""
+    out = {}
+    for s in lists2:
+        if s >= na:
+            continue
+        for a in range(b):
+            if u2f(after[s, a, 2]) == 0 or after[s, a, 0] == F.NONE:
+                out[int(s)] = a
+                break
+    return out
+
+
+def test_heal_counts_offsets_and_lists_brute_force():
+    o = sorted_oracle(remodel_scene(13))
+    before = o.download(F.FELASTIDX)
+    o.ComputeBondChangesCL(3)
+    after = o.download(F.FELASTIDX)
+    na = active(o)
+    gcell = o.download(F.FGCELL)
+    want = expected_heal(before, after, living_slots(o), na, gcell, o.b)
+    assert len(want) > 50
+    c = o.c
+    counts = np.zeros(c, np.uint32)
+    for s in want:
+        counts[gcell[s]] += 1
+    cc = o.download(F.FBIN_COUNT_CHANGES)
+    assert np.array_equal(cc[0], counts) and not cc[1:].any()
+    # change words: 0 up to the flagged slot, 1 there, unchanged after; living particles
+    # without a flagged slot have every word cleared; others keep theirs
+    liv = set(int(s) for s in living_slots(o) if s < na)
+    for s in range(o.n):
+        for a in range(o.b):
+            w = after[s, a, 8]
+            if s in want:
+                assert w == (1 if a == want[s] else (0 if a < want[s] else before[s, a, 8]))
+            elif s in liv:
+                assert w == 0
+            else:
+                assert w == before[s, a, 8]
+    o.PrefixSumChangesCL(1)
+    off = o.download(F.FBIN_OFFSET_CHANGES)
+    lens = o.download(F.FDENSE_LIST_LENGTHS_CHANGES)
+    excl = np.concatenate([[0], np.cumsum(counts)[:-1]]).astype(np.uint32)
+    assert np.array_equal(off[0], excl) and not off[1:].any()
+    assert lens[0] == len(want) and not lens[1:].any()
+    o.CountingSortChangesCL()
+    lst = o.download(F.FDENSE_LISTS_CHANGES)
+    # slots are sorted by cell, so cell order then slot order is ascending slot order
+    order = sorted(want)
+    assert lst[0, 0, : len(order)].tolist() == order
+    assert lst[0, 1, : len(order)].tolist() == [want[s] for s in order]
+    assert (lst[0, :, len(order):] == F.NONE).all() and (lst[1:] == F.NONE).all()
+
+
+def test_stale_change_bits_are_not_listed():
+    """A change bit left on a slot after the counted one (from an earlier call) stays in the
+    record but does not enter the heal list."""
+    sc = remodel_scene(14, p_broken=0.0, p_empty=0.0)
+    rec = sc.buffers[F.FELASTIDX]
+    epi = sc.buffers[F.FEPIGEN]
+    k = int(np.flatnonzero((epi[2] > 0) & (rec[:, 3, 0] != F.NONE))[0])
+    rec[k, 0, 2] = f2u(0.0)        # slot 0 broken -> counted
+    rec[k, 3, 8] = 1               # stale heal bit on slot 3
+    o = sorted_oracle(sc)
+    s = int(np.flatnonzero(o.download(F.FPARTICLE_ID) == k + 1)[0])
+    o.ComputeBondChangesCL(2)
+    o.PrefixSumChangesCL(1)
+    o.CountingSortChangesCL()
+    after = o.download(F.FELASTIDX)
+    assert after[s, 0, 8] == 1 and after[s, 3, 8] == 1
+    lst = o.download(F.FDENSE_LISTS_CHANGES)
+    n0 = int(o.download(F.FDENSE_LIST_LENGTHS_CHANGES)[0])
+    pairs = list(zip(lst[0, 0, :n0].tolist(), lst[0, 1, :n0].tolist()))
+    assert (s, 0) in pairs and (s, 3) not in pairs
+
+
+# ------------------------------------------------------------------ heal
+def test_heal_brute_force_search():
+    from oracle import mg_oracle as O
+
+    seed = 99
+    o = sorted_oracle(remodel_scene(15))
+    o.ComputeBondChangesCL(3)
+    o.PrefixSumChangesCL(1)
+    o.CountingSortChangesCL()
+    mid = o.download(F.FELASTIDX)
+    lst = o.download(F.FDENSE_LISTS_CHANGES)
+    n0 = int(o.download(F.FDENSE_LIST_LENGTHS_CHANGES)[0])
+    pos = o.download(F.FPOS)[:, :3].astype(F32)
+    pid = o.download(F.FPARTICLE_ID)
+    gcell = o.download(F.FGCELL)
+    cnt, off = o.download(F.FBIN_COUNT), o.download(F.FBIN_OFFSET)
+    grid, params, gen = o.scene.grid, o.scene.params, o.scene.genome
+    o.ComputeParticleChangesCL(seed)
+    after = o.download(F.FELASTIDX)
+    rd2 = F32(params.rd2)
+    max_len = np.sqrt(rd2, dtype=F32)
+    adj = [int(grid.adj[c]) for c in range(grid.adj_cnt)]
+    elastin = f2u([gen.param[0][8], gen.param[0][9], gen.param[0][10], gen.param[0][11]])
+    healed = missed = 0
+    for k in range(n0):
+        s, a = int(lst[0, 0, k]), int(lst[0, 1, k])
+        r = O.heal_word(seed, int(pid[s]))
+        x = pos[s].copy()
+        for ax, (lo, sb) in enumerate(((0, 3), (4, 7), (8, 11))):
+            step = max_len / F32(4 + ((r >> lo) & 7))
+            x[ax] = x[ax] + step * (F32(-1.0) * F32((r >> sb) & 1))
+        cands = []
+        for cadj in adj:
+            cell = int(gcell[s]) + cadj
+            if cell >= o.c or cnt[cell] == 0:
+                continue
+            for j in range(int(off[cell]), int(off[cell] + cnt[cell])):
+                if pid[j] == 0:
+                    continue
+                u = (x - pos[j]).astype(F32)
+                dsq = F32(u[0] * u[0] + u[1] * u[1]) + F32(u[2] * u[2])
+                if 0 < dsq < rd2:
+                    cands.append((j, u, dsq))
+        if not cands:
+            assert np.array_equal(after[s, a], mid[s, a])
+            missed += 1
+            continue
+        got = int(after[s, a, 0])
+        if a == 0:
+            best = min(c[2] for c in cands)
+            assert got in [c[0] for c in cands if c[2] == best]
+        else:
+            def score(j, u):
+                t = 0.0
+                for bb in range(o.b):
+                    kk = int(mid[s, bb, 0])
+                    if kk == j:
+                        return np.inf
+                    if kk < o.n:
+                        v = (x - pos[kk]).astype(np.float64)
+                        uu = u.astype(np.float64)
+                        cs = np.dot(uu, v) / (np.linalg.norm(uu) * np.linalg.norm(v))
+                        t += abs(1.91 - np.arccos(np.clip(cs, -1, 1)))
+                return t
+            scores = {j: score(j, u) for j, u, _ in cands}
+            assert scores[got] <= min(scores.values()) + 1e-5
+        rec = after[s, a]
+        assert rec[0] == got and np.array_equal(rec[1:5], elastin)
+        assert rec[5] == pid[got] and rec[6] == 0 and rec[7] == 0 and rec[8] == 0
+        healed += 1
+    assert healed > 50
+
+
+def test_other_change_lists_are_refused_and_zero_steps_rejected():
+    o = sorted_oracle(remodel_scene(16, n_side=5, b=4))
+    with pytest.raises(F.MgError):
+        o.ComputeBondChangesCL(0)
+    o.ComputeBondChangesCL(1)
+    cc = o.download(F.FBIN_COUNT_CHANGES)
+    cc[3, 5] = 2
+    o.upload(F.FBIN_COUNT_CHANGES, cc)
+    o.PrefixSumChangesCL(1)
+    assert o.download(F.FDENSE_LIST_LENGTHS_CHANGES)[3] == 2
+    before = o.download(F.FELASTIDX)
+    with pytest.raises(F.MgError):
+        o.ComputeParticleChangesCL(1)
+    assert np.array_equal(o.download(F.FELASTIDX), before)
