@@ -340,6 +340,91 @@ def run_bonds(args):
     print(json.dumps(out), flush=True)
 
 
+def remodel_scene(n_side):
+    """Bonded tissue (B = 32) for the remodelling workload: gene 2 ("living") on 70% of the
+    particles, the tissue genes 6 / 7 / 9 / 10 on 10-15% each, 5% of the bonds broken (rest
+    length 0), strain integrators from the first steps, and gentle tanh remodelling rows."""
+    import numpy as np
+
+    from morphogenesis_amd import _ffi as F
+    from morphogenesis_amd import scenes
+
+    sc = scenes.tissue(n_side, 32, seed=1)
+    g = scenes.example_genome()
+    rows = [(1.0, 0.05, 10.0, 0.5, 1.0, 0.03, 40.0, 0.2), (1.01, 0.04, 6.0, 1.0, 0.99, 0.02, 20.0, 0.1),
+            (0.995, 0.02, 3.0, 0.3, 1.02, 0.05, 8.0, 0.4)]
+    for r, row in enumerate(rows):
+        for k, v in enumerate(row):
+            g.tanh_param[r][k] = v
+    sc.genome = g
+    rng = np.random.default_rng(11)
+    epi = sc.buffers[F.FEPIGEN]
+    for gi, frac in ((2, 0.7), (6, 0.1), (7, 0.1), (9, 0.15), (10, 0.1)):
+        epi[gi] = np.where(rng.random(sc.n) < frac, 2147483647, 0).astype(np.uint32)
+    rec = sc.buffers[F.FELASTIDX]
+    broken = (rec[:, :, 0] != F.NONE) & (rng.random(rec.shape[:2]) < 0.05)
+    rec[:, :, 2][broken] = 0
+    return sc
+
+
+def run_remodel(args):
+    """Remodelling (Run2Remodelling = mg_compute_bond_changes, mg_prefix_sum_changes,
+    mg_counting_sort_changes, mg_compute_particle_changes) on the 262,144-particle tissue of
+    remodel_scene after 4 steps. `value` = bond slots of the living particles processed per
+    second over whole calls (kernels + the host rebuild of the incoming-bond lists after heal);
+    the CPU baseline is the oracle's four calls on the same 262,144-particle state (under a
+    second of CPU work per call, so the whole workload is the sample)."""
+    from morphogenesis_amd import _ffi as F
+    from morphogenesis_amd.fluid_system import FluidSystem
+
+    sc = remodel_scene(64)
+    sim = FluidSystem(sc.n, sc.b)
+    sim.load_scene(sc)
+    sim.Run2PhysicalSort()
+    sim.step(4, sc.dt, sc.flags | F.STEP_GRAPH)
+    living = int(sim.download(F.FDENSE_LIST_LENGTHS)[2])
+    sim.Run2Remodelling(4, seed=0)
+    healed0 = int(sim.download(F.FDENSE_LIST_LENGTHS_CHANGES)[0])
+    sim.sync()
+    reps = max(args.steps // 40, 3)
+    parts = {"bond_changes": 0.0, "prefix_sum": 0.0, "counting_sort": 0.0, "heal": 0.0}
+    t0 = time.perf_counter()
+    for r in range(reps):
+        for name, fn in (("bond_changes", lambda: sim.ComputeBondChangesCL(4)),
+                         ("prefix_sum", lambda: sim.PrefixSumChangesCL(1)),
+                         ("counting_sort", sim.CountingSortChangesCL),
+                         ("heal", lambda: sim.ComputeParticleChangesCL(r + 1))):
+            t = time.perf_counter()
+            fn()
+            sim.sync()
+            parts[name] += time.perf_counter() - t
+    dt = (time.perf_counter() - t0) / reps
+    out = {"metric": "bond slots remodelled/s (Run2Remodelling), 262k bonded tissue",
+           "value": living * sc.b / dt, "unit": "slots/s", "n_gpus": 1, "steps": reps, "warmup": 1,
+           "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "fp32 (fp64 angle sum in heal)", "data": "synthetic",
+           "config": {"workload": args.workload, "particles": sc.n, "living": living, "bonds_per_particle": sc.b,
+                      "heal_list_first_call": healed0,
+                      "ms_per_call": {k: round(v / reps * 1e3, 3) for k, v in parts.items()}}}
+    if not args.no_cpu:
+        from oracle.mg_oracle import Oracle
+
+        o = Oracle(sc.n, sc.b)
+        o.load_scene(sc)
+        o.Run2PhysicalSort()
+        o.step(4, sc.dt, sc.flags)
+        ls = int(o.download(F.FDENSE_LIST_LENGTHS)[2])
+        cpu_reps = 3
+        t = time.perf_counter()
+        for r in range(cpu_reps):
+            o.Run2Remodelling(4, r)
+        ct = (time.perf_counter() - t) / cpu_reps
+        out["cpu_baseline"] = {"value": ls * sc.b / ct, "unit": "slots/s", "cores": 1, "kind": "port",
+                               "sample": f"{cpu_reps} Run2Remodelling calls over the same {ls} living particles "
+                               f"of the {sc.n}-particle tissue, {ct:.3f} s per call, FP32 oracle, 1 thread"}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -373,6 +458,9 @@ def main():
         return
     if args.workload == "tissue_262k_bonds":
         run_bonds(args)
+        return
+    if args.workload == "tissue_262k_remodel":
+        run_remodel(args)
         return
     builders = {"tissue_262k": lambda: scenes.tissue(64, 32, seed=1),
                 "fluid_block_32k": lambda: scenes.fluid_block(32, seed=1),

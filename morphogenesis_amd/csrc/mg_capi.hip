@@ -262,6 +262,7 @@ struct mg_handle {
     uint32_t* FBIN = nullptr;
     uint32_t *INC_OFF = nullptr, *INC = nullptr;
     size_t inc_cap = 0;
+    uint32_t* incw = nullptr; /* device build of INC: counts / cursors [N], tile sums, maximum */
     uint64_t* LIVE = nullptr; /* per home: bit a set when bond slot a holds a partner */
     uint64_t* HM = nullptr;   /* hit masks, pressure -> force of a fused step: [row][slot] */
     int hm_rows = 0;          /* rows HM holds per slot */
@@ -273,6 +274,7 @@ struct mg_handle {
     /* remodelling change lists, allocated on first use: counts and offsets [9][C], lengths [9],
      * (slot, bond) pairs [9][2][N], and per slot the bond computeBondChanges counted for heal */
     uint32_t *CCNT = nullptr, *COFF = nullptr, *CLEN = nullptr, *CL = nullptr, *HS = nullptr;
+    uint32_t* csum = nullptr; /* their scan's tile sums and prefixes [2][9][tiles] */
     /* OFF[gcur] is the exclusive prefix sum of CNT[gcur] (set by a scan, cleared by an insert
      * or an upload of either); with a row-structured gridAdj the sweeps walk rows */
     bool grid_scanned = false;
@@ -430,8 +432,10 @@ int ready(mg_handle* h)
 /* Rebuild the incoming-bond CSR from BPARTNER (home space): entries of particle j are the
  * (owner home, slot) pairs whose partner is j, ordered by the owner's particle ID, then its
  * home, then the slot. The ID order makes the force's summation order a property of the
- * state itself: homes are renumbered by uploads and re-homing, IDs travel with particles. */
-int rebuild_inc(mg_handle* h)
+ * state itself: homes are renumbered by uploads and re-homing, IDs travel with particles.
+ * rebuild_inc builds it on the device; this host build serves particles with more than
+ * INC_RUN_MAX incoming bonds. */
+int rebuild_inc_host(mg_handle* h)
 {
     const int N = h->N, B = h->B;
     std::vector<uint32_t> partner((size_t)N * B);
@@ -475,6 +479,41 @@ int rebuild_inc(mg_handle* h)
     HIPCHK(hipMemcpyAsync(h->INC, ent.data(), ent.size() * 4, hipMemcpyHostToDevice, h->st));
     HIPCHK(hipMemcpyAsync(h->LIVE, live.data(), live.size() * 8, hipMemcpyHostToDevice, h->st));
     HIPCHK(hipStreamSynchronize(h->st));
+    return MG_OK;
+}
+
+int rebuild_inc(mg_handle* h)
+{
+    const int N = h->N, B = h->B;
+    if (B == 0) return rebuild_inc_host(h);
+    const unsigned tiles = (unsigned)((N + SCAN_TILE - 1) / SCAN_TILE);
+    if (!h->incw) CHK(dalloc(h, &h->incw, (size_t)N + 2 * tiles + 2));
+    uint32_t *cnt = h->incw, *sums = cnt + N, *excl = sums + tiles, *stat = excl + tiles;
+    CHK(fill_u32(h, cnt, 0, (size_t)N));
+    CHK(fill_u32(h, stat, 0, 1));
+    hipLaunchKernelGGL(k_inc_count, dim3(nblk(N)), dim3(256), 0, h->st, h->BPARTNER, cnt, h->LIVE, N, B);
+    hipLaunchKernelGGL(k_tile_sums, dim3(tiles), dim3(256), 0, h->st, cnt, N, sums, stat);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, h->st, sums, excl, (int)tiles, h->ctr, 0, 0, h->INC_OFF + N);
+    hipLaunchKernelGGL(k_tile_scan, dim3(tiles), dim3(256), 0, h->st, cnt, h->INC_OFF, N, excl);
+    HIPCHK(hipGetLastError());
+    uint32_t total = 0, longest = 0;
+    HIPCHK(hipMemcpyAsync(&total, h->INC_OFF + N, 4, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipMemcpyAsync(&longest, stat, 4, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    if (longest > INC_RUN_MAX) return rebuild_inc_host(h);
+    if (std::max<size_t>(total, 1) > h->inc_cap) {
+        dfree(h, h->INC);
+        h->INC = nullptr;
+        CHK(dalloc(h, &h->INC, std::max<size_t>(total, 1)));
+        h->inc_cap = std::max<size_t>(total, 1);
+    }
+    h->epoch++;
+    if (total == 0) return MG_OK;
+    CHK(fill_u32(h, cnt, 0, (size_t)N));
+    hipLaunchKernelGGL(k_inc_fill, dim3(nblk(N)), dim3(256), 0, h->st, h->BPARTNER, h->INC_OFF, cnt, h->INC, N, B);
+    const unsigned blocks = (unsigned)std::min<size_t>(((size_t)N + 3) / 4, 8192);
+    hipLaunchKernelGGL(k_inc_order, dim3(blocks), dim3(256), 0, h->st, h->INC_OFF, h->INC, h->PID, N, B);
+    HIPCHK(hipGetLastError());
     return MG_OK;
 }
 
@@ -1185,8 +1224,8 @@ int mg_set_grid(mg_handle* h, const MgGrid* g)
     if (h->have_grid && g->total != h->C) {
         for (int k = 0; k < 2; k++) { dfree(h, h->CNT[k]); dfree(h, h->OFF[k]); }
         dfree(h, h->GCNT); dfree(h, h->GOFF);
-        dfree(h, h->CCNT); dfree(h, h->COFF);
-        h->CCNT = h->COFF = nullptr;
+        dfree(h, h->CCNT); dfree(h, h->COFF); dfree(h, h->csum);
+        h->CCNT = h->COFF = h->csum = nullptr;
         h->have_grid = false;
     }
     if (!h->have_grid) {
@@ -1540,8 +1579,17 @@ int mg_prefix_sum_changes(mg_handle* h, int zero_offsets)
     (void)zero_offsets;
     CHK(ready(h));
     CHK(ensure_changes(h));
-    hipLaunchKernelGGL(k_scan, dim3(MG_NUM_CHANGES), dim3(1024), 0, h->st, h->CCNT, h->COFF, h->C, h->ctr, 0, 0,
+    /* the two-level scan per list: tile sums, one block scanning each list's tile sums (and
+     * writing its length), tile prefixes */
+    const int C = h->C;
+    const unsigned tiles = (unsigned)((C + SCAN_TILE - 1) / SCAN_TILE);
+    if (!h->csum) CHK(dalloc(h, &h->csum, (size_t)2 * MG_NUM_CHANGES * tiles));
+    uint32_t *sums = h->csum, *excl = sums + (size_t)MG_NUM_CHANGES * tiles;
+    hipLaunchKernelGGL(k_tile_sums, dim3(tiles, MG_NUM_CHANGES), dim3(256), 0, h->st, h->CCNT, C, sums,
+                       (uint32_t*)nullptr);
+    hipLaunchKernelGGL(k_scan, dim3(MG_NUM_CHANGES), dim3(1024), 0, h->st, sums, excl, (int)tiles, h->ctr, 0, 0,
                        h->CLEN);
+    hipLaunchKernelGGL(k_tile_scan, dim3(tiles, MG_NUM_CHANGES), dim3(256), 0, h->st, h->CCNT, h->COFF, C, excl);
     HIPCHK(hipGetLastError());
     return MG_OK;
 }

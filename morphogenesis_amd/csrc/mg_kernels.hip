@@ -247,6 +247,128 @@ __global__ __launch_bounds__(256) void k_scan_lookback(const uint32_t* __restric
     scan_lookback_body(in, out, n, status, ticket, ctr, gsm, gridDim.x);
 }
 
+/* ------------------------------------------------ two-level exclusive scan
+ * For arrays the cell scans do not cover (incoming-bond and change-list offsets), one segment
+ * of n words per blockIdx.y: k_tile_sums reduces tiles of 4096 words (and, when `maxw` is
+ * given, folds their maximum into *maxw), k_scan scans each segment's tile sums in one block,
+ * k_tile_scan writes each tile's exclusive prefix. */
+__global__ __launch_bounds__(256) void k_tile_sums(const uint32_t* __restrict__ in, int n,
+                                                   uint32_t* __restrict__ sums, uint32_t* maxw)
+{
+    __shared__ uint32_t ws[4], wm[4];
+    in += (size_t)blockIdx.y * n;
+    sums += (size_t)blockIdx.y * gridDim.x;
+    const int t = threadIdx.x, base = (int)blockIdx.x * SCAN_TILE + t * 16;
+    uint32_t s = 0, m = 0;
+    for (int k = 0; k < 16; k++) {
+        const uint32_t v = base + k < n ? in[base + k] : 0u;
+        s += v;
+        m = max(m, v);
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        s += __shfl_xor(s, o);
+        m = max(m, (uint32_t)__shfl_xor(m, o));
+    }
+    if ((t & 63) == 0) { ws[t >> 6] = s; wm[t >> 6] = m; }
+    __syncthreads();
+    if (t == 0) {
+        sums[blockIdx.x] = ws[0] + ws[1] + ws[2] + ws[3];
+        if (maxw) atomicMax(maxw, max(max(wm[0], wm[1]), max(wm[2], wm[3])));
+    }
+}
+
+__global__ __launch_bounds__(256) void k_tile_scan(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
+                                                   int n, const uint32_t* __restrict__ tile_excl)
+{
+    __shared__ uint32_t wsum[4];
+    in += (size_t)blockIdx.y * n;
+    out += (size_t)blockIdx.y * n;
+    tile_excl += (size_t)blockIdx.y * gridDim.x;
+    const int t = threadIdx.x, base = (int)blockIdx.x * SCAN_TILE + t * 16;
+    uint32_t v[16], local = 0;
+    for (int k = 0; k < 16; k++) {
+        v[k] = base + k < n ? in[base + k] : 0u;
+        local += v[k];
+    }
+    const int lane = t & 63, w = t >> 6;
+    uint32_t x = local;
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    uint32_t run = tile_excl[blockIdx.x] + x - local;
+    for (int k = 0; k < w; k++) run += wsum[k];
+    for (int k = 0; k < 16; k++) {
+        if (base + k < n) out[base + k] = run;
+        run += v[k];
+    }
+}
+
+/* ------------------------------------------------ incoming-bond lists
+ * INC_OFF[j] .. INC_OFF[j + 1] lists the bonds whose partner is home j as owner_home * B + slot,
+ * ordered by (owner's particle ID, owner home, slot), so the force sums a particle's incoming
+ * terms in an order that is a property of the state, not of the launch. k_inc_count counts
+ * per partner and records each owner's live-slot mask; after the scan k_inc_fill scatters the
+ * entries (any order within a partner) and k_inc_order puts each partner's run in key order:
+ * one wavefront per partner, each entry's position is the number of smaller keys (keys are
+ * distinct), runs of up to 256 entries (longer runs take the host build). */
+__global__ LAUNCH_BOUNDS_256 void k_inc_count(const uint32_t* __restrict__ BPARTNER, uint32_t* __restrict__ cnt,
+                                              uint64_t* __restrict__ LIVE, int N, int B)
+{
+    const uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= (uint32_t)N) return;
+    uint64_t live = 0;
+    for (int a = 0; a < B; a++) {
+        const uint32_t j = BPARTNER[(size_t)a * N + h];
+        if (j < (uint32_t)N) {
+            atomicAdd(&cnt[j], 1u);
+            if (a < 64) live |= 1ull << a;
+        }
+    }
+    LIVE[h] = live;
+}
+
+__global__ LAUNCH_BOUNDS_256 void k_inc_fill(const uint32_t* __restrict__ BPARTNER, const uint32_t* __restrict__ OFF,
+                                             uint32_t* __restrict__ cursor, uint32_t* __restrict__ ENT, int N, int B)
+{
+    const uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= (uint32_t)N) return;
+    for (int a = 0; a < B; a++) {
+        const uint32_t j = BPARTNER[(size_t)a * N + h];
+        if (j < (uint32_t)N) ENT[OFF[j] + atomicAdd(&cursor[j], 1u)] = h * (uint32_t)B + (uint32_t)a;
+    }
+}
+
+#define INC_RUN_MAX 256
+__global__ __launch_bounds__(256) void k_inc_order(const uint32_t* __restrict__ OFF, uint32_t* __restrict__ ENT,
+                                                   const uint32_t* __restrict__ PID, int N, int B)
+{
+    __shared__ unsigned long long keys[4][INC_RUN_MAX];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int base = (int)blockIdx.x * 4; base < N; base += (int)gridDim.x * 4) {
+        const int j = base + w;
+        uint32_t o = 0, len = 0;
+        if (j < N) {
+            o = OFF[j];
+            len = min(OFF[j + 1] - o, (uint32_t)INC_RUN_MAX);
+        }
+        for (uint32_t k = (uint32_t)lane; k < len; k += 64) {
+            const uint32_t e = ENT[o + k];
+            keys[w][k] = ((unsigned long long)PID[e / (uint32_t)B] << 32) | e;
+        }
+        __syncthreads();
+        for (uint32_t k = (uint32_t)lane; k < len; k += 64) {
+            const unsigned long long key = keys[w][k];
+            uint32_t rank = 0;
+            for (uint32_t q = 0; q < len; q++) rank += keys[w][q] < key ? 1u : 0u;
+            ENT[o + rank] = (uint32_t)key;
+        }
+        __syncthreads();
+    }
+}
+
 /* ------------------------------------------------------ stable placement
  * PERM[OFF[cell] + slot] = s, then each cell's run of PERM is sorted ascending so the
  * intra-cell order is the pre-sort index order; FGNDX becomes that rank. */
@@ -1452,12 +1574,12 @@ __global__ LAUNCH_BOUNDS_256 void k_sort_changes(DS d, const uint32_t* __restric
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t C = (uint32_t)d.C, N = (uint32_t)d.N;
     if (c >= C) return;
-    uint32_t cap[MG_NUM_CHANGES], any = 0;
+    uint32_t cap[MG_NUM_CHANGES], others = 0;
     for (int l = 0; l < MG_NUM_CHANGES; l++) {
         cap[l] = CCNT[(size_t)l * C + c];
-        any |= cap[l];
+        if (l > 0) others |= cap[l];
     }
-    if (any == 0u) return; /* nothing can be written */
+    if ((cap[0] | others) == 0u) return; /* nothing can be written */
     const uint32_t count = d.CNT[c];
     if (count == 0u) return;
     const uint32_t off = d.OFF[c];
@@ -1474,6 +1596,13 @@ __global__ LAUNCH_BOUNDS_256 void k_sort_changes(DS d, const uint32_t* __restric
         CL[(size_t)l * 2 * N + at] = s;
         CL[(size_t)l * 2 * N + N + at] = a;
     };
+    if (others == 0u) { /* the usual case: only the heal list, no change words to read */
+        for (uint32_t s = off; s < off + count; s++) {
+            const uint32_t hs = HS[s];
+            if (hs < (uint32_t)d.B) put(0, s, hs);
+        }
+        return;
+    }
     for (uint32_t s = off; s < off + count; s++) {
         const uint32_t hs = HS[s];
         const uint32_t h = d.HOME[s];
