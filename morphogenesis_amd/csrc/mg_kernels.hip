@@ -1431,23 +1431,90 @@ __device__ __forceinline__ uint32_t find_bond(const DS& d, const MgGrid& g, cons
             const f3 u = x - xyz(d.P[j]);
             const float dsq = u.x * u.x + u.y * u.y + u.z * u.z;
             if (!(dsq < rd2 && dsq > 0.0f)) continue;
-            float theta = 0.0f;
-            for (int b = 0; b < B; b++) {
-                const uint32_t kh = d.BPARTNER[(size_t)b * N + h];
-                const uint32_t k = kh == MG_NONE ? MG_NONE : d.SLOT[kh];
-                if (j == k) {
-                    theta = FLT_MAX;
-                    break;
-                }
-                if (k < N) {
-                    const f3 v = x - xyz(d.P[k]);
-                    const float lu = sqrtf(u.x * u.x + u.y * u.y + u.z * u.z);
-                    const float lv = sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
-                    const float cosang = (u.x * v.x + u.y * v.y + u.z * v.z) / (lu * lv);
-                    theta = (float)((double)theta + fabs(1.91 - (double)mg_acosf(cosang)));
+            float theta = dsq; /* slot 0's score whatever the angle sum */
+            if (bond != 0) {
+                theta = 0.0f;
+                for (int b = 0; b < B; b++) {
+                    const uint32_t kh = d.BPARTNER[(size_t)b * N + h];
+                    const uint32_t k = kh == MG_NONE ? MG_NONE : d.SLOT[kh];
+                    if (j == k) {
+                        theta = FLT_MAX;
+                        break;
+                    }
+                    if (k < N) {
+                        const f3 v = x - xyz(d.P[k]);
+                        const float lu = sqrtf(u.x * u.x + u.y * u.y + u.z * u.z);
+                        const float lv = sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
+                        const float cosang = (u.x * v.x + u.y * v.y + u.z * v.z) / (lu * lv);
+                        theta = (float)((double)theta + fabs(1.91 - (double)mg_acosf(cosang)));
+                    }
                 }
             }
-            if (bond == 0) theta = dsq;
+            if (theta < best) {
+                best = theta;
+                to = j;
+            }
+        }
+    }
+    return to;
+}
+
+/* find_bond with the particle's current partners staged once per search: kb[b * BOND_STAGE_W]
+ * holds slot b's partner slot (MG_NONE when empty or unplaced) and vb[b * BOND_STAGE_W] =
+ * (x - x_k, |x - x_k|), the thread's column of its block's LDS. The operations and their order
+ * are find_bond's, so the result is the same bit for bit; slot 0's score is dsq whatever the
+ * angle sum, so its angle loop is skipped. */
+#define BOND_STAGE_W 64
+#define BOND_STAGE_MAX_B 48 /* 48 slots x 64 threads x 20 B = 60 KB of LDS per block */
+__device__ __forceinline__ void stage_bond(const DS& d, uint32_t h, f3 x, int b, float4* vb, uint32_t* kb)
+{
+    const uint32_t N = (uint32_t)d.N;
+    const uint32_t kh = d.BPARTNER[(size_t)b * N + h];
+    const uint32_t k = kh == MG_NONE ? MG_NONE : d.SLOT[kh];
+    float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (k < N) {
+        const f3 w = x - xyz(d.P[k]);
+        v = make_float4(w.x, w.y, w.z, sqrtf(w.x * w.x + w.y * w.y + w.z * w.z));
+    }
+    kb[b * BOND_STAGE_W] = k;
+    vb[b * BOND_STAGE_W] = v;
+}
+__device__ __forceinline__ uint32_t find_bond_staged(const DS& d, const MgGrid& g, const uint32_t* __restrict__ PID,
+                                                     uint32_t gc, f3 x, int bond, float rd2, const float4* vb,
+                                                     const uint32_t* kb)
+{
+    const uint32_t N = (uint32_t)d.N, C = (uint32_t)d.C;
+    const int B = d.B;
+    float best = FLT_MAX;
+    uint32_t to = MG_NONE;
+    for (int c = 0; c < g.adj_cnt; c++) {
+        const uint32_t cell = gc + (uint32_t)g.adj[c];
+        if (cell >= C) continue;
+        const uint32_t cn = d.CNT[cell];
+        if (cn == 0u) continue;
+        const uint32_t c0 = d.OFF[cell];
+        for (uint32_t j = c0; j < c0 + cn && j < N; j++) {
+            if (PID[d.HOME[j]] == 0u) continue;
+            const f3 u = x - xyz(d.P[j]);
+            const float dsq = u.x * u.x + u.y * u.y + u.z * u.z;
+            if (!(dsq < rd2 && dsq > 0.0f)) continue;
+            float theta = dsq;
+            if (bond != 0) {
+                theta = 0.0f;
+                const float lu = sqrtf(dsq);
+                for (int b = 0; b < B; b++) {
+                    const uint32_t k = kb[b * BOND_STAGE_W];
+                    if (j == k) {
+                        theta = FLT_MAX;
+                        break;
+                    }
+                    if (k < N) {
+                        const float4 v = vb[b * BOND_STAGE_W];
+                        const float cosang = (u.x * v.x + u.y * v.y + u.z * v.z) / (lu * v.w);
+                        theta = (float)((double)theta + fabs(1.91 - (double)mg_acosf(cosang)));
+                    }
+                }
+            }
             if (theta < best) {
                 best = theta;
                 to = j;
@@ -1476,9 +1543,11 @@ __device__ __forceinline__ void bond_types(const DS& d, uint32_t h, int bt4[4])
  * list ("solid") below mActivePoints: for each bond slot in turn the best candidate of
  * find_bond is written into the slot with the genome's default parameters of the slot's bond
  * type. Each particle writes only its own records. */
-__global__ LAUNCH_BOUNDS_256 void k_initialize_bonds(DS d, MgParams p, MgGrid g, MgGenome gen,
-                                                     const uint32_t* __restrict__ PID, uint2* BCOLD)
+__global__ __launch_bounds__(BOND_STAGE_W) void k_initialize_bonds(DS d, MgParams p, MgGrid g, MgGenome gen,
+                                                                   const uint32_t* __restrict__ PID, uint2* BCOLD,
+                                                                   int staged)
 {
+    extern __shared__ float4 mg_lds[];
     const uint32_t li = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t N = (uint32_t)d.N;
     const int B = d.B;
@@ -1490,10 +1559,15 @@ __global__ LAUNCH_BOUNDS_256 void k_initialize_bonds(DS d, MgParams p, MgGrid g,
     if (gc == MG_GRID_UNDEF) return;
     const uint32_t h = d.HOME[i];
     const f3 ipos = xyz(d.P[i]);
+    float4* vb = mg_lds + threadIdx.x;
+    uint32_t* kb = (uint32_t*)(mg_lds + (size_t)B * BOND_STAGE_W) + threadIdx.x;
+    if (staged)
+        for (int b = 0; b < B; b++) stage_bond(d, h, ipos, b, vb, kb);
     int bt4[4];
     bond_types(d, h, bt4);
     for (int bond = 0; bond < B; bond++) {
-        const uint32_t to = find_bond(d, g, PID, gc, h, ipos, bond, p.rd2);
+        const uint32_t to = staged ? find_bond_staged(d, g, PID, gc, ipos, bond, p.rd2, vb, kb)
+                                   : find_bond(d, g, PID, gc, h, ipos, bond, p.rd2);
         if (to < na) {
             const int bt = bond < 4 ? bt4[bond] : 0;
             const size_t bi = (size_t)bond * N + h;
@@ -1502,6 +1576,7 @@ __global__ LAUNCH_BOUNDS_256 void k_initialize_bonds(DS d, MgParams p, MgGrid g,
             d.BPAR[bi] = make_float4(gen.param[bt][8], gen.param[bt][9], gen.param[bt][10], gen.param[bt][11]);
             d.BINT[bi] = make_float2(0.0f, 0.0f);
             BCOLD[bi].x = PID[th];
+            if (staged) stage_bond(d, h, ipos, bond, vb, kb); /* the slot's new partner */
         }
     }
 }
@@ -1622,10 +1697,13 @@ __global__ LAUNCH_BOUNDS_256 void k_sort_changes(DS d, const uint32_t* __restric
  * default parameters (makeBond, :1573-1596; the partner's incoming list is derived from the
  * owners afterwards). Each listed particle appears once, so each thread writes only its own
  * record. */
-__global__ LAUNCH_BOUNDS_256 void k_heal(DS d, MgParams p, MgGrid g, float4 elastin, const uint32_t* __restrict__ CL,
-                                         const uint32_t* __restrict__ CLEN, const uint32_t* __restrict__ PID,
-                                         uint2* BCOLD, unsigned long long seed)
+__global__ __launch_bounds__(BOND_STAGE_W) void k_heal(DS d, MgParams p, MgGrid g, float4 elastin,
+                                                       const uint32_t* __restrict__ CL,
+                                                       const uint32_t* __restrict__ CLEN,
+                                                       const uint32_t* __restrict__ PID, uint2* BCOLD,
+                                                       unsigned long long seed, int staged)
 {
+    extern __shared__ float4 mg_lds[];
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t N = (uint32_t)d.N;
     if (k >= N || k >= CLEN[0]) return;
@@ -1642,7 +1720,16 @@ __global__ LAUNCH_BOUNDS_256 void k_heal(DS d, MgParams p, MgGrid g, float4 elas
     x.x = x.x + (max_len / (float)(4u + (r & 7u))) * (-1.0f * (float)(1u & (r >> 3)));
     x.y = x.y + (max_len / (float)(4u + ((r >> 4) & 7u))) * (-1.0f * (float)(1u & (r >> 7)));
     x.z = x.z + (max_len / (float)(4u + ((r >> 8) & 7u))) * (-1.0f * (float)(1u & (r >> 11)));
-    const uint32_t to = find_bond(d, g, PID, gc, h, x, (int)bond, p.rd2);
+    uint32_t to;
+    if (staged) {
+        float4* vb = mg_lds + threadIdx.x;
+        uint32_t* kb = (uint32_t*)(mg_lds + (size_t)d.B * BOND_STAGE_W) + threadIdx.x;
+        if (bond != 0u)
+            for (int b = 0; b < d.B; b++) stage_bond(d, h, x, b, vb, kb);
+        to = find_bond_staged(d, g, PID, gc, x, (int)bond, p.rd2, vb, kb);
+    } else {
+        to = find_bond(d, g, PID, gc, h, x, (int)bond, p.rd2);
+    }
     if (to < N) {
         const size_t bi = (size_t)bond * N + h;
         const uint32_t th = d.HOME[to];
