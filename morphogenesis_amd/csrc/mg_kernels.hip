@@ -1382,9 +1382,14 @@ __device__ __forceinline__ void diffuse_genes_insert_body(const DS& d, const MgP
             const uint32_t m = d.GMASK[i];
             if (genes && m != 0 && d.ctr[MG_C_LISTS]) {
                 const bool tally = (m & 4u) && i < d.ctr[MG_C_ACTIVE];
-                genes_one(p, gen, m, tally, epi, out);
+                uint32_t was[MG_NUM_GENES];
 #pragma unroll
-                for (int k = 0; k < MG_NUM_GENES; k++) d.EPIGEN[(size_t)k * d.N + h] = epi[k];
+                for (int k = 0; k < MG_NUM_GENES; k++) was[k] = epi[k];
+                genes_one(p, gen, m, tally, epi, out);
+                /* most states do not change in a step: store only those that did */
+#pragma unroll
+                for (int k = 0; k < MG_NUM_GENES; k++)
+                    if (epi[k] != was[k]) d.EPIGEN[(size_t)k * d.N + h] = epi[k];
             }
 #pragma unroll
             for (int k = 0; k < MG_NUM_GENES; k++)
