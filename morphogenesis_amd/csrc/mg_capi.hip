@@ -1548,7 +1548,7 @@ int mg_initialize_bonds(mg_handle* h)
     }
     CHK(materialize_gene_grid(h));
     const int staged = h->B <= BOND_STAGE_MAX_B;
-    const size_t lds = staged ? (size_t)h->B * BOND_STAGE_W * (sizeof(float4) + sizeof(uint32_t)) : 0;
+    const size_t lds = staged ? (size_t)h->B * BOND_STAGE_W * sizeof(uint32_t) : 0;
     hipLaunchKernelGGL(k_initialize_bonds, dim3(nblk(h->N, BOND_STAGE_W)), dim3(BOND_STAGE_W), lds, h->st,
                        make_ds(h), h->p, h->g, h->gen, h->PID, h->BCOLD, staged);
     HIPCHK(hipGetLastError());
@@ -1624,9 +1624,10 @@ int mg_compute_particle_changes(mg_handle* h, uint64_t seed)
     if (h->B == 0 || len[0] == 0) return MG_OK;
     const float4 elastin = make_float4(h->gen.param[0][8], h->gen.param[0][9], h->gen.param[0][10], h->gen.param[0][11]);
     const uint32_t n = std::min<uint32_t>(len[0], (uint32_t)h->N);
-    /* one search per particle: staging its partners would not pay for itself (measured slower) */
-    hipLaunchKernelGGL(k_heal, dim3(nblk(n, BOND_STAGE_W)), dim3(BOND_STAGE_W), 0, h->st, make_ds(h), h->p, h->g,
-                       elastin, h->CL, h->CLEN, h->PID, h->BCOLD, (unsigned long long)seed, 0);
+    const int staged = h->B <= BOND_STAGE_MAX_B;
+    const size_t lds = staged ? (size_t)h->B * BOND_STAGE_W * sizeof(uint32_t) : 0;
+    hipLaunchKernelGGL(k_heal, dim3(nblk(n, BOND_STAGE_W)), dim3(BOND_STAGE_W), lds, h->st, make_ds(h), h->p, h->g,
+                       elastin, h->CL, h->CLEN, h->PID, h->BCOLD, (unsigned long long)seed, staged);
     HIPCHK(hipGetLastError());
     return rebuild_inc(h); /* new partners: incoming-bond CSR and live-slot masks */
 }

@@ -1464,29 +1464,24 @@ __device__ __forceinline__ uint32_t find_bond(const DS& d, const MgGrid& g, cons
     return to;
 }
 
-/* find_bond with the particle's current partners staged once per search: kb[b * BOND_STAGE_W]
- * holds slot b's partner slot (MG_NONE when empty or unplaced) and vb[b * BOND_STAGE_W] =
- * (x - x_k, |x - x_k|), the thread's column of its block's LDS. The operations and their order
- * are find_bond's, so the result is the same bit for bit; slot 0's score is dsq whatever the
- * angle sum, so its angle loop is skipped. */
+/* find_bond with the particle's partner slots staged in LDS once per search: kb[b * BOND_STAGE_W]
+ * holds slot b's partner slot (MG_NONE when empty or unplaced), the thread's column of its
+ * block's LDS (4 B per slot and thread, 64-thread blocks). Each angle term then needs one
+ * position load instead of the partner home -> slot -> position chain. The operations and their
+ * order are find_bond's, so the result is the same bit for bit; slot 0's score is dsq whatever
+ * the angle sum, so its loop is skipped. */
 #define BOND_STAGE_W 64
-#define BOND_STAGE_MAX_B 48 /* 48 slots x 64 threads x 20 B = 60 KB of LDS per block */
-__device__ __forceinline__ void stage_bond(const DS& d, uint32_t h, f3 x, int b, float4* vb, uint32_t* kb)
+#define BOND_STAGE_MAX_B 192 /* 192 slots x 64 threads x 4 B = 48 KB of LDS per block */
+__device__ __forceinline__ void stage_partners(const DS& d, uint32_t h, uint32_t* kb)
 {
     const uint32_t N = (uint32_t)d.N;
-    const uint32_t kh = d.BPARTNER[(size_t)b * N + h];
-    const uint32_t k = kh == MG_NONE ? MG_NONE : d.SLOT[kh];
-    float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    if (k < N) {
-        const f3 w = x - xyz(d.P[k]);
-        v = make_float4(w.x, w.y, w.z, sqrtf(w.x * w.x + w.y * w.y + w.z * w.z));
+    for (int b = 0; b < d.B; b++) {
+        const uint32_t kh = d.BPARTNER[(size_t)b * N + h];
+        kb[b * BOND_STAGE_W] = kh == MG_NONE ? MG_NONE : d.SLOT[kh];
     }
-    kb[b * BOND_STAGE_W] = k;
-    vb[b * BOND_STAGE_W] = v;
 }
-__device__ __forceinline__ uint32_t find_bond_staged(const DS& d, const MgGrid& g, const uint32_t* __restrict__ PID,
-                                                     uint32_t gc, f3 x, int bond, float rd2, const float4* vb,
-                                                     const uint32_t* kb)
+__device__ __forceinline__ uint32_t find_bond_kstaged(const DS& d, const MgGrid& g, const uint32_t* __restrict__ PID,
+                                                      uint32_t gc, f3 x, int bond, float rd2, const uint32_t* kb)
 {
     const uint32_t N = (uint32_t)d.N, C = (uint32_t)d.C;
     const int B = d.B;
@@ -1514,8 +1509,9 @@ __device__ __forceinline__ uint32_t find_bond_staged(const DS& d, const MgGrid& 
                         break;
                     }
                     if (k < N) {
-                        const float4 v = vb[b * BOND_STAGE_W];
-                        const float cosang = (u.x * v.x + u.y * v.y + u.z * v.z) / (lu * v.w);
+                        const f3 v = x - xyz(d.P[k]);
+                        const float lv = sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
+                        const float cosang = (u.x * v.x + u.y * v.y + u.z * v.z) / (lu * lv);
                         theta = (float)((double)theta + fabs(1.91 - (double)mg_acosf(cosang)));
                     }
                 }
@@ -1564,14 +1560,12 @@ __global__ __launch_bounds__(BOND_STAGE_W) void k_initialize_bonds(DS d, MgParam
     if (gc == MG_GRID_UNDEF) return;
     const uint32_t h = d.HOME[i];
     const f3 ipos = xyz(d.P[i]);
-    float4* vb = mg_lds + threadIdx.x;
-    uint32_t* kb = (uint32_t*)(mg_lds + (size_t)B * BOND_STAGE_W) + threadIdx.x;
-    if (staged)
-        for (int b = 0; b < B; b++) stage_bond(d, h, ipos, b, vb, kb);
+    uint32_t* kb = (uint32_t*)mg_lds + threadIdx.x;
+    if (staged) stage_partners(d, h, kb);
     int bt4[4];
     bond_types(d, h, bt4);
     for (int bond = 0; bond < B; bond++) {
-        const uint32_t to = staged ? find_bond_staged(d, g, PID, gc, ipos, bond, p.rd2, vb, kb)
+        const uint32_t to = staged ? find_bond_kstaged(d, g, PID, gc, ipos, bond, p.rd2, kb)
                                    : find_bond(d, g, PID, gc, h, ipos, bond, p.rd2);
         if (to < na) {
             const int bt = bond < 4 ? bt4[bond] : 0;
@@ -1581,7 +1575,7 @@ __global__ __launch_bounds__(BOND_STAGE_W) void k_initialize_bonds(DS d, MgParam
             d.BPAR[bi] = make_float4(gen.param[bt][8], gen.param[bt][9], gen.param[bt][10], gen.param[bt][11]);
             d.BINT[bi] = make_float2(0.0f, 0.0f);
             BCOLD[bi].x = PID[th];
-            if (staged) stage_bond(d, h, ipos, bond, vb, kb); /* the slot's new partner */
+            if (staged) kb[bond * BOND_STAGE_W] = to; /* the slot's new partner */
         }
     }
 }
@@ -1727,11 +1721,9 @@ __global__ __launch_bounds__(BOND_STAGE_W) void k_heal(DS d, MgParams p, MgGrid 
     x.z = x.z + (max_len / (float)(4u + ((r >> 8) & 7u))) * (-1.0f * (float)(1u & (r >> 11)));
     uint32_t to;
     if (staged) {
-        float4* vb = mg_lds + threadIdx.x;
-        uint32_t* kb = (uint32_t*)(mg_lds + (size_t)d.B * BOND_STAGE_W) + threadIdx.x;
-        if (bond != 0u)
-            for (int b = 0; b < d.B; b++) stage_bond(d, h, x, b, vb, kb);
-        to = find_bond_staged(d, g, PID, gc, x, (int)bond, p.rd2, vb, kb);
+        uint32_t* kb = (uint32_t*)mg_lds + threadIdx.x;
+        if (bond != 0u) stage_partners(d, h, kb);
+        to = find_bond_kstaged(d, g, PID, gc, x, (int)bond, p.rd2, kb);
     } else {
         to = find_bond(d, g, PID, gc, h, x, (int)bond, p.rd2);
     }

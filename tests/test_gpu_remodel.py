@@ -88,3 +88,34 @@ def test_remodelling_errors(built):
     with pytest.raises(F.MgError):
         g.ComputeParticleChangesCL(1)
     assert np.array_equal(g.download(F.FELASTIDX), before)
+
+
+def test_remodelling_demo_scene_from_reference_files(built):
+    """The reference's demo inputs (20 living particles, B = 6, its genome: the elastin
+    lengthen row is all zero, so every remodel breaks each elastin bond and heal reconnects
+    it) through the reference's loop with remodelling, 12 cycles."""
+    from morphogenesis_amd import fileio as fio
+    from morphogenesis_amd.fluid_system import FluidSystem
+    from oracle.mg_oracle import Oracle
+    from tests.test_fileio import GENOME, PARAMS, POINTS
+
+    say("case: demo scene with remodelling (SimParams.txt, genome.csv, point file)")
+    g = FluidSystem.from_reference_files(PARAMS, POINTS, GENOME)
+    sc = fio.load_scene(PARAMS, POINTS, GENOME)
+    o = Oracle(sc.n, sc.b)
+    o.load_scene(sc)
+    for sim in (g, o):
+        sim.Run2PhysicalSort()
+    healed = 0
+    for cycle in range(12):
+        for sim in (g, o):
+            for _ in range(3):
+                sim.Run2InnerPhysicalLoop(sc.dt)
+            sim.Run2GeneAction()
+            sim.Run2Remodelling(3, seed=cycle)
+        compare_all(g, o, f"demo cycle {cycle}: remodelled", ALL)
+        healed += int(o.download(F.FDENSE_LIST_LENGTHS_CHANGES)[0])
+        for sim in (g, o):
+            sim.Run2PhysicalSort()
+        compare_all(g, o, f"demo cycle {cycle}: sort", ALL)
+    assert healed >= 12 * 18

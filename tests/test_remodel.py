@@ -361,3 +361,30 @@ def test_other_change_lists_are_refused_and_zero_steps_rejected():
     with pytest.raises(F.MgError):
         o.ComputeParticleChangesCL(1)
     assert np.array_equal(o.download(F.FELASTIDX), before)
+
+
+def test_demo_genome_breaks_and_heals_every_elastin_bond():
+    """The reference's demo genome has an all-zero elastin lengthen row: a remodel multiplies
+    every elastin rest length by 0 + 0 * tanh(..) = 0, so each living particle's first bond is
+    broken and listed for heal, and heal gives it a fresh elastin bond."""
+    from morphogenesis_amd import fileio as fio
+    from oracle.mg_oracle import Oracle
+    from tests.test_fileio import GENOME, PARAMS, POINTS
+
+    sc = fio.load_scene(PARAMS, POINTS, GENOME)
+    o = Oracle(sc.n, sc.b)
+    o.load_scene(sc)
+    o.Run2PhysicalSort()
+    assert int(o.download(F.FDENSE_LIST_LENGTHS)[2]) == 20
+    elastin = f2u([sc.genome.param[0][8], sc.genome.param[0][9], sc.genome.param[0][10], sc.genome.param[0][11]])
+    o.Run2Remodelling(3, 0)                       # no bonds yet: slot 0 of all 20 is empty
+    assert o.download(F.FDENSE_LIST_LENGTHS_CHANGES)[0] == 20
+    rec = o.download(F.FELASTIDX)
+    bonded = rec[:, 0, 0] != F.NONE
+    assert bonded.sum() >= 18 and (rec[bonded, 0, 1:5] == elastin).all()
+    o.ComputeBondChangesCL(3)
+    after = o.download(F.FELASTIDX)
+    assert (u2f(after[bonded, 0, 2]) == 0).all()  # rest length * 0
+    assert (after[bonded, 0, 8] == 1).all()
+    cc = o.download(F.FBIN_COUNT_CHANGES)
+    assert cc[0].sum() == 20
