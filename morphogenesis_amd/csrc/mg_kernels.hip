@@ -456,6 +456,7 @@ __device__ __forceinline__ void sort_gather_body(const DS& d, float vmax, uint32
     if (dd < nump) {
         const uint32_t s = d.PERM[dd];
         const uint32_t h = d.HOME[s];
+        d.SLOT[h] = dd;
         d.P2[dd] = d.P[s];
         d.V2[dd] = d.V[s];
         d.VEL2[dd] = d.VEL[s];
@@ -472,7 +473,6 @@ __device__ __forceinline__ void sort_gather_body(const DS& d, float vmax, uint32
                 if (d.EPIGEN[(size_t)gene * d.N + h] > 0u) m |= 1u << gene;
         }
         d.GMASK2[dd] = m;
-        d.SLOT[h] = dd;
     } else {
         float4 p = d.P[dd];
         d.P2[dd] = make_float4(vmax, vmax, vmax, p.w);
