@@ -1547,10 +1547,15 @@ int mg_initialize_bonds(mg_handle* h)
         return MG_ERR_STATE;
     }
     CHK(materialize_gene_grid(h));
+    /* eight lanes per listed particle share its candidate scan (one lane when unstaged) */
     const int staged = h->B <= BOND_STAGE_MAX_B;
     const size_t lds = staged ? (size_t)h->B * BOND_STAGE_W * sizeof(uint32_t) : 0;
-    hipLaunchKernelGGL(k_initialize_bonds, dim3(nblk(h->N, BOND_STAGE_W)), dim3(BOND_STAGE_W), lds, h->st,
-                       make_ds(h), h->p, h->g, h->gen, h->PID, h->BCOLD, staged);
+    if (staged)
+        hipLaunchKernelGGL(k_initialize_bonds<8>, dim3(nblk((size_t)h->N * 8, BOND_STAGE_W)), dim3(BOND_STAGE_W), lds,
+                           h->st, make_ds(h), h->p, h->g, h->gen, h->PID, h->BCOLD, 1);
+    else
+        hipLaunchKernelGGL(k_initialize_bonds<1>, dim3(nblk(h->N, BOND_STAGE_W)), dim3(BOND_STAGE_W), 0, h->st,
+                           make_ds(h), h->p, h->g, h->gen, h->PID, h->BCOLD, 0);
     HIPCHK(hipGetLastError());
     return rebuild_inc(h); /* new partners: incoming-bond CSR and live-slot masks */
 }
@@ -1624,10 +1629,16 @@ int mg_compute_particle_changes(mg_handle* h, uint64_t seed)
     if (h->B == 0 || len[0] == 0) return MG_OK;
     const float4 elastin = make_float4(h->gen.param[0][8], h->gen.param[0][9], h->gen.param[0][10], h->gen.param[0][11]);
     const uint32_t n = std::min<uint32_t>(len[0], (uint32_t)h->N);
+    /* four lanes per listed particle share its candidate scan (one lane when unstaged) */
     const int staged = h->B <= BOND_STAGE_MAX_B;
     const size_t lds = staged ? (size_t)h->B * BOND_STAGE_W * sizeof(uint32_t) : 0;
-    hipLaunchKernelGGL(k_heal, dim3(nblk(n, BOND_STAGE_W)), dim3(BOND_STAGE_W), lds, h->st, make_ds(h), h->p, h->g,
-                       elastin, h->CL, h->CLEN, h->PID, h->BCOLD, (unsigned long long)seed, staged);
+    if (staged)
+        hipLaunchKernelGGL(k_heal<4>, dim3(nblk((size_t)n * 4, BOND_STAGE_W)), dim3(BOND_STAGE_W), lds, h->st,
+                           make_ds(h), h->p, h->g, elastin, h->CL, h->CLEN, h->PID, h->BCOLD,
+                           (unsigned long long)seed, 1);
+    else
+        hipLaunchKernelGGL(k_heal<1>, dim3(nblk(n, BOND_STAGE_W)), dim3(BOND_STAGE_W), 0, h->st, make_ds(h), h->p,
+                           h->g, elastin, h->CL, h->CLEN, h->PID, h->BCOLD, (unsigned long long)seed, 0);
     HIPCHK(hipGetLastError());
     return rebuild_inc(h); /* new partners: incoming-bond CSR and live-slot masks */
 }

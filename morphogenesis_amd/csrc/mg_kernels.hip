@@ -1464,12 +1464,11 @@ __device__ __forceinline__ uint32_t find_bond(const DS& d, const MgGrid& g, cons
     return to;
 }
 
-/* find_bond with the particle's partner slots staged in LDS once per search: kb[b * BOND_STAGE_W]
- * holds slot b's partner slot (MG_NONE when empty or unplaced), the thread's column of its
- * block's LDS (4 B per slot and thread, 64-thread blocks). Each angle term then needs one
- * position load instead of the partner home -> slot -> position chain. The operations and their
- * order are find_bond's, so the result is the same bit for bit; slot 0's score is dsq whatever
- * the angle sum, so its loop is skipped. */
+/* The searches stage the particle's partner slots in LDS once: kb[b * BOND_STAGE_W] holds slot
+ * b's partner slot (MG_NONE when empty or unplaced), the thread's column of its block's LDS (4 B
+ * per slot and thread, 64-thread blocks). Each angle term then needs one position load instead
+ * of the partner home -> slot -> position chain; slot 0's score is dsq whatever the angle sum,
+ * so its loop is skipped. */
 #define BOND_STAGE_W 64
 #define BOND_STAGE_MAX_B 192 /* 192 slots x 64 threads x 4 B = 48 KB of LDS per block */
 __device__ __forceinline__ void stage_partners(const DS& d, uint32_t h, uint32_t* kb)
@@ -1480,20 +1479,31 @@ __device__ __forceinline__ void stage_partners(const DS& d, uint32_t h, uint32_t
         kb[b * BOND_STAGE_W] = kh == MG_NONE ? MG_NONE : d.SLOT[kh];
     }
 }
-__device__ __forceinline__ uint32_t find_bond_kstaged(const DS& d, const MgGrid& g, const uint32_t* __restrict__ PID,
-                                                      uint32_t gc, f3 x, int bond, float rd2, const uint32_t* kb)
+/* The staged search split over the S consecutive lanes of a particle's group (S a power of two,
+ * all S lanes call it together): the candidates are numbered q = 0, 1, ... in the reference's
+ * visit order (cells in gridAdj order, slots ascending) and lane `sub` scores those with
+ * q = sub mod S, keeping its first strictly best; the group then takes the smallest score,
+ * ties to the smallest q. That is the candidate the serial loop keeps (the first one reaching
+ * the minimum; FLT_MAX and NaN scores never win), and each score is computed by the same
+ * operations in the same order, so the result is bit-equal to find_bond. */
+template <int S>
+__device__ __forceinline__ uint32_t find_bond_split(const DS& d, const MgGrid& g, const uint32_t* __restrict__ PID,
+                                                    uint32_t gc, f3 x, int bond, float rd2, const uint32_t* kb,
+                                                    uint32_t sub)
 {
     const uint32_t N = (uint32_t)d.N, C = (uint32_t)d.C;
     const int B = d.B;
     float best = FLT_MAX;
-    uint32_t to = MG_NONE;
+    uint32_t bq = 0xFFFFFFFFu, to = MG_NONE, qbase = 0;
     for (int c = 0; c < g.adj_cnt; c++) {
         const uint32_t cell = gc + (uint32_t)g.adj[c];
         if (cell >= C) continue;
         const uint32_t cn = d.CNT[cell];
         if (cn == 0u) continue;
         const uint32_t c0 = d.OFF[cell];
-        for (uint32_t j = c0; j < c0 + cn && j < N; j++) {
+        const uint32_t c1 = min(c0 + cn, N);
+        if (c0 >= c1) continue;
+        for (uint32_t j = c0 + ((sub + (uint32_t)S - qbase % (uint32_t)S) % (uint32_t)S); j < c1; j += (uint32_t)S) {
             if (PID[d.HOME[j]] == 0u) continue;
             const f3 u = x - xyz(d.P[j]);
             const float dsq = u.x * u.x + u.y * u.y + u.z * u.z;
@@ -1518,8 +1528,21 @@ __device__ __forceinline__ uint32_t find_bond_kstaged(const DS& d, const MgGrid&
             }
             if (theta < best) {
                 best = theta;
+                bq = qbase + (j - c0);
                 to = j;
             }
+        }
+        qbase += c1 - c0;
+    }
+#pragma unroll
+    for (int o = 1; o < S; o <<= 1) {
+        const float ot = __shfl_xor(best, o);
+        const uint32_t oq = (uint32_t)__shfl_xor((int)bq, o);
+        const uint32_t oj = (uint32_t)__shfl_xor((int)to, o);
+        if (ot < best || (ot == best && oq < bq)) {
+            best = ot;
+            bq = oq;
+            to = oj;
         }
     }
     return to;
@@ -1544,15 +1567,17 @@ __device__ __forceinline__ void bond_types(const DS& d, uint32_t h, int bt4[4])
  * list ("solid") below mActivePoints: for each bond slot in turn the best candidate of
  * find_bond is written into the slot with the genome's default parameters of the slot's bond
  * type. Each particle writes only its own records. */
+template <int S>
 __global__ __launch_bounds__(BOND_STAGE_W) void k_initialize_bonds(DS d, MgParams p, MgGrid g, MgGenome gen,
                                                                    const uint32_t* __restrict__ PID, uint2* BCOLD,
                                                                    int staged)
 {
     extern __shared__ float4 mg_lds[];
-    const uint32_t li = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t li = (blockIdx.x * blockDim.x + threadIdx.x) / (uint32_t)S;
+    const uint32_t sub = threadIdx.x % (uint32_t)S;
     const uint32_t N = (uint32_t)d.N;
     const int B = d.B;
-    if (li >= N || li >= d.LENS[1]) return;
+    if (li >= N || li >= d.LENS[1]) return; /* whole groups leave together */
     const uint32_t i = d.LISTS[(size_t)1 * N + li];
     const uint32_t na = d.ctr[MG_C_ACTIVE];
     if (i >= na) return;
@@ -1565,17 +1590,19 @@ __global__ __launch_bounds__(BOND_STAGE_W) void k_initialize_bonds(DS d, MgParam
     int bt4[4];
     bond_types(d, h, bt4);
     for (int bond = 0; bond < B; bond++) {
-        const uint32_t to = staged ? find_bond_kstaged(d, g, PID, gc, ipos, bond, p.rd2, kb)
+        const uint32_t to = staged ? find_bond_split<S>(d, g, PID, gc, ipos, bond, p.rd2, kb, sub)
                                    : find_bond(d, g, PID, gc, h, ipos, bond, p.rd2);
         if (to < na) {
-            const int bt = bond < 4 ? bt4[bond] : 0;
-            const size_t bi = (size_t)bond * N + h;
-            const uint32_t th = d.HOME[to];
-            d.BPARTNER[bi] = th;
-            d.BPAR[bi] = make_float4(gen.param[bt][8], gen.param[bt][9], gen.param[bt][10], gen.param[bt][11]);
-            d.BINT[bi] = make_float2(0.0f, 0.0f);
-            BCOLD[bi].x = PID[th];
-            if (staged) kb[bond * BOND_STAGE_W] = to; /* the slot's new partner */
+            if (sub == 0u) {
+                const int bt = bond < 4 ? bt4[bond] : 0;
+                const size_t bi = (size_t)bond * N + h;
+                const uint32_t th = d.HOME[to];
+                d.BPARTNER[bi] = th;
+                d.BPAR[bi] = make_float4(gen.param[bt][8], gen.param[bt][9], gen.param[bt][10], gen.param[bt][11]);
+                d.BINT[bi] = make_float2(0.0f, 0.0f);
+                BCOLD[bi].x = PID[th];
+            }
+            if (staged) kb[bond * BOND_STAGE_W] = to; /* the slot's new partner, every lane's copy */
         }
     }
 }
@@ -1696,6 +1723,7 @@ __global__ LAUNCH_BOUNDS_256 void k_sort_changes(DS d, const uint32_t* __restric
  * default parameters (makeBond, :1573-1596; the partner's incoming list is derived from the
  * owners afterwards). Each listed particle appears once, so each thread writes only its own
  * record. */
+template <int S>
 __global__ __launch_bounds__(BOND_STAGE_W) void k_heal(DS d, MgParams p, MgGrid g, float4 elastin,
                                                        const uint32_t* __restrict__ CL,
                                                        const uint32_t* __restrict__ CLEN,
@@ -1703,7 +1731,8 @@ __global__ __launch_bounds__(BOND_STAGE_W) void k_heal(DS d, MgParams p, MgGrid 
                                                        unsigned long long seed, int staged)
 {
     extern __shared__ float4 mg_lds[];
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t k = (blockIdx.x * blockDim.x + threadIdx.x) / (uint32_t)S;
+    const uint32_t sub = threadIdx.x % (uint32_t)S;
     const uint32_t N = (uint32_t)d.N;
     if (k >= N || k >= CLEN[0]) return;
     const uint32_t i = CL[k];
@@ -1723,11 +1752,11 @@ __global__ __launch_bounds__(BOND_STAGE_W) void k_heal(DS d, MgParams p, MgGrid 
     if (staged) {
         uint32_t* kb = (uint32_t*)mg_lds + threadIdx.x;
         if (bond != 0u) stage_partners(d, h, kb);
-        to = find_bond_kstaged(d, g, PID, gc, x, (int)bond, p.rd2, kb);
+        to = find_bond_split<S>(d, g, PID, gc, x, (int)bond, p.rd2, kb, sub);
     } else {
         to = find_bond(d, g, PID, gc, h, x, (int)bond, p.rd2);
     }
-    if (to < N) {
+    if (to < N && sub == 0u) {
         const size_t bi = (size_t)bond * N + h;
         const uint32_t th = d.HOME[to];
         d.BPARTNER[bi] = th;
