@@ -119,3 +119,31 @@ def test_remodelling_demo_scene_from_reference_files(built):
             sim.Run2PhysicalSort()
         compare_all(g, o, f"demo cycle {cycle}: sort", ALL)
     assert healed >= 12 * 18
+
+
+@pytest.mark.parametrize("spokes", [200, 300])
+def test_hub_particle_incoming_lists(built, spokes):
+    """One particle that every other one bonds to: 200 incoming bonds take the device build's
+    multi-pass rank sort, 300 take the host build (runs over 256 entries). Steps bit-equal to
+    the oracle either way, particle IDs shuffled so the incoming order is not the home order."""
+    from tests.helpers import mini_scene
+
+    rng = np.random.default_rng(spokes)
+    n = spokes + 1
+    hub = np.array([6.0, 6.0, 6.0])
+    pos = [hub] + [hub + rng.normal(0, 1, 3) / 1.0 for _ in range(spokes)]
+    pos = np.clip(np.array(pos), 3.5, 8.5)
+    sc = mini_scene(pos, b=2)
+    ids = rng.permutation(n).astype(np.uint32) + 1
+    sc.buffers[F.FPARTICLE_ID][:] = ids
+    rec = sc.buffers[F.FELASTIDX]          # FPARTICLEIDX cannot hold more than B incoming: left empty
+    f = lambda x: np.float32(x).view(np.uint32)
+    for k in range(1, n):
+        rest = float(np.linalg.norm(pos[k] - hub)) + 0.01
+        rec[k, 0] = [0, f(4.0), f(rest), f(5.0), f(0.1), ids[0], f(0), f(0), 0]
+    say("case: hub particle with", spokes, "incoming bonds")
+    g, o = pair(sc)
+    for sim in (g, o):
+        sim.Run2PhysicalSort()
+        sim.step(12, sc.dt, sc.flags | (F.STEP_GRAPH if sim is g else 0))
+    compare_all(g, o, f"hub {spokes}: 12 steps")
