@@ -265,4 +265,24 @@ int mgfs_run(mgfs_handle* f, int nsteps, int genes)
     }
 }
 
+/* Run2Simulation's loop with remodelling (src/fluid_system.cpp:2671-2684): a sort, then
+ * `cycles` x [steps x Run2InnerPhysicalLoop, Run2GeneAction (genes), Run2Remodelling(steps),
+ * Run2PhysicalSort]. */
+int mgfs_run_remodel(mgfs_handle* f, int cycles, int steps, int genes)
+{
+    auto* fs = reinterpret_cast<mg::FluidSystem*>(f);
+    try {
+        fs->Run2PhysicalSort();
+        for (int c = 0; c < cycles; c++) {
+            for (int s = 0; s < steps; s++) fs->Run2InnerPhysicalLoop();
+            if (genes) fs->Run2GeneAction();
+            fs->Run2Remodelling((uint32_t)steps);
+            fs->Run2PhysicalSort();
+        }
+        return MG_OK;
+    } catch (const mg::Error& e) {
+        return e.code;
+    }
+}
+
 } // extern "C"

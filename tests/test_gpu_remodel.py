@@ -147,3 +147,49 @@ def test_hub_particle_incoming_lists(built, spokes):
         sim.Run2PhysicalSort()
         sim.step(12, sc.dt, sc.flags | (F.STEP_GRAPH if sim is g else 0))
     compare_all(g, o, f"hub {spokes}: 12 steps")
+
+
+def test_cpp_facade_remodelling(built):
+    """The C++ FluidSystem's Run2Remodelling (seed = its call count) inside the reference's loop
+    equals the same sequence through the C-ABI."""
+    import ctypes as C
+
+    from morphogenesis_amd.fluid_system import FluidSystem
+    from tests.helpers import assert_bits_equal
+
+    lib = F.load_library()
+    lib.mgfs_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_int]
+    lib.mgfs_set_host.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t]
+    lib.mgfs_get_host.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t]
+    lib.mgfs_setup.argtypes = [C.c_void_p, C.POINTER(F.MgGrid), C.POINTER(F.MgParams),
+                               C.POINTER(F.MgGenome), C.c_double]
+    lib.mgfs_transfer_to.argtypes = [C.c_void_p]
+    lib.mgfs_transfer_from.argtypes = [C.c_void_p]
+    lib.mgfs_run_remodel.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
+    lib.mgfs_destroy.argtypes = [C.c_void_p]
+    sc = remodel_scene(25, n_side=8, b=16)
+    say("case: C++ facade Run2Remodelling")
+    fh = C.c_void_p()
+    assert lib.mgfs_create(C.byref(fh), sc.n, sc.b) == 0
+    assert lib.mgfs_setup(fh, C.byref(sc.grid), C.byref(sc.params), C.byref(sc.genome), sc.dt) == 0
+    for b, arr in sc.buffers.items():
+        a = np.ascontiguousarray(arr)
+        assert lib.mgfs_set_host(fh, b, a.ctypes.data_as(C.c_void_p), a.nbytes) == 0
+    assert lib.mgfs_transfer_to(fh) == 0
+    assert lib.mgfs_run_remodel(fh, 3, 4, 1) == 0
+    assert lib.mgfs_transfer_from(fh) == 0
+    g = FluidSystem(sc.n, sc.b)
+    g.load_scene(sc)
+    g.Run2PhysicalSort()
+    for cycle in range(3):
+        for _ in range(4):
+            g.Run2InnerPhysicalLoop(sc.dt)
+        g.Run2GeneAction()
+        g.Run2Remodelling(4, seed=cycle)
+        g.Run2PhysicalSort()
+    for b in (F.FPOS, F.FVEL, F.FCONC, F.FEPIGEN, F.FELASTIDX, F.FGCELL):
+        shape, dt = F.buffer_shape(b, sc.n, sc.c, sc.b)
+        a = np.empty(shape, dt)
+        assert lib.mgfs_get_host(fh, b, a.ctypes.data_as(C.c_void_p), a.nbytes) == 0
+        assert_bits_equal(a, g.download(b), f"facade remodel {F.BUFFER_NAMES[b]}")
+    lib.mgfs_destroy(fh)
