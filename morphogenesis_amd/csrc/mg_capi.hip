@@ -274,7 +274,7 @@ struct mg_handle {
     /* remodelling change lists, allocated on first use: counts and offsets [9][C], lengths [9],
      * (slot, bond) pairs [9][2][N], and per slot the bond computeBondChanges counted for heal */
     uint32_t *CCNT = nullptr, *COFF = nullptr, *CLEN = nullptr, *CL = nullptr, *HS = nullptr;
-    uint32_t* csum = nullptr; /* their scan's tile sums and prefixes [2][9][tiles] */
+    uint32_t* tsum = nullptr; /* tile sums and prefixes of the segmented scans [2][16][tiles] */
     /* OFF[gcur] is the exclusive prefix sum of CNT[gcur] (set by a scan, cleared by an insert
      * or an upload of either); with a row-structured gridAdj the sweeps walk rows */
     bool grid_scanned = false;
@@ -744,18 +744,31 @@ int grid_scan_max_index(const MgGrid& g)
     return (g.scan_max[1] * g.res[2] + g.scan_max[2]) * g.res[0] + g.scan_max[0];
 }
 
+/* Exclusive scans of `segs` (<= 16) consecutive segments of C words, in -> out, each segment's
+ * total into lens[seg] (count[C-1] + offset[C-1], tally_denselist_lengths): tile sums, one block
+ * per segment scanning its tile sums, tile prefixes. */
+int seg_scan(mg_handle* h, const uint32_t* in, uint32_t* out, int segs, uint32_t* lens)
+{
+    const int C = h->C;
+    const unsigned tiles = (unsigned)((C + SCAN_TILE - 1) / SCAN_TILE);
+    if (!h->tsum) CHK(dalloc(h, &h->tsum, (size_t)2 * MG_NUM_GENES * tiles));
+    uint32_t *sums = h->tsum, *excl = sums + (size_t)MG_NUM_GENES * tiles;
+    hipLaunchKernelGGL(k_tile_sums, dim3(tiles, segs), dim3(256), 0, h->st, in, C, sums, (uint32_t*)nullptr);
+    hipLaunchKernelGGL(k_scan, dim3(segs), dim3(1024), 0, h->st, sums, excl, (int)tiles, h->ctr, 0, 0, lens);
+    hipLaunchKernelGGL(k_tile_scan, dim3(tiles, segs), dim3(256), 0, h->st, in, out, C, excl);
+    HIPCHK(hipGetLastError());
+    return MG_OK;
+}
+
 int do_scan(mg_handle* h, int genes)
 {
     PhaseScope ps(h, MG_PH_SCAN);
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, h->st, h->CNT[h->gcur], h->OFF[h->gcur], h->C, h->ctr, 1,
-                       grid_scan_max_index(h->g), (uint32_t*)nullptr);
+    const unsigned tiles = (unsigned)((h->C + SCAN_TILE - 1) / SCAN_TILE);
+    hipLaunchKernelGGL(k_scan_lookback, dim3(tiles), dim3(256), 0, h->st, h->CNT[h->gcur], h->OFF[h->gcur], h->C,
+                       h->scan_status, h->scan_ticket, h->ctr, grid_scan_max_index(h->g));
     HIPCHK(hipGetLastError());
     h->grid_scanned = true;
-    if (genes) {
-        hipLaunchKernelGGL(k_scan, dim3(MG_NUM_GENES), dim3(1024), 0, h->st, h->GCNT, h->GOFF,
-                           h->C, h->ctr, 0, 0, h->LENS);
-        HIPCHK(hipGetLastError());
-    }
+    if (genes) CHK(seg_scan(h, h->GCNT, h->GOFF, MG_NUM_GENES, h->LENS));
     return MG_OK;
 }
 
@@ -790,9 +803,6 @@ int do_sort(mg_handle* h, int lists)
     return MG_OK;
 }
 
-/* Rebuild FBIN_COUNT_ACTIVE_GENES / FBIN_OFFSET_ACTIVE_GENES / FDENSE_LIST_LENGTHS /
- * FDENSE_LISTS from the sorted state after fused steps (which skip them): the values the
- * reference's insert + scan + dense-list kernels leave after a sort. */
 /* The change-list buffers, allocated and cleared (pairs to MG_NONE) on first use. */
 int ensure_changes(mg_handle* h)
 {
@@ -806,6 +816,9 @@ int ensure_changes(mg_handle* h)
     return MG_OK;
 }
 
+/* Rebuild FBIN_COUNT_ACTIVE_GENES / FBIN_OFFSET_ACTIVE_GENES / FDENSE_LIST_LENGTHS /
+ * FDENSE_LISTS from the sorted state after fused steps and Run2PhysicalSort (which skip them):
+ * the values the reference's insert + scan + dense-list kernels leave after a sort. */
 int materialize_gene_grid(mg_handle* h)
 {
     if (!h->gene_grid_stale) return MG_OK;
@@ -813,8 +826,7 @@ int materialize_gene_grid(mg_handle* h)
     DS d = make_ds(h);
     CHK(fill_u32(h, h->GCNT, 0, (size_t)MG_NUM_GENES * h->C));
     hipLaunchKernelGGL(k_gene_counts_sorted, dim3(nblk(h->N)), dim3(256), 0, h->st, d);
-    hipLaunchKernelGGL(k_scan, dim3(MG_NUM_GENES), dim3(1024), 0, h->st, h->GCNT, h->GOFF, h->C,
-                       h->ctr, 0, 0, h->LENS);
+    CHK(seg_scan(h, h->GCNT, h->GOFF, MG_NUM_GENES, h->LENS));
     CHK(fill_u32(h, h->LISTS, MG_NONE, (size_t)MG_NUM_GENES * h->N));
     hipLaunchKernelGGL(k_dense_lists, dim3(nblk(h->C)), dim3(256), 0, h->st, d);
     HIPCHK(hipGetLastError());
@@ -1224,8 +1236,8 @@ int mg_set_grid(mg_handle* h, const MgGrid* g)
     if (h->have_grid && g->total != h->C) {
         for (int k = 0; k < 2; k++) { dfree(h, h->CNT[k]); dfree(h, h->OFF[k]); }
         dfree(h, h->GCNT); dfree(h, h->GOFF);
-        dfree(h, h->CCNT); dfree(h, h->COFF); dfree(h, h->csum);
-        h->CCNT = h->COFF = h->csum = nullptr;
+        dfree(h, h->CCNT); dfree(h, h->COFF); dfree(h, h->tsum);
+        h->CCNT = h->COFF = h->tsum = nullptr;
         h->have_grid = false;
     }
     if (!h->have_grid) {
@@ -1586,17 +1598,7 @@ int mg_prefix_sum_changes(mg_handle* h, int zero_offsets)
     (void)zero_offsets;
     CHK(ready(h));
     CHK(ensure_changes(h));
-    /* the two-level scan per list: tile sums, one block scanning each list's tile sums (and
-     * writing its length), tile prefixes */
-    const int C = h->C;
-    const unsigned tiles = (unsigned)((C + SCAN_TILE - 1) / SCAN_TILE);
-    if (!h->csum) CHK(dalloc(h, &h->csum, (size_t)2 * MG_NUM_CHANGES * tiles));
-    uint32_t *sums = h->csum, *excl = sums + (size_t)MG_NUM_CHANGES * tiles;
-    hipLaunchKernelGGL(k_tile_sums, dim3(tiles, MG_NUM_CHANGES), dim3(256), 0, h->st, h->CCNT, C, sums,
-                       (uint32_t*)nullptr);
-    hipLaunchKernelGGL(k_scan, dim3(MG_NUM_CHANGES), dim3(1024), 0, h->st, sums, excl, (int)tiles, h->ctr, 0, 0,
-                       h->CLEN);
-    hipLaunchKernelGGL(k_tile_scan, dim3(tiles, MG_NUM_CHANGES), dim3(256), 0, h->st, h->CCNT, h->COFF, C, excl);
+    CHK(seg_scan(h, h->CCNT, h->COFF, MG_NUM_CHANGES, h->CLEN));
     HIPCHK(hipGetLastError());
     return MG_OK;
 }
@@ -1646,10 +1648,13 @@ int mg_compute_particle_changes(mg_handle* h, uint64_t seed)
 int mg_physical_sort(mg_handle* h)
 {
     CHK(ready(h));
-    CHK(materialize_gene_grid(h));
-    CHK(do_insert(h, 1));
-    CHK(do_scan(h, 1));
-    return do_sort(h, 1);
+    /* the gene counts, offsets and dense lists are observable only through downloads and the
+     * calls that read them, which rebuild them from the sorted state (materialize_gene_grid) */
+    CHK(do_insert(h, 0));
+    CHK(do_scan(h, 0));
+    CHK(do_sort(h, 0));
+    h->gene_grid_stale = true;
+    return MG_OK;
 }
 
 int mg_inner_physical_loop(mg_handle* h, float time, float dt)
@@ -1698,9 +1703,10 @@ int mg_step(mg_handle* h, int nsteps, float dt, unsigned flags)
             CHK(do_diffusion(h));
             CHK(do_genes(h));
         }
-        CHK(do_insert(h, 1));
-        CHK(do_scan(h, 1));
-        CHK(do_sort(h, 1));
+        CHK(do_insert(h, 0));
+        CHK(do_scan(h, 0));
+        CHK(do_sort(h, 0));
+        h->gene_grid_stale = true;
     }
     return MG_OK;
 }
