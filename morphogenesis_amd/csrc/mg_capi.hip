@@ -1022,19 +1022,24 @@ int ensure_force(mg_handle* h)
     return MG_OK;
 }
 
-int do_pressure(mg_handle* h)
+/* hm: hit masks to record (pressure) or to visit (force); only a pressure and a force of the
+ * same positions and grid may share them, as inside one inner loop */
+int do_pressure(mg_handle* h, uint64_t* hm = nullptr)
 {
     PhaseScope ps(h, MG_PH_PRESSURE);
-    hipLaunchKernelGGL(k_pressure, dim3(nblk(h->N)), dim3(256), band_lds<0>(256), h->st, make_ds(h), h->p, h->g);
+    DS d = make_ds(h);
+    d.HMW = hm;
+    hipLaunchKernelGGL(k_pressure, dim3(nblk(h->N)), dim3(256), band_lds<0>(256), h->st, d, h->p, h->g);
     HIPCHK(hipGetLastError());
     return MG_OK;
 }
 
-int do_force(mg_handle* h)
+int do_force(mg_handle* h, const uint64_t* hm = nullptr)
 {
     CHK(ensure_force(h));
     PhaseScope ps(h, MG_PH_FORCE);
     DS d = make_ds(h);
+    d.HMR = hm;
     hipLaunchKernelGGL(k_force, dim3(nblk(h->N)), dim3(256), band_lds<1>(256), h->st, d, h->p, h->g, 1);
     HIPCHK(hipGetLastError());
     return MG_OK;
@@ -1661,8 +1666,10 @@ int mg_inner_physical_loop(mg_handle* h, float time, float dt)
 {
     (void)time;
     CHK(ready(h));
-    CHK(do_pressure(h));
-    CHK(do_force(h));
+    ensure_hit_masks(h);
+    uint64_t* hm = hit_masks(h); /* pressure and force of the same positions: share the masks */
+    CHK(do_pressure(h, hm));
+    CHK(do_force(h, hm));
     return do_advance(h, dt);
 }
 
@@ -1695,8 +1702,9 @@ int mg_step(mg_handle* h, int nsteps, float dt, unsigned flags)
             continue;
         }
         CHK(materialize_gene_grid(h));
-        CHK(do_pressure(h));
-        CHK(do_force(h));
+        uint64_t* hm = hit_masks(h);
+        CHK(do_pressure(h, hm));
+        CHK(do_force(h, hm));
         CHK(do_advance(h, dt, flags));
         h->time += dt;
         if (flags & MG_STEP_GENES) {
