@@ -1348,10 +1348,15 @@ __global__ LAUNCH_BOUNDS_256 void k_genes(DS d, MgParams p, MgGenome gen)
     float* cp = d.CONC + (size_t)s * MG_NUM_TF;
     load16(cp, conc);
     const bool tally = (m & 4u) && s < d.ctr[MG_C_ACTIVE];
+    uint32_t was[MG_NUM_GENES];
+#pragma unroll
+    for (int g = 0; g < MG_NUM_GENES; g++) was[g] = epi[g];
     genes_one(p, gen, m, tally, epi, conc);
     if (tally) store16(cp, conc);
+    /* most states do not change in a step: store only those that did */
 #pragma unroll
-    for (int g = 0; g < MG_NUM_GENES; g++) d.EPIGEN[(size_t)g * d.N + h] = epi[g];
+    for (int g = 0; g < MG_NUM_GENES; g++)
+        if (epi[g] != was[g]) d.EPIGEN[(size_t)g * d.N + h] = epi[g];
 }
 
 /* Fused ComputeDiffusionCL + ComputeGenesCL + InsertParticlesCL of mg_step. Reads positions
