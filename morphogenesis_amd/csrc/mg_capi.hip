@@ -1566,7 +1566,7 @@ int mg_initialize_bonds(mg_handle* h)
     CHK(materialize_gene_grid(h));
     /* eight lanes per listed particle share its candidate scan (one lane when unstaged) */
     const int staged = h->B <= BOND_STAGE_MAX_B;
-    const size_t lds = staged ? (size_t)h->B * BOND_STAGE_W * sizeof(uint32_t) : 0;
+    const size_t lds = staged ? (size_t)h->B * (BOND_STAGE_W / 8) * (sizeof(float4) + sizeof(uint32_t)) : 0;
     if (staged)
         hipLaunchKernelGGL(k_initialize_bonds<8>, dim3(nblk((size_t)h->N * 8, BOND_STAGE_W)), dim3(BOND_STAGE_W), lds,
                            h->st, make_ds(h), h->p, h->g, h->gen, h->PID, h->BCOLD, 1);
@@ -1636,11 +1636,11 @@ int mg_compute_particle_changes(mg_handle* h, uint64_t seed)
     if (h->B == 0 || len[0] == 0) return MG_OK;
     const float4 elastin = make_float4(h->gen.param[0][8], h->gen.param[0][9], h->gen.param[0][10], h->gen.param[0][11]);
     const uint32_t n = std::min<uint32_t>(len[0], (uint32_t)h->N);
-    /* four lanes per listed particle share its candidate scan (one lane when unstaged) */
+    /* eight lanes per listed particle share its candidate scan (one lane when unstaged) */
     const int staged = h->B <= BOND_STAGE_MAX_B;
-    const size_t lds = staged ? (size_t)h->B * BOND_STAGE_W * sizeof(uint32_t) : 0;
+    const size_t lds = staged ? (size_t)h->B * (BOND_STAGE_W / 8) * (sizeof(float4) + sizeof(uint32_t)) : 0;
     if (staged)
-        hipLaunchKernelGGL(k_heal<4>, dim3(nblk((size_t)n * 4, BOND_STAGE_W)), dim3(BOND_STAGE_W), lds, h->st,
+        hipLaunchKernelGGL(k_heal<8>, dim3(nblk((size_t)n * 8, BOND_STAGE_W)), dim3(BOND_STAGE_W), lds, h->st,
                            make_ds(h), h->p, h->g, elastin, h->CL, h->CLEN, h->PID, h->BCOLD,
                            (unsigned long long)seed, 1);
     else

@@ -1469,32 +1469,44 @@ __device__ __forceinline__ uint32_t find_bond(const DS& d, const MgGrid& g, cons
     return to;
 }
 
-/* The searches stage the particle's partner slots in LDS once: kb[b * BOND_STAGE_W] holds slot
- * b's partner slot (MG_NONE when empty or unplaced), the thread's column of its block's LDS (4 B
- * per slot and thread, 64-thread blocks). Each angle term then needs one position load instead
- * of the partner home -> slot -> position chain; slot 0's score is dsq whatever the angle sum,
- * so its loop is skipped. */
+/* The searches stage the particle's current bonds in LDS, once per particle and shared by the S
+ * lanes of its group: kb[b] holds slot b's partner slot (MG_NONE when empty or unplaced) and
+ * vb[b] = (x - x_k, |x - x_k|) for the search point x. Each angle term then reads LDS instead of
+ * walking the partner home -> slot -> position chain; slot 0's score is dsq whatever the angle
+ * sum, so its loop is skipped. 20 B per slot and group; 64-thread blocks. */
 #define BOND_STAGE_W 64
-#define BOND_STAGE_MAX_B 192 /* 192 slots x 64 threads x 4 B = 48 KB of LDS per block */
-__device__ __forceinline__ void stage_partners(const DS& d, uint32_t h, uint32_t* kb)
+#define BOND_STAGE_MAX_B 192 /* at 8 lanes per particle: 8 groups x 192 slots x 20 B = 30 KB */
+__device__ __forceinline__ void stage_bond(const DS& d, f3 x, uint32_t k, int b, float4* vb, uint32_t* kb)
+{
+    float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (k < (uint32_t)d.N) {
+        const f3 w = x - xyz(d.P[k]);
+        v = make_float4(w.x, w.y, w.z, sqrtf(w.x * w.x + w.y * w.y + w.z * w.z));
+    }
+    kb[b] = k;
+    vb[b] = v;
+}
+/* lanes sub, sub + S, ... of the group stage slots sub, sub + S, ...; the caller syncs */
+template <int S>
+__device__ __forceinline__ void stage_bonds(const DS& d, uint32_t h, f3 x, uint32_t sub, float4* vb, uint32_t* kb)
 {
     const uint32_t N = (uint32_t)d.N;
-    for (int b = 0; b < d.B; b++) {
+    for (int b = (int)sub; b < d.B; b += S) {
         const uint32_t kh = d.BPARTNER[(size_t)b * N + h];
-        kb[b * BOND_STAGE_W] = kh == MG_NONE ? MG_NONE : d.SLOT[kh];
+        stage_bond(d, x, kh == MG_NONE ? MG_NONE : d.SLOT[kh], b, vb, kb);
     }
 }
-/* The staged search split over the S consecutive lanes of a particle's group (S a power of two,
- * all S lanes call it together): the candidates are numbered q = 0, 1, ... in the reference's
- * visit order (cells in gridAdj order, slots ascending) and lane `sub` scores those with
- * q = sub mod S, keeping its first strictly best; the group then takes the smallest score,
+/* find_bond over the staged bonds, split over the S consecutive lanes of a particle's group (S a
+ * power of two, all S lanes call it together): the candidates are numbered q = 0, 1, ... in the
+ * reference's visit order (cells in gridAdj order, slots ascending) and lane `sub` scores those
+ * with q = sub mod S, keeping its first strictly best; the group then takes the smallest score,
  * ties to the smallest q. That is the candidate the serial loop keeps (the first one reaching
  * the minimum; FLT_MAX and NaN scores never win), and each score is computed by the same
  * operations in the same order, so the result is bit-equal to find_bond. */
 template <int S>
 __device__ __forceinline__ uint32_t find_bond_split(const DS& d, const MgGrid& g, const uint32_t* __restrict__ PID,
-                                                    uint32_t gc, f3 x, int bond, float rd2, const uint32_t* kb,
-                                                    uint32_t sub)
+                                                    uint32_t gc, f3 x, int bond, float rd2, const float4* vb,
+                                                    const uint32_t* kb, uint32_t sub)
 {
     const uint32_t N = (uint32_t)d.N, C = (uint32_t)d.C;
     const int B = d.B;
@@ -1518,15 +1530,14 @@ __device__ __forceinline__ uint32_t find_bond_split(const DS& d, const MgGrid& g
                 theta = 0.0f;
                 const float lu = sqrtf(dsq);
                 for (int b = 0; b < B; b++) {
-                    const uint32_t k = kb[b * BOND_STAGE_W];
+                    const uint32_t k = kb[b];
                     if (j == k) {
                         theta = FLT_MAX;
                         break;
                     }
                     if (k < N) {
-                        const f3 v = x - xyz(d.P[k]);
-                        const float lv = sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
-                        const float cosang = (u.x * v.x + u.y * v.y + u.z * v.z) / (lu * lv);
+                        const float4 v = vb[b];
+                        const float cosang = (u.x * v.x + u.y * v.y + u.z * v.z) / (lu * v.w);
                         theta = (float)((double)theta + fabs(1.91 - (double)mg_acosf(cosang)));
                     }
                 }
@@ -1579,7 +1590,7 @@ __global__ __launch_bounds__(BOND_STAGE_W) void k_initialize_bonds(DS d, MgParam
 {
     extern __shared__ float4 mg_lds[];
     const uint32_t li = (blockIdx.x * blockDim.x + threadIdx.x) / (uint32_t)S;
-    const uint32_t sub = threadIdx.x % (uint32_t)S;
+    const uint32_t sub = threadIdx.x % (uint32_t)S, grp = threadIdx.x / (uint32_t)S;
     const uint32_t N = (uint32_t)d.N;
     const int B = d.B;
     if (li >= N || li >= d.LENS[1]) return; /* whole groups leave together */
@@ -1590,25 +1601,28 @@ __global__ __launch_bounds__(BOND_STAGE_W) void k_initialize_bonds(DS d, MgParam
     if (gc == MG_GRID_UNDEF) return;
     const uint32_t h = d.HOME[i];
     const f3 ipos = xyz(d.P[i]);
-    uint32_t* kb = (uint32_t*)mg_lds + threadIdx.x;
-    if (staged) stage_partners(d, h, kb);
+    float4* vb = mg_lds + (size_t)grp * B;
+    uint32_t* kb = (uint32_t*)(mg_lds + (size_t)(BOND_STAGE_W / S) * B) + (size_t)grp * B;
+    if (staged) {
+        stage_bonds<S>(d, h, ipos, sub, vb, kb);
+        wave_lds_sync();
+    }
     int bt4[4];
     bond_types(d, h, bt4);
     for (int bond = 0; bond < B; bond++) {
-        const uint32_t to = staged ? find_bond_split<S>(d, g, PID, gc, ipos, bond, p.rd2, kb, sub)
+        const uint32_t to = staged ? find_bond_split<S>(d, g, PID, gc, ipos, bond, p.rd2, vb, kb, sub)
                                    : find_bond(d, g, PID, gc, h, ipos, bond, p.rd2);
-        if (to < na) {
-            if (sub == 0u) {
-                const int bt = bond < 4 ? bt4[bond] : 0;
-                const size_t bi = (size_t)bond * N + h;
-                const uint32_t th = d.HOME[to];
-                d.BPARTNER[bi] = th;
-                d.BPAR[bi] = make_float4(gen.param[bt][8], gen.param[bt][9], gen.param[bt][10], gen.param[bt][11]);
-                d.BINT[bi] = make_float2(0.0f, 0.0f);
-                BCOLD[bi].x = PID[th];
-            }
-            if (staged) kb[bond * BOND_STAGE_W] = to; /* the slot's new partner, every lane's copy */
+        if (to < na && sub == 0u) {
+            const int bt = bond < 4 ? bt4[bond] : 0;
+            const size_t bi = (size_t)bond * N + h;
+            const uint32_t th = d.HOME[to];
+            d.BPARTNER[bi] = th;
+            d.BPAR[bi] = make_float4(gen.param[bt][8], gen.param[bt][9], gen.param[bt][10], gen.param[bt][11]);
+            d.BINT[bi] = make_float2(0.0f, 0.0f);
+            BCOLD[bi].x = PID[th];
+            if (staged) stage_bond(d, ipos, to, bond, vb, kb); /* the slot's new partner */
         }
+        if (staged) wave_lds_sync();
     }
 }
 
@@ -1755,9 +1769,14 @@ __global__ __launch_bounds__(BOND_STAGE_W) void k_heal(DS d, MgParams p, MgGrid 
     x.z = x.z + (max_len / (float)(4u + ((r >> 8) & 7u))) * (-1.0f * (float)(1u & (r >> 11)));
     uint32_t to;
     if (staged) {
-        uint32_t* kb = (uint32_t*)mg_lds + threadIdx.x;
-        if (bond != 0u) stage_partners(d, h, kb);
-        to = find_bond_split<S>(d, g, PID, gc, x, (int)bond, p.rd2, kb, sub);
+        const uint32_t grp = threadIdx.x / (uint32_t)S;
+        float4* vb = mg_lds + (size_t)grp * d.B;
+        uint32_t* kb = (uint32_t*)(mg_lds + (size_t)(BOND_STAGE_W / S) * d.B) + (size_t)grp * d.B;
+        if (bond != 0u) {
+            stage_bonds<S>(d, h, x, sub, vb, kb);
+            wave_lds_sync();
+        }
+        to = find_bond_split<S>(d, g, PID, gc, x, (int)bond, p.rd2, vb, kb, sub);
     } else {
         to = find_bond(d, g, PID, gc, h, x, (int)bond, p.rd2);
     }
