@@ -1564,10 +1564,16 @@ int mg_initialize_bonds(mg_handle* h)
         return MG_ERR_STATE;
     }
     CHK(materialize_gene_grid(h));
-    /* eight lanes per listed particle share its candidate scan (one lane when unstaged) */
+    /* up to 32 slots: 32 lanes per listed particle, one candidate each, angle terms kept from
+     * slot to slot; else eight lanes share its candidate scan (one lane when unstaged) */
     const int staged = h->B <= BOND_STAGE_MAX_B;
     const size_t lds = staged ? (size_t)h->B * (BOND_STAGE_W / 8) * (sizeof(float4) + sizeof(uint32_t)) : 0;
-    if (staged)
+    if (h->B <= BOND_TAB_B)
+        hipLaunchKernelGGL(k_initialize_bonds_tab, dim3(nblk((size_t)h->N * BOND_TAB_S, BOND_STAGE_W)),
+                           dim3(BOND_STAGE_W),
+                           (size_t)(BOND_STAGE_W / BOND_TAB_S) * (h->B + BOND_TAB_S) * (sizeof(float4) + sizeof(uint32_t)),
+                           h->st, make_ds(h), h->p, h->g, h->gen, h->PID, h->BCOLD);
+    else if (staged)
         hipLaunchKernelGGL(k_initialize_bonds<8>, dim3(nblk((size_t)h->N * 8, BOND_STAGE_W)), dim3(BOND_STAGE_W), lds,
                            h->st, make_ds(h), h->p, h->g, h->gen, h->PID, h->BCOLD, 1);
     else

@@ -1626,6 +1626,193 @@ __global__ __launch_bounds__(BOND_STAGE_W) void k_initialize_bonds(DS d, MgParam
     }
 }
 
+/* Bond formation with the angle terms kept from slot to slot (at most 32 bond slots). At slot
+ * s a candidate's score sums |1.91 - acos(u.v / (|u| |v|))| over the particle's bonds in slot
+ * order, and from slot s to s + 1 only slot s's partner changes. So 32 lanes serve a particle,
+ * lane q holding candidate q of the search (visit order) with a[b] = the acos of its angle to
+ * slot b's current partner (BT_EMPTY for an empty slot, BT_PARTNER when the candidate is that
+ * partner) and pre = the float sum over the slots before s, which no later slot changes. Each
+ * score is then produced by find_bond's operations in find_bond's order, and the smallest
+ * score over the lanes, ties to the smallest q, is find_bond's candidate. A particle with more
+ * than 32 candidates runs find_bond_split over its staged bonds instead. */
+#define BOND_TAB_S 32
+#define BOND_TAB_B 32
+#define BT_EMPTY (-2.0f)
+#define BT_PARTNER (-1.0f)
+
+__device__ __forceinline__ float bond_angle(float4 u, float4 v) /* w: the vector's length */
+{
+    return mg_acosf((u.x * v.x + u.y * v.y + u.z * v.z) / (u.w * v.w));
+}
+/* theta of find_bond plus one slot's term */
+__device__ __forceinline__ float bond_score_add(float theta, float a)
+{
+    return a < 0.0f ? theta : (float)((double)theta + fabs(1.91 - (double)a));
+}
+
+/* the candidates of find_bond for the 32-lane group `grp` of the wavefront, in visit order:
+ * the first 32 into cu (x - x_j and |x - x_j|) and cj (j); returns how many there are */
+__device__ __forceinline__ uint32_t gather_bond_candidates(const DS& d, const MgGrid& g,
+                                                           const uint32_t* __restrict__ PID, uint32_t gc, f3 x,
+                                                           float rd2, uint32_t sub, uint32_t grp, float4* cu,
+                                                           uint32_t* cj)
+{
+    const uint32_t N = (uint32_t)d.N, C = (uint32_t)d.C;
+    uint32_t q = 0;
+    for (int c = 0; c < g.adj_cnt; c++) {
+        const uint32_t cell = gc + (uint32_t)g.adj[c];
+        if (cell >= C) continue;
+        const uint32_t cn = d.CNT[cell];
+        if (cn == 0u) continue;
+        const uint32_t c0 = d.OFF[cell];
+        const uint32_t c1 = min(c0 + cn, N);
+        for (uint32_t base = c0; base < c1; base += BOND_TAB_S) {
+            const uint32_t j = base + sub;
+            bool ok = false;
+            f3 u = mk3(0.0f, 0.0f, 0.0f);
+            float dsq = 0.0f;
+            if (j < c1 && PID[d.HOME[j]] != 0u) {
+                u = x - xyz(d.P[j]);
+                dsq = u.x * u.x + u.y * u.y + u.z * u.z;
+                ok = dsq < rd2 && dsq > 0.0f;
+            }
+            /* lanes of the other group, or not in this loop, contribute no bits */
+            const uint32_t m = (uint32_t)(__ballot(ok) >> (grp * BOND_TAB_S));
+            if (ok) {
+                const uint32_t at = q + (uint32_t)__popc(m & ((1u << sub) - 1u));
+                if (at < BOND_TAB_S) {
+                    cu[at] = make_float4(u.x, u.y, u.z, sqrtf(dsq));
+                    cj[at] = j;
+                }
+            }
+            q += (uint32_t)__popc(m);
+        }
+    }
+    return q;
+}
+
+__device__ __forceinline__ void write_bond(const DS& d, const MgGenome& gen, const int bt4[4], int bond, uint32_t h,
+                                           uint32_t to, const uint32_t* __restrict__ PID, uint2* BCOLD)
+{
+    const int bt = bond < 4 ? bt4[bond] : 0;
+    const size_t bi = (size_t)bond * d.N + h;
+    const uint32_t th = d.HOME[to];
+    d.BPARTNER[bi] = th;
+    d.BPAR[bi] = make_float4(gen.param[bt][8], gen.param[bt][9], gen.param[bt][10], gen.param[bt][11]);
+    d.BINT[bi] = make_float2(0.0f, 0.0f);
+    BCOLD[bi].x = PID[th];
+}
+
+/* LDS per 32-lane group: B staged bonds (float4 + uint) and 32 candidates (float4 + uint) */
+__global__ __launch_bounds__(BOND_STAGE_W) void k_initialize_bonds_tab(DS d, MgParams p, MgGrid g, MgGenome gen,
+                                                                       const uint32_t* __restrict__ PID,
+                                                                       uint2* BCOLD)
+{
+    extern __shared__ float4 mg_lds[];
+    constexpr uint32_t G = BOND_STAGE_W / BOND_TAB_S;
+    const uint32_t li = (blockIdx.x * blockDim.x + threadIdx.x) / BOND_TAB_S;
+    const uint32_t sub = threadIdx.x % BOND_TAB_S, grp = threadIdx.x / BOND_TAB_S;
+    const uint32_t N = (uint32_t)d.N;
+    const int B = d.B;
+    if (li >= N || li >= d.LENS[1]) return; /* whole groups leave together */
+    const uint32_t i = d.LISTS[(size_t)1 * N + li];
+    const uint32_t na = d.ctr[MG_C_ACTIVE];
+    if (i >= na) return;
+    const uint32_t gc = d.GCELL[i];
+    if (gc == MG_GRID_UNDEF) return;
+    const uint32_t h = d.HOME[i];
+    const f3 ipos = xyz(d.P[i]);
+    float4* vb = mg_lds + (size_t)grp * B;
+    float4* cu = mg_lds + (size_t)G * B + (size_t)grp * BOND_TAB_S;
+    uint32_t* kb = (uint32_t*)(mg_lds + (size_t)G * (B + BOND_TAB_S)) + (size_t)grp * B;
+    uint32_t* cj = (uint32_t*)(mg_lds + (size_t)G * (B + BOND_TAB_S)) + (size_t)G * B + (size_t)grp * BOND_TAB_S;
+    stage_bonds<BOND_TAB_S>(d, h, ipos, sub, vb, kb);
+    const uint32_t nq = gather_bond_candidates(d, g, PID, gc, ipos, p.rd2, sub, grp, cu, cj);
+    wave_lds_sync();
+    int bt4[4];
+    bond_types(d, h, bt4);
+    if (nq > BOND_TAB_S) { /* dense neighbourhood */
+        for (int bond = 0; bond < B; bond++) {
+            const uint32_t to = find_bond_split<BOND_TAB_S>(d, g, PID, gc, ipos, bond, p.rd2, vb, kb, sub);
+            if (to < na && sub == 0u) {
+                write_bond(d, gen, bt4, bond, h, to, PID, BCOLD);
+                stage_bond(d, ipos, to, bond, vb, kb);
+            }
+            wave_lds_sync();
+        }
+        return;
+    }
+    const bool have = sub < nq;
+    float4 u = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+    uint32_t j = MG_NONE;
+    if (have) {
+        u = cu[sub];
+        j = cj[sub];
+    }
+    const float dsq = u.x * u.x + u.y * u.y + u.z * u.z; /* the candidate test's value, bit for bit */
+    float a[BOND_TAB_B];
+#pragma unroll
+    for (int b = 0; b < BOND_TAB_B; b++) {
+        float t = BT_EMPTY;
+        if (b < B) {
+            const uint32_t k = kb[b];
+            if (have && k == j)
+                t = BT_PARTNER;
+            else if (k < N)
+                t = bond_angle(u, vb[b]);
+        }
+        a[b] = t;
+    }
+    float pre = 0.0f; /* the score over slots 0 .. bond-1 */
+    bool pex = false; /* the candidate is the partner of one of those slots */
+    for (int bond = 0; bond < B; bond++) {
+        float th = dsq; /* slot 0's score whatever the angle sum */
+        if (bond != 0) {
+            float s = pre;
+            bool ex = pex;
+#pragma unroll
+            for (int b = 1; b < BOND_TAB_B; b++)
+                if (b >= bond && b < B) {
+                    ex = ex || a[b] == BT_PARTNER;
+                    s = bond_score_add(s, a[b]);
+                }
+            th = ex ? FLT_MAX : s;
+        }
+        float best = FLT_MAX;
+        uint32_t bq = 0xFFFFFFFFu, to = MG_NONE;
+        if (have && th < best) {
+            best = th;
+            bq = sub;
+            to = j;
+        }
+#pragma unroll
+        for (int o = 1; o < BOND_TAB_S; o <<= 1) {
+            const float ot = __shfl_xor(best, o);
+            const uint32_t oq = (uint32_t)__shfl_xor((int)bq, o);
+            const uint32_t oj = (uint32_t)__shfl_xor((int)to, o);
+            if (ot < best || (ot == best && oq < bq)) {
+                best = ot;
+                bq = oq;
+                to = oj;
+            }
+        }
+        if (to < na) {
+            if (sub == 0u) write_bond(d, gen, bt4, bond, h, to, PID, BCOLD);
+            /* the winner's (x - x_to, |x - x_to|) is what stage_bond computes for the new partner */
+            const float t = j == to ? BT_PARTNER : bond_angle(u, cu[bq]);
+#pragma unroll
+            for (int b = 0; b < BOND_TAB_B; b++)
+                if (b == bond) a[b] = t;
+        }
+        float t = BT_EMPTY;
+#pragma unroll
+        for (int b = 0; b < BOND_TAB_B; b++)
+            if (b == bond) t = a[b];
+        pex = pex || t == BT_PARTNER;
+        pre = bond_score_add(pre, t);
+    }
+}
+
 /* ------------------------------------------------------------- remodelling
  * Change lists (src/fluid.h:103): list l of cell c holds the (slot, bond) pairs of the cell's
  * particles whose change word (bond record word 8) has bit l set; list 0 is heal. */

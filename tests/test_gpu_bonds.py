@@ -61,3 +61,55 @@ def test_initialize_bonds_needs_the_sorted_state(built):
         g.InitializeBondsCL()
     g.Run2PhysicalSort()
     g.InitializeBondsCL()
+
+
+@pytest.mark.parametrize("seed,b", [(8, 32), (9, 40), (10, 64)])
+def test_initialize_bonds_slot_counts(built, seed, b):
+    """B = 32 takes the search that keeps each candidate's angle terms from slot to slot, B = 40
+    and 64 the eight-lane search over staged bonds; all bit-equal."""
+    sc = bond_scene(seed, b=b)
+    say("case: initialize_bonds", sc.name, "seed", seed, "B", b)
+    g, o = pair(sc)
+    for sim in (g, o):
+        sim.Run2PhysicalSort()
+        sim.step(10, sc.dt, sc.flags)
+    g.InitializeBondsCL()
+    o.InitializeBondsCL()
+    compare_all(g, o, f"B {b}: after initialize_bonds")
+    for sim in (g, o):
+        sim.InitializeBondsCL()            # again, over the bonds just formed
+    compare_all(g, o, f"B {b}: second initialize_bonds")
+    for sim in (g, o):
+        sim.step(10, sc.dt, sc.flags | (F.STEP_GRAPH if sim is g else 0))
+    compare_all(g, o, f"B {b}: 10 steps after initialize_bonds")
+
+
+def test_initialize_bonds_dense_and_sparse_neighbourhoods(built):
+    """A dense cluster (~100 candidates per search, more than the 32 a particle's lanes hold)
+    beside a sparse lattice: both searches in one launch, bit-equal to the oracle."""
+    from morphogenesis_amd.fluid_system import FluidSystem
+    from oracle.mg_oracle import Oracle
+    from tests.helpers import mini_scene
+
+    rng = np.random.default_rng(11)
+    dense = rng.uniform(4.0, 6.5, (1200, 3))
+    ax = np.arange(7.0, 10.5, 0.5)
+    sparse = np.stack(np.meshgrid(ax, ax, ax, indexing="ij"), -1).reshape(-1, 3)
+    sparse += rng.uniform(-0.05, 0.05, sparse.shape)
+    pos = np.concatenate([dense, sparse])
+    n = len(pos)
+    epi = np.zeros((F.NUM_GENES, n), np.uint32)
+    for gi, frac in ((1, 0.7), (6, 0.1), (9, 0.15)):
+        epi[gi] = np.where(rng.random(n) < frac, INT_MAX, 0).astype(np.uint32)
+    sc = mini_scene(pos, b=16, epigen=epi, genome=scenes.example_genome())
+    say("case: initialize_bonds, dense cluster and sparse lattice,", n, "particles, B 16")
+    g = FluidSystem(sc.n, sc.b)
+    g.load_scene(sc)
+    o = Oracle(sc.n, sc.b)
+    o.load_scene(sc)
+    for sim in (g, o):
+        sim.Run2PhysicalSort()
+        sim.InitializeBondsCL()
+    compare_all(g, o, "dense and sparse: after initialize_bonds")
+    rec = o.download(F.FELASTIDX)
+    assert np.sum(rec[:, :, 2] != 0) > 0.5 * n * 16 * 0.7
