@@ -1571,7 +1571,8 @@ int mg_initialize_bonds(mg_handle* h)
     if (h->B <= BOND_TAB_B)
         hipLaunchKernelGGL(k_initialize_bonds_tab, dim3(nblk((size_t)h->N * BOND_TAB_S, BOND_STAGE_W)),
                            dim3(BOND_STAGE_W),
-                           (size_t)(BOND_STAGE_W / BOND_TAB_S) * (h->B + BOND_TAB_S) * (sizeof(float4) + sizeof(uint32_t)),
+                           (size_t)(BOND_STAGE_W / BOND_TAB_S) *
+                               ((h->B + BOND_TAB_S) * (sizeof(float4) + sizeof(uint32_t)) + 2 * BOND_TAB_S * sizeof(uint32_t)),
                            h->st, make_ds(h), h->p, h->g, h->gen, h->PID, h->BCOLD);
     else if (staged)
         hipLaunchKernelGGL(k_initialize_bonds<8>, dim3(nblk((size_t)h->N * 8, BOND_STAGE_W)), dim3(BOND_STAGE_W), lds,

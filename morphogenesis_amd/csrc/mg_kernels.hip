@@ -1629,26 +1629,24 @@ __global__ __launch_bounds__(BOND_STAGE_W) void k_initialize_bonds(DS d, MgParam
 /* Bond formation with the angle terms kept from slot to slot (at most 32 bond slots). At slot
  * s a candidate's score sums |1.91 - acos(u.v / (|u| |v|))| over the particle's bonds in slot
  * order, and from slot s to s + 1 only slot s's partner changes. So 32 lanes serve a particle,
- * lane q holding candidate q of the search (visit order) with a[b] = the acos of its angle to
- * slot b's current partner (BT_EMPTY for an empty slot, BT_PARTNER when the candidate is that
- * partner) and pre = the float sum over the slots before s, which no later slot changes. Each
- * score is then produced by find_bond's operations in find_bond's order, and the smallest
- * score over the lanes, ties to the smallest q, is find_bond's candidate. A particle with more
- * than 32 candidates runs find_bond_split over its staged bonds instead. */
+ * lane q holding candidate q of the search (visit order) with e[b] = the double term
+ * |1.91 - acos| of its angle to slot b's current partner, and pre = the float sum over the
+ * slots before s, which no later slot changes. Each score step is find_bond's
+ * (float)((double)theta + term). An empty slot holds +0, which leaves theta's bits as they are
+ * (find_bond skips it); a slot whose partner is the candidate holds +inf, which makes the score
+ * +inf or NaN, so, like find_bond's FLT_MAX, it never wins. The smallest score over the lanes,
+ * ties to the smallest q, is then find_bond's candidate. A particle with more than 32
+ * candidates runs find_bond_split over its staged bonds instead. */
 #define BOND_TAB_S 32
 #define BOND_TAB_B 32
-#define BT_EMPTY (-2.0f)
-#define BT_PARTNER (-1.0f)
+#define BT_PARTNER_TERM __builtin_inf()
+typedef double d16 __attribute__((ext_vector_type(16)));
 
 __device__ __forceinline__ float bond_angle(float4 u, float4 v) /* w: the vector's length */
 {
     return mg_acosf((u.x * v.x + u.y * v.y + u.z * v.z) / (u.w * v.w));
 }
-/* theta of find_bond plus one slot's term */
-__device__ __forceinline__ float bond_score_add(float theta, float a)
-{
-    return a < 0.0f ? theta : (float)((double)theta + fabs(1.91 - (double)a));
-}
+__device__ __forceinline__ double bond_term(float angle) { return fabs(1.91 - (double)angle); }
 
 /* the candidates of find_bond for the 32-lane group `grp` of the wavefront, in visit order:
  * the first 32 into cu (x - x_j and |x - x_j|) and cj (j); returns how many there are */
@@ -1691,6 +1689,95 @@ __device__ __forceinline__ uint32_t gather_bond_candidates(const DS& d, const Mg
     return q;
 }
 
+/* The same candidates with the 27 cells' counts and offsets loaded at once (lane c: cell c,
+ * at most 32 cells) and their slots taken as one concatenated range, 32 per round: lane `sub`
+ * of a round finds its cell by binary search over the cells' exclusive starts in LDS (cs: start,
+ * cb: first slot). The concatenation keeps the visit order. */
+__device__ __forceinline__ uint32_t gather_bond_candidates_flat(const DS& d, const MgGrid& g,
+                                                                const uint32_t* __restrict__ PID, uint32_t gc, f3 x,
+                                                                float rd2, uint32_t sub, uint32_t grp, float4* cu,
+                                                                uint32_t* cj, uint32_t* cs, uint32_t* cb)
+{
+    const uint32_t N = (uint32_t)d.N, C = (uint32_t)d.C;
+    const uint32_t nc = (uint32_t)g.adj_cnt; /* <= 32 */
+    uint32_t len = 0, c0 = 0;
+    if (sub < nc) {
+        const uint32_t cell = gc + (uint32_t)g.adj[sub];
+        if (cell < C) {
+            const uint32_t cn = d.CNT[cell];
+            if (cn != 0u) {
+                c0 = d.OFF[cell];
+                const uint32_t c1 = min(c0 + cn, N);
+                len = c1 > c0 ? c1 - c0 : 0u;
+            }
+        }
+    }
+    uint32_t incl = len;
+#pragma unroll
+    for (int o = 1; o < BOND_TAB_S; o <<= 1) {
+        const uint32_t v = (uint32_t)__shfl_up((int)incl, o, BOND_TAB_S);
+        if (sub >= (uint32_t)o) incl += v;
+    }
+    const uint32_t total = (uint32_t)__shfl((int)incl, BOND_TAB_S - 1, BOND_TAB_S);
+    cs[sub] = incl - len;
+    cb[sub] = c0;
+    wave_lds_sync();
+    uint32_t q = 0;
+    for (uint32_t base = 0; base < total; base += BOND_TAB_S) {
+        const uint32_t gi = base + sub;
+        bool ok = false;
+        f3 u = mk3(0.0f, 0.0f, 0.0f);
+        float dsq = 0.0f;
+        uint32_t j = 0;
+        if (gi < total) {
+            /* the last cell whose start is <= gi: a cell of length > 0 */
+            uint32_t c = 0;
+#pragma unroll
+            for (uint32_t step = BOND_TAB_S / 2; step > 0; step >>= 1)
+                if (c + step < nc && cs[c + step] <= gi) c += step;
+            j = cb[c] + (gi - cs[c]);
+            if (PID[d.HOME[j]] != 0u) {
+                u = x - xyz(d.P[j]);
+                dsq = u.x * u.x + u.y * u.y + u.z * u.z;
+                ok = dsq < rd2 && dsq > 0.0f;
+            }
+        }
+        const uint32_t m = (uint32_t)(__ballot(ok) >> (grp * BOND_TAB_S));
+        if (ok) {
+            const uint32_t at = q + (uint32_t)__popc(m & ((1u << sub) - 1u));
+            if (at < BOND_TAB_S) {
+                cu[at] = make_float4(u.x, u.y, u.z, sqrtf(dsq));
+                cj[at] = j;
+            }
+        }
+        q += (uint32_t)__popc(m);
+    }
+    return q;
+}
+
+/* minimum of a 64-bit key over the 32 lanes of a group, in every lane of it: DPP exchanges
+ * inside 16-lane rows (lane ^ 1, lane ^ 2 in quads; the mirrored lane in the half row, then in
+ * the row), one swap of the two rows */
+template <int CTRL>
+__device__ __forceinline__ uint64_t key_min_dpp(uint64_t k)
+{
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(k >> 32), CTRL, 0xF, 0xF, false);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)k, CTRL, 0xF, 0xF, false);
+    const uint64_t o = ((uint64_t)hi << 32) | lo;
+    return o < k ? o : k;
+}
+__device__ __forceinline__ uint64_t group_key_min(uint64_t k)
+{
+    k = key_min_dpp<0xB1>(k);  /* quad_perm [1,0,3,2] */
+    k = key_min_dpp<0x4E>(k);  /* quad_perm [2,3,0,1] */
+    k = key_min_dpp<0x141>(k); /* row_half_mirror */
+    k = key_min_dpp<0x140>(k); /* row_mirror */
+    const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(k >> 32), 16, BOND_TAB_S);
+    const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)k, 16, BOND_TAB_S);
+    const uint64_t o = ((uint64_t)hi << 32) | lo;
+    return o < k ? o : k;
+}
+
 __device__ __forceinline__ void write_bond(const DS& d, const MgGenome& gen, const int bt4[4], int bond, uint32_t h,
                                            uint32_t to, const uint32_t* __restrict__ PID, uint2* BCOLD)
 {
@@ -1703,7 +1790,8 @@ __device__ __forceinline__ void write_bond(const DS& d, const MgGenome& gen, con
     BCOLD[bi].x = PID[th];
 }
 
-/* LDS per 32-lane group: B staged bonds (float4 + uint) and 32 candidates (float4 + uint) */
+/* LDS per 32-lane group: B staged bonds (float4 + uint), 32 candidates (float4 + uint) and
+ * the 32 cells' starts and first slots (uint, uint) */
 __global__ __launch_bounds__(BOND_STAGE_W) void k_initialize_bonds_tab(DS d, MgParams p, MgGrid g, MgGenome gen,
                                                                        const uint32_t* __restrict__ PID,
                                                                        uint2* BCOLD)
@@ -1726,8 +1814,12 @@ __global__ __launch_bounds__(BOND_STAGE_W) void k_initialize_bonds_tab(DS d, MgP
     float4* cu = mg_lds + (size_t)G * B + (size_t)grp * BOND_TAB_S;
     uint32_t* kb = (uint32_t*)(mg_lds + (size_t)G * (B + BOND_TAB_S)) + (size_t)grp * B;
     uint32_t* cj = (uint32_t*)(mg_lds + (size_t)G * (B + BOND_TAB_S)) + (size_t)G * B + (size_t)grp * BOND_TAB_S;
+    uint32_t* cs = cj + (size_t)G * BOND_TAB_S;
+    uint32_t* cb = cs + (size_t)G * BOND_TAB_S;
     stage_bonds<BOND_TAB_S>(d, h, ipos, sub, vb, kb);
-    const uint32_t nq = gather_bond_candidates(d, g, PID, gc, ipos, p.rd2, sub, grp, cu, cj);
+    const uint32_t nq = g.adj_cnt <= BOND_TAB_S
+                            ? gather_bond_candidates_flat(d, g, PID, gc, ipos, p.rd2, sub, grp, cu, cj, cs, cb)
+                            : gather_bond_candidates(d, g, PID, gc, ipos, p.rd2, sub, grp, cu, cj);
     wave_lds_sync();
     int bt4[4];
     bond_types(d, h, bt4);
@@ -1750,67 +1842,54 @@ __global__ __launch_bounds__(BOND_STAGE_W) void k_initialize_bonds_tab(DS d, MgP
         j = cj[sub];
     }
     const float dsq = u.x * u.x + u.y * u.y + u.z * u.z; /* the candidate test's value, bit for bit */
-    float a[BOND_TAB_B];
+    /* e[b] in two vectors of 16 indexed by a wave-uniform slot number (register indexing) */
+    d16 e0, e1;
 #pragma unroll
     for (int b = 0; b < BOND_TAB_B; b++) {
-        float t = BT_EMPTY;
+        double t = 0.0;
         if (b < B) {
             const uint32_t k = kb[b];
             if (have && k == j)
-                t = BT_PARTNER;
+                t = BT_PARTNER_TERM;
             else if (k < N)
-                t = bond_angle(u, vb[b]);
+                t = bond_term(bond_angle(u, vb[b]));
         }
-        a[b] = t;
+        if (b < 16)
+            e0[b] = t;
+        else
+            e1[b - 16] = t;
     }
-    float pre = 0.0f; /* the score over slots 0 .. bond-1 */
-    bool pex = false; /* the candidate is the partner of one of those slots */
+    float pre = 0.0f;         /* the score over slots 0 .. bond-1 */
+    uint32_t win = MG_NONE;   /* lane b: slot b's new partner */
+    const int B0 = min(B, 16);
     for (int bond = 0; bond < B; bond++) {
         float th = dsq; /* slot 0's score whatever the angle sum */
         if (bond != 0) {
-            float s = pre;
-            bool ex = pex;
-#pragma unroll
-            for (int b = 1; b < BOND_TAB_B; b++)
-                if (b >= bond && b < B) {
-                    ex = ex || a[b] == BT_PARTNER;
-                    s = bond_score_add(s, a[b]);
-                }
-            th = ex ? FLT_MAX : s;
+            th = pre;
+            for (int b = bond; b < B0; b++) th = (float)((double)th + e0[b]);
+            for (int b = max(bond, 16); b < B; b++) th = (float)((double)th + e1[b - 16]);
         }
-        float best = FLT_MAX;
-        uint32_t bq = 0xFFFFFFFFu, to = MG_NONE;
-        if (have && th < best) {
-            best = th;
-            bq = sub;
-            to = j;
-        }
-#pragma unroll
-        for (int o = 1; o < BOND_TAB_S; o <<= 1) {
-            const float ot = __shfl_xor(best, o);
-            const uint32_t oq = (uint32_t)__shfl_xor((int)bq, o);
-            const uint32_t oj = (uint32_t)__shfl_xor((int)to, o);
-            if (ot < best || (ot == best && oq < bq)) {
-                best = ot;
-                bq = oq;
-                to = oj;
+        /* (score, q) as one key: scores that can win are finite and >= 0, whose bits order
+         * as the values do; the smallest key is the smallest score, ties to the smallest q */
+        uint64_t key = ~0ull;
+        if (have && th < FLT_MAX) key = ((uint64_t)__float_as_uint(th) << 32) | sub;
+        key = group_key_min(key);
+        if (key != ~0ull) {
+            const uint32_t bq = (uint32_t)key, to = cj[bq];
+            if (to < na) {
+                if (sub == (uint32_t)bond) win = to;
+                /* the winner's (x - x_to, |x - x_to|) is what stage_bond computes for the new partner */
+                const double t = j == to ? BT_PARTNER_TERM : bond_term(bond_angle(u, cu[bq]));
+                if (bond < 16)
+                    e0[bond] = t;
+                else
+                    e1[bond - 16] = t;
             }
         }
-        if (to < na) {
-            if (sub == 0u) write_bond(d, gen, bt4, bond, h, to, PID, BCOLD);
-            /* the winner's (x - x_to, |x - x_to|) is what stage_bond computes for the new partner */
-            const float t = j == to ? BT_PARTNER : bond_angle(u, cu[bq]);
-#pragma unroll
-            for (int b = 0; b < BOND_TAB_B; b++)
-                if (b == bond) a[b] = t;
-        }
-        float t = BT_EMPTY;
-#pragma unroll
-        for (int b = 0; b < BOND_TAB_B; b++)
-            if (b == bond) t = a[b];
-        pex = pex || t == BT_PARTNER;
-        pre = bond_score_add(pre, t);
+        pre = (float)((double)pre + (bond < 16 ? e0[bond] : e1[bond - 16]));
     }
+    /* each slot is written once, after the searches (none of them reads the records) */
+    if ((int)sub < B && win < na) write_bond(d, gen, bt4, (int)sub, h, win, PID, BCOLD);
 }
 
 /* ------------------------------------------------------------- remodelling
