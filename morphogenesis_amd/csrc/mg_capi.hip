@@ -1643,13 +1643,14 @@ int mg_compute_particle_changes(mg_handle* h, uint64_t seed)
     if (h->B == 0 || len[0] == 0) return MG_OK;
     const float4 elastin = make_float4(h->gen.param[0][8], h->gen.param[0][9], h->gen.param[0][10], h->gen.param[0][11]);
     const uint32_t n = std::min<uint32_t>(len[0], (uint32_t)h->N);
-    /* eight lanes per listed particle share its candidate scan (one lane when unstaged) */
+    /* 32 lanes per listed particle, one candidate each (one lane when the bonds are not staged) */
     const int staged = h->B <= BOND_STAGE_MAX_B;
-    const size_t lds = staged ? (size_t)h->B * (BOND_STAGE_W / 8) * (sizeof(float4) + sizeof(uint32_t)) : 0;
+    const size_t lds = (size_t)(BOND_STAGE_W / BOND_TAB_S) *
+                       ((h->B + BOND_TAB_S) * (sizeof(float4) + sizeof(uint32_t)) + 2 * BOND_TAB_S * sizeof(uint32_t));
     if (staged)
-        hipLaunchKernelGGL(k_heal<8>, dim3(nblk((size_t)n * 8, BOND_STAGE_W)), dim3(BOND_STAGE_W), lds, h->st,
-                           make_ds(h), h->p, h->g, elastin, h->CL, h->CLEN, h->PID, h->BCOLD,
-                           (unsigned long long)seed, 1);
+        hipLaunchKernelGGL(k_heal_flat, dim3(nblk((size_t)n * BOND_TAB_S, BOND_STAGE_W)), dim3(BOND_STAGE_W), lds,
+                           h->st, make_ds(h), h->p, h->g, elastin, h->CL, h->CLEN, h->PID, h->BCOLD,
+                           (unsigned long long)seed);
     else
         hipLaunchKernelGGL(k_heal<1>, dim3(nblk(n, BOND_STAGE_W)), dim3(BOND_STAGE_W), 0, h->st, make_ds(h), h->p,
                            h->g, elastin, h->CL, h->CLEN, h->PID, h->BCOLD, (unsigned long long)seed, 0);

@@ -2056,6 +2056,83 @@ __global__ __launch_bounds__(BOND_STAGE_W) void k_heal(DS d, MgParams p, MgGrid 
     }
 }
 
+/* heal as k_heal with 32 lanes per listed particle: the candidates gathered in one pass
+ * (gather_bond_candidates_flat), lane q scoring candidate q over the bonds staged for the
+ * shifted point in slot order with find_bond's operations, and the smallest (score, q) key of
+ * the group taking the slot. More than 32 candidates (or more than 32 search cells): the
+ * staged search of find_bond_split instead. LDS as k_initialize_bonds_tab. */
+__global__ __launch_bounds__(BOND_STAGE_W) void k_heal_flat(DS d, MgParams p, MgGrid g, float4 elastin,
+                                                            const uint32_t* __restrict__ CL,
+                                                            const uint32_t* __restrict__ CLEN,
+                                                            const uint32_t* __restrict__ PID, uint2* BCOLD,
+                                                            unsigned long long seed)
+{
+    extern __shared__ float4 mg_lds[];
+    constexpr uint32_t G = BOND_STAGE_W / BOND_TAB_S;
+    const uint32_t k = (blockIdx.x * blockDim.x + threadIdx.x) / BOND_TAB_S;
+    const uint32_t sub = threadIdx.x % BOND_TAB_S, grp = threadIdx.x / BOND_TAB_S;
+    const uint32_t N = (uint32_t)d.N;
+    const int B = d.B;
+    if (k >= N || k >= CLEN[0]) return; /* whole groups leave together */
+    const uint32_t i = CL[k];
+    if (i >= d.ctr[MG_C_ACTIVE]) return;
+    const uint32_t bond = CL[N + k];
+    if (bond >= (uint32_t)B) return;
+    const uint32_t gc = d.GCELL[i];
+    if (gc == MG_GRID_UNDEF) return;
+    const uint32_t h = d.HOME[i];
+    const uint32_t r = mg_heal_word(seed, PID[h]);
+    const float max_len = sqrtf(p.rd2);
+    f3 x = xyz(d.P[i]);
+    x.x = x.x + (max_len / (float)(4u + (r & 7u))) * (-1.0f * (float)(1u & (r >> 3)));
+    x.y = x.y + (max_len / (float)(4u + ((r >> 4) & 7u))) * (-1.0f * (float)(1u & (r >> 7)));
+    x.z = x.z + (max_len / (float)(4u + ((r >> 8) & 7u))) * (-1.0f * (float)(1u & (r >> 11)));
+    float4* vb = mg_lds + (size_t)grp * B;
+    float4* cu = mg_lds + (size_t)G * B + (size_t)grp * BOND_TAB_S;
+    uint32_t* kb = (uint32_t*)(mg_lds + (size_t)G * (B + BOND_TAB_S)) + (size_t)grp * B;
+    uint32_t* cj = (uint32_t*)(mg_lds + (size_t)G * (B + BOND_TAB_S)) + (size_t)G * B + (size_t)grp * BOND_TAB_S;
+    uint32_t* cs = cj + (size_t)G * BOND_TAB_S;
+    uint32_t* cb = cs + (size_t)G * BOND_TAB_S;
+    if (bond != 0u) stage_bonds<BOND_TAB_S>(d, h, x, sub, vb, kb); /* slot 0 scores by distance */
+    const uint32_t nq = g.adj_cnt <= BOND_TAB_S
+                            ? gather_bond_candidates_flat(d, g, PID, gc, x, p.rd2, sub, grp, cu, cj, cs, cb)
+                            : gather_bond_candidates(d, g, PID, gc, x, p.rd2, sub, grp, cu, cj);
+    wave_lds_sync();
+    uint32_t to = MG_NONE;
+    if (nq > BOND_TAB_S) {
+        to = find_bond_split<BOND_TAB_S>(d, g, PID, gc, x, (int)bond, p.rd2, vb, kb, sub);
+    } else {
+        uint64_t key = ~0ull;
+        if (sub < nq) {
+            const float4 u = cu[sub];
+            const uint32_t j = cj[sub];
+            float th = u.x * u.x + u.y * u.y + u.z * u.z; /* slot 0: the candidate test's dsq */
+            if (bond != 0u) {
+                th = 0.0f;
+                for (int b = 0; b < B; b++) {
+                    const uint32_t kk = kb[b];
+                    if (j == kk) {
+                        th = FLT_MAX;
+                        break;
+                    }
+                    if (kk < N) th = (float)((double)th + bond_term(bond_angle(u, vb[b])));
+                }
+            }
+            if (th < FLT_MAX) key = ((uint64_t)__float_as_uint(th) << 32) | sub;
+        }
+        key = group_key_min(key);
+        if (key != ~0ull) to = cj[(uint32_t)key];
+    }
+    if (to < N && sub == 0u) {
+        const size_t bi = (size_t)bond * N + h;
+        const uint32_t th = d.HOME[to];
+        d.BPARTNER[bi] = th;
+        d.BPAR[bi] = elastin;
+        d.BINT[bi] = make_float2(0.0f, 0.0f);
+        BCOLD[bi] = make_uint2(PID[th], 0u);
+    }
+}
+
 /* ZeroVelCL (src/host_CL.cpp:2175-2180) over the whole FVEL / FVEVAL buffers. */
 __global__ LAUNCH_BOUNDS_256 void k_zero_vel(DS d)
 {

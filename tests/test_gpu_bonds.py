@@ -84,11 +84,8 @@ def test_initialize_bonds_slot_counts(built, seed, b):
     compare_all(g, o, f"B {b}: 10 steps after initialize_bonds")
 
 
-def test_initialize_bonds_dense_and_sparse_neighbourhoods(built):
-    """A dense cluster (~100 candidates per search, more than the 32 a particle's lanes hold)
-    beside a sparse lattice: both searches in one launch, bit-equal to the oracle."""
-    from morphogenesis_amd.fluid_system import FluidSystem
-    from oracle.mg_oracle import Oracle
+def dense_sparse_scene(b=16, living=0.0):
+    """A dense cluster (~100 candidates per bond search) beside a sparse jittered lattice."""
     from tests.helpers import mini_scene
 
     rng = np.random.default_rng(11)
@@ -99,9 +96,19 @@ def test_initialize_bonds_dense_and_sparse_neighbourhoods(built):
     pos = np.concatenate([dense, sparse])
     n = len(pos)
     epi = np.zeros((F.NUM_GENES, n), np.uint32)
-    for gi, frac in ((1, 0.7), (6, 0.1), (9, 0.15)):
+    for gi, frac in ((1, 0.7), (2, living), (6, 0.1), (9, 0.15)):
         epi[gi] = np.where(rng.random(n) < frac, INT_MAX, 0).astype(np.uint32)
-    sc = mini_scene(pos, b=16, epigen=epi, genome=scenes.example_genome())
+    return mini_scene(pos, b=b, epigen=epi, genome=scenes.example_genome())
+
+
+def test_initialize_bonds_dense_and_sparse_neighbourhoods(built):
+    """A dense cluster (~100 candidates per search, more than the 32 a particle's lanes hold)
+    beside a sparse lattice: both searches in one launch, bit-equal to the oracle."""
+    from morphogenesis_amd.fluid_system import FluidSystem
+    from oracle.mg_oracle import Oracle
+
+    sc = dense_sparse_scene()
+    n = sc.n
     say("case: initialize_bonds, dense cluster and sparse lattice,", n, "particles, B 16")
     g = FluidSystem(sc.n, sc.b)
     g.load_scene(sc)

@@ -121,6 +121,35 @@ def test_remodelling_demo_scene_from_reference_files(built):
     assert healed >= 12 * 18
 
 
+def test_heal_dense_and_sparse_neighbourhoods(built):
+    """Heal where a dense cluster gives ~100 candidates per search (the staged fallback) and a
+    sparse lattice gives few (one candidate per lane): bonds formed, one slot in three of the
+    living particles broken, Run2Remodelling; bit-equal to the oracle."""
+    from morphogenesis_amd.fluid_system import FluidSystem
+    from oracle.mg_oracle import Oracle
+    from tests.test_gpu_bonds import dense_sparse_scene
+
+    sc = dense_sparse_scene(b=12, living=0.8)
+    say("case: heal, dense cluster and sparse lattice,", sc.n, "particles, B 12")
+    g = FluidSystem(sc.n, sc.b)
+    g.load_scene(sc)
+    o = Oracle(sc.n, sc.b)
+    o.load_scene(sc)
+    for sim in (g, o):
+        sim.Run2PhysicalSort()
+        sim.InitializeBondsCL()
+    compare_all(g, o, "dense and sparse: bonds formed", ALL)
+    rec = o.download(F.FELASTIDX)
+    rng = np.random.default_rng(3)
+    broken = rng.random(sc.n) < 0.33
+    rec[broken, rng.integers(0, sc.b, sc.n)[broken], 2] = 0
+    for sim in (g, o):
+        sim.upload(F.FELASTIDX, rec)
+        sim.Run2Remodelling(2, seed=5)
+    compare_all(g, o, "dense and sparse: remodelled", ALL)
+    assert o.download(F.FDENSE_LIST_LENGTHS_CHANGES)[0] > 300
+
+
 @pytest.mark.parametrize("spokes", [200, 300])
 def test_hub_particle_incoming_lists(built, spokes):
     """One particle that every other one bonds to: 200 incoming bonds take the device build's
