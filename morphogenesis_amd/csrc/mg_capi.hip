@@ -262,7 +262,8 @@ struct mg_handle {
     uint32_t* FBIN = nullptr;
     uint32_t *INC_OFF = nullptr, *INC = nullptr;
     size_t inc_cap = 0;
-    uint32_t* incw = nullptr; /* device build of INC: counts / cursors [N], tile sums, maximum */
+    uint32_t* incw = nullptr;   /* device build of INC: counts [N], tile sums, maximum */
+    uint32_t* incpos = nullptr; /* device build of INC: each bond's place in its partner's run [B][N] */
     uint64_t* LIVE = nullptr; /* per home: bit a set when bond slot a holds a partner */
     uint64_t* HM = nullptr;   /* hit masks, pressure -> force of a fused step: [row][slot] */
     int hm_rows = 0;          /* rows HM holds per slot */
@@ -488,10 +489,11 @@ int rebuild_inc(mg_handle* h)
     if (B == 0) return rebuild_inc_host(h);
     const unsigned tiles = (unsigned)((N + SCAN_TILE - 1) / SCAN_TILE);
     if (!h->incw) CHK(dalloc(h, &h->incw, (size_t)N + 2 * tiles + 2));
+    if (!h->incpos) CHK(dalloc(h, &h->incpos, (size_t)N * B));
     uint32_t *cnt = h->incw, *sums = cnt + N, *excl = sums + tiles, *stat = excl + tiles;
     CHK(fill_u32(h, cnt, 0, (size_t)N));
     CHK(fill_u32(h, stat, 0, 1));
-    hipLaunchKernelGGL(k_inc_count, dim3(nblk(N)), dim3(256), 0, h->st, h->BPARTNER, cnt, h->LIVE, N, B);
+    hipLaunchKernelGGL(k_inc_count, dim3(nblk(N)), dim3(256), 0, h->st, h->BPARTNER, cnt, h->incpos, h->LIVE, N, B);
     hipLaunchKernelGGL(k_tile_sums, dim3(tiles), dim3(256), 0, h->st, cnt, N, sums, stat);
     hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, h->st, sums, excl, (int)tiles, h->ctr, 0, 0, h->INC_OFF + N);
     hipLaunchKernelGGL(k_tile_scan, dim3(tiles), dim3(256), 0, h->st, cnt, h->INC_OFF, N, excl);
@@ -509,9 +511,9 @@ int rebuild_inc(mg_handle* h)
     }
     h->epoch++;
     if (total == 0) return MG_OK;
-    CHK(fill_u32(h, cnt, 0, (size_t)N));
-    hipLaunchKernelGGL(k_inc_fill, dim3(nblk(N)), dim3(256), 0, h->st, h->BPARTNER, h->INC_OFF, cnt, h->INC, N, B);
-    const unsigned blocks = (unsigned)std::min<size_t>(((size_t)N + 3) / 4, 8192);
+    hipLaunchKernelGGL(k_inc_fill, dim3(nblk(N)), dim3(256), 0, h->st, h->BPARTNER, h->INC_OFF, h->incpos, h->INC, N,
+                       B);
+    const unsigned blocks = (unsigned)(((size_t)N + 4 * INC_PER_WAVE - 1) / (4 * INC_PER_WAVE));
     hipLaunchKernelGGL(k_inc_order, dim3(blocks), dim3(256), 0, h->st, h->INC_OFF, h->INC, h->PID, N, B);
     HIPCHK(hipGetLastError());
     return MG_OK;

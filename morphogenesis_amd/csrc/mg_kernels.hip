@@ -310,20 +310,23 @@ __global__ __launch_bounds__(256) void k_tile_scan(const uint32_t* __restrict__ 
  * INC_OFF[j] .. INC_OFF[j + 1] lists the bonds whose partner is home j as owner_home * B + slot,
  * ordered by (owner's particle ID, owner home, slot), so the force sums a particle's incoming
  * terms in an order that is a property of the state, not of the launch. k_inc_count counts
- * per partner and records each owner's live-slot mask; after the scan k_inc_fill scatters the
- * entries (any order within a partner) and k_inc_order puts each partner's run in key order:
- * one wavefront per partner, each entry's position is the number of smaller keys (keys are
- * distinct), runs of up to 256 entries (longer runs take the host build). */
+ * per partner, keeps each bond's place among its partner's (the count's atomic, any order) and
+ * records each owner's live-slot mask; after the scan k_inc_fill scatters the entries to those
+ * places and k_inc_order puts each partner's run in key order:
+ * one wavefront per four partners (their runs are adjacent), each entry's position is the
+ * number of smaller keys in its run (keys are distinct), runs of up to 256 entries (longer
+ * runs take the host build). */
 __global__ LAUNCH_BOUNDS_256 void k_inc_count(const uint32_t* __restrict__ BPARTNER, uint32_t* __restrict__ cnt,
-                                              uint64_t* __restrict__ LIVE, int N, int B)
+                                              uint32_t* __restrict__ POS, uint64_t* __restrict__ LIVE, int N, int B)
 {
     const uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
     if (h >= (uint32_t)N) return;
     uint64_t live = 0;
     for (int a = 0; a < B; a++) {
-        const uint32_t j = BPARTNER[(size_t)a * N + h];
+        const size_t bi = (size_t)a * N + h;
+        const uint32_t j = BPARTNER[bi];
         if (j < (uint32_t)N) {
-            atomicAdd(&cnt[j], 1u);
+            POS[bi] = atomicAdd(&cnt[j], 1u); /* the bond's place in its partner's run */
             if (a < 64) live |= 1ull << a;
         }
     }
@@ -331,41 +334,48 @@ __global__ LAUNCH_BOUNDS_256 void k_inc_count(const uint32_t* __restrict__ BPART
 }
 
 __global__ LAUNCH_BOUNDS_256 void k_inc_fill(const uint32_t* __restrict__ BPARTNER, const uint32_t* __restrict__ OFF,
-                                             uint32_t* __restrict__ cursor, uint32_t* __restrict__ ENT, int N, int B)
+                                             const uint32_t* __restrict__ POS, uint32_t* __restrict__ ENT, int N, int B)
 {
     const uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
     if (h >= (uint32_t)N) return;
     for (int a = 0; a < B; a++) {
-        const uint32_t j = BPARTNER[(size_t)a * N + h];
-        if (j < (uint32_t)N) ENT[OFF[j] + atomicAdd(&cursor[j], 1u)] = h * (uint32_t)B + (uint32_t)a;
+        const size_t bi = (size_t)a * N + h;
+        const uint32_t j = BPARTNER[bi];
+        if (j < (uint32_t)N) ENT[OFF[j] + POS[bi]] = h * (uint32_t)B + (uint32_t)a;
     }
 }
 
 #define INC_RUN_MAX 256
+#define INC_PER_WAVE 4 /* partners per wavefront: their runs are one contiguous range of ENT */
 __global__ __launch_bounds__(256) void k_inc_order(const uint32_t* __restrict__ OFF, uint32_t* __restrict__ ENT,
                                                    const uint32_t* __restrict__ PID, int N, int B)
 {
-    __shared__ unsigned long long keys[4][INC_RUN_MAX];
+    __shared__ unsigned long long keys[4][INC_PER_WAVE * INC_RUN_MAX];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    for (int base = (int)blockIdx.x * 4; base < N; base += (int)gridDim.x * 4) {
-        const int j = base + w;
-        uint32_t o = 0, len = 0;
-        if (j < N) {
-            o = OFF[j];
-            len = min(OFF[j + 1] - o, (uint32_t)INC_RUN_MAX);
-        }
-        for (uint32_t k = (uint32_t)lane; k < len; k += 64) {
-            const uint32_t e = ENT[o + k];
-            keys[w][k] = ((unsigned long long)PID[e / (uint32_t)B] << 32) | e;
-        }
-        __syncthreads();
-        for (uint32_t k = (uint32_t)lane; k < len; k += 64) {
-            const unsigned long long key = keys[w][k];
-            uint32_t rank = 0;
-            for (uint32_t q = 0; q < len; q++) rank += keys[w][q] < key ? 1u : 0u;
-            ENT[o + rank] = (uint32_t)key;
-        }
-        __syncthreads();
+    const uint32_t j0 = ((uint32_t)blockIdx.x * 4u + (uint32_t)w) * INC_PER_WAVE;
+    if (j0 >= (uint32_t)N) return;
+    uint32_t st[INC_PER_WAVE + 1]; /* the runs' bounds relative to the first run's start */
+    const uint32_t o = OFF[j0];
+#pragma unroll
+    for (int k = 0; k <= INC_PER_WAVE; k++) st[k] = OFF[min(j0 + (uint32_t)k, (uint32_t)N)] - o;
+    const uint32_t tot = st[INC_PER_WAVE];
+    for (uint32_t t = (uint32_t)lane; t < tot; t += 64) {
+        const uint32_t e = ENT[o + t];
+        keys[w][t] = ((unsigned long long)PID[e / (uint32_t)B] << 32) | e;
+    }
+    wave_lds_sync();
+    for (uint32_t t = (uint32_t)lane; t < tot; t += 64) {
+        uint32_t rs = 0, re = st[1];
+#pragma unroll
+        for (int k = 1; k < INC_PER_WAVE; k++)
+            if (t >= st[k]) {
+                rs = st[k];
+                re = st[k + 1];
+            }
+        const unsigned long long key = keys[w][t];
+        uint32_t rank = 0;
+        for (uint32_t q = rs; q < re; q++) rank += keys[w][q] < key ? 1u : 0u;
+        ENT[o + rs + rank] = (uint32_t)key;
     }
 }
 
