@@ -899,7 +899,8 @@ __global__ LAUNCH_BOUNDS_SWEEP void k_pressure(DS d, MgParams p, MgGrid g)
  *   FFORCE[i] = ((F0 + own) + in_1) + in_2 ...
  * Updates the bond integrators and breaks bonds as the reference does. The live bond slots
  * (LIVE, kept with the bond records) are fetched 4 at a time in slot order; incoming bonds
- * are fetched 4 at a time. */
+ * are fetched 4 at a time; in both loops the next group's records are fetched while the
+ * current group's partner positions load. */
 __device__ __forceinline__ f3 force_one(const DS& d, const MgParams& p, const MgGrid& g, Band<1>& wb,
                                         bool valid, uint32_t i, uint32_t na, uint32_t gc, f3 F0)
 {
@@ -919,15 +920,20 @@ __device__ __forceinline__ f3 force_one(const DS& d, const MgParams& p, const Mg
     const bool freeze = p.freeze != 0;
 
     f3 force = mk3(0.0f, 0.0f, 0.0f);
-    /* live slots in groups of 4, ascending */
+    /* live slots in groups of 4, ascending; a group's partner slots and records are fetched
+     * while the previous group's partner positions load (software pipeline) */
     uint64_t live = valid ? d.LIVE[h] : 0ull;
-    while (live) {
-        const int q = (__ffsll((long long)live) - 1) & ~3;
-        const uint32_t g4 = (uint32_t)(live >> q) & 0xFu;
-        live &= ~(0xFull << q);
-        uint32_t j[4];
-        float4 bp[4];
-        float2 in[4];
+    int q = 0;
+    uint32_t g4 = 0, j[4];
+    float4 bp[4];
+    float2 in[4];
+    auto fetch_own = [&]() {
+        g4 = 0;
+        if (live) {
+            q = (__ffsll((long long)live) - 1) & ~3;
+            g4 = (uint32_t)(live >> q) & 0xFu;
+            live &= ~(0xFull << q);
+        }
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             j[k] = MG_NONE;
@@ -940,29 +946,40 @@ __device__ __forceinline__ f3 force_one(const DS& d, const MgParams& p, const Mg
                 in[k] = d.BINT[bi];
             }
         }
+    };
+    fetch_own();
+    while (g4) {
+        const int qc = q;
+        uint32_t jc[4];
+        float4 bpc[4];
+        float2 inc[4];
         bool use[4];
         float4 pj[4], vj[4];
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            use[k] = j[k] < na && !((double)bp[k].y < 0.000000001);
+            jc[k] = j[k];
+            bpc[k] = bp[k];
+            inc[k] = in[k];
+            use[k] = jc[k] < na && !((double)bpc[k].y < 0.000000001);
             pj[k] = vj[k] = make_float4(0.f, 0.f, 0.f, 0.f);
             if (use[k]) {
-                pj[k] = d.P[j[k]];
-                vj[k] = d.V[j[k]];
+                pj[k] = d.P[jc[k]];
+                vj[k] = d.V[jc[k]];
             }
         }
+        fetch_own();
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             if (!use[k]) continue;
-            const size_t bi = (size_t)(q + k) * N + h;
+            const size_t bi = (size_t)(qc + k) * N + h;
             float strain, abs_dist;
-            f3 eterm = bond_term(ipos, ivev, xyz(pj[k]), xyz(vj[k]), bp[k], p.pmass, &strain, &abs_dist);
-            float2 v = in[k];
+            f3 eterm = bond_term(ipos, ivev, xyz(pj[k]), xyz(vj[k]), bpc[k], p.pmass, &strain, &abs_dist);
+            float2 v = inc[k];
             v.x = v.x + strain * strain;
             v.y = v.y + strain;
             d.BINT[bi] = v;
             if (!has_nan3(eterm)) force = force - eterm;
-            if (abs_dist >= bp[k].x && !freeze) ((float*)&d.BPAR[bi])[1] = 0.0f;
+            if (abs_dist >= bpc[k].x && !freeze) ((float*)&d.BPAR[bi])[1] = 0.0f;
         }
     }
 
@@ -984,9 +1001,9 @@ __device__ __forceinline__ f3 force_one(const DS& d, const MgParams& p, const Mg
     force = force + fsum * p.pmass;
 
     f3 F = F0 + force;
-    for (uint32_t e = e0; valid && e < e1; e += 4) {
-        uint32_t kk[4];
-        float4 bp[4];
+    /* incoming bonds 4 at a time, the next 4 fetched while this group's positions load */
+    uint32_t kk[4];
+    auto fetch_in = [&](uint32_t e) {
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             const uint32_t packed = d.INC[min(e + (uint32_t)k, e1 - 1u)];
@@ -994,22 +1011,27 @@ __device__ __forceinline__ f3 force_one(const DS& d, const MgParams& p, const Mg
             kk[k] = d.SLOT[kh];
             bp[k] = d.BPAR[(size_t)b * N + kh];
         }
+    };
+    if (valid && e0 < e1) fetch_in(e0);
+    for (uint32_t e = e0; valid && e < e1; e += 4) {
         bool use[4];
-        float4 pk[4], vk[4];
+        float4 bpc[4], pk[4], vk[4];
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            use[k] = e + (uint32_t)k < e1 && kk[k] < na && !((double)bp[k].y < 0.000000001);
+            bpc[k] = bp[k];
+            use[k] = e + (uint32_t)k < e1 && kk[k] < na && !((double)bpc[k].y < 0.000000001);
             pk[k] = vk[k] = make_float4(0.f, 0.f, 0.f, 0.f);
             if (use[k]) {
                 pk[k] = d.P[kk[k]];
                 vk[k] = d.V[kk[k]];
             }
         }
+        if (e + 4u < e1) fetch_in(e + 4u);
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             if (!use[k]) continue;
             float strain, abs_dist;
-            f3 eterm = bond_term(xyz(pk[k]), xyz(vk[k]), ipos, ivev, bp[k], p.pmass, &strain, &abs_dist);
+            f3 eterm = bond_term(xyz(pk[k]), xyz(vk[k]), ipos, ivev, bpc[k], p.pmass, &strain, &abs_dist);
             if (!has_nan3(eterm)) F = F + eterm;
         }
     }
