@@ -983,6 +983,10 @@ __device__ __forceinline__ f3 force_one(const DS& d, const MgParams& p, const Mg
         }
     }
 
+    uint32_t first[4] = {0u, 0u, 0u, 0u}; /* the first 4 incoming entries */
+    if (valid && e0 < e1)
+#pragma unroll
+        for (int k = 0; k < 4; k++) first[k] = d.INC[min(e0 + (uint32_t)k, e1 - 1u)];
     /* fluid terms: per-cell sums added in gridAdj order */
     const float sr = p.psmoothradius, st = p.psurface_t;
     f3 fsum = mk3(0.0f, 0.0f, 0.0f), cf = mk3(0.0f, 0.0f, 0.0f);
@@ -1001,18 +1005,22 @@ __device__ __forceinline__ f3 force_one(const DS& d, const MgParams& p, const Mg
     force = force + fsum * p.pmass;
 
     f3 F = F0 + force;
-    /* incoming bonds 4 at a time, the next 4 fetched while this group's positions load */
+    /* incoming bonds 4 at a time, the next 4 fetched while this group's positions load (the
+     * first 4 entries are read before the fluid sweep) */
     uint32_t kk[4];
-    auto fetch_in = [&](uint32_t e) {
+    auto fetch_in = [&](const uint32_t packed[4]) {
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            const uint32_t packed = d.INC[min(e + (uint32_t)k, e1 - 1u)];
-            const uint32_t kh = packed / (uint32_t)B, b = packed - kh * (uint32_t)B;
+            const uint32_t kh = packed[k] / (uint32_t)B, b = packed[k] - kh * (uint32_t)B;
             kk[k] = d.SLOT[kh];
             bp[k] = d.BPAR[(size_t)b * N + kh];
         }
     };
-    if (valid && e0 < e1) fetch_in(e0);
+    auto entries = [&](uint32_t e, uint32_t packed[4]) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) packed[k] = d.INC[min(e + (uint32_t)k, e1 - 1u)];
+    };
+    if (valid && e0 < e1) fetch_in(first);
     for (uint32_t e = e0; valid && e < e1; e += 4) {
         bool use[4];
         float4 bpc[4], pk[4], vk[4];
@@ -1026,7 +1034,11 @@ __device__ __forceinline__ f3 force_one(const DS& d, const MgParams& p, const Mg
                 vk[k] = d.V[kk[k]];
             }
         }
-        if (e + 4u < e1) fetch_in(e + 4u);
+        if (e + 4u < e1) {
+            uint32_t packed[4];
+            entries(e + 4u, packed);
+            fetch_in(packed);
+        }
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             if (!use[k]) continue;
