@@ -921,13 +921,13 @@ __device__ __forceinline__ f3 force_one(const DS& d, const MgParams& p, const Mg
 
     f3 force = mk3(0.0f, 0.0f, 0.0f);
     /* live slots in groups of 4, ascending; a group's partner slots and records are fetched
-     * while the previous group's partner positions load (software pipeline) */
+     * while the previous group's partner positions load (software pipeline). The own bond
+     * terms are summed after the fluid sweep (into their own total, as before it), and the
+     * first group's partner slots are fetched before the sweep. */
     uint64_t live = valid ? d.LIVE[h] : 0ull;
     int q = 0;
     uint32_t g4 = 0, j[4];
-    float4 bp[4];
-    float2 in[4];
-    auto fetch_own = [&]() {
+    auto next_group = [&]() {
         g4 = 0;
         if (live) {
             q = (__ffsll((long long)live) - 1) & ~3;
@@ -935,53 +935,9 @@ __device__ __forceinline__ f3 force_one(const DS& d, const MgParams& p, const Mg
             live &= ~(0xFull << q);
         }
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            j[k] = MG_NONE;
-            bp[k] = make_float4(0.f, 0.f, 0.f, 0.f);
-            in[k] = make_float2(0.f, 0.f);
-            if (g4 & (1u << k)) {
-                const size_t bi = (size_t)(q + k) * N + h;
-                j[k] = d.SLOT[d.BPARTNER[bi]];
-                bp[k] = d.BPAR[bi];
-                in[k] = d.BINT[bi];
-            }
-        }
+        for (int k = 0; k < 4; k++) j[k] = (g4 & (1u << k)) ? d.SLOT[d.BPARTNER[(size_t)(q + k) * N + h]] : MG_NONE;
     };
-    fetch_own();
-    while (g4) {
-        const int qc = q;
-        uint32_t jc[4];
-        float4 bpc[4];
-        float2 inc[4];
-        bool use[4];
-        float4 pj[4], vj[4];
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            jc[k] = j[k];
-            bpc[k] = bp[k];
-            inc[k] = in[k];
-            use[k] = jc[k] < na && !((double)bpc[k].y < 0.000000001);
-            pj[k] = vj[k] = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (use[k]) {
-                pj[k] = d.P[jc[k]];
-                vj[k] = d.V[jc[k]];
-            }
-        }
-        fetch_own();
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            if (!use[k]) continue;
-            const size_t bi = (size_t)(qc + k) * N + h;
-            float strain, abs_dist;
-            f3 eterm = bond_term(ipos, ivev, xyz(pj[k]), xyz(vj[k]), bpc[k], p.pmass, &strain, &abs_dist);
-            float2 v = inc[k];
-            v.x = v.x + strain * strain;
-            v.y = v.y + strain;
-            d.BINT[bi] = v;
-            if (!has_nan3(eterm)) force = force - eterm;
-            if (abs_dist >= bpc[k].x && !freeze) ((float*)&d.BPAR[bi])[1] = 0.0f;
-        }
-    }
+    next_group();
 
     uint32_t first[4] = {0u, 0u, 0u, 0u}; /* the first 4 incoming entries */
     if (valid && e0 < e1)
@@ -1002,6 +958,57 @@ __device__ __forceinline__ f3 force_one(const DS& d, const MgParams& p, const Mg
             cf = mk3(0.0f, 0.0f, 0.0f);
         },
         i);
+
+    float4 bp[4];
+    float2 in[4];
+    auto records = [&]() {
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            bp[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+            in[k] = make_float2(0.f, 0.f);
+            if (g4 & (1u << k)) {
+                const size_t bi = (size_t)(q + k) * N + h;
+                bp[k] = d.BPAR[bi];
+                in[k] = d.BINT[bi];
+            }
+        }
+    };
+    records();
+    while (g4) {
+        const int qc = q;
+        uint32_t jc[4];
+        float4 bpc[4];
+        float2 inc[4];
+        bool use[4];
+        float4 pj[4], vj[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            jc[k] = j[k];
+            bpc[k] = bp[k];
+            inc[k] = in[k];
+            use[k] = jc[k] < na && !((double)bpc[k].y < 0.000000001);
+            pj[k] = vj[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (use[k]) {
+                pj[k] = d.P[jc[k]];
+                vj[k] = d.V[jc[k]];
+            }
+        }
+        next_group();
+        records();
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if (!use[k]) continue;
+            const size_t bi = (size_t)(qc + k) * N + h;
+            float strain, abs_dist;
+            f3 eterm = bond_term(ipos, ivev, xyz(pj[k]), xyz(vj[k]), bpc[k], p.pmass, &strain, &abs_dist);
+            float2 v = inc[k];
+            v.x = v.x + strain * strain;
+            v.y = v.y + strain;
+            d.BINT[bi] = v;
+            if (!has_nan3(eterm)) force = force - eterm;
+            if (abs_dist >= bpc[k].x && !freeze) ((float*)&d.BPAR[bi])[1] = 0.0f;
+        }
+    }
     force = force + fsum * p.pmass;
 
     f3 F = F0 + force;
