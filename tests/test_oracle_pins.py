@@ -169,26 +169,58 @@ def test_wendland_normalisation_constant():
     assert abs(p.wendlandC2kern - 1 / integral) < 1e-6
 
 
+def jittered_block(n_side, spacing, origin, seed, jitter=0.05):
+    rng = np.random.default_rng(seed)
+    g = np.stack(np.meshgrid(*[np.arange(n_side)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    return (np.asarray(origin) + g * spacing + rng.uniform(-jitter, jitter, g.shape)).astype(np.float32)
+
+
+def density_bruteforce(pos, sr, params):
+    """O(N^2) float64 Wendland-C2 density (src/fluid_system_cuda.cu:653-668, 693-700):
+    sum over 0 < dsq < sr^2 of (1-q)^4 (4q+1), q = r/sr; times pmass * 21/(2*pi); 0 -> 1."""
+    pos = pos.astype(np.float64)
+    d2 = ((pos[:, None, :] - pos[None, :, :]) ** 2).sum(-1)
+    m = (d2 < float(np.float32(sr) * np.float32(sr))) & (d2 > 0)
+    q = np.where(m, np.sqrt(d2) / sr, 0.0)
+    w = np.where(m, (1 - q) ** 4 * (4 * q + 1), 0.0).sum(1)
+    dsum = w * params.pmass * params.wendlandC2kern
+    dsum[dsum == 0] = 1.0
+    return dsum
+
+
 def test_pressure_matches_bruteforce(built):
     sc = scenes.fluid_block(n_side=9, seed=8)
     o = make(sc)
     o.Run2PhysicalSort()
     o.ComputePressureCL()
     nump, na = o.counts()
-    pos = o.download(F.FPOS)[:na, :3].astype(np.float64)
-    d2 = ((pos[:, None, :] - pos[None, :, :]) ** 2).sum(-1)
-    r = np.sqrt(d2)
-    m = (d2 < 1.0) & (d2 > 0)
-    q = np.where(m, r, 0.0)
-    w = np.where(m, (1 - q) ** 4 * (4 * q + 1), 0.0).sum(1)
+    pos = o.download(F.FPOS)[:na, :3]
     p = sc.params
-    dsum = w * p.pmass * p.wendlandC2kern
-    dsum[dsum == 0] = 1.0
+    dsum = density_bruteforce(pos, 1.0, p)
     dens = o.download(F.FDENSITY)[:na]
     press = o.download(F.FPRESS)[:na]
     np.testing.assert_allclose(1.0 / dens, dsum, rtol=2e-6)
     np.testing.assert_allclose(press, (dsum - p.prest_dens) * p.pintstiff, rtol=1e-4,
                                atol=2e-6 * p.prest_dens)
+
+
+@pytest.mark.parametrize("sr", [0.75, 1.5, 2.0])
+def test_pressure_bruteforce_smoothing_radius(built, sr):
+    """q = r / sr and the cut-off sr^2 for smoothing radii other than 1 (the grid cell follows
+    sr, as the reference's set-up derives it)."""
+    pos = jittered_block(7, 0.5 * sr, (5.0 * sr, 5.0 * sr, 5.0 * sr), seed=31)
+    sc = mini_scene(pos, b=0, vmax=14.0 * sr, sr=sr)
+    # the FP64 oracle's buffers download as float32: 1/density carries one float32 rounding
+    for prec, rtol in ((64, 2e-7), (32, 2e-6)):
+        o = make(sc, precision=prec)
+        o.Run2PhysicalSort()
+        nump, na = o.counts()
+        assert na == len(pos)
+        o.ComputePressureCL()
+        got = o.download(F.FPOS)[:na, :3]
+        dsum = density_bruteforce(got, sr, sc.params)
+        np.testing.assert_allclose(1.0 / o.download(F.FDENSITY)[:na].astype(np.float64), dsum,
+                                   rtol=rtol)
 
 
 def test_isolated_particle_density_rule(built):
@@ -213,6 +245,145 @@ def test_force_conserves_momentum(built):
     total = frc.sum(0)
     scale = np.abs(frc).sum(0)
     assert np.all(np.abs(total) <= 1e-6 * scale + 1e-9), (total, scale)
+
+
+def fluid_force_bruteforce(pos, veval, press, dens, sr, m, sigma):
+    """O(N^2) float64 fluid force, written from src/fluid_system_cuda.cu:3203-3221 and
+    :3332-3340: for every pair with 0 < |x_i - x_j|^2 < 1, with r = |x_i - x_j| and
+    t = sr - r,
+        pterm = dens_i dens_j 100 (x_i - x_j)/r (100 (p_i + p_j) t^3 - sigma t^2)
+        vterm = 1e5 (veval_j - veval_i) t^3,
+    and the force on i is pmass * sum_j (pterm + vterm). FDENSITY holds 1/density."""
+    x = pos.astype(np.float64)
+    v = veval.astype(np.float64)
+    p = press.astype(np.float64)
+    rho_inv = dens.astype(np.float64)
+    n = len(x)
+    out = np.zeros((n, 3))
+    scale = np.zeros((n, 3))
+    for i in range(n):
+        d = x[i] - x
+        dsq = (d * d).sum(1)
+        hit = (dsq < 1.0) & (dsq > 0.0)
+        d, dsq = d[hit], dsq[hit]
+        r = np.sqrt(dsq)
+        t = sr - r
+        coef = rho_inv[i] * rho_inv[hit] * 100.0 * (100.0 * (p[i] + p[hit]) * t ** 3 - sigma * t ** 2)
+        pterm = d / r[:, None] * coef[:, None]
+        vterm = 1e5 * (v[hit] - v[i]) * (t ** 3)[:, None]
+        terms = m * (pterm + vterm)
+        out[i] = terms.sum(0)
+        scale[i] = np.abs(terms).sum(0)
+    return out, scale
+
+
+@pytest.mark.parametrize("sr", [1.0, 1.25])
+def test_fluid_force_matches_bruteforce(built, sr):
+    """The whole fluid force of a 512-particle jittered block with random velocities, non-zero
+    pressure and surface tension, against the float64 brute force (no bonds)."""
+    rng = np.random.default_rng(17)
+    pos = jittered_block(8, 0.5, (5.0 * sr, 5.0 * sr, 5.0 * sr), seed=13, jitter=0.08)
+    vel = rng.uniform(-0.3, 0.3, pos.shape)
+    sc = mini_scene(pos, b=0, vmax=14.0 * sr, sr=sr, vel=vel,
+                    params_over={"surface_t": 0.35, "istiff": 3.0})
+    # FP64: the float32 download of FFORCE, FPRESS and FDENSITY bounds the agreement
+    for prec, tol in ((64, 5e-7), (32, 2e-5)):
+        o = make(sc, precision=prec)
+        o.Run2PhysicalSort()
+        nump, na = o.counts()
+        assert na == len(pos)
+        o.ComputePressureCL()
+        o.ComputeForceCL()
+        x = o.download(F.FPOS)[:na, :3]
+        ve = o.download(F.FVEVAL)[:na, :3]
+        pr, de = o.download(F.FPRESS)[:na], o.download(F.FDENSITY)[:na]
+        assert np.ptp(pr) > 0 and np.abs(pr).max() > 1e-3
+        want, scale = fluid_force_bruteforce(x, ve, pr, de, sr, sc.params.pmass, sc.params.psurface_t)
+        got = o.download(F.FFORCE)[:na, :3].astype(np.float64)
+        assert np.all(np.abs(got - want) <= tol * scale + 1e-30), (
+            prec, np.max(np.abs(got - want) / (scale + 1e-30)))
+
+
+# FP64 oracle against float64 closed forms: the buffers download as float32 (FFORCE, and the
+# FDENSITY / FPRESS values the closed forms are built from), so agreement is a few float32 ulps
+RT64 = 5e-7
+
+
+def pair_scene(r, sr, vel1=(0.0, 0.0, 0.0), **over):
+    u = np.array([0.36, 0.48, 0.8])                 # unit vector
+    x0 = np.array([5.0, 5.0, 5.0]) * sr
+    pos = np.array([x0, x0 + r * u])
+    vel = np.array([[0.0, 0.0, 0.0], vel1])
+    return mini_scene(pos, b=0, vmax=12.0 * sr, sr=sr, vel=vel, params_over=over), pos
+
+
+def pair_force_on_0(sc):
+    o = make(sc, precision=64)
+    o.Run2PhysicalSort()
+    o.ComputePressureCL()
+    o.ComputeForceCL()
+    x = o.download(F.FPOS)[:, :3].astype(np.float64)
+    pid = o.download(F.FPARTICLE_ID)
+    i0, i1 = (0, 1) if pid[0] == 1 else (1, 0)      # IDs follow creation order
+    f = o.download(F.FFORCE)[:, :3].astype(np.float64)
+    dens = o.download(F.FDENSITY).astype(np.float64)
+    press = o.download(F.FPRESS).astype(np.float64)
+    return f[i0], f[i1], x[i0] - x[i1], dens[i0] * dens[i1], press[i0] + press[i1]
+
+
+@pytest.mark.parametrize("sr", [1.0, 1.5])
+def test_fluid_pair_pressure_term_is_cubic(built, sr):
+    """sigma = 0, no relative velocity: the pair force is along x_0 - x_1 and, divided by
+    pmass 100 100 dens_0 dens_1 (p_0 + p_1), equals (sr - r)^3 at each radius."""
+    m = None
+    g = {}
+    for r in (0.3, 0.55, 0.8):
+        sc, _ = pair_scene(r, sr, surface_t=0.0, istiff=50.0)
+        m = sc.params.pmass
+        f0, f1, d, dd, pp = pair_force_on_0(sc)
+        assert abs(pp) > 0
+        np.testing.assert_allclose(f0, -f1, rtol=RT64, atol=0)
+        dn = d / np.linalg.norm(d)
+        np.testing.assert_allclose(f0, np.dot(f0, dn) * dn, atol=RT64 * np.abs(f0).max())
+        g[r] = np.dot(f0, dn) / (m * 1e4 * dd * pp)
+        rr = np.linalg.norm(d)
+        np.testing.assert_allclose(g[r], (sr - rr) ** 3, rtol=RT64)
+    np.testing.assert_allclose(g[0.3] / g[0.8], ((sr - 0.3) / (sr - 0.8)) ** 3, rtol=1e-5)
+
+
+@pytest.mark.parametrize("sr", [1.0, 1.5])
+def test_fluid_pair_surface_term_attracts_quadratically(built, sr):
+    """Zero pressure (stiffness 0), no relative velocity: the surface-tension term alone pulls
+    the pair together with magnitude pmass 100 sigma dens_0 dens_1 (sr - r)^2."""
+    sigma = 0.4
+    for r in (0.3, 0.55, 0.8):
+        sc, _ = pair_scene(r, sr, surface_t=sigma, istiff=0.0)
+        f0, f1, d, dd, pp = pair_force_on_0(sc)
+        assert pp == 0.0
+        rr = np.linalg.norm(d)
+        assert np.dot(f0, d) < 0, "surface tension must attract"
+        want = -sc.params.pmass * 100.0 * np.float32(sigma) * dd * (sr - rr) ** 2 * d / rr
+        np.testing.assert_allclose(f0, want, rtol=RT64)
+        np.testing.assert_allclose(f1, -want, rtol=RT64)
+
+
+@pytest.mark.parametrize("sr", [1.0, 1.5])
+def test_fluid_pair_viscous_term_is_linear_in_dv(built, sr):
+    """Zero pressure and sigma: the force on particle 0 is pmass 1e5 (v_1 - v_0) (sr - r)^3,
+    linear in the relative velocity (and independent of density)."""
+    for r in (0.3, 0.7):
+        prev = None
+        for k in (1.0, 2.0, -3.0):
+            dv = k * np.array([0.02, -0.05, 0.03], np.float32)
+            sc, _ = pair_scene(r, sr, vel1=dv, surface_t=0.0, istiff=0.0)
+            f0, f1, d, dd, pp = pair_force_on_0(sc)
+            rr = np.linalg.norm(d)
+            want = sc.params.pmass * 1e5 * dv.astype(np.float64) * (sr - rr) ** 3
+            np.testing.assert_allclose(f0, want, rtol=RT64)
+            np.testing.assert_allclose(f1, -want, rtol=RT64)
+            if prev is not None:
+                np.testing.assert_allclose(f0 / k, prev, rtol=RT64)
+            prev = f0 / k
 
 
 def pair_force(d, rest, k, c, dv, m):
@@ -354,20 +525,40 @@ def test_diffusion_conserves_and_freezes_nondiffusing(built):
     assert np.abs(c1[:, diffusing] - c0[:, diffusing]).max() > 1e-4
 
 
-def test_two_particle_diffusion_analytic(built):
+@pytest.mark.parametrize("sr", [1.0, 1.5])
+def test_two_particle_diffusion_analytic(built, sr):
+    """Two particles at distance r: new_t = K_t (c_t + D_t (sr^2 - r^2) (c'_t - c_t)) for a
+    diffusing TF and K_t c_t for one that does not diffuse, with integer breakdown rates K_t
+    other than 1 (src/fluid_system_cuda.cu:3046-3058, 3102-3104)."""
     g = scenes.empty_genome()
     g.tf_diffusability[3] = 2
+    g.tf_diffusability[5] = 1
+    g.tf_breakdown_rate[3] = 2
+    g.tf_breakdown_rate[5] = 1
+    g.tf_breakdown_rate[7] = 0
+    g.tf_breakdown_rate[9] = 3
     conc = np.zeros((2, F.NUM_TF), np.float32)
     conc[0, 3], conc[1, 3] = 1.0, 0.25
-    pos = np.array([[5.0, 5.0, 5.0], [5.0, 5.6, 5.0]])
-    sc = mini_scene(pos, b=0, genome=g, conc=conc, params_over={})
+    conc[0, 5], conc[1, 5] = 0.5, 0.125
+    conc[:, 7] = [0.75, 0.5]
+    conc[:, 9] = [0.25, 0.0625]
+    r = 0.6
+    pos = np.array([[5.0, 5.0, 5.0], [5.0, 5.0 + r, 5.0]]) * [1, 1, 1]
+    pos = pos * sr
+    sc = mini_scene(pos, b=0, genome=g, conc=conc, sr=sr, vmax=12.0 * sr)
     sc.params.diff_scale = 0.5
     o = make(sc, precision=64)
     o.Run2PhysicalSort(); o.ComputeDiffusionCL()
-    c = o.download(F.FCONC)[:, 3].astype(np.float64)
-    cc = 1.0 - 0.6 ** 2
-    D = 2 * 0.5
-    np.testing.assert_allclose(c, [1.0 + D * cc * (0.25 - 1.0), 0.25 + D * cc * (1.0 - 0.25)], rtol=1e-6)
+    c = o.download(F.FCONC).astype(np.float64)
+    x = o.download(F.FPOS)[:, :3].astype(np.float64)
+    dsq = ((x[0] - x[1]) ** 2).sum()
+    cc = np.float32(sr) ** 2 - dsq
+    for t, D, K in ((3, 2 * 0.5, 2), (5, 1 * 0.5, 1)):
+        np.testing.assert_allclose(c[:, t], [K * (conc[0, t] + D * cc * (conc[1, t] - conc[0, t])),
+                                             K * (conc[1, t] + D * cc * (conc[0, t] - conc[1, t]))],
+                                   rtol=1e-6)
+    np.testing.assert_array_equal(c[:, 7], [0.0, 0.0])
+    np.testing.assert_allclose(c[:, 9], 3 * conc[:, 9].astype(np.float64), rtol=0)
 
 
 # ------------------------------------------------------------------ gene action
