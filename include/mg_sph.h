@@ -49,6 +49,22 @@ int mg_fluid_param(mg_handle* h, float ss, float sr, float pr, float mass, float
                    float a_p);
 /* Same, from a prepared MgParams (including the diff_scale / secrete_scale additions). */
 int mg_set_params(mg_handle* h, const MgParams* p);
+/* Host-only derivations (no device work, callable without a GPU): the MgParams that
+ * FluidParamCL (src/fluid_system.cpp:521-610) derives from its argument list, exactly as
+ * mg_fluid_param stores them, and the MgGrid of SetupGrid (src/fluid_system.cpp:2073-2131)
+ * followed by FluidSetupCL (src/host_CL.cpp:39-106). FFI callers use these instead of
+ * restating the float32 host arithmetic. */
+int mg_derive_params(MgParams* out, float ss, float sr, float pr, float mass, float rest,
+                     const float bmin[3], const float bmax[3], float estiff, float istiff,
+                     float visc, float surface_t, float damp, float fmin, float fmax, float ffreq,
+                     float gslope, float gx, float gy, float gz, float al, float vl, float a_f,
+                     float a_p);
+int mg_derive_grid(MgGrid* out, const float vmin[3], const float vmax[3], float sim_scale,
+                   float cell_size, float smooth_radius);
+/* The parameters and grid a handle holds: the reference's host copies m_FParams and
+ * m_GridRes / m_GridMin / ... (src/fluid_system.h:741, 756-760). */
+int mg_get_params(mg_handle* h, MgParams* out);
+int mg_get_grid(mg_handle* h, MgGrid* out);
 /* Replaces UpdateGenome (src/fluid_system.cpp:494-502); MgGenome has FGenome's layout. */
 int mg_set_genome(mg_handle* h, const MgGenome* g);
 /* Replaces setFreeze (src/fluid_system.cpp:2020-2035). */
@@ -139,6 +155,11 @@ enum {
 };
 int mg_step_timed(mg_handle* h, int nsteps, float dt, unsigned flags, double* ms_out,
                   int* calls_out);
+
+/* Test hook (no reference counterpart): sets the cell scan's 64-bit tile ticket, so a test
+ * can run scans across the point where a 32-bit ticket would wrap. The value must be a
+ * multiple of the scan's tile count, ceil(C / 4096), as it always is between launches. */
+int mg_debug_set_scan_ticket(mg_handle* h, uint64_t ticket);
 
 /* ---- batched instances (no counterpart in the reference, which steps one FluidSystem) ----
  * A batch steps many independent handles (each with its own buffers, grid, parameters and

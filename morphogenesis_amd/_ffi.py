@@ -100,79 +100,45 @@ def f32(x) -> float:
     return float(np.float32(x))
 
 
+_libm = None
+
+
+def powf(x, y) -> np.float32:
+    """The C library's powf, the function the reference's host pow(float, float) calls
+    (src/fluid_system.cpp:548, 2529-2534). numpy's float32 power can differ by an ulp."""
+    global _libm
+    if _libm is None:
+        _libm = C.CDLL("libm.so.6")
+        _libm.powf.restype = C.c_float
+        _libm.powf.argtypes = [C.c_float, C.c_float]
+    return np.float32(_libm.powf(float(np.float32(x)), float(np.float32(y))))
+
+
 def params_from_fluid_param(ss, sr, pr, mass, rest, bmin, bmax, estiff, istiff, visc, surface_t,
                             damp, fmin, fmax, ffreq, gslope, gx, gy, gz, al, vl, a_f=0.0,
                             a_p=1.0) -> MgParams:
-    """FluidParamCL (src/fluid_system.cpp:521-610) in float32, as mg_params_from_fluid_param."""
-    F = np.float32
+    """FluidParamCL (src/fluid_system.cpp:521-610): the MgParams that mg_fluid_param stores,
+    derived by the library's own host code (mg_derive_params, no device work)."""
+    lib = load_library()
     p = MgParams()
-    sr_, ss_ = F(sr), F(ss)
-    p.psimscale = ss_
-    p.psmoothradius = sr_
-    p.pradius = F(pr)
-    p.r2 = F(sr_ * sr_)
-    p.pmass = F(mass)
-    p.prest_dens = F(rest)
-    for k in range(3):
-        p.pboundmin[k] = F(bmin[k])
-        p.pboundmax[k] = F(bmax[k])
-    p.pextstiff, p.pintstiff, p.pvisc = F(estiff), F(istiff), F(visc)
-    p.psurface_t, p.pdamp = F(surface_t), F(damp)
-    p.pforce_min, p.pforce_max, p.pforce_freq, p.pground_slope = F(fmin), F(fmax), F(ffreq), F(gslope)
-    p.pgravity[0], p.pgravity[1], p.pgravity[2] = F(gx), F(gy), F(gz)
-    p.AL, p.VL = F(al), F(vl)
-    p.AL2 = F(F(al) * F(al))
-    p.VL2 = F(F(vl) * F(vl))
-    pi = F(3.141592)
-    p.pdist = F(np.power(F(p.pmass) / F(p.prest_dens), F(1) / F(3)))
-    p.poly6kern = F(F(315.0) / (F(64.0) * pi * np.power(sr_, F(9.0))))
-    p.wendlandC2kern = F(F(21) / (F(2) * pi))
-    p.spikykern = F(F(-45.0) / (pi * np.power(sr_, F(6.0))))
-    p.lapkern = F(F(45.0) / (pi * np.power(sr_, F(6.0))))
-    p.gausskern = F(F(1.0) / np.power(pi * F(2.0) * sr_ * sr_, F(1.5)))
-    p.H = F(sr_ / ss_)
-    p.d2 = F(ss_ * ss_)
-    p.rd2 = F(F(p.r2) / F(p.d2))
-    p.vterm = F(F(p.lapkern) * F(p.pvisc))
-    p.actuation_factor, p.actuation_period = F(a_f), F(a_p)
-    p.diff_scale = 1.0
-    p.secrete_scale = 1.0
-    p.freeze = 0
-    p.frame = 0
+    fl = C.c_float * 3
+    r = lib.mg_derive_params(C.byref(p), ss, sr, pr, mass, rest, fl(*bmin), fl(*bmax), estiff,
+                             istiff, visc, surface_t, damp, fmin, fmax, ffreq, gslope, gx, gy, gz,
+                             al, vl, a_f, a_p)
+    if r != MG_OK:
+        raise MgError(f"mg_derive_params -> {ERRORS.get(r, r)}: {lib.mg_last_error().decode()}")
     return p
 
 
 def grid_setup(vmin, vmax, sim_scale, cell_size, smooth_radius) -> MgGrid:
     """SetupGrid (src/fluid_system.cpp:2073-2131) + FluidSetupCL (src/host_CL.cpp:39-106),
-    float32 host arithmetic, as mg_grid_setup in include/mg_types.h."""
-    F = np.float32
+    derived by the library's own host code (mg_derive_grid, no device work)."""
+    lib = load_library()
     g = MgGrid()
-    world_cellsize = F(F(cell_size) / F(sim_scale))
-    res = []
-    for k in range(3):
-        size = F(F(vmax[k]) - F(vmin[k]))
-        r = int(np.ceil(F(size / world_cellsize)))
-        size = F(F(F(r) * F(cell_size)) / F(sim_scale))
-        g.res[k] = r
-        g.size[k] = size
-        g.delta[k] = F(F(r) / size)
-        g.gmin[k] = F(vmin[k])
-        g.gmax[k] = F(vmax[k])
-        res.append(r)
-    gsrch = int(np.floor(F(F(F(2.0) * F(F(smooth_radius) / F(sim_scale))) / world_cellsize)) + F(1.0))
-    gsrch = max(gsrch, 2)
-    for k in range(3):
-        g.scan_max[k] = res[k] - gsrch
-    g.total = res[0] * res[1] * res[2]
-    g.srch = gsrch
-    g.adj_cnt = gsrch ** 3
-    g.chk = 0
-    cell = 0
-    for y in range(gsrch):
-        for z in range(gsrch):
-            for x in range(gsrch):
-                g.adj[cell] = (y * res[2] + z) * res[0] + x
-                cell += 1
+    fl = C.c_float * 3
+    r = lib.mg_derive_grid(C.byref(g), fl(*vmin), fl(*vmax), sim_scale, cell_size, smooth_radius)
+    if r != MG_OK:
+        raise MgError(f"mg_derive_grid -> {ERRORS.get(r, r)}: {lib.mg_last_error().decode()}")
     return g
 
 
@@ -259,6 +225,12 @@ def declare(lib, prefix: str):
         sig["fluid_setup"] = [h, C.c_int, C.c_int, P(C.c_int32), P(C.c_float), P(C.c_float),
                               P(C.c_float), P(C.c_float), C.c_int, C.c_int]
         sig["step_timed"] = [h, C.c_int, C.c_float, C.c_uint, P(C.c_double), P(C.c_int)]
+        sig["get_params"] = [h, P(MgParams)]
+        sig["get_grid"] = [h, P(MgGrid)]
+        sig["debug_set_scan_ticket"] = [h, C.c_uint64]
+        fl3 = P(C.c_float)
+        sig["derive_params"] = [P(MgParams)] + [C.c_float] * 5 + [fl3, fl3] + [C.c_float] * 16
+        sig["derive_grid"] = [P(MgGrid), fl3, fl3, C.c_float, C.c_float, C.c_float]
         sig["batch_create"] = [P(h), P(h), C.c_int]
         sig["batch_destroy"] = [h]
         sig["batch_step"] = [h, C.c_int, C.c_float, C.c_uint]

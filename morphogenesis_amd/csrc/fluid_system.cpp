@@ -235,6 +235,53 @@ int mgfs_setup(mgfs_handle* f, const MgGrid* g, const MgParams* p, const MgGenom
     }
 }
 
+/* the facade's SetupGrid (src/fluid_system.cpp:2073-2131, then FluidSetupCL) */
+int mgfs_setup_grid(mgfs_handle* f, const float vmin[3], const float vmax[3], float sim_scale,
+                    float cell_size, float smooth_radius)
+{
+    try {
+        reinterpret_cast<mg::FluidSystem*>(f)->SetupGrid(vmin, vmax, sim_scale, cell_size, smooth_radius);
+        return MG_OK;
+    } catch (const mg::Error& e) {
+        return e.code;
+    }
+}
+
+/* the facade's FluidParamCL (src/fluid_system.cpp:521-610) */
+int mgfs_fluid_param(mgfs_handle* f, float ss, float sr, float pr, float mass, float rest,
+                     const float bmin[3], const float bmax[3], float estiff, float istiff,
+                     float visc, float surface_t, float damp, float fmin, float fmax, float ffreq,
+                     float gslope, float gx, float gy, float gz, float al, float vl, float a_f,
+                     float a_p)
+{
+    try {
+        reinterpret_cast<mg::FluidSystem*>(f)->FluidParamCL(ss, sr, pr, mass, rest, bmin, bmax,
+                                                            estiff, istiff, visc, surface_t, damp,
+                                                            fmin, fmax, ffreq, gslope, gx, gy, gz,
+                                                            al, vl, a_f, a_p);
+        return MG_OK;
+    } catch (const mg::Error& e) {
+        return e.code;
+    }
+}
+
+/* the genome and time step without touching the grid or parameters */
+int mgfs_set_genome_dt(mgfs_handle* f, const MgGenome* genome, double dt)
+{
+    auto* fs = reinterpret_cast<mg::FluidSystem*>(f);
+    try {
+        fs->SetGenome(*genome);
+        fs->UpdateGenome();
+        fs->SetDT(dt);
+        return MG_OK;
+    } catch (const mg::Error& e) {
+        return e.code;
+    }
+}
+
+/* the C-ABI handle the facade drives (for mg_get_params / mg_get_grid / mg_download) */
+mg_handle* mgfs_handle_of(mgfs_handle* f) { return reinterpret_cast<mg::FluidSystem*>(f)->handle(); }
+
 int mgfs_transfer_to(mgfs_handle* f)
 {
     try { reinterpret_cast<mg::FluidSystem*>(f)->TransferToCL(); return MG_OK; }

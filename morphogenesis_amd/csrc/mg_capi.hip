@@ -286,7 +286,7 @@ struct mg_handle {
     bool pos_binned = false; /* GCELL / CNT come from the current positions (insert since the last move) */
     uint32_t* ctr = nullptr;
     unsigned long long* scan_status = nullptr;  /* decoupled look-back status, one word per tile */
-    uint32_t* scan_ticket = nullptr;
+    unsigned long long* scan_ticket = nullptr; /* 64-bit: never wraps (see k_scan_lookback) */
     void* staging = nullptr;
     size_t staging_bytes = 0;
     bool normalized = true;
@@ -1259,7 +1259,7 @@ int mg_set_grid(mg_handle* h, const MgGrid* g)
         if (!h->scan_ticket) CHK(dalloc(h, &h->scan_ticket, 1));
         CHK(dalloc(h, &h->scan_status, (size_t)(h->C + SCAN_TILE - 1) / SCAN_TILE));
         CHK(fill_u32(h, h->scan_status, 0, 2 * ((size_t)(h->C + SCAN_TILE - 1) / SCAN_TILE)));
-        CHK(fill_u32(h, h->scan_ticket, 0, 1));
+        CHK(fill_u32(h, h->scan_ticket, 0, 2));
         CHK(dalloc(h, &h->GCNT, (size_t)MG_NUM_GENES * h->C));
         CHK(dalloc(h, &h->GOFF, (size_t)MG_NUM_GENES * h->C));
         CHK(fill_u32(h, h->GCNT, 0, (size_t)MG_NUM_GENES * h->C));
@@ -1316,6 +1316,68 @@ int mg_fluid_param(mg_handle* h, float ss, float sr, float pr, float mass, float
         p.secrete_scale = h->p.secrete_scale;
     }
     return mg_set_params(h, &p);
+}
+
+int mg_derive_params(MgParams* out, float ss, float sr, float pr, float mass, float rest,
+                     const float bmin[3], const float bmax[3], float estiff, float istiff,
+                     float visc, float surface_t, float damp, float fmin, float fmax, float ffreq,
+                     float gslope, float gx, float gy, float gz, float al, float vl, float a_f,
+                     float a_p)
+{
+    if (!out || !bmin || !bmax) { set_err("mg_derive_params: null argument"); return MG_ERR_ARG; }
+    mg_params_from_fluid_param(out, ss, sr, pr, mass, rest, bmin, bmax, estiff, istiff, visc,
+                               surface_t, damp, fmin, fmax, ffreq, gslope, gx, gy, gz, al, vl, a_f,
+                               a_p);
+    return MG_OK;
+}
+
+int mg_derive_grid(MgGrid* out, const float vmin[3], const float vmax[3], float sim_scale,
+                   float cell_size, float smooth_radius)
+{
+    if (!out || !vmin || !vmax) { set_err("mg_derive_grid: null argument"); return MG_ERR_ARG; }
+    if (!(sim_scale > 0.0f) || !(cell_size > 0.0f) || !(smooth_radius > 0.0f)) {
+        set_err("mg_derive_grid: sim_scale, cell_size and smooth_radius must be positive");
+        return MG_ERR_ARG;
+    }
+    if (mg_grid_setup(out, vmin, vmax, sim_scale, cell_size, smooth_radius) != MG_OK) {
+        set_err("mg_derive_grid: the search width exceeds %d cells", MG_MAX_ADJ);
+        return MG_ERR_ARG;
+    }
+    return MG_OK;
+}
+
+int mg_get_params(mg_handle* h, MgParams* out)
+{
+    if (!h || !out) { set_err("mg_get_params: null argument"); return MG_ERR_ARG; }
+    if (!h->have_params) { set_err("mg_get_params: no parameters set"); return MG_ERR_STATE; }
+    *out = h->p;
+    return MG_OK;
+}
+
+int mg_get_grid(mg_handle* h, MgGrid* out)
+{
+    if (!h || !out) { set_err("mg_get_grid: null argument"); return MG_ERR_ARG; }
+    if (!h->have_grid) { set_err("mg_get_grid: no grid set"); return MG_ERR_STATE; }
+    *out = h->g;
+    return MG_OK;
+}
+
+int mg_debug_set_scan_ticket(mg_handle* h, uint64_t ticket)
+{
+    if (!h) { set_err("mg_debug_set_scan_ticket: null handle"); return MG_ERR_ARG; }
+    if (!h->scan_ticket || !h->have_grid) { set_err("mg_debug_set_scan_ticket: no grid set"); return MG_ERR_STATE; }
+    /* every scan launch takes exactly `tiles` tickets, so the ticket is always a multiple of
+     * the tile count between launches; another value would split a launch over two epochs */
+    const uint64_t tiles = (uint64_t)(h->C + SCAN_TILE - 1) / SCAN_TILE;
+    if (ticket % tiles != 0) {
+        set_err("mg_debug_set_scan_ticket: %llu is not a multiple of the %llu scan tiles",
+                (unsigned long long)ticket, (unsigned long long)tiles);
+        return MG_ERR_ARG;
+    }
+    unsigned long long t = ticket;
+    HIPCHK(hipMemcpyAsync(h->scan_ticket, &t, sizeof(t), hipMemcpyHostToDevice, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    return MG_OK;
 }
 
 int mg_set_genome(mg_handle* h, const MgGenome* g)

@@ -146,24 +146,30 @@ __global__ __launch_bounds__(1024) void k_scan(const uint32_t* in, uint32_t* out
  * Parallel Prefix Scan with Decoupled Look-back", 2016) for the cell counts of the fused
  * step. Tiles of 4096 counts (256 threads x 16); each tile publishes its aggregate, then
  * walks back over predecessors' published aggregates / inclusive prefixes until it meets an
- * inclusive one. Tile ids come from a monotonic ticket taken at block start, so a tile only
- * ever waits on tiles that started before it (no deadlock), and the launch number
+ * inclusive one. Tile ids come from a monotonic 64-bit ticket taken at block start, so a tile
+ * only ever waits on tiles that started before it (no deadlock), and the launch number
  * (ticket / tiles) tags every status word, so status words from earlier launches read as
- * "not ready" without any reset (also under hipGraph replay).
+ * "not ready" without any reset (also under hipGraph replay). The ticket is 64 bits wide so
+ * that it never wraps: a 32-bit ticket would, after 2^32 tiles, split a launch over two
+ * epochs whenever the tile count does not divide 2^32. The tag keeps the launch number's low
+ * 30 bits; a reader only ever meets the previous launch's words or its own launch's, since
+ * every launch rewrites every tile's word.
  * status word: [63:34] launch epoch, [33:32] flag (1 aggregate, 2 inclusive), [31:0] value. */
 #define SCAN_TILE 4096
 __device__ __forceinline__ void scan_lookback_body(const uint32_t* __restrict__ in,
                                                    uint32_t* __restrict__ out, int n,
-                                                   unsigned long long* status, uint32_t* ticket,
-                                                   uint32_t* ctr, int gsm, uint32_t tiles)
+                                                   unsigned long long* status,
+                                                   unsigned long long* ticket, uint32_t* ctr,
+                                                   int gsm, uint32_t tiles)
 {
-    __shared__ uint32_t s_ticket, s_excl;
+    __shared__ unsigned long long s_ticket;
+    __shared__ uint32_t s_excl;
     __shared__ uint32_t wsum[4];
     const int t = threadIdx.x;
-    if (t == 0) s_ticket = atomicAdd(ticket, 1u);
+    if (t == 0) s_ticket = atomicAdd(ticket, 1ull);
     __syncthreads();
-    const uint32_t tile = s_ticket % tiles;
-    const unsigned long long epoch = (unsigned long long)(s_ticket / tiles + 1u) & 0x3FFFFFFFull;
+    const uint32_t tile = (uint32_t)(s_ticket % tiles);
+    const unsigned long long epoch = (s_ticket / tiles + 1ull) & 0x3FFFFFFFull;
     const int base = (int)tile * SCAN_TILE + t * 16;
     uint32_t v[16];
     if (base + 15 < n) {
@@ -242,7 +248,8 @@ __device__ __forceinline__ void scan_lookback_body(const uint32_t* __restrict__ 
 __global__ __launch_bounds__(256) void k_scan_lookback(const uint32_t* __restrict__ in,
                                                        uint32_t* __restrict__ out, int n,
                                                        unsigned long long* status,
-                                                       uint32_t* ticket, uint32_t* ctr, int gsm)
+                                                       unsigned long long* ticket, uint32_t* ctr,
+                                                       int gsm)
 {
     scan_lookback_body(in, out, n, status, ticket, ctr, gsm, gridDim.x);
 }
@@ -2209,7 +2216,7 @@ struct BatchInst {
     const uint32_t* scan_in;
     uint32_t* scan_out;
     unsigned long long* scan_status;
-    uint32_t* scan_ticket;
+    unsigned long long* scan_ticket;
     uint32_t* ctr;
     int C, gsm, genes;
     float vmax, dt;

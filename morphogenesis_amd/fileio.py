@@ -217,12 +217,14 @@ def write_sim_params(sp: SimParams, directory: str) -> str:
     return path
 
 
-def fluid_setup(sp: SimParams, actuation_factor: float = 0.0, actuation_period: float = 1.0):
+def fluid_setup_args(sp: SimParams, actuation_factor: float = 0.0, actuation_period: float = 1.0):
     """The set-up ReadPointsCSV2 runs before it reads particles (src/fileCL_IO.cpp:351-366):
     PGRIDSIZE = 2*PSMOOTHRADIUS/PGRID_DENSITY, SetupSPH_Kernels, SetupSpacing
-    (src/fluid_system.cpp:2521-2546), SetupGrid (src/fluid_system.cpp:2073-2131) and
-    UpdateParams -> FluidParamCL (src/fluid_system.cpp:612-628), in float32 host arithmetic.
-    Returns (MgParams, MgGrid, SimParams with the derived entries updated)."""
+    (src/fluid_system.cpp:2521-2546), then the argument lists it hands to SetupGrid
+    (src/fluid_system.cpp:2073-2131) and, through UpdateParams, to FluidParamCL
+    (src/fluid_system.cpp:612-628), in float32 host arithmetic. Returns (SetupGrid arguments
+    (vmin, vmax, sim_scale, cell_size, smooth_radius), FluidParamCL arguments, SimParams with
+    the derived entries updated)."""
     f = np.float32
     out = sp.copy()
     P = out.param
@@ -231,7 +233,7 @@ def fluid_setup(sp: SimParams, actuation_factor: float = 0.0, actuation_period: 
     P["PGRIDSIZE"] = float(gridsize)
     mass, rest = f(P["PMASS"]), f(P["PRESTDENSITY"])
     with np.errstate(divide="ignore", invalid="ignore"):
-        pdist = np.power(f(mass / rest), f(f(1) / f(3)))
+        pdist = F.powf(f(mass / rest), f(f(1) / f(3)))
     if f(P["PSPACING"]) == 0:
         # spacing from density
         spacing = f(f(pdist * f(0.87)) / ss)
@@ -239,7 +241,7 @@ def fluid_setup(sp: SimParams, actuation_factor: float = 0.0, actuation_period: 
     else:
         # density from spacing
         pdist = f(f(f(P["PSPACING"]) * ss) / f(0.87))
-        rest = f(mass / np.power(pdist, f(3)))
+        rest = f(mass / F.powf(pdist, 3))
     P["PDIST"] = float(pdist)
     P["PRESTDENSITY"] = float(rest)
     off = 2.0 * float(f(gridsize / ss))           # float quotient, doubled in double
@@ -249,15 +251,23 @@ def fluid_setup(sp: SimParams, actuation_factor: float = 0.0, actuation_period: 
     bmax = [float(f(float(f(v)) - off)) for v in vmax]
     out.vec["PBOUNDMIN"] = tuple(bmin)
     out.vec["PBOUNDMAX"] = tuple(bmax)
-    grid = F.grid_setup(vmin, vmax, float(ss), float(gridsize), float(sr))
+    grid_args = (tuple(vmin), tuple(vmax), float(ss), float(gridsize), float(sr))
     gdir = out.vec["PPLANE_GRAV_DIR"]
     g = [float(f(f(c) * f(P["PGRAV"]))) for c in gdir]
-    params = F.params_from_fluid_param(
-        float(ss), float(sr), P["PRADIUS"], P["PMASS"], float(rest), bmin, bmax,
+    param_args = (
+        float(ss), float(sr), P["PRADIUS"], P["PMASS"], float(rest), tuple(bmin), tuple(bmax),
         P["PEXTSTIFF"], P["PINTSTIFF"], P["PVISC"], P["PSURFACE_TENSION"], P["PEXTDAMP"],
         P["PFORCE_MIN"], P["PFORCE_MAX"], P["PFORCE_FREQ"], P["PGROUND_SLOPE"], g[0], g[1], g[2],
         P["PACCEL_LIMIT"], P["PVEL_LIMIT"], actuation_factor, actuation_period)
-    return params, grid, out
+    return grid_args, param_args, out
+
+
+def fluid_setup(sp: SimParams, actuation_factor: float = 0.0, actuation_period: float = 1.0):
+    """fluid_setup_args, then SetupGrid and FluidParamCL through the library's host derivation
+    (mg_derive_grid / mg_derive_params). Returns (MgParams, MgGrid, SimParams with the derived
+    entries updated)."""
+    grid_args, param_args, out = fluid_setup_args(sp, actuation_factor, actuation_period)
+    return F.params_from_fluid_param(*param_args), F.grid_setup(*grid_args), out
 
 
 # ---------------------------------------------------------------------------- genome

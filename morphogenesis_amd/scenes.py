@@ -54,7 +54,7 @@ def rest_density(mass: float, spacing: float, ss: float = 1.0) -> float:
     """SetupSpacing (src/fluid_system.cpp:2531-2535): density from spacing."""
     f = np.float32
     pdist = f(f(spacing) * f(ss) / f(0.87))
-    return float(f(f(mass) / f(np.power(pdist, f(3.0)))))
+    return float(f(f(mass) / F.powf(pdist, 3.0)))
 
 
 def make_params(vmin, vmax, spacing=0.5, **over) -> F.MgParams:

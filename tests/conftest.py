@@ -12,12 +12,24 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP library)")
 
 
-@pytest.fixture(scope="session")
-def built():
-    """Build the oracle (and the HIP library when missing) once per session. On the GPU box
-    the prebuilt in-tree libraries are used as shipped."""
+def _ensure_built():
+    """Build the oracle and the HIP library when either is missing (hipcc cross-compiles for
+    gfx950 without a GPU). On the GPU box the prebuilt in-tree libraries are used as shipped."""
+    from morphogenesis_amd import build as mb
     from oracle import build as ob
 
     if not os.path.exists(os.path.join(REPO, "oracle", "lib", "libmg_oracle.so")):
         ob.build_oracle()
+    if not os.path.exists(os.path.join(REPO, "morphogenesis_amd", "lib", "libmg_sph.so")):
+        mb.build_library()
+
+
+def pytest_sessionstart(session):
+    # scenes derive their parameters and grids through the library's host entry points
+    _ensure_built()
+
+
+@pytest.fixture(scope="session")
+def built():
+    _ensure_built()
     return True

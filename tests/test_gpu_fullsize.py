@@ -1,0 +1,122 @@
+"""Correctness at the benchmark configurations' own sizes, bit for bit against the FP32 oracle
+(or, for the batch, against each instance stepped alone):
+
+  tissue_262k       262,144 particles, B = 32, genes on: first sort, then 100 steps (the
+                    north star's 100-step bar), compared every 25 steps;
+  mixed_4m          4,194,304 particles (bonded slab + fluid column with dense droplets):
+                    first sort, then 2 steps;
+  tissue_batch_256  256 bonded tissues of log-uniform size (4.58M particles): 4 batched
+                    steps (hipGraph) of every instance against the same instance stepped
+                    alone, and the smallest and largest instance against the oracle.
+
+Time budget on one MI355X box with 16 host threads for the oracle: about 40 s, 60 s and 60 s
+(most of it the oracle and the scene builders); each test carries its own limit."""
+import sys
+
+import numpy as np
+import pytest
+
+from morphogenesis_amd import _ffi as F
+from morphogenesis_amd import scenes
+from tests.helpers import assert_bits_equal
+
+pytestmark = pytest.mark.gpu
+
+STATE = [F.FPOS, F.FVEL, F.FVEVAL, F.FPRESS, F.FDENSITY, F.FCONC, F.FEPIGEN, F.FELASTIDX,
+         F.FGCELL, F.FBIN_COUNT, F.FBIN_OFFSET, F.FPARTICLE_ID]
+SORT = [F.FBIN_COUNT, F.FBIN_OFFSET, F.FGCELL, F.FGNDX, F.FPARTICLE_ID, F.FELASTIDX, F.FPARTICLEIDX]
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def pair(scene):
+    from morphogenesis_amd.fluid_system import FluidSystem
+    from oracle.mg_oracle import Oracle
+
+    g = FluidSystem(scene.n, scene.b)
+    o = Oracle(scene.n, scene.b)
+    g.load_scene(scene)
+    o.load_scene(scene)
+    return g, o
+
+
+def compare(g, o, what, bufs):
+    assert g.counts() == o.counts(), (what, g.counts(), o.counts())
+    for b in bufs:
+        assert_bits_equal(g.download(b), o.download(b), f"{what}: {F.BUFFER_NAMES[b]}")
+
+
+@pytest.mark.timeout(900)
+def test_headline_tissue_262k_100_steps(built):
+    sc = scenes.tissue(64, 32, seed=1)
+    say("case: tissue_262k x 100 steps", sc.n)
+    g, o = pair(sc)
+    for sim in (g, o):
+        sim.Run2PhysicalSort()
+    compare(g, o, "first sort", SORT)
+    for chunk in range(4):
+        g.step(25, sc.dt, sc.flags | F.STEP_GRAPH)
+        o.step(25, sc.dt, sc.flags)
+        say("  steps", 25 * (chunk + 1))
+        compare(g, o, f"after {25 * (chunk + 1)} steps", STATE)
+    g.close()
+
+
+@pytest.mark.timeout(900)
+def test_mixed_4m_first_sort_and_2_steps(built):
+    sc = scenes.mixed(4194304, 32, seed=2)
+    say("case: mixed_4m", sc.n, "slab", sc.info["n_slab"], "droplets", sc.info["n_droplets"])
+    g, o = pair(sc)
+    for sim in (g, o):
+        sim.Run2PhysicalSort()
+    compare(g, o, "mixed_4m first sort", SORT)
+    cnt = g.download(F.FBIN_COUNT)
+    assert cnt.max() >= 64, "the scene should reach dense cells"
+    g.step(2, sc.dt, sc.flags | F.STEP_GRAPH)
+    o.step(2, sc.dt, sc.flags)
+    compare(g, o, "mixed_4m 2 steps", STATE)
+    g.close()
+    del o
+
+
+@pytest.mark.timeout(900)
+def test_batch_256_instances(built):
+    from morphogenesis_amd.batch import Batch
+    from morphogenesis_amd.fluid_system import FluidSystem
+    from oracle.mg_oracle import Oracle
+
+    scs = scenes.tissue_batch(256, 32, seed=3)
+    total = sum(sc.n for sc in scs)
+    say("case: tissue_batch_256", len(scs), "instances,", total, "particles")
+    solo, inb = [], []
+    for sc in scs:
+        for lst in (solo, inb):
+            s = FluidSystem(sc.n, sc.b)
+            s.load_scene(sc)
+            s.Run2PhysicalSort()
+            lst.append(s)
+    bat = Batch(inb)
+    dt, fl = scs[0].dt, scs[0].flags
+    bat.step(4, dt, fl | F.STEP_GRAPH)
+    bat.sync()
+    for s in solo:
+        s.step(4, dt, fl)
+    bufs = [F.FPOS, F.FVEL, F.FVEVAL, F.FPRESS, F.FDENSITY, F.FCONC, F.FEPIGEN, F.FELASTIDX,
+            F.FGCELL, F.FBIN_COUNT, F.FBIN_OFFSET]
+    for k, (a, b) in enumerate(zip(solo, inb)):
+        assert a.counts() == b.counts(), k
+        for buf in bufs:
+            assert_bits_equal(b.download(buf), a.download(buf), f"instance {k}: {F.BUFFER_NAMES[buf]}")
+    sizes = [sc.n for sc in scs]
+    for k in (int(np.argmin(sizes)), int(np.argmax(sizes))):
+        o = Oracle(scs[k].n, scs[k].b)
+        o.load_scene(scs[k])
+        o.Run2PhysicalSort()
+        o.step(4, dt, fl)
+        compare(inb[k], o, f"batched instance {k} ({scs[k].n} particles) vs oracle", STATE)
+    bat.close()
+    for s in solo + inb:
+        s.close()
