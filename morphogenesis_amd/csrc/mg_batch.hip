@@ -94,6 +94,7 @@ void fill_inst(BatchInst& t, mg_handle* h, const MgGenome* dgen, float dt, unsig
     t.diffuse = make_ds(h, 1, 0);
     t.diffuse.GMN = h->GMN;
     t.diffuse.cull = 0;
+    t.diffuse.HMR = t.pressure.HMW;
     t.place = make_ds(h);
     t.place.CNT = t.place.CNTN;
     t.place.OFF = t.place.OFFN;

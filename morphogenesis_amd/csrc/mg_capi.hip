@@ -863,9 +863,11 @@ bool graph_ok(const mg_handle* h, unsigned flags)
     return !(flags & MG_STEP_SPECIAL) || h->p.actuation_factor == 0.0f;
 }
 
-/* Hit masks for the fused step: the force sweep reuses the pressure sweep's neighbour sets
- * when both sweeps walk rows of the sorted grid (rows of srch <= 4 cells) and test the same
- * cut-off (pressure: dsq < r2, force: dsq < 1). ensure_hit_masks allocates the buffer outside
+/* Hit masks for the fused step: the force and diffusion sweeps reuse the pressure sweep's
+ * neighbour sets (recorded within a skin of 0.01 around the cut-off) when the sweeps walk rows
+ * of the sorted grid (rows of srch <= 4 cells) and share the cut-off (pressure: dsq < r2,
+ * force: dsq < 1, diffusion: dsq < r2 at the advanced positions, while no particle moved
+ * farther than the skin allows). ensure_hit_masks allocates the buffer outside
  * any stream capture (a new buffer bumps the epoch, so captured graphs are rebuilt);
  * hit_masks returns it when the current state may use it. */
 bool hm_eligible(const mg_handle* h) { return h->adj_rows && h->p.r2 == 1.0f && h->g.srch <= 4; }
@@ -922,6 +924,7 @@ int do_fused_step(mg_handle* h, float dt, unsigned flags, const mg_net* net = nu
         DS d = make_ds(h, 1, 0);
         d.GMN = h->GMN;
         d.cull = 0; /* positions moved since the binning */
+        d.HMR = hm; /* within the skin, unless the force-advance raised MG_C_SKIN */
         const DiffPlan dp = diff_plan(h);
         if (dp.ng <= 1)
             hipLaunchKernelGGL(k_diffuse_genes_insert<1>, dim3(nb), dim3(256), band_lds<1>(256), h->st, d, h->p, h->g, h->gen, dp, genes);

@@ -33,8 +33,22 @@ enum {
     MG_C_SCAN_NUM = 5,    /* FBIN_OFFSET[C-1] of the last scan (committed by the sort) */
     MG_C_SCAN_ACTIVE = 6, /* FBIN_OFFSET[grid_ScanMax] of the last scan               */
     MG_C_ERR = 8,         /* nonzero: a kernel gave up a bounded wait (result invalid)  */
+    MG_C_SKIN = 9,        /* nonzero: a particle of this fused step moved farther than
+                           * the hit masks' skin allows (the diffusion sweep tests its
+                           * candidates itself)                                        */
     MG_C_COUNT = 16
 };
+
+/* Hit-mask skin. The pressure sweep records the candidates with dsq < MG_SKIN_LIM, i.e.
+ * within 1 + 0.01 of the particle, while the cut-off of the pressure, force and diffusion
+ * sums is 1. The pressure and force sweeps run on the same positions; the diffusion sweep
+ * runs after the advance, and a pair outside the skin can only come within the cut-off if
+ * the two particles moved 0.01 together. The fused force-advance raises MG_C_SKIN when any
+ * particle moves by more than sqrt(MG_SKIN_DISP2) = 0.004 in the step (two of them: 0.008,
+ * leaving 0.002 for rounding, far above the float rounding of positions and squared
+ * distances), and the diffusion sweep then tests its candidates instead of using the masks. */
+#define MG_SKIN_LIM 1.0201f
+#define MG_SKIN_DISP2 1.6e-5f
 
 struct DS {
     /* sorted, current */
@@ -69,10 +83,11 @@ struct DS {
     /* learned coarse step: per-slot velocity correction (xyz) and use flag (w != 0) added
      * by the advance; null on plain steps */
     const float4* DVc;
-    /* Hit masks handed from the pressure sweep to the force sweep of the same step (same
-     * positions, same cut-off): row s of slot i at [s * N + i], bit k = slot OFF[c0] + k is
-     * a neighbour; ~0 = the row's window exceeds 64 slots (the force tests that row). Null
-     * when not in use. */
+    /* Hit masks handed from the pressure sweep to the force sweep and the diffusion sweep of
+     * the same step: row s of slot i at [s * N + i], bit k = slot OFF[c0] + k is within the
+     * skin (dsq < MG_SKIN_LIM at the pressure sweep's positions); ~0 = the row's window
+     * exceeds 64 slots (the reader tests that row itself). Readers test dsq against their own
+     * cut-off for every recorded candidate. Null when not in use. */
     uint64_t* HMW;
     const uint64_t* HMR;
     uint32_t* ctr;

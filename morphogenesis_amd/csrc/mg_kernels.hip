@@ -665,7 +665,8 @@ __device__ __forceinline__ void cell_candidates(uint32_t lo, uint32_t hi, f3 pi,
 template <int K, int NX, int HM = 0, class XLoad, class Visit, class EndCell>
 __device__ __forceinline__ void sweep(const DS& d, const MgGrid& g, Band<NX>& wb, bool valid,
                                       uint32_t gc0, f3 pi, float lim, XLoad xload, Visit visit,
-                                      EndCell end_cell, uint32_t self = 0u)
+                                      EndCell end_cell, uint32_t self = 0u,
+                                      float rec_lim = 0.0f, bool hm_ok = true)
 {
     constexpr int NXX = NX > 0 ? NX : 1;
     const uint32_t lane = __lane_id();
@@ -721,10 +722,11 @@ __device__ __forceinline__ void sweep(const DS& d, const MgGrid& g, Band<NX>& wb
     bool fits = band_of(0, blo, bhi);
     fetch(fits, blo, bhi);
     RowBounds nxt = row_bounds(d, g, valid ? gc0 : cmin, 0);
-    /* HM == 1: record the hits of each row (pressure); HM == 2: visit the recorded hits
-     * instead of testing the candidates (force) */
+    /* HM == 1: record the candidates of each row within rec_lim (>= lim) and visit those
+     * within lim (pressure); HM == 2: test only the recorded candidates (force, diffusion) */
     const bool hm_w = HM == 1 && rows && d.HMW != nullptr;
-    const bool hm_r = HM == 2 && rows && d.HMR != nullptr;
+    const bool hm_r = HM == 2 && rows && d.HMR != nullptr && hm_ok;
+    const float tlim = hm_w ? fmaxf(rec_lim, lim) : lim; /* candidate test and culling */
     uint64_t hm_next = ~0ull;
     if (hm_r && valid) hm_next = d.HMR[self];
     for (int s = 0; s < nseg; s++) {
@@ -774,18 +776,18 @@ __device__ __forceinline__ void sweep(const DS& d, const MgGrid& g, Band<NX>& wb
                     end_cell();
                     cnext = next_bound(j);
                 }
-                float4 pj, ex[NXX];
+                const float4 pj = cfits ? wb.p[j - cblo] : d.P[j];
+                const f3 dist = pi - xyz(pj);
+                const float dsq = dist.x * dist.x + dist.y * dist.y + dist.z * dist.z;
+                if (!(dsq < lim && dsq > 0.0f)) continue; /* recorded within the skin only */
+                float4 ex[NXX];
                 if (cfits) {
-                    pj = wb.p[j - cblo];
 #pragma unroll
                     for (int x = 0; x < NX; x++) ex[x] = wb.x[x][j - cblo];
                 } else {
-                    pj = d.P[j];
 #pragma unroll
                     for (int x = 0; x < NX; x++) ex[x] = xload(j, x);
                 }
-                const f3 dist = pi - xyz(pj);
-                const float dsq = dist.x * dist.x + dist.y * dist.y + dist.z * dist.z;
                 visit(j, pj, dist, dsq, ex);
             }
             end_cell();
@@ -797,7 +799,7 @@ __device__ __forceinline__ void sweep(const DS& d, const MgGrid& g, Band<NX>& wb
         uint64_t rec = 0ull;
         auto visit_rec = [&](uint32_t j, float4 pj, f3 dist, float dsq, const float4* x) {
             if (!wide) rec |= 1ull << ((j - wlo) & 63u);
-            visit(j, pj, dist, dsq, x);
+            if (dsq < lim) visit(j, pj, dist, dsq, x);
         };
         float gyz = 0.0f;
         if (cull) {
@@ -810,17 +812,17 @@ __device__ __forceinline__ void sweep(const DS& d, const MgGrid& g, Band<NX>& wb
             bool live = k < ncell;
             if (cull && live) {
                 const float gx = axis_gap(fr.x, k - 1, inv.x);
-                live = gyz + gx * gx < lim;
+                live = gyz + gx * gx < tlim;
             }
             if (!live) continue;
             const uint32_t lo = rb.b[k], hi = max(rb.b[k + 1], lo);
             if (cfits) {
                 cell_candidates<K, NX>(
-                    lo, hi, pi, lim, [&](uint32_t j) { return wb.p[j - cblo]; },
+                    lo, hi, pi, tlim, [&](uint32_t j) { return wb.p[j - cblo]; },
                     [&](uint32_t j, int x) { return wb.x[x][j - cblo]; }, visit_rec);
             } else {
                 cell_candidates<K, NX>(
-                    lo, hi, pi, lim, [&](uint32_t j) { return d.P[j]; },
+                    lo, hi, pi, tlim, [&](uint32_t j) { return d.P[j]; },
                     [&](uint32_t j, int x) { return xload(j, x); }, visit_rec);
             }
             end_cell();
@@ -840,6 +842,8 @@ __device__ __forceinline__ void pressure_body(const DS& d, const MgParams& p, co
         for (uint32_t z = bid * blockDim.x + threadIdx.x; z < d.zero_n; z += nblk * blockDim.x)
             d.ZERO[z] = 0u;
     }
+    /* a new set of masks: the force-advance of this step raises the flag again if needed */
+    if (d.HMW && bid == 0 && threadIdx.x == 0) d.ctr[MG_C_SKIN] = 0u;
     const uint32_t i = bid * blockDim.x + threadIdx.x;
     uint32_t gc = (i < d.ctr[MG_C_ACTIVE]) ? d.GCELL[i] : MG_GRID_UNDEF;
     const bool valid = gc != MG_GRID_UNDEF;
@@ -868,7 +872,7 @@ __device__ __forceinline__ void pressure_body(const DS& d, const MgParams& p, co
                 b *= b;
                 csum += b * (4.0f * q + 1.0f);
             },
-            flush, i);
+            flush, i, MG_SKIN_LIM);
     } else {
         sweep<8, 0, 1>(
             d, g, bands[threadIdx.x >> 6], valid, gc, pi, p.r2, none,
@@ -880,7 +884,7 @@ __device__ __forceinline__ void pressure_body(const DS& d, const MgParams& p, co
                 b *= b;
                 csum += b * (4.0f * q + 1.0f);
             },
-            flush, i);
+            flush, i, MG_SKIN_LIM);
     }
     if (!valid) return;
     sum = sum * p.pmass * p.wendlandC2kern;
@@ -1213,6 +1217,12 @@ __device__ __forceinline__ void force_advance_body(const DS& d, const MgParams& 
         advance_one(p, dt, xyz(P_i), xyz(V_i), xyz(VEL_i), F, &np, &ve, &vn);
     }
     if (special) special_one(d, i, zoff, xyz(P_i), xyz(V_i), xyz(VEL_i), &np, &ve, &vn);
+    if (d.HMR) {
+        /* the masks serve the diffusion sweep on the advanced positions only while every
+         * particle stays inside half the skin (NaN and inf displacements raise the flag too) */
+        const f3 dx = np - xyz(P_i);
+        if (!(dot3(dx, dx) <= MG_SKIN_DISP2)) atomicOr(&d.ctr[MG_C_SKIN], 1u);
+    }
     d.P2[i] = make_float4(np.x, np.y, np.z, P_i.w);
     d.V2[i] = make_float4(ve.x, ve.y, ve.z, V_i.w);
     d.VEL2[i] = make_float4(vn.x, vn.y, vn.z, 0.0f);
@@ -1288,7 +1298,11 @@ __device__ __forceinline__ void diffuse_sweep(const DS& d, const MgParams& p, co
     }
 #pragma unroll
     for (int t = 0; t < 4 * NG; t++) nw[t] = 0.0f;
-    sweep<(NG >= 4 ? 1 : (NG >= 2 ? 2 : 4)), NG>(
+    /* the step's hit masks (d.HMR, recorded within the skin before the advance) stand for
+     * the candidate tests unless a particle moved too far (MG_C_SKIN): either way the visits
+     * are the candidates within r2 at the advanced positions, in the same order */
+    const bool hm_ok = d.HMR != nullptr && d.ctr[MG_C_SKIN] == 0u && r2 == 1.0f;
+    sweep<(NG >= 4 ? 1 : (NG >= 2 ? 2 : 4)), NG, 2>(
         d, g, wb, valid, gc, pi, r2,
         [&](uint32_t j, int q) { return ((const float4*)(d.CONC + (size_t)j * MG_NUM_TF))[dp.grp[q]]; },
         [&](uint32_t, float4, f3, float dsq, const float4* x) {
@@ -1298,7 +1312,7 @@ __device__ __forceinline__ void diffuse_sweep(const DS& d, const MgParams& p, co
             for (int t = 0; t < 4 * NG; t++)
                 if (dp.dm & (1u << t)) nw[t] += dp.D[t] * cc * (cn[t] - cur[t]);
         },
-        []() {});
+        []() {}, i, 0.0f, hm_ok);
 }
 
 /* contributeDiffusion/computeDiffusion, src/fluid_system_cuda.cu:3026-3105, as a Jacobi update

@@ -375,3 +375,23 @@ def test_smoothing_radius_not_a_power_of_two(built):
         sim.Run2PhysicalSort()
         sim.step(10, sc.dt, sc.flags)
     compare_all(g, o, "sr 0.75", [F.FPOS, F.FVEL, F.FPRESS, F.FDENSITY, F.FCONC])
+
+
+@pytest.mark.parametrize("speed", [0.0, 60.0])
+def test_diffusion_skin_masks_and_fallback(built, speed):
+    """The diffusion sweep of the fused step reads the pressure sweep's masks (recorded within a
+    skin of 0.01) when no particle moves more than 0.004 in the step, and tests its candidates
+    itself otherwise: a particle at 60 units/s (0.006 per step) takes the second path. Both
+    match the oracle bit for bit, through hipGraph replay."""
+    sc = scenes.tissue(n_side=12, b=16, seed=31)
+    if speed:
+        sc.buffers[F.FVEL][100, :3] = [speed, 0.0, 0.0]
+        sc.buffers[F.FVEVAL][100, :3] = [speed, 0.0, 0.0]
+    say("case: diffusion skin, fast particle speed", speed)
+    g, o = pair(sc)
+    for sim in (g, o):
+        sim.Run2PhysicalSort()
+    g.step(10, sc.dt, sc.flags | F.STEP_GRAPH)
+    o.step(10, sc.dt, sc.flags)
+    compare_all(g, o, f"skin speed {speed}", [F.FPOS, F.FVEL, F.FCONC, F.FEPIGEN, F.FPRESS, F.FDENSITY,
+                                               F.FGCELL, F.FBIN_OFFSET])
