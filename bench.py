@@ -259,74 +259,70 @@ def cpu_baseline_batch(scs, budget_s: float = 15.0) -> dict:
                       f"FP32 oracle (CPU restatement), {threads} threads"}
 
 
-def run_learned(args):
-    """The learned 4 x dt step on the 262,144-particle tissue: train the MLP on the GPU from
-    fine/coarse pairs along a fine trajectory of a second tissue (other seed), then from the
-    same start run 4K fine steps of dt, K learned steps of 4 dt and K plain steps of 4 dt
-    (the ablation) and compare positions by particle ID. `value` counts a learned step as the
-    4 fine steps it replaces (fine-equivalent particle-steps/s)."""
+def learned_measurement(coarse_steps: int = 1000, progress=None) -> dict:
+    """The learned 4 x dt step on the 262,144-particle tissue. Training data: 200 fine
+    trajectory segments (4 steps of dt each) along the fine trajectories of four other 262k
+    tissues (seeds 21-24, 50 start states each, 80 fine steps apart, 65,536 random rows per
+    segment); Adam on the GPU. Then from the sorted start state of the seed-1 tissue:
+    4 K fine steps of dt, K learned steps of 4 dt and K plain steps of 4 dt (the ablation),
+    positions compared by particle ID."""
     from morphogenesis_amd import scenes
     from morphogenesis_amd.learned import Net, make_training_data, rollout_errors
 
+    say = progress or (lambda msg: None)
+    t0 = time.perf_counter()
+    train = [scenes.tissue(64, 32, seed=s) for s in (21, 22, 23, 24)]
+    data = make_training_data(train, samples=50, ratio=4, gap=80, rows_per_sample=65536)
+    del train
+    gen_s = time.perf_counter() - t0
+    say(f"training data: {data.segments} segments, {data.X.shape[0]} rows in {gen_s:.1f} s")
+    net = Net(seed=5)
+    t1 = time.perf_counter()
+    loss = net.fit(data, steps=3000, batch=16384, lr=1e-3)
+    train_s = time.perf_counter() - t1
+    say(f"trained in {train_s:.1f} s, loss {loss[:5].mean():.3f} -> {loss[-100:].mean():.3f}")
+    rows = int(data.X.shape[0])
+    del data
+    sc = scenes.tissue(64, 32, seed=1)
+    r = rollout_errors(sc, net, coarse_steps, ratio=4)
+    net.close()
+    return {"speedup_vs_fine": r["speedup_vs_fine"],
+            "speedup_plain_coarse_vs_fine": r["fine_s"] / r["coarse_s"],
+            "rel_error_vs_fine_disp": r["learned_rel_disp"],
+            "rel_error_vs_fine_pos": r["learned_rel_pos"],
+            "ablation_plain_coarse_rel_error_disp": r["coarse_rel_disp"],
+            "ablation_plain_coarse_rel_error_pos": r["coarse_rel_pos"],
+            "error_ratio_learned_over_plain": r["learned_rel_disp"] / max(r["coarse_rel_disp"], 1e-30),
+            "fallback_particle_steps": r["fallbacks"], "coarse_steps": coarse_steps,
+            "fine_s": r["fine_s"], "learned_s": r["learned_s"], "coarse_s": r["coarse_s"],
+            "train_segments": 200, "train_rows": rows, "train_data_s": gen_s, "train_s": train_s,
+            "fallback_rule": "normalised input beyond 6 sigma of the training data, or a drift correction moving the particle more than 1e-3 in the step",
+            "train_loss_first": float(loss[:5].mean()), "train_loss_last": float(loss[-100:].mean()),
+            "error_norm": "|x - x_fine| / |x_fine - x_0| over all particles (2-norms), by particle ID",
+            "particles": sc.n, "dt_fine": sc.dt, "dt_coarse": 4 * sc.dt}
+
+
+def run_learned(args):
+    """The learned workload as its own line: `value` counts a learned step as the 4 fine
+    steps it replaces (fine-equivalent particle-steps/s)."""
     def progress(msg):
         print(f"[bench learned] {msg}", file=sys.stderr, flush=True)
 
-    K = args.steps
-    t0 = time.perf_counter()
-    train = scenes.tissue(64, 32, seed=21)
-    data = make_training_data(train, samples=50, ratio=4, gap=80, warm=0)
-    progress(f"training data {data.X.shape} in {time.perf_counter() - t0:.1f} s")
-    net = Net(seed=5)
-    t1 = time.perf_counter()
-    loss = net.fit(data, steps=3000, batch=16384, lr=1e-3, in_limit=12.0, out_limit=12.0)
-    train_s = time.perf_counter() - t1
-    progress(f"trained in {train_s:.1f} s, loss {loss[:5].mean():.3f} -> {loss[-100:].mean():.3f}")
-    sc = scenes.tissue(64, 32, seed=1)
-    r = rollout_errors(sc, net, K, ratio=4)
+    lm = learned_measurement(args.steps, progress)
+    n = lm["particles"]
     out = {
-        "metric": METRIC, "value": sc.n * 4 * K / r["learned_s"], "unit": "particle-steps/s",
-        "n_gpus": 1, "steps": K, "warmup": 0, "ms_per_step": r["learned_s"] / K * 1e3,
+        "metric": METRIC, "value": n * 4 * args.steps / lm["learned_s"], "unit": "particle-steps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": 0, "ms_per_step": lm["learned_s"] / args.steps * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16+fp32",
         "data": "synthetic",
-        "config": {"workload": args.workload, "particles": sc.n, "bonds_per_particle": 32,
-                   "dt_fine": sc.dt, "dt_coarse": 4 * sc.dt, "coarse_steps": K, "hipgraph": True,
-                   "mlp": "50(64)-128-128-3(16) SiLU, bf16 MFMA, fp32 accumulate",
+        "config": {"workload": args.workload, "particles": n, "bonds_per_particle": 32,
+                   "dt_fine": lm["dt_fine"], "dt_coarse": lm["dt_coarse"], "coarse_steps": args.steps,
+                   "hipgraph": True,
+                   "mlp": "53(64)-128-128-6(16) SiLU, bf16 MFMA, fp32 accumulate; outputs: drift and veval corrections",
                    "value_counts": "each learned step as 4 fine-equivalent steps"},
-        "learned": {"speedup_vs_fine": r["speedup_vs_fine"],
-                    "speedup_plain_coarse_vs_fine": r["fine_s"] / r["coarse_s"],
-                    "rel_error_vs_fine_pos": r["learned_rel_pos"],
-                    "rel_error_vs_fine_disp": r["learned_rel_disp"],
-                    "ablation_plain_coarse_rel_error_pos": r["coarse_rel_pos"],
-                    "ablation_plain_coarse_rel_error_disp": r["coarse_rel_disp"],
-                    "fallback_particle_steps": r["fallbacks"], "fine_s": r["fine_s"],
-                    "learned_s": r["learned_s"], "coarse_s": r["coarse_s"],
-                    "train_rows": int(data.X.shape[0]), "train_s": train_s,
-                    "train_loss_first": float(loss[:5].mean()), "train_loss_last": float(loss[-100:].mean()),
-                    "zero_prediction_loss": 3.0},
+        "learned": lm,
     }
-    net.close()
     print(json.dumps(out))
-
-
-def roofline(dom: str, scene, stats: dict, dom_ms: float, workload: str) -> dict:
-    """The dominant kernel against HBM bandwidth. `achieved` counts the compulsory bytes of
-    the reference's layout for the work the kernel does (reference_layout_bytes, terms
-    listed); `layout` counts the compulsory bytes of this build's own layout
-    (algorithmic_bytes), which move far fewer bytes per particle; `traffic` is the measured
-    HBM traffic per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated)."""
-    ref_bytes, terms = reference_layout_bytes(dom, scene, stats)
-    lay = algorithmic_bytes(dom, scene, stats)
-    ach = ref_bytes / (dom_ms * 1e-3) / 1e9
-    lay_ach = lay / (dom_ms * 1e-3) / 1e9
-    return {"bound": "hbm", "kernel": KERNEL_OF_PHASE[dom], "achieved": ach, "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-            "traffic": load_traffic(KERNEL_OF_PHASE[dom], workload),
-            "algorithmic_bytes_per_launch": ref_bytes,
-            "algorithmic_basis": "reference layout, per particle: " +
-                                 ", ".join(f"{k} {v:.1f} B" for k, v in terms.items()),
-            "avg_launch_ms": dom_ms,
-            "layout": {"bytes_per_launch": lay, "achieved": lay_ach, "frac": lay_ach / HBM_PEAK_GBS,
-                       "basis": "this build's HBM layout, every element once"}}
 
 
 def load_traffic(kernel: str, workload: str):
@@ -485,6 +481,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--timed-steps", type=int, default=20)
     ap.add_argument("--no-graph", action="store_true", help="launch the step kernels eagerly")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="leave out the learned 4 x dt measurement of the default line")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run a plain device copy of this size (traffic counter calibration)")
     args = ap.parse_args()
@@ -593,6 +591,10 @@ def main():
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(scene)
+    if args.workload == "tissue_262k" and not args.no_learned:
+        sim.close()
+        out["learned"] = learned_measurement(1000, lambda msg: print(f"[bench learned] {msg}",
+                                                                      file=sys.stderr, flush=True))
     print(json.dumps(out))
 
 

@@ -191,13 +191,15 @@ int mg_batch_step_timed(mg_batch* b, int nsteps, float dt, unsigned flags, doubl
                         int* calls_out);
 
 /* ---- learned coarse step (no counterpart in the reference) ----
- * An MLP (50 inputs: 8 bonded neighbours' relative position and velocity, density,
- * pressure; two hidden layers of 128 SiLU units; 3 outputs) predicts for each particle the
- * velocity correction that four fine steps would add to one coarse step, and
- * mg_step_learned applies it inside the fused step's advance. BF16 weights, FP32
- * accumulation on MFMA. Parameter vector (fp32, mg_net_nparams() floats, row-major
- * [in][out]): W1[64][128] b1[128] W2[128][128] b2[128] W3[128][16] b3[16]; inputs 50..63
- * and outputs 3..15 are zero padding. */
+ * An MLP (53 inputs: 8 bonded neighbours' relative position and velocity, density, pressure
+ * and the coarse step's acceleration; two hidden layers of 128 SiLU units; 6 outputs)
+ * predicts for each particle two velocity corrections for one coarse step, each the value
+ * four fine steps of dt / 4 reach minus the plain coarse step's: the drift c_d (the position
+ * moves by (v_next + c_d) dt) and veval c_e ((v + v_next) / 2 + c_e is stored); the velocity
+ * stays v_next. mg_step_learned runs the step's pressure and force passes, the network, then
+ * the advance. BF16 weights, FP32 accumulation on MFMA. Parameter vector (fp32,
+ * mg_net_nparams() floats, row-major [in][out]): W1[64][128] b1[128] W2[128][128] b2[128]
+ * W3[128][16] b3[16]; inputs 53..63 and outputs 6..15 are zero padding. */
 typedef struct mg_net mg_net;
 size_t mg_net_nparams(void);
 /* Xavier-uniform weights from a seed, zero biases, identity normalisation. */
@@ -205,12 +207,13 @@ int mg_net_create(mg_net** out, uint64_t seed);
 int mg_net_destroy(mg_net* n);
 int mg_net_get_params(mg_net* n, float* out, size_t count);
 int mg_net_set_params(mg_net* n, const float* in, size_t count);
-/* Input i is used as (x_i - mu[i]) * inv_sigma[i] (64 entries each); the correction is
- * y * out_scale. A particle whose normalised input exceeds in_limit or whose |y| exceeds
- * out_limit in magnitude takes the plain step (counted by mg_learned_fallbacks). */
-int mg_net_set_norm(mg_net* n, const float* mu, const float* inv_sigma, float out_scale,
+/* Input i is used as (x_i - mu[i]) * inv_sigma[i] (64 entries each); output c is
+ * y_c * out_scale[c] (6 entries). A particle whose normalised input exceeds in_limit, whose
+ * |y_c| exceeds out_limit, or whose drift correction would move it more than 1e-3 (world
+ * units) in the step takes the plain step (counted by mg_learned_fallbacks). */
+int mg_net_set_norm(mg_net* n, const float* mu, const float* inv_sigma, const float* out_scale,
                     float in_limit, float out_limit);
-/* Training set: X rows x 64 raw inputs, T rows x 3 targets (in output units, i.e. already
+/* Training set: X rows x 64 raw inputs, T rows x 6 targets (in output units, i.e. already
  * divided by out_scale), mask rows (0 = ignore the row). */
 int mg_net_load_data(mg_net* n, const float* X, const float* T, const float* mask, size_t rows);
 /* `steps` Adam updates (lr, beta 0.9 / 0.999) on minibatches of `batch` consecutive rows of
@@ -219,15 +222,17 @@ int mg_net_train(mg_net* n, int steps, size_t batch, float lr, uint64_t seed, fl
 /* One forward and backward pass over training rows [row0, row0 + rows): gradients
  * (mg_net_nparams() floats) and the mean loss; no update. */
 int mg_net_grad(mg_net* n, size_t row0, size_t rows, float* grads_out, float* loss_out);
-/* Network outputs y (3 per row, before out_scale) of training rows [row0, row0 + rows). */
+/* Network outputs y (6 per row, before out_scale) of training rows [row0, row0 + rows). */
 int mg_net_forward(mg_net* n, size_t row0, size_t rows, float* y_out);
 /* Raw inputs of every slot of the handle's current sorted state (N x 64; zero rows for
- * inactive slots); density and pressure are read as the last pressure pass left them. */
+ * inactive slots), after the pressure and force passes a learned step runs first (the force
+ * pass changes no bond record; the state stays as a step finds it). Needs the sorted state. */
 int mg_net_features(mg_handle* h, float* X_out);
-/* The correction a learned step would apply to the current state: N x 4 floats (dv, used). */
-int mg_net_infer(mg_handle* h, mg_net* n, float* DV_out);
-/* nsteps learned steps of dt (the coarse step): mg_step with the correction added to each
- * active particle's new velocity before its position update. */
+/* The corrections a learned step of length dt would apply to the current sorted state:
+ * N x 8 floats (c_d, used, c_e, 0). */
+int mg_net_infer(mg_handle* h, mg_net* n, float dt, float* DV_out);
+/* nsteps learned steps of dt (the coarse step): mg_step with each active particle's
+ * corrections applied in its advance. Needs the sorted state. */
 int mg_step_learned(mg_handle* h, mg_net* n, int nsteps, float dt, unsigned flags);
 /* Particles that took the plain step inside learned steps since the last call. */
 int mg_learned_fallbacks(mg_handle* h, uint64_t* count);

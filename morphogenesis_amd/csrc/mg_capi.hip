@@ -415,6 +415,7 @@ DS make_ds(mg_handle* h, int kin = 0, int conc = 0)
     d.DVc = nullptr;
     d.HMW = nullptr;
     d.HMR = nullptr;
+    d.dry = 0;
     d.ctr = h->ctr;
     d.N = h->N; d.B = h->B; d.C = h->C;
     return d;
@@ -842,7 +843,7 @@ int materialize_gene_grid(mg_handle* h)
 
 /* learned steps (mg_learn.hip) */
 struct mg_net;
-int launch_mlp_infer(mg_handle* h, const mg_net* net);
+int launch_mlp_infer(mg_handle* h, const mg_net* net, float dt);
 uint64_t net_version(const mg_net* net);
 
 namespace {
@@ -905,17 +906,32 @@ int do_fused_step(mg_handle* h, float dt, unsigned flags, const mg_net* net = nu
         d.HMW = hm;
         hipLaunchKernelGGL(k_pressure, dim3(nb), dim3(256), band_lds<0>(256), h->st, d, h->p, h->g);
     }
+    const int special = (flags & MG_STEP_SPECIAL) ? 1 : 0;
+    const float zoff = special ? actuation_offset(h, dt) : 0.0f;
     if (net) {
-        PhaseScope ps(h, MG_PH_LEARN);
-        CHK(launch_mlp_infer(h, net));
-    }
-    {
+        /* learned coarse step: the network reads the step's acceleration, so the force pass
+         * (FFORCE = 0 + force), the inference and the advance are separate launches */
+        {
+            PhaseScope ps(h, MG_PH_FORCE);
+            DS d = make_ds(h);
+            d.HMR = hm;
+            hipLaunchKernelGGL(k_force, dim3(nb), dim3(256), band_lds<1>(256), h->st, d, h->p, h->g, 0);
+        }
+        {
+            PhaseScope ps(h, MG_PH_LEARN);
+            CHK(launch_mlp_infer(h, net, dt));
+        }
+        {
+            PhaseScope ps(h, MG_PH_ADVANCE);
+            DS d = make_ds(h);
+            d.DVc = h->DV;
+            d.HMR = hm;
+            hipLaunchKernelGGL(k_advance_learned, dim3(nb), dim3(256), 0, h->st, d, h->p, dt, special, zoff);
+        }
+    } else {
         PhaseScope ps(h, MG_PH_FORCE);
         DS d = make_ds(h);
-        d.DVc = net ? h->DV : nullptr;
         d.HMR = hm;
-        const int special = (flags & MG_STEP_SPECIAL) ? 1 : 0;
-        const float zoff = special ? actuation_offset(h, dt) : 0.0f;
         hipLaunchKernelGGL(k_force_advance, dim3(nb), dim3(256), band_lds<1>(256), h->st, d, h->p, h->g, dt,
                            special, zoff);
     }

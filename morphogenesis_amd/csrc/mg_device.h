@@ -80,8 +80,8 @@ struct DS {
     /* 1 when the slots are sorted by cell (CNT / OFF / GCELL of the last sort) and the
      * positions have not changed since: sweeps may skip cells out of reach */
     int cull;
-    /* learned coarse step: per-slot velocity correction (xyz) and use flag (w != 0) added
-     * by the advance; null on plain steps */
+    /* learned coarse step: per slot two float4 corrections (drift c_d with the use flag in w,
+     * veval c_e) applied by the advance; null on plain steps */
     const float4* DVc;
     /* Hit masks handed from the pressure sweep to the force sweep and the diffusion sweep of
      * the same step: row s of slot i at [s * N + i], bit k = slot OFF[c0] + k is within the
@@ -90,6 +90,9 @@ struct DS {
      * cut-off for every recorded candidate. Null when not in use. */
     uint64_t* HMW;
     const uint64_t* HMR;
+    /* 1: a force pass that changes no bond record (no integrators, no breaking), for the
+     * learned step's training inputs */
+    int dry;
     uint32_t* ctr;
     int N, B, C;
 };

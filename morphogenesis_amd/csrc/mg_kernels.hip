@@ -1012,12 +1012,14 @@ __device__ __forceinline__ f3 force_one(const DS& d, const MgParams& p, const Mg
             const size_t bi = (size_t)(qc + k) * N + h;
             float strain, abs_dist;
             f3 eterm = bond_term(ipos, ivev, xyz(pj[k]), xyz(vj[k]), bpc[k], p.pmass, &strain, &abs_dist);
-            float2 v = inc[k];
-            v.x = v.x + strain * strain;
-            v.y = v.y + strain;
-            d.BINT[bi] = v;
+            if (!d.dry) {
+                float2 v = inc[k];
+                v.x = v.x + strain * strain;
+                v.y = v.y + strain;
+                d.BINT[bi] = v;
+            }
             if (!has_nan3(eterm)) force = force - eterm;
-            if (abs_dist >= bpc[k].x && !freeze) ((float*)&d.BPAR[bi])[1] = 0.0f;
+            if (abs_dist >= bpc[k].x && !freeze && !d.dry) ((float*)&d.BPAR[bi])[1] = 0.0f;
         }
     }
     force = force + fsum * p.pmass;
@@ -1083,11 +1085,11 @@ __global__ LAUNCH_BOUNDS_SWEEP void k_force(DS d, MgParams p, MgGrid g, int accu
 }
 
 /* ------------------------------------------------------------- advance (a11)
- * advanceParticles, src/fluid_system_cuda.cu:3470-3614, for one particle with acceleration
- * `accel` (FFORCE). Returns the new position, veval and velocity. */
-__device__ __forceinline__ void advance_one(const MgParams& p, float dt, f3 pos, f3 vev, f3 v,
-                                            f3 accel, f3* np_out, f3* ve_out, f3* vnext_out,
-                                            const float4* corr = nullptr)
+ * advanceParticles, src/fluid_system_cuda.cu:3470-3614, for one particle, in two parts:
+ * advance_accel is the acceleration the step applies (FFORCE plus the wall penalties, gravity
+ * and the acceleration limit, :3497-3576), advance_one the (no-op) velocity limit and the
+ * leapfrog update that return the new position, veval and velocity. */
+__device__ __forceinline__ f3 advance_accel(const MgParams& p, f3 pos, f3 vev, f3 accel)
 {
     const float eps = MG_EPSILON, prad = p.pradius, slope = p.pground_slope;
     const float estiff = p.pextstiff, damp = p.pdamp;
@@ -1141,13 +1143,29 @@ __device__ __forceinline__ void advance_one(const MgParams& p, float dt, f3 pos,
     accel = accel + mk3(p.pgravity[0], p.pgravity[1], p.pgravity[2]);
     speed = accel.x * accel.x + accel.y * accel.y + accel.z * accel.z;
     if (speed > p.AL2) accel = accel * (p.AL / sqrtf(speed));
-    speed = v.x * v.x + v.y * v.y + v.z * v.z;
+    return accel;
+}
+
+/* The rest of advanceParticles (src/fluid_system_cuda.cu:3577-3600). A learned step's
+ * corrections (corr[0].w != 0; two float4: drift c_d, veval c_e) move the position by
+ * (v_next + c_d) dt and store veval (v + v_next) / 2 + c_e; the velocity stays v_next. */
+__device__ __forceinline__ void advance_one(const MgParams& p, float dt, f3 pos, f3 vev, f3 v,
+                                            f3 accel, f3* np_out, f3* ve_out, f3* vnext_out,
+                                            const float4* corr = nullptr)
+{
+    accel = advance_accel(p, pos, vev, accel);
+    float speed = v.x * v.x + v.y * v.y + v.z * v.z;
     if (speed > p.VL2) {
         speed = p.VL2;
         v = v * (p.VL / sqrtf(speed)); /* no-op limit, src/fluid_system_cuda.cu:3577-3579 */
     }
-    f3 vnext = accel * dt + v;
-    if (corr && corr->w != 0.0f) vnext = vnext + mk3(corr->x, corr->y, corr->z); /* learned step */
+    const f3 vnext = accel * dt + v;
+    if (corr && corr[0].w != 0.0f) { /* learned step */
+        *np_out = pos + (vnext + xyz(corr[0])) * dt;
+        *ve_out = (v + vnext) * 0.5f + xyz(corr[1]);
+        *vnext_out = vnext;
+        return;
+    }
     *ve_out = (v + vnext) * 0.5f;
     *np_out = pos + vnext * dt;
     *vnext_out = vnext;
@@ -1188,19 +1206,13 @@ __global__ LAUNCH_BOUNDS_256 void k_advance(DS d, MgParams p, float dt, int spec
     d.VEL[i] = make_float4(vn.x, vn.y, vn.z, 0.0f);
 }
 
-/* Fused ComputeForceCL + AdvanceCL for a step that starts from a sort (FFORCE = 0): the force
- * stays in registers and the advanced state goes to the other half of the ping-pong
- * (P2/V2/VEL2) so neighbours keep reading the pre-step state. Inactive slots copy through. */
-__device__ __forceinline__ void force_advance_body(const DS& d, const MgParams& p, const MgGrid& g,
-                                                   Band<1>* bands, float dt, uint32_t bid,
-                                                   int special = 0, float zoff = 0.0f)
+/* Advance slot i with acceleration F into the other half of the ping-pong (P2/V2/VEL2), so
+ * neighbours keep reading the pre-step state; an inactive slot copies through. With d.DVc (a
+ * learned step) the slot's corrections, when marked used, enter the update. With
+ * d.HMR (hit masks in use) a displacement beyond the masks' skin raises MG_C_SKIN. */
+__device__ __forceinline__ void advance_other_half(const DS& d, const MgParams& p, float dt, uint32_t i,
+                                                   bool valid, f3 F, int special, float zoff)
 {
-    const uint32_t i = bid * blockDim.x + threadIdx.x;
-    const uint32_t na = d.ctr[MG_C_ACTIVE];
-    const uint32_t gc = (i < na) ? d.GCELL[i] : MG_GRID_UNDEF;
-    const bool valid = gc != MG_GRID_UNDEF;
-    const f3 F = force_one(d, p, g, bands[threadIdx.x >> 6], valid, i, na, gc, mk3(0.0f, 0.0f, 0.0f));
-    if (i >= (uint32_t)d.N) return;
     const float4 P_i = d.P[i], V_i = d.V[i], VEL_i = d.VEL[i];
     if (!valid) {
         d.P2[i] = P_i;
@@ -1210,9 +1222,8 @@ __device__ __forceinline__ void force_advance_body(const DS& d, const MgParams& 
     }
     f3 np, ve, vn;
     if (d.DVc) {
-        /* learned coarse step: the correction joins v_next before veval and the position */
-        const float4 c = d.DVc[i];
-        advance_one(p, dt, xyz(P_i), xyz(V_i), xyz(VEL_i), F, &np, &ve, &vn, &c);
+        const float4 c[2] = {d.DVc[2 * (size_t)i], d.DVc[2 * (size_t)i + 1]};
+        advance_one(p, dt, xyz(P_i), xyz(V_i), xyz(VEL_i), F, &np, &ve, &vn, c);
     } else {
         advance_one(p, dt, xyz(P_i), xyz(V_i), xyz(VEL_i), F, &np, &ve, &vn);
     }
@@ -1226,6 +1237,33 @@ __device__ __forceinline__ void force_advance_body(const DS& d, const MgParams& 
     d.P2[i] = make_float4(np.x, np.y, np.z, P_i.w);
     d.V2[i] = make_float4(ve.x, ve.y, ve.z, V_i.w);
     d.VEL2[i] = make_float4(vn.x, vn.y, vn.z, 0.0f);
+}
+
+/* The learned coarse step's advance: FFORCE from this step's force pass (k_force), the
+ * network's corrections (DVc), the advanced state into the other half. */
+__global__ LAUNCH_BOUNDS_256 void k_advance_learned(DS d, MgParams p, float dt, int special, float zoff)
+{
+    const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
+    if (i >= (uint32_t)d.N) return;
+    const uint32_t na = d.ctr[MG_C_ACTIVE];
+    const bool valid = i < na && d.GCELL[i] != MG_GRID_UNDEF;
+    advance_other_half(d, p, dt, i, valid, valid ? xyz(d.F[i]) : mk3(0.f, 0.f, 0.f), special, zoff);
+}
+
+/* Fused ComputeForceCL + AdvanceCL for a step that starts from a sort (FFORCE = 0): the force
+ * stays in registers and the advanced state goes to the other half of the ping-pong
+ * (P2/V2/VEL2) so neighbours keep reading the pre-step state. Inactive slots copy through. */
+__device__ __forceinline__ void force_advance_body(const DS& d, const MgParams& p, const MgGrid& g,
+                                                   Band<1>* bands, float dt, uint32_t bid,
+                                                   int special = 0, float zoff = 0.0f)
+{
+    const uint32_t i = bid * blockDim.x + threadIdx.x;
+    const uint32_t na = d.ctr[MG_C_ACTIVE];
+    const uint32_t gc = (i < na) ? d.GCELL[i] : MG_GRID_UNDEF;
+    const bool valid = gc != MG_GRID_UNDEF;
+    const f3 F = force_one(d, p, g, bands[threadIdx.x >> 6], valid, i, na, gc, mk3(0.0f, 0.0f, 0.0f));
+    if (i >= (uint32_t)d.N) return;
+    advance_other_half(d, p, dt, i, valid, F, special, zoff);
 }
 
 __global__ LAUNCH_BOUNDS_SWEEP void k_force_advance(DS d, MgParams p, MgGrid g, float dt, int special,

@@ -24,7 +24,8 @@ struct mg_net {
     int adam_t = 0;
     uint64_t version = 1; /* bumped on every parameter / normalisation change */
     float mu[MLP_IN], inv_sigma[MLP_IN];
-    float out_scale = 1.f, in_limit = 1e30f, out_limit = 1e30f;
+    float out_scale[MLP_OUT] = {1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f, 1.f};
+    float in_limit = 1e30f, out_limit = 1e30f;
     /* training set */
     float *X = nullptr, *T = nullptr, *mask = nullptr;
     size_t rows = 0;
@@ -65,7 +66,7 @@ int net_sync(mg_net* n)
     HIPCHK(hipMemcpyAsync(n->dev->b3, n->params + kOffB3, MLP_OUT * 4, hipMemcpyDeviceToDevice, n->st));
     HIPCHK(hipMemcpyAsync(n->dev->mu, n->mu, sizeof(n->mu), hipMemcpyHostToDevice, n->st));
     HIPCHK(hipMemcpyAsync(n->dev->inv_sigma, n->inv_sigma, sizeof(n->inv_sigma), hipMemcpyHostToDevice, n->st));
-    HIPCHK(hipMemcpyAsync(&n->dev->out_scale, &n->out_scale, 4, hipMemcpyHostToDevice, n->st));
+    HIPCHK(hipMemcpyAsync(n->dev->out_scale, n->out_scale, sizeof(n->out_scale), hipMemcpyHostToDevice, n->st));
     HIPCHK(hipMemcpyAsync(&n->dev->in_limit, &n->in_limit, 4, hipMemcpyHostToDevice, n->st));
     HIPCHK(hipMemcpyAsync(&n->dev->out_limit, &n->out_limit, 4, hipMemcpyHostToDevice, n->st));
     HIPCHK(hipStreamSynchronize(n->st)); /* the host-side sources above must stay valid */
@@ -187,20 +188,21 @@ int ensure_learn_buffers(mg_handle* h)
 {
     if (h->DV) return MG_OK;
     void* q = nullptr;
-    HIPCHK(hipMalloc(&q, (size_t)h->N * sizeof(float4)));
+    HIPCHK(hipMalloc(&q, (size_t)h->N * 2 * sizeof(float4)));
     h->allocs.push_back(q);
     h->DV = (float4*)q;
     HIPCHK(hipMalloc(&q, 256));
     h->allocs.push_back(q);
     h->fallbacks = (uint32_t*)q;
     HIPCHK(hipMemsetAsync(h->fallbacks, 0, 4, h->st));
-    HIPCHK(hipMemsetAsync(h->DV, 0, (size_t)h->N * sizeof(float4), h->st));
+    HIPCHK(hipMemsetAsync(h->DV, 0, (size_t)h->N * 2 * sizeof(float4), h->st));
     h->epoch++;
     return MG_OK;
 }
 
-/* the MLP pass of a learned step (called from do_fused_step) */
-int launch_mlp_infer(mg_handle* h, const mg_net* net)
+/* the MLP pass of a learned step of length dt (called from do_fused_step after the force
+ * pass has written FFORCE) */
+int launch_mlp_infer(mg_handle* h, const mg_net* net, float dt)
 {
     if (!h->DV) {
         set_err("learned step: correction buffer missing");
@@ -208,8 +210,8 @@ int launch_mlp_infer(mg_handle* h, const mg_net* net)
     }
     const unsigned tiles = (unsigned)((h->N + 63) / 64);
     const unsigned blocks = std::max(1u, std::min((tiles + MLP_WAVES - 1) / MLP_WAVES, 1024u));
-    hipLaunchKernelGGL(k_mlp_infer, dim3(blocks), dim3(256), sizeof(MlpLds), h->st, make_ds(h), net->dev, h->DV,
-                       h->fallbacks);
+    hipLaunchKernelGGL(k_mlp_infer, dim3(blocks), dim3(256), sizeof(MlpLds), h->st, make_ds(h), h->p, net->dev,
+                       h->DV, h->fallbacks, dt);
     HIPCHK(hipGetLastError());
     return MG_OK;
 }
@@ -295,13 +297,13 @@ int mg_net_set_params(mg_net* n, const float* in, size_t count)
     return net_sync(n);
 }
 
-int mg_net_set_norm(mg_net* n, const float* mu, const float* inv_sigma, float out_scale, float in_limit,
-                    float out_limit)
+int mg_net_set_norm(mg_net* n, const float* mu, const float* inv_sigma, const float* out_scale,
+                    float in_limit, float out_limit)
 {
-    if (!n || !mu || !inv_sigma) return MG_ERR_ARG;
+    if (!n || !mu || !inv_sigma || !out_scale) return MG_ERR_ARG;
     std::memcpy(n->mu, mu, sizeof(n->mu));
     std::memcpy(n->inv_sigma, inv_sigma, sizeof(n->inv_sigma));
-    n->out_scale = out_scale;
+    for (int c = 0; c < MLP_OUT; c++) n->out_scale[c] = c < MLP_OUT_REAL ? out_scale[c] : 0.0f;
     n->in_limit = in_limit;
     n->out_limit = out_limit;
     return net_sync(n);
@@ -373,14 +375,30 @@ int mg_net_train(mg_net* n, int steps, size_t batch, float lr, uint64_t seed, fl
     return net_sync(n);
 }
 
-/* raw MLP inputs of every slot of h's current state (N x 64 floats; inactive rows zero) */
+/* Raw MLP inputs of every slot of h's current sorted state (N x 64 floats; inactive rows
+ * zero), from the pressure and force passes a learned step would run first. The force pass
+ * changes no bond record (dry) and FFORCE keeps counting as zero, so the state is left as a
+ * step would find it. */
 int mg_net_features(mg_handle* h, float* X_out)
 {
     CHK(ready(h));
     if (!X_out) return MG_ERR_ARG;
+    if (!h->force_zero) {
+        set_err("mg_net_features: the handle must be in the sorted state (after a physical sort or a step)");
+        return MG_ERR_STATE;
+    }
+    uint64_t* hm = hit_masks(h);
+    CHK(do_pressure(h, hm));
+    {
+        DS d = make_ds(h);
+        d.HMR = hm;
+        d.dry = 1;
+        hipLaunchKernelGGL(k_force, dim3(nblk(h->N)), dim3(256), band_lds<1>(256), h->st, d, h->p, h->g, 0);
+        HIPCHK(hipGetLastError());
+    }
     float* d = nullptr;
     HIPCHK(hipMalloc(&d, (size_t)h->N * MLP_IN * 4));
-    hipLaunchKernelGGL(k_mlp_features, dim3(nblk(h->N)), dim3(256), 0, h->st, make_ds(h), d);
+    hipLaunchKernelGGL(k_mlp_features, dim3(nblk(h->N)), dim3(256), 0, h->st, make_ds(h), h->p, d);
     hipError_t e = hipMemcpyAsync(X_out, d, (size_t)h->N * MLP_IN * 4, hipMemcpyDeviceToHost, h->st);
     hipError_t e2 = hipStreamSynchronize(h->st);
     (void)hipFree(d);
@@ -389,15 +407,30 @@ int mg_net_features(mg_handle* h, float* X_out)
     return MG_OK;
 }
 
-/* the learned step's correction for h's current state: DV_out (N x 4 floats: dv, used) */
-int mg_net_infer(mg_handle* h, mg_net* n, float* DV_out)
+/* The corrections a learned step of length dt would apply to h's current sorted state:
+ * DV_out (N x 8 floats: c_d, used, c_e, 0), after the same pressure and (dry) force passes
+ * as mg_net_features. */
+int mg_net_infer(mg_handle* h, mg_net* n, float dt, float* DV_out)
 {
     CHK(ready(h));
     if (!n || !DV_out) return MG_ERR_ARG;
+    if (!h->force_zero) {
+        set_err("mg_net_infer: the handle must be in the sorted state (after a physical sort or a step)");
+        return MG_ERR_STATE;
+    }
     HIPCHK(hipStreamSynchronize(n->st));
     CHK(ensure_learn_buffers(h));
-    CHK(launch_mlp_infer(h, n));
-    HIPCHK(hipMemcpyAsync(DV_out, h->DV, (size_t)h->N * 16, hipMemcpyDeviceToHost, h->st));
+    uint64_t* hm = hit_masks(h);
+    CHK(do_pressure(h, hm));
+    {
+        DS d = make_ds(h);
+        d.HMR = hm;
+        d.dry = 1;
+        hipLaunchKernelGGL(k_force, dim3(nblk(h->N)), dim3(256), band_lds<1>(256), h->st, d, h->p, h->g, 0);
+        HIPCHK(hipGetLastError());
+    }
+    CHK(launch_mlp_infer(h, n, dt));
+    HIPCHK(hipMemcpyAsync(DV_out, h->DV, (size_t)h->N * 32, hipMemcpyDeviceToHost, h->st));
     HIPCHK(hipStreamSynchronize(h->st));
     return MG_OK;
 }

@@ -1,13 +1,21 @@
 /*
  * mg_mlp.hip — the learned coarse-step correction (no counterpart in the reference).
  *
- * A per-particle MLP 64 -> 128 -> 128 -> 16 (SiLU hidden layers; 50 real inputs, 3 real
- * outputs, the rest zero padding to MFMA tile sizes) predicts the velocity correction
- * dv_i = v_fine - v_coarse that four fine steps of dt would add to one coarse step of 4 dt.
- * Inputs, from the particle's sorted state after the pressure pass:
+ * A per-particle MLP 64 -> 128 -> 128 -> 16 (SiLU hidden layers; 53 real inputs, 6 real
+ * outputs, the rest zero padding to MFMA tile sizes) predicts two velocity corrections for
+ * one coarse step of 4 dt, each the fine segment's value minus the plain coarse step's:
+ *   drift c_d: the position moves by (v_next + c_d) 4 dt, as four fine steps move it;
+ *   veval c_e: veval is the plain step's (v + v_next) / 2 + c_e, the fine segment's last.
+ * The plain coarse step overshoots four fine steps by about 6 dt^2 a per step in position,
+ * because the leapfrog drift uses the end-of-step velocity, and its veval lags by about
+ * 1.5 dt a; the stored velocity itself agrees with the fine one to a difference that the
+ * local state does not predict, so it keeps the plain step's value.
+ * Inputs, from the particle's sorted state after the step's pressure and force passes:
  *   8 bonded neighbours (live own bond slots in slot order, then incoming bonds in CSR
- *   order): (x_j - x_i) and (v_j - v_i), 48 values; density and pressure; zeros to 64.
- * Each input is normalised as (x - mu) * inv_sigma; the output is y * out_scale.
+ *   order): (x_j - x_i) and (v_j - v_i), 48 values; density and pressure; the coarse step's
+ *   acceleration (FFORCE with walls, gravity and the acceleration limit, as the advance
+ *   applies it), 3 values; zeros to 64.
+ * Each input is normalised as (x - mu) * inv_sigma; output c is y_c * out_scale[c].
  *
  * Weights are bf16, accumulation fp32, on gfx950 MFMA (v_mfma_f32_16x16x32_bf16): for a
  * 16 x 32 A tile lane l holds A[l % 16][8 (l / 16) + e], for a 32 x 16 B tile lane l holds
@@ -19,10 +27,13 @@
  */
 
 #define MLP_IN 64
-#define MLP_IN_REAL 50
+#define MLP_IN_REAL 53
+/* a correction that would move a particle farther than this in one step (world units, a
+ * thousandth of the unit smoothing radius) is refused: the particle takes the plain step */
+#define MLP_MAX_SHIFT 1e-3f
 #define MLP_H 128
 #define MLP_OUT 16
-#define MLP_OUT_REAL 3
+#define MLP_OUT_REAL 6
 #define MLP_NBR 8
 
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8_t;
@@ -35,12 +46,20 @@ __device__ __forceinline__ f32x4_t mfma16(bf16x8_t a, bf16x8_t b, f32x4_t c)
 
 __device__ __forceinline__ __bf16 to_bf16(float x) { return (__bf16)x; }
 
-__device__ __forceinline__ float silu(float z) { return z / (1.0f + expf(-z)); }
+/* sigmoid with the hardware exp2 and reciprocal (a few ulp; the network runs in bf16). The
+ * same function serves training and inference. */
+__device__ __forceinline__ float sigmoid_fast(float z)
+{
+    const float e = __builtin_amdgcn_exp2f(-1.44269504f * z);
+    return __builtin_amdgcn_rcpf(1.0f + e);
+}
+
+__device__ __forceinline__ float silu(float z) { return z * sigmoid_fast(z); }
 
 /* d silu / dz = s (1 + z (1 - s)), s = sigmoid(z) */
 __device__ __forceinline__ float silu_grad(float z)
 {
-    const float s = 1.0f / (1.0f + expf(-z));
+    const float s = sigmoid_fast(z);
     return s * (1.0f + z * (1.0f - s));
 }
 
@@ -53,14 +72,16 @@ struct MlpDev {
     __bf16 w3f[(MLP_H / 32) * (MLP_OUT / 16) * 16 * 32];
     float b1[MLP_H], b2[MLP_H], b3[MLP_OUT];
     float mu[MLP_IN], inv_sigma[MLP_IN];
-    float out_scale;
+    float out_scale[MLP_OUT];
     float in_limit;  /* |normalised input| above this -> fallback (plain coarse step) */
     float out_limit; /* |y| above this -> fallback */
 };
 
 /* ------------------------------------------------------------------ features
- * The 64 raw inputs of active particle i (sorted state after pressure). */
-__device__ __forceinline__ void mlp_features(const DS& d, uint32_t i, uint32_t na, float x[MLP_IN])
+ * The 64 raw inputs of active particle i (sorted state after the pressure and force passes;
+ * FFORCE in d.F). */
+__device__ __forceinline__ void mlp_features(const DS& d, const MgParams& p, uint32_t i, uint32_t na,
+                                             float x[MLP_IN])
 {
     const int N = d.N, B = d.B;
 #pragma unroll
@@ -94,19 +115,24 @@ __device__ __forceinline__ void mlp_features(const DS& d, uint32_t i, uint32_t n
         if (k >= na || (double)d.BPAR[(size_t)b * N + kh].y < 0.000000001) continue;
         add(k);
     }
-    x[48] = 1.0f / d.V[i].w; /* density */
-    x[49] = Pi.w;            /* pressure */
+    const float4 Vi_eval = d.V[i];
+    x[48] = 1.0f / Vi_eval.w; /* density */
+    x[49] = Pi.w;             /* pressure */
+    const f3 a = advance_accel(p, xyz(Pi), xyz(Vi_eval), xyz(d.F[i]));
+    x[50] = a.x;
+    x[51] = a.y;
+    x[52] = a.z;
 }
 
 /* raw features of every active slot (rows of inactive slots are zero), for training data */
-__global__ LAUNCH_BOUNDS_256 void k_mlp_features(DS d, float* __restrict__ X)
+__global__ LAUNCH_BOUNDS_256 void k_mlp_features(DS d, MgParams p, float* __restrict__ X)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (uint32_t)d.N) return;
     float x[MLP_IN];
     const uint32_t na = d.ctr[MG_C_ACTIVE];
     if (i < na && d.GCELL[i] != MG_GRID_UNDEF) {
-        mlp_features(d, i, na, x);
+        mlp_features(d, p, i, na, x);
     } else {
 #pragma unroll
         for (int k = 0; k < MLP_IN; k++) x[k] = 0.0f;
@@ -119,7 +145,9 @@ __global__ LAUNCH_BOUNDS_256 void k_mlp_features(DS d, float* __restrict__ X)
 /* --------------------------------------------------------------- inference
  * One wavefront per 64 particles: each lane gathers and normalises its particle's inputs
  * into an LDS tile, then the wave runs the three layers as 16-row MFMA tiles through LDS.
- * DV[i] = (dv, 1) for corrected particles, (0, 0, 0, 0) for inactive or fallback ones;
+ * DV[2i] = (c_d, 1), DV[2i + 1] = (c_e, 0) for corrected particles, zeros for inactive or
+ * fallback ones (a normalised input beyond in_limit, an output beyond out_limit, or a drift
+ * correction that would move the particle more than MLP_MAX_SHIFT in the step of length dt);
  * fallbacks are counted in *fallbacks. Weights are staged in LDS once per block. */
 #define MLP_WAVES 4
 struct MlpLds {
@@ -134,8 +162,9 @@ struct MlpLds {
 
 __device__ __forceinline__ bf16x8_t lds8(const __bf16* p) { return *(const bf16x8_t*)p; }
 
-__global__ __launch_bounds__(256) void k_mlp_infer(DS d, const MlpDev* __restrict__ net, float4* __restrict__ DV,
-                                                   uint32_t* __restrict__ fallbacks)
+__global__ __launch_bounds__(256) void k_mlp_infer(DS d, MgParams p, const MlpDev* __restrict__ net,
+                                                   float4* __restrict__ DV, uint32_t* __restrict__ fallbacks,
+                                                   float dt)
 {
     extern __shared__ float4 mg_lds[];
     MlpLds& L = *(MlpLds*)mg_lds;
@@ -163,7 +192,7 @@ __global__ __launch_bounds__(256) void k_mlp_infer(DS d, const MlpDev* __restric
         {
             float x[MLP_IN];
             if (act) {
-                mlp_features(d, i, na, x);
+                mlp_features(d, p, i, na, x);
             } else {
 #pragma unroll
                 for (int k = 0; k < MLP_IN; k++) x[k] = 0.0f;
@@ -227,7 +256,7 @@ __global__ __launch_bounds__(256) void k_mlp_infer(DS d, const MlpDev* __restric
                 const int prow = rt * 16 + 4 * kq + r; /* particle row within the 64-tile */
                 if (col < MLP_OUT_REAL) {
                     if (!(fabsf(y) <= net->out_limit)) L.ok[w][prow] = 0;
-                    ((float*)&L.h[w][0][0])[prow * 4 + col] = y; /* layer-1 tile is free now */
+                    ((float*)&L.h[w][0][0])[prow * 16 + col] = y * net->out_scale[col]; /* layer-1 tile is free now */
                 }
             }
             wave_lds_sync();
@@ -236,12 +265,17 @@ __global__ __launch_bounds__(256) void k_mlp_infer(DS d, const MlpDev* __restric
                 const int prow = l;
                 const uint32_t ii = tile * 64u + (uint32_t)prow;
                 if (ii < (uint32_t)d.N) {
-                    const float* y = &((float*)&L.h[w][0][0])[prow * 4];
-                    const bool use = L.ok[w][prow] != 0;
-                    if (use)
-                        DV[ii] = make_float4(y[0] * net->out_scale, y[1] * net->out_scale, y[2] * net->out_scale, 1.0f);
-                    else
-                        DV[ii] = make_float4(0.f, 0.f, 0.f, 0.f);
+                    const float* y = &((float*)&L.h[w][0][0])[prow * 16];
+                    const f3 cd = mk3(y[0], y[1], y[2]), ce = mk3(y[3], y[4], y[5]);
+                    /* the residual the network may apply: a shift of at most MLP_MAX_SHIFT */
+                    const bool use = L.ok[w][prow] != 0 && sqrtf(dot3(cd, cd)) * dt <= MLP_MAX_SHIFT;
+                    if (use) {
+                        DV[2 * (size_t)ii] = make_float4(cd.x, cd.y, cd.z, 1.0f);
+                        DV[2 * (size_t)ii + 1] = make_float4(ce.x, ce.y, ce.z, 0.0f);
+                    } else {
+                        DV[2 * (size_t)ii] = make_float4(0.f, 0.f, 0.f, 0.f);
+                        DV[2 * (size_t)ii + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
+                    }
                     const bool wasact = ii < na && d.GCELL[ii] != MG_GRID_UNDEF;
                     if (wasact && !use) nfall++;
                 }

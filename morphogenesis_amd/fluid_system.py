@@ -188,8 +188,8 @@ class FluidSystem:
         self._call("gene_action", self.h)
 
     def step_learned(self, net, nsteps: int, dt: float, flags: int = 0):
-        """nsteps learned coarse steps of dt (mg_step_learned): the fused step with the
-        net's velocity correction applied in the advance."""
+        """nsteps learned coarse steps of dt (mg_step_learned): the step's pressure and force
+        passes, the network, and the advance with the net's drift-velocity correction."""
         from .learned import _declare
 
         _declare(self.lib)

@@ -11,7 +11,7 @@ from __future__ import annotations
 
 import numpy as np
 
-IN, H, OUT, OUT_REAL = 64, 128, 16, 3
+IN, H, OUT, OUT_REAL = 64, 128, 16, 6
 SHAPES = [("W1", (IN, H)), ("b1", (H,)), ("W2", (H, H)), ("b2", (H,)), ("W3", (H, OUT)), ("b3", (OUT,))]
 NPARAMS = sum(int(np.prod(s)) for _, s in SHAPES)
 
