@@ -105,6 +105,16 @@ int mg_compute_genes(mg_handle* h);
 /* ZeroVelCL (src/host_CL.cpp:2175-2180). */
 int mg_zero_vel(mg_handle* h);
 
+/* No counterpart in the reference: mg_prefix_sum_cells' cell scan (the exclusive prefix sum
+ * FBIN_OFFSET of FBIN_COUNT) for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc). It allocates
+ * nothing: cnt (n counts) and off (n offsets) are device buffers of the caller, the scan's
+ * scratch is a fixed array in the library (one per GPU), and the work is queued on `stream`
+ * (a hipStream_t; null = the null stream). n <= 67,108,864 (16,384 tiles of 4,096), else
+ * MG_ERR_SIZE. report (device, may be null) receives {sum of the counts, nonzero if the
+ * scan's bounded wait expired}. Calls that overlap on one GPU share the scratch and race. */
+int mg_prefix_sum_cells_noalloc(const uint32_t* cnt, uint32_t* off, int n, uint32_t* report, void* stream);
+
 /* InitializeBondsCL (src/host_CUDA.cpp:650-660, src/host_CL.cpp:1884-1916): the
  * initialize_bonds kernel (src/fluid_system_cuda.cu:1938-2044) over gene 1's dense list. Needs
  * the sorted state (after a physical sort or a step). */

@@ -14,6 +14,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1611,6 +1612,86 @@ int mg_prefix_sum_cells(mg_handle* h, int zero_offsets)
     }
     CHK(materialize_gene_grid(h));
     return do_scan(h, 1);
+}
+
+/* ---- the cell scan without allocation (mg_prefix_sum_cells_noalloc) ----
+ * The look-back scan's scratch lives in one file-scope device array instead of a handle's
+ * allocations. Layout (64-bit words): [0] the tile ticket, [1, 9) the scan counters (16
+ * words of 32 bits, MG_C_*), [9, 9 + kScanNoAllocTiles) one status word per tile. Each call
+ * clears the ticket, counters and the status words of its tiles before the scan (a fill
+ * kernel, so a captured call holds no memset node), so every call starts from zero; calls
+ * that overlap on one GPU share the array and race. Every GPU the library runs on holds its
+ * own copy until the process ends; host code reaches it only through the address
+ * hipGetSymbolAddress returns for the current device, looked up once per device. */
+namespace {
+constexpr unsigned kScanNoAllocTiles = 16384; /* up to 16384 x 4096 = 67,108,864 cells */
+constexpr unsigned kScanNoAllocHead = 9;       /* ticket + 8 words of counters */
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned long long* g_scratch_ptr[kMaxDevices] = {};
+} // namespace
+
+static __device__ __align__(256) unsigned long long scratch[kScanNoAllocHead + kScanNoAllocTiles];
+
+/* the device address of `scratch` on the current device (set-up on the first call there) */
+static int scratch_on_device(unsigned long long** out)
+{
+    int dev = -1;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) {
+        set_err("mg_prefix_sum_cells_noalloc: device %d outside the supported range", dev);
+        return MG_ERR_UNSUPPORTED;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            set_err("mg_prefix_sum_cells_noalloc: hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+            return MG_ERR_HIP;
+        }
+        g_scratch_ptr[dev] = (unsigned long long*)ptr;
+    }
+    *out = g_scratch_ptr[dev];
+    return MG_OK;
+}
+
+/* report[0] = the total (the last cell's exclusive offset plus its count), report[1] = the
+ * bounded-wait flag */
+__global__ void k_scan_report(const uint32_t* __restrict__ cnt, const uint32_t* __restrict__ off, int n,
+                              const uint32_t* __restrict__ ctr, uint32_t* __restrict__ out)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        out[0] = off[n - 1] + cnt[n - 1];
+        out[1] = ctr[MG_C_ERR];
+    }
+}
+
+int mg_prefix_sum_cells_noalloc(const uint32_t* cnt, uint32_t* off, int n, uint32_t* report, void* stream)
+{
+    if (!cnt || !off || n <= 0) {
+        set_err("mg_prefix_sum_cells_noalloc: null buffer or n <= 0");
+        return MG_ERR_ARG;
+    }
+    const unsigned tiles = (unsigned)(((size_t)n + SCAN_TILE - 1) / SCAN_TILE);
+    if (tiles > kScanNoAllocTiles) {
+        set_err("mg_prefix_sum_cells_noalloc: n = %d exceeds the maximum of %u cells", n,
+                kScanNoAllocTiles * (unsigned)SCAN_TILE);
+        return MG_ERR_SIZE;
+    }
+    unsigned long long* base = nullptr;
+    CHK(scratch_on_device(&base));
+    hipStream_t st = (hipStream_t)stream;
+    unsigned long long* ticket = base;
+    uint32_t* ctr = (uint32_t*)(base + 1);
+    unsigned long long* status = base + kScanNoAllocHead;
+    const size_t words = 2 * ((size_t)kScanNoAllocHead + tiles);
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)std::min<size_t>((words + 255) / 256, 1024)), dim3(256), 0, st,
+                       (uint32_t*)base, 0u, words);
+    hipLaunchKernelGGL(k_scan_lookback, dim3(tiles), dim3(256), 0, st, cnt, off, n, status, ticket, ctr, -1);
+    if (report) hipLaunchKernelGGL(k_scan_report, dim3(1), dim3(64), 0, st, cnt, off, n, ctr, report);
+    HIPCHK(hipGetLastError());
+    return MG_OK;
 }
 
 int mg_counting_sort_full(mg_handle* h)
