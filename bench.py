@@ -179,7 +179,8 @@ def run_batch(args, count: int = 256):
     dt, flags = scs[0].dt, scs[0].flags
     bat = Batch(systems)
     chosen = bat.autotune(2, dt, flags)
-    progress(f"autotuned: {chosen}")
+    persistent, workgroups = bat.launch_config()
+    progress(f"autotuned: block sizes {chosen}, persistent launch {persistent} ({workgroups} workgroups)")
     bat.step(args.warmup, dt, flags | F.STEP_GRAPH)
     bat.sync()
     torch.cuda.synchronize()
@@ -188,6 +189,19 @@ def run_batch(args, count: int = 256):
     bat.sync()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    # the other launch form over the same number of steps, for comparison (not the value)
+    forms = {("persistent" if persistent else "dispatch"): elapsed / args.steps * 1e3}
+    if persistent or bat.launch_config()[0] == 0:
+        bat.set_persistent(not persistent)
+        other = bat.launch_config()[0]
+        if other != persistent:
+            bat.step(2, dt, flags | F.STEP_GRAPH)
+            bat.sync()
+            t1 = time.perf_counter()
+            bat.step(args.steps, dt, flags | F.STEP_GRAPH)
+            bat.sync()
+            forms["persistent" if other else "dispatch"] = (time.perf_counter() - t1) / args.steps * 1e3
+        bat.set_persistent(bool(persistent))
     ms = (C.c_double * len(PHASES))()
     calls = (C.c_int * len(PHASES))()
     bat._call("batch_step_timed", bat.h, args.timed_steps, dt, flags, ms, calls)
@@ -207,8 +221,11 @@ def run_batch(args, count: int = 256):
         if p in kern and bts > 0:
             kernels[kern[p]] = {"ms": round(v, 5), "alg_bytes": int(bts), "GB/s": round(bts / (v * 1e-3) / 1e9, 1)}
     dom = max((p for p in per_step if p in kern and p != "place"), key=lambda p: per_step[p])
-    alg = sum(algorithmic_bytes(dom, sc, st) for sc, st in zip(scs, stats))
-    achieved = alg / (per_step[dom] * 1e-3) / 1e9
+    ref = sum(reference_layout_bytes(dom, sc, st)[0] for sc, st in zip(scs, stats))
+    terms = reference_layout_bytes(dom, scs[0], stats[0])[1]
+    lay = sum(algorithmic_bytes(dom, sc, st) for sc, st in zip(scs, stats))
+    achieved = ref / (per_step[dom] * 1e-3) / 1e9
+    lay_ach = lay / (per_step[dom] * 1e-3) / 1e9
     classes = bat.classes()
     out = {
         "metric": METRIC, "value": total * args.steps / elapsed, "unit": "particle-steps/s",
@@ -217,13 +234,24 @@ def run_batch(args, count: int = 256):
         "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
         "config": {"workload": args.workload, "instances": count, "particles": total,
                    "bonds_per_particle": 32, "dt": dt, "genes": bool(flags & F.STEP_GENES),
-                   "hipgraph": True, "size_classes": [{"instances": m, "threads": t} for m, t in classes],
+                   "hipgraph": True,
+                   "launch": {"persistent": bool(persistent), "workgroups": workgroups,
+                              "ms_per_step_by_form": {k: round(v, 4) for k, v in forms.items()}},
+                   "size_classes": [{"instances": m, "threads": t} for m, t in classes],
                    "autotuned_threads": chosen,
-                   "step": "pressure+force+advance+diffusion+genes+insert+scan+sort, one launch per phase and size class"},
+                   "step": ("pressure+force+advance+diffusion+genes+insert+scan+sort, "
+                            + ("one persistent launch per step for all instances" if persistent
+                               else "one launch per phase and size class"))},
         "roofline": {"bound": "hbm", "kernel": kern[dom], "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": load_traffic(kern[dom], args.workload),
-                     "algorithmic_bytes_per_launch": alg, "avg_launch_ms": per_step[dom]},
+                     "algorithmic_bytes_per_launch": ref,
+                     "algorithmic_basis": "reference layout, per particle: " +
+                                          ", ".join(f"{k} {v:.1f} B" for k, v in terms.items()),
+                     "avg_launch_ms": per_step[dom],
+                     "timing": "per-phase times of the dispatch form (HIP events around each launch group)",
+                     "layout": {"bytes_per_launch": lay, "achieved": lay_ach, "frac": lay_ach / HBM_PEAK_GBS,
+                                "basis": "this build's HBM layout, every element once"}},
         "step_breakdown_ms": {p: round(v, 5) for p, v in per_step.items()},
         "kernels": kernels,
     }

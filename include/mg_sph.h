@@ -173,8 +173,10 @@ int mg_debug_set_scan_ticket(mg_handle* h, uint64_t ticket);
 
 /* ---- batched instances (no counterpart in the reference, which steps one FluidSystem) ----
  * A batch steps many independent handles (each with its own buffers, grid, parameters and
- * genome) with one launch per phase and size class instead of one per handle; every
- * instance evolves bit for bit as under mg_step. While a handle belongs to a batch its work
+ * genome) together: by default one persistent launch per step covers every phase of every
+ * instance (resident workgroups, per-phase work queues, grid barriers); the alternative
+ * dispatch form runs one launch per phase and size class. Every instance evolves bit for bit
+ * as under mg_step. While a handle belongs to a batch its work
  * is queued on the batch's stream (uploads, downloads and mg_step keep working); destroy the
  * batch before any of its handles. Handles must be in the sorted state (after
  * mg_physical_sort or a step) when mg_batch_step is called. */
@@ -190,10 +192,19 @@ int mg_batch_sync(mg_batch* b);
  * number of instances and the block size of its particle and cell launches (arrays of at
  * least *nclass entries, either may be null). */
 int mg_batch_classes(mg_batch* b, int* nclass, int* members, unsigned* threads);
-/* Measures `steps` steps at each block size (256, 128, 64) and keeps, per size class, the one
- * with the least kernel time; threads_out (one entry per class, may be null) receives the
- * choice. The steps are real steps of the batch. */
+/* Measures `steps` steps of the dispatch form at each block size (256, 128, 64) and keeps,
+ * per size class, the one with the least kernel time (threads_out, one entry per class, may be
+ * null, receives it); then times `steps` whole steps of that dispatch form and of the
+ * persistent form at each workgroup count per CU that stays resident, and keeps the fastest.
+ * The steps are real steps of the batch. */
 int mg_batch_autotune(mg_batch* b, int steps, float dt, unsigned flags, unsigned* threads_out);
+/* The launch form in use: *persistent = 1 for one persistent launch per step (with
+ * *workgroups workgroups of 256 threads), 0 for the dispatch form. Either may be null. */
+int mg_batch_launch_config(mg_batch* b, int* persistent, unsigned* workgroups);
+/* Chooses the form: on = 1 persistent (workgroups_per_cu workgroups on each CU, 0 = all that
+ * stay resident; batches whose instances exceed 65,535 blocks of 256 use the dispatch form),
+ * on = 0 the dispatch form. */
+int mg_batch_set_persistent(mg_batch* b, int on, unsigned workgroups_per_cu);
 /* Like mg_batch_step without graph replay, with HIP events around every launch group:
  * ms_out[phase] accumulates milliseconds summed over the size classes and calls_out[phase]
  * the launch groups, phase ids MG_PH_* (MG_PH_COUNT entries each; either may be null). */

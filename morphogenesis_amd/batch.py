@@ -46,6 +46,16 @@ class Batch:
         self._call("batch_autotune", self.h, int(steps), float(dt), int(flags), out)
         return list(out)
 
+    def launch_config(self):
+        """(persistent, workgroups): 1 and the workgroup count for one persistent launch per
+        step, 0 for the dispatch form."""
+        on, wg = C.c_int(), C.c_uint()
+        self._call("batch_launch_config", self.h, C.byref(on), C.byref(wg))
+        return int(on.value), int(wg.value)
+
+    def set_persistent(self, on: bool, workgroups_per_cu: int = 0):
+        self._call("batch_set_persistent", self.h, int(bool(on)), int(workgroups_per_cu))
+
     def close(self):
         if self.h:
             self.lib.mg_batch_destroy(self.h)

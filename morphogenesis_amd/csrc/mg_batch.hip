@@ -1,15 +1,17 @@
 /*
  * mg_batch.hip — many independent handles stepped together (included by mg_capi.hip).
  *
- * Instances are grouped into size classes by particle count; each class runs the fused step
- * as seven launches (pressure, force + advance, diffusion + genes + insert, scan, place,
- * cellsort, gather) that cover all of its instances, so a step of 256 instances costs
- * 7 x (number of classes) dispatches instead of 7 x 256. The kernels look up their instance
- * in a device table (BatchInst) built from each handle exactly as its own fused step builds
- * its kernel arguments, so every instance evolves bit for bit as it would alone. Two tables
- * per class cover the two buffer parities of consecutive steps; two consecutive steps are
- * captured into one hipGraph and replayed. The block size of each class's particle and cell
- * launches is chosen by mg_batch_autotune from measured per-class kernel time.
+ * A step of every instance is one persistent launch (kb_step): resident workgroups walk the
+ * seven phases of the fused step (pressure, force + advance, diffusion + genes + insert,
+ * scan, place, cellsort, gather) over per-phase queues of (instance, block) items, with grid
+ * barriers between phases. The alternative is the dispatch form: instances grouped into
+ * size classes by particle count, each class running the seven phases as launches that cover
+ * all of its instances (7 x classes dispatches per step). Both look up their instance in a
+ * device table (BatchInst) built from each handle exactly as its own fused step builds its
+ * kernel arguments, so every instance evolves bit for bit as it would alone. Tables cover the
+ * two buffer parities of consecutive steps; two consecutive steps are captured into one
+ * hipGraph and replayed. mg_batch_autotune measures the dispatch form at each block size per
+ * class and the persistent form at each workgroup count per CU, and keeps the fastest.
  */
 
 struct mg_batch {
@@ -34,6 +36,18 @@ struct mg_batch {
     unsigned sig_flags = 0;
     /* graphs of two steps, by start parity (0: the parity the tables were built at) */
     hipGraphExec_t graph[2] = {nullptr, nullptr};
+    /* the persistent form: one kb_step launch per step (when every instance fits its
+     * 16-bit block and instance fields) */
+    bool persistent = true;
+    bool persist_ok = false;
+    int persist_ng = 1;
+    unsigned persist_per_cu = 0;      /* resident workgroups per CU (occupancy) */
+    unsigned persist_use_per_cu = 0;  /* chosen workgroups per CU (0: all resident) */
+    int cus = 0;
+    BatchInst* PI[2] = {};            /* every instance, by parity */
+    uint32_t* pitems[PERSIST_PHASES] = {};
+    uint32_t pnitems[PERSIST_PHASES] = {};
+    PersistCtl* pctl = nullptr;
     /* per-class timing during autotuning */
     bool timing = false;
     std::vector<hipEvent_t> ev;
@@ -75,6 +89,13 @@ void batch_free_tables(mg_batch* b)
         }
     for (void* q : b->allocs) (void)hipFree(q);
     b->allocs.clear();
+    b->PI[0] = b->PI[1] = nullptr;
+    for (int k = 0; k < PERSIST_PHASES; k++) {
+        b->pitems[k] = nullptr;
+        b->pnitems[k] = 0;
+    }
+    b->pctl = nullptr;
+    b->persist_ok = false;
     for (auto& c : b->cls) {
         c.I[0] = c.I[1] = nullptr;
         c.pmap = c.cmap = c.tmap = nullptr;
@@ -114,6 +135,118 @@ void fill_inst(BatchInst& t, mg_handle* h, const MgGenome* dgen, float dt, unsig
     t.genes = (flags & MG_STEP_GENES) ? 1 : 0;
     t.vmax = std::max(std::max(h->g.gmax[0], h->g.gmax[1]), h->g.gmax[2]);
     t.dt = dt;
+}
+
+template <int NG> size_t persist_lds() { return band_lds<(NG > 1 ? NG : 1)>(256); }
+
+template <int NG> int persist_occupancy(unsigned* per_cu)
+{
+    int nb = 0;
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void*)kb_step<NG>, 256, persist_lds<NG>()));
+    *per_cu = (unsigned)std::max(nb, 0);
+    return MG_OK;
+}
+
+/* Tables and item lists of the persistent form: every instance (largest first), its blocks of
+ * 256 particles / cells and its scan tiles, for both parities. */
+int persist_build(mg_batch* b, float dt, unsigned flags)
+{
+    const int n = (int)b->hs.size();
+    std::vector<int> order(n);
+    for (int i = 0; i < n; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return b->hs[x]->N > b->hs[y]->N; });
+    std::vector<BatchInst> tab[2];
+    std::vector<uint32_t> items[PERSIST_PHASES];
+    int ng = 1;
+    bool fits = n <= 65535;
+    for (int k = 0; k < n && fits; k++) {
+        mg_handle* h = b->hs[order[k]];
+        const uint32_t pnb = nblk(h->N, 256), cnb = nblk(h->C, 256);
+        const uint32_t tnb = (uint32_t)((h->C + SCAN_TILE - 1) / SCAN_TILE);
+        if (pnb > 65536 || cnb > 65536 || tnb > 65536) { fits = false; break; }
+        for (int q = 0; q < 2; q++) {
+            const int ic = h->icur, gc = h->gcur;
+            const bool gs = h->grid_scanned, pb = h->pos_binned, cf = h->cells_fresh;
+            if (q == 1) {
+                h->icur = 1 - ic;
+                h->gcur = 1 - gc;
+                h->grid_scanned = h->pos_binned = h->cells_fresh = true;
+            }
+            BatchInst t;
+            fill_inst(t, h, b->dgen[order[k]], dt, flags);
+            t.pb0 = 0; t.pnb = pnb;
+            t.cb0 = 0; t.cnb = cnb;
+            t.tb0 = 0; t.tnb = tnb;
+            tab[q].push_back(t);
+            h->icur = ic;
+            h->gcur = gc;
+            h->grid_scanned = gs;
+            h->pos_binned = pb;
+            h->cells_fresh = cf;
+        }
+        const int g = tab[0].back().dp.ng;
+        ng = std::max(ng, g <= 1 ? 1 : (g == 2 ? 2 : 4));
+        const uint32_t base = (uint32_t)k << 16;
+        for (uint32_t x = 0; x < pnb; x++)
+            for (int ph : {0, 1, 2, 4, 6}) items[ph].push_back(base | x);
+        for (uint32_t x = 0; x < tnb; x++) items[3].push_back(base | x);
+        for (uint32_t x = 0; x < cnb; x++) items[5].push_back(base | x);
+    }
+    b->persist_ok = false;
+    if (!fits) return MG_OK; /* the dispatch form serves this batch */
+    b->persist_ng = ng;
+    if (ng <= 1) CHK(persist_occupancy<1>(&b->persist_per_cu));
+    else if (ng == 2) CHK(persist_occupancy<2>(&b->persist_per_cu));
+    else CHK(persist_occupancy<4>(&b->persist_per_cu));
+    if (b->persist_per_cu == 0) return MG_OK;
+    if (!b->cus) {
+        int dev = 0;
+        HIPCHK(hipGetDevice(&dev));
+        HIPCHK(hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, dev));
+    }
+    for (int q = 0; q < 2; q++) {
+        CHK(balloc(b, &b->PI[q], tab[q].size()));
+        HIPCHK(hipMemcpyAsync(b->PI[q], tab[q].data(), tab[q].size() * sizeof(BatchInst), hipMemcpyHostToDevice, b->st));
+    }
+    for (int ph = 0; ph < PERSIST_PHASES; ph++) {
+        CHK(balloc(b, &b->pitems[ph], items[ph].size()));
+        HIPCHK(hipMemcpyAsync(b->pitems[ph], items[ph].data(), items[ph].size() * 4, hipMemcpyHostToDevice, b->st));
+        b->pnitems[ph] = (uint32_t)items[ph].size();
+    }
+    CHK(balloc(b, &b->pctl, 1));
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, b->st, (uint32_t*)b->pctl, 0u,
+                       sizeof(PersistCtl) / sizeof(uint32_t));
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(b->st)); /* the host tables go away with this scope */
+    b->persist_ok = true;
+    return MG_OK;
+}
+
+unsigned persist_workgroups(const mg_batch* b)
+{
+    const unsigned per_cu = b->persist_use_per_cu ? std::min(b->persist_use_per_cu, b->persist_per_cu)
+                                                  : b->persist_per_cu;
+    return per_cu * (unsigned)b->cus;
+}
+
+int persist_launch(mg_batch* b, int q)
+{
+    PersistPlan P;
+    P.I = b->PI[q];
+    for (int ph = 0; ph < PERSIST_PHASES; ph++) {
+        P.items[ph] = b->pitems[ph];
+        P.nitems[ph] = b->pnitems[ph];
+    }
+    P.ctl = b->pctl;
+    const unsigned G = persist_workgroups(b);
+    if (b->persist_ng <= 1)
+        hipLaunchKernelGGL(kb_step<1>, dim3(G), dim3(256), persist_lds<1>(), b->st, P);
+    else if (b->persist_ng == 2)
+        hipLaunchKernelGGL(kb_step<2>, dim3(G), dim3(256), persist_lds<2>(), b->st, P);
+    else
+        hipLaunchKernelGGL(kb_step<4>, dim3(G), dim3(256), persist_lds<4>(), b->st, P);
+    HIPCHK(hipGetLastError());
+    return MG_OK;
 }
 
 int batch_build(mg_batch* b, float dt, unsigned flags)
@@ -180,6 +313,7 @@ int batch_build(mg_batch* b, float dt, unsigned flags)
         HIPCHK(hipStreamSynchronize(b->st));
     }
     HIPCHK(hipStreamSynchronize(b->st));
+    CHK(persist_build(b, dt, flags));
     b->sig.resize(b->hs.size());
     for (size_t i = 0; i < b->hs.size(); i++) b->sig[i] = handle_sig(b->hs[i]);
     b->sig_dt = dt;
@@ -223,9 +357,11 @@ void batch_mark(mg_batch* b, int cls, int ph, bool start)
     }
 }
 
-/* one fused step of every instance, tables of parity q */
+/* one fused step of every instance, tables of parity q: the persistent launch, or the dispatch
+ * form (always when timing per phase) */
 int batch_launch(mg_batch* b, int q)
 {
+    if (b->persistent && b->persist_ok && !b->timing) return persist_launch(b, q);
     for (size_t ci = 0; ci < b->cls.size(); ci++) {
         auto& c = b->cls[ci];
         if (c.members.empty()) continue;
@@ -445,6 +581,14 @@ int mg_batch_sync(mg_batch* b)
 {
     if (!b) return MG_ERR_ARG;
     HIPCHK(hipStreamSynchronize(b->st));
+    if (b->pctl) {
+        uint32_t err = 0;
+        HIPCHK(hipMemcpy(&err, &b->pctl->err, 4, hipMemcpyDeviceToHost));
+        if (err) {
+            set_err("mg_batch_sync: a grid barrier of the persistent step gave up its bounded wait; results are invalid");
+            return MG_ERR_HIP;
+        }
+    }
     for (mg_handle* h : b->hs) {
         uint32_t err = 0;
         HIPCHK(hipMemcpy(&err, h->ctr + MG_C_ERR, 4, hipMemcpyDeviceToHost));
@@ -502,6 +646,62 @@ int mg_batch_autotune(mg_batch* b, int steps, float dt, unsigned flags, unsigned
         if (threads_out) threads_out[c] = pick[c];
     }
     b->built = false;
+    /* the dispatch form with these block sizes against the persistent form at each workgroup
+     * count per CU: whole steps between two events */
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    auto time_steps = [&](double* ms) -> int {
+        HIPCHK(hipEventRecord(e0, b->st));
+        CHK(batch_steps(b, steps, dt, flags, false));
+        HIPCHK(hipEventRecord(e1, b->st));
+        HIPCHK(hipEventSynchronize(e1));
+        float t = 0.f;
+        HIPCHK(hipEventElapsedTime(&t, e0, e1));
+        *ms = t;
+        return MG_OK;
+    };
+    double best_ms = 1e300;
+    bool best_persistent = false;
+    unsigned best_per_cu = 0;
+    b->persistent = false;
+    int r = time_steps(&best_ms);
+    if (r == MG_OK && b->persist_ok) {
+        const unsigned maxc = b->persist_per_cu;
+        b->persistent = true;
+        for (unsigned per_cu = 1; per_cu <= maxc && r == MG_OK; per_cu++) {
+            b->persist_use_per_cu = per_cu;
+            double ms = 0.0;
+            r = time_steps(&ms);
+            if (r == MG_OK) r = mg_batch_sync(b);
+            if (r == MG_OK && ms < best_ms) {
+                best_ms = ms;
+                best_persistent = true;
+                best_per_cu = per_cu;
+            }
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    b->persistent = best_persistent;
+    b->persist_use_per_cu = best_per_cu;
+    return r;
+}
+
+int mg_batch_launch_config(mg_batch* b, int* persistent, unsigned* workgroups)
+{
+    if (!b) return MG_ERR_ARG;
+    const bool on = b->persistent && b->persist_ok;
+    if (persistent) *persistent = on ? 1 : 0;
+    if (workgroups) *workgroups = on ? persist_workgroups(b) : 0u;
+    return MG_OK;
+}
+
+int mg_batch_set_persistent(mg_batch* b, int on, unsigned workgroups_per_cu)
+{
+    if (!b) return MG_ERR_ARG;
+    b->persistent = on != 0;
+    b->persist_use_per_cu = workgroups_per_cu;
     return MG_OK;
 }
 

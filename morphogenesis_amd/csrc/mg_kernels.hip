@@ -2330,4 +2330,95 @@ __global__ LAUNCH_BOUNDS_256 void kb_sort_gather(const BatchInst* __restrict__ I
     sort_gather_body(in.sort, in.vmax, b - in.pb0);
 }
 
+/* --------------------------------------------------- persistent batched step
+ * One launch per step for every instance of a batch: a grid of workgroups that stay resident
+ * (sized from the occupancy of this kernel) walks the seven phases of the fused step, each
+ * phase a queue of (instance, block) items taken with one atomic per item (largest instances
+ * first), with a grid barrier between phases. Each item runs the same body with the same
+ * block-local index as the instance's own launch, so every instance evolves bit for bit as
+ * it would alone.
+ *
+ * Barrier: sense reversal. The last workgroup to arrive resets the arrival count and the
+ * queue of the phase that just ended (every workgroup has left it), then flips `sense`; the
+ * others wait for the flip with an agent-scope acquire, so the phase's writes, released by
+ * every arriving workgroup, are visible to all XCDs after it. Queue 6 (the last phase) is
+ * reset at the first barrier of the next launch. Nothing needs clearing between launches,
+ * so the launch replays from a graph as it is. A wait that exceeds its bound raises `err`
+ * and gives up (the step's results are then invalid) instead of spinning forever, e.g.
+ * when the GPU cannot keep the whole grid resident. */
+#define PERSIST_PHASES 7
+struct PersistCtl {
+    unsigned count, sense, err, pad;
+    unsigned q[PERSIST_PHASES + 1];
+};
+
+struct PersistPlan {
+    const BatchInst* I;
+    const uint32_t* items[PERSIST_PHASES]; /* (instance << 16) | block, per phase */
+    uint32_t nitems[PERSIST_PHASES];
+    PersistCtl* ctl;
+};
+
+__device__ __forceinline__ void persist_barrier(PersistCtl* ctl, unsigned& sense, int ended)
+{
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        sense ^= 1u;
+        __atomic_thread_fence(__ATOMIC_RELEASE);
+        const unsigned arrived =
+            __hip_atomic_fetch_add(&ctl->count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) + 1u;
+        if (arrived == gridDim.x) {
+            __hip_atomic_store(&ctl->q[ended], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (ended == 0) __hip_atomic_store(&ctl->q[PERSIST_PHASES - 1], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&ctl->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&ctl->sense, sense, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            uint32_t spins = 0;
+            while (__hip_atomic_load(&ctl->sense, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != sense) {
+                if (++spins > (1u << 22)) {
+                    atomicOr(&ctl->err, 1u);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+            }
+        }
+        __atomic_thread_fence(__ATOMIC_ACQUIRE);
+    }
+    __syncthreads();
+}
+
+template <int NG>
+__global__ __launch_bounds__(256) void kb_step(PersistPlan P)
+{
+    extern __shared__ float4 mg_lds[];
+    __shared__ uint32_t s_item;
+    unsigned sense = 0;
+    if (threadIdx.x == 0) sense = __hip_atomic_load(&P.ctl->sense, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    for (int ph = 0; ph < PERSIST_PHASES; ph++) {
+        for (;;) {
+            if (threadIdx.x == 0) s_item = atomicAdd(&P.ctl->q[ph], 1u);
+            __syncthreads();
+            const uint32_t k = s_item;
+            __syncthreads();
+            if (k >= P.nitems[ph]) break;
+            const uint32_t it = P.items[ph][k];
+            const BatchInst& in = P.I[it >> 16];
+            const uint32_t bid = it & 0xFFFFu;
+            switch (ph) {
+            case 0: pressure_body(in.pressure, in.p, in.g, (Band<0>*)mg_lds, bid, in.pnb); break;
+            case 1: force_advance_body(in.force, in.p, in.g, (Band<1>*)mg_lds, in.dt, bid); break;
+            case 2:
+                diffuse_genes_insert_body<NG>(in.diffuse, in.p, in.g, *in.gen, in.dp, in.genes, (Band<NG>*)mg_lds, bid);
+                break;
+            case 3: scan_lookback_body(in.scan_in, in.scan_out, in.C, in.scan_status, in.scan_ticket, in.ctr, in.gsm, in.tnb); break;
+            case 4: place_body(in.place, bid); break;
+            case 5: cellsort_body(in.place, bid); break;
+            default: sort_gather_body(in.sort, in.vmax, bid); break;
+            }
+            __syncthreads(); /* the bodies' LDS is reused by the next item */
+        }
+        if (ph + 1 < PERSIST_PHASES) persist_barrier(P.ctl, sense, ph);
+    }
+}
+
 #include "mg_mlp.hip"

@@ -6,8 +6,9 @@
   mixed_4m          4,194,304 particles (bonded slab + fluid column with dense droplets):
                     first sort, then 2 steps;
   tissue_batch_256  256 bonded tissues of log-uniform size (4.58M particles): 4 batched
-                    steps (hipGraph) of every instance against the same instance stepped
-                    alone, and the smallest and largest instance against the oracle.
+                    steps (one persistent launch per step, hipGraph) of every instance
+                    against the same instance stepped alone, and the smallest and largest
+                    instance against the oracle.
 
 Time budget on one MI355X box with 16 host threads for the oracle: about 40 s, 60 s and 60 s
 (most of it the oracle and the scene builders); each test carries its own limit."""
@@ -102,6 +103,7 @@ def test_batch_256_instances(built):
     dt, fl = scs[0].dt, scs[0].flags
     bat.step(4, dt, fl | F.STEP_GRAPH)
     bat.sync()
+    assert bat.launch_config()[0] == 1, "the 256 instances step in one persistent launch per step"
     for s in solo:
         s.step(4, dt, fl)
     bufs = [F.FPOS, F.FVEL, F.FVEVAL, F.FPRESS, F.FDENSITY, F.FCONC, F.FEPIGEN, F.FELASTIDX,

@@ -304,8 +304,8 @@ def test_phase_calls_after_advance(built):
 def test_batch_matches_single(built):
     """Instances of three size classes (different genomes, bond counts and grids) stepped as
     one batch match the same instances stepped alone, bit for bit, through graph replay, odd
-    step counts, an autotuning pass and a second batch call; one instance is also checked
-    against the oracle."""
+    step counts, an autotuning pass (the solo instances then restart from the batched state)
+    and a second batch call; one instance is also checked against the oracle."""
     from morphogenesis_amd.batch import Batch
     from morphogenesis_amd.fluid_system import FluidSystem
     from oracle.mg_oracle import Oracle
@@ -323,28 +323,48 @@ def test_batch_matches_single(built):
     bat = Batch(inb)
     assert [m for m, _ in bat.classes()] == [3, 1, 1]
     dt, fl = scs[0].dt, scs[0].flags
+    bufs = [F.FPOS, F.FVEL, F.FVEVAL, F.FPRESS, F.FDENSITY, F.FCONC, F.FEPIGEN, F.FELASTIDX,
+            F.FGCELL, F.FGNDX, F.FBIN_COUNT, F.FBIN_OFFSET, F.FPARTICLE_ID]
+
+    def compare(what):
+        for k, (a, b) in enumerate(zip(solo, inb)):
+            assert a.counts() == b.counts(), (what, k)
+            for buf in bufs:
+                assert_bits_equal(b.download(buf), a.download(buf), f"{what}, instance {k}: {F.BUFFER_NAMES[buf]}")
+
     bat.step(7, dt, fl | F.STEP_GRAPH)
+    bat.sync()
     for s in solo:
         s.step(7, dt, fl)
+    compare("7 steps")
+    # autotuning takes real steps of the batch (both launch forms); the solo instances restart
+    # from the batched state
     chosen = bat.autotune(2, dt, fl)
     assert all(t in (64, 128, 256) for t in chosen)
-    for s in solo:
-        s.step(6, dt, fl)          # the autotune pass took 3 x 2 steps
+    bat.sync()
+    for k, sc in enumerate(scs):
+        solo[k].close()
+        s = FluidSystem(sc.n, sc.b)
+        s.set_grid(sc.grid)
+        s.set_params(sc.params)
+        s.set_genome(sc.genome)
+        s.TransferToCL(inb[k].TransferFromCL())
+        s.Run2PhysicalSort()
+        solo[k] = s
     bat.step(5, dt, fl | F.STEP_GRAPH)
     bat.sync()
     for s in solo:
         s.step(5, dt, fl)
-    bufs = [F.FPOS, F.FVEL, F.FVEVAL, F.FPRESS, F.FDENSITY, F.FCONC, F.FEPIGEN, F.FELASTIDX,
-            F.FGCELL, F.FGNDX, F.FBIN_COUNT, F.FBIN_OFFSET, F.FPARTICLE_ID]
-    for k, (a, b) in enumerate(zip(solo, inb)):
-        assert a.counts() == b.counts(), k
-        for buf in bufs:
-            assert_bits_equal(b.download(buf), a.download(buf), f"instance {k}: {F.BUFFER_NAMES[buf]}")
+    compare("after autotuning, 5 steps")
     o = Oracle(scs[1].n, scs[1].b)
-    o.load_scene(scs[1])
+    o.set_grid(scs[1].grid)
+    o.set_params(scs[1].params)
+    o.set_genome(scs[1].genome)
+    o.TransferToCL(solo[1].TransferFromCL())
     o.Run2PhysicalSort()
-    o.step(18, dt, fl)
-    compare_all(inb[1], o, "batched instance vs oracle", [F.FPOS, F.FVEL, F.FCONC, F.FGCELL])
+    solo[1].step(3, dt, fl)
+    o.step(3, dt, fl)
+    compare_all(solo[1], o, "restarted instance vs oracle", [F.FPOS, F.FVEL, F.FCONC, F.FGCELL])
     bat.close()
 
 
@@ -395,3 +415,41 @@ def test_diffusion_skin_masks_and_fallback(built, speed):
     o.step(10, sc.dt, sc.flags)
     compare_all(g, o, f"skin speed {speed}", [F.FPOS, F.FVEL, F.FCONC, F.FEPIGEN, F.FPRESS, F.FDENSITY,
                                                F.FGCELL, F.FBIN_OFFSET])
+
+
+@pytest.mark.parametrize("persistent", [1, 0])
+def test_batch_launch_forms_match_single(built, persistent):
+    """The persistent launch (one kb_step per step: every phase of every instance, grid barriers
+    between phases) and the dispatch form, each forced, match the same instances stepped alone
+    bit for bit, through graph replay and eager steps of both parities."""
+    from morphogenesis_amd.batch import Batch
+    from morphogenesis_amd.fluid_system import FluidSystem
+
+    specs = [(9, 6, 51), (12, 32, 52), (21, 8, 53), (30, 8, 54), (13, 4, 55), (10, 16, 56)]
+    scs = [scenes.tissue(n_side=n, b=b, seed=sd) for n, b, sd in specs]
+    say("case: batch launch form persistent =", persistent, [sc.n for sc in scs])
+    solo, inb = [], []
+    for sc in scs:
+        for lst in (solo, inb):
+            s = FluidSystem(sc.n, sc.b)
+            s.load_scene(sc)
+            s.Run2PhysicalSort()
+            lst.append(s)
+    bat = Batch(inb)
+    bat.set_persistent(bool(persistent))
+    dt, fl = scs[0].dt, scs[0].flags
+    bat.step(6, dt, fl | F.STEP_GRAPH)
+    bat.step(3, dt, fl)
+    bat.step(4, dt, fl | F.STEP_GRAPH)
+    bat.sync()
+    on, wg = bat.launch_config()
+    assert on == persistent and (wg > 0) == bool(persistent)
+    for s in solo:
+        s.step(13, dt, fl)
+    bufs = [F.FPOS, F.FVEL, F.FVEVAL, F.FPRESS, F.FDENSITY, F.FCONC, F.FEPIGEN, F.FELASTIDX,
+            F.FGCELL, F.FGNDX, F.FBIN_COUNT, F.FBIN_OFFSET, F.FPARTICLE_ID]
+    for k, (a, b) in enumerate(zip(solo, inb)):
+        assert a.counts() == b.counts(), k
+        for buf in bufs:
+            assert_bits_equal(b.download(buf), a.download(buf), f"instance {k}: {F.BUFFER_NAMES[buf]}")
+    bat.close()
