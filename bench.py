@@ -120,6 +120,38 @@ def scene_stats(scene, neighbours: bool = True) -> dict:
     return {"bonds_out": bo, "bonds_in": bi, "neighbours": nbrs}
 
 
+# the step count each workload's configuration names (the 262k tissue: 1000 steps of 1e-4)
+CONFIG_STEPS = {"tissue_262k": 1000, "fluid_block_32k": 100, "mixed_4m": 200}
+
+
+def config_run(scene, steps: int, flags: int) -> dict:
+    """The configuration's whole run from its initial state: `steps` steps on a fresh system,
+    timed in chunks. The state changes along the run (the tissue settles and compresses under
+    gravity, so cells fill up and the neighbour sweeps slow down); `value` above is the K-step
+    window after W warmup steps that the benchmark contract times."""
+    from morphogenesis_amd.fluid_system import FluidSystem
+
+    sim = FluidSystem(scene.n, scene.b)
+    sim.load_scene(scene)
+    sim.Run2PhysicalSort()
+    chunk = max(steps // 10, 1)
+    done, total_s, per_chunk = 0, 0.0, []
+    while done < steps:
+        k = min(chunk, steps - done)
+        sim.sync()
+        t = time.perf_counter()
+        sim.step(k, scene.dt, flags)
+        sim.sync()
+        el = time.perf_counter() - t
+        total_s += el
+        done += k
+        per_chunk.append(round(el / k * 1e3, 4))
+    sim.close()
+    return {"steps": steps, "value": scene.n * steps / total_s, "unit": "particle-steps/s",
+            "ms_per_step": total_s / steps * 1e3, "ms_per_step_by_chunk": per_chunk,
+            "chunk_steps": chunk, "note": "from the initial state; includes graph capture and re-homing"}
+
+
 def cpu_baseline(scene, budget_s: float = 15.0) -> dict:
     """The FP32 oracle (a plain C++ restatement of the reference step, OpenMP over all host
     cores) on a bounded sample of the same workload: the same scene, a few steps."""
@@ -179,6 +211,8 @@ def run_batch(args, count: int = 256):
     dt, flags = scs[0].dt, scs[0].flags
     bat = Batch(systems)
     chosen = bat.autotune(2, dt, flags)
+    if args.batch_form != "auto":
+        bat.set_persistent(args.batch_form == "persistent")
     persistent, workgroups = bat.launch_config()
     progress(f"autotuned: block sizes {chosen}, persistent launch {persistent} ({workgroups} workgroups)")
     bat.step(args.warmup, dt, flags | F.STEP_GRAPH)
@@ -353,6 +387,27 @@ def run_learned(args):
     print(json.dumps(out))
 
 
+def roofline(dom: str, scene, stats: dict, dom_ms: float, workload: str) -> dict:
+    """The dominant kernel against HBM bandwidth. `achieved` counts the compulsory bytes of
+    the reference's layout for the work the kernel does (reference_layout_bytes, terms
+    listed); `layout` counts the compulsory bytes of this build's own layout
+    (algorithmic_bytes), which move far fewer bytes per particle; `traffic` is the measured
+    HBM traffic per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated)."""
+    ref_bytes, terms = reference_layout_bytes(dom, scene, stats)
+    lay = algorithmic_bytes(dom, scene, stats)
+    ach = ref_bytes / (dom_ms * 1e-3) / 1e9
+    lay_ach = lay / (dom_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "kernel": KERNEL_OF_PHASE[dom], "achieved": ach, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+            "traffic": load_traffic(KERNEL_OF_PHASE[dom], workload),
+            "algorithmic_bytes_per_launch": ref_bytes,
+            "algorithmic_basis": "reference layout, per particle: " +
+                                 ", ".join(f"{k} {v:.1f} B" for k, v in terms.items()),
+            "avg_launch_ms": dom_ms,
+            "layout": {"bytes_per_launch": lay, "achieved": lay_ach, "frac": lay_ach / HBM_PEAK_GBS,
+                       "basis": "this build's HBM layout, every element once"}}
+
+
 def load_traffic(kernel: str, workload: str):
     p = os.path.join(REPO, "profiles", f"traffic_{kernel}_{workload}.json")
     if not os.path.exists(p):
@@ -509,6 +564,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--timed-steps", type=int, default=20)
     ap.add_argument("--no-graph", action="store_true", help="launch the step kernels eagerly")
+    ap.add_argument("--batch-form", choices=("auto", "persistent", "dispatch"), default="auto",
+                    help="tissue_batch_256: the launch form (auto: the autotuner's choice)")
+    ap.add_argument("--no-config-run", action="store_true",
+                    help="leave out the timing of the configuration's whole run")
     ap.add_argument("--no-learned", action="store_true",
                     help="leave out the learned 4 x dt measurement of the default line")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
@@ -617,6 +676,8 @@ def main():
         "kernels": kernels,
         "mean_neighbours": round(stats["neighbours"], 2),
     }
+    if args.workload in CONFIG_STEPS and not args.no_config_run:
+        out["config_run"] = config_run(scene, CONFIG_STEPS[args.workload], flags)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(scene)
     if args.workload == "tissue_262k" and not args.no_learned:
