@@ -53,8 +53,8 @@ def algorithmic_bytes(phase: str, scene, stats: dict) -> float:
         return n * (56 + hm + 4 * b + bo * 36 + 8 + 4 * bi + 48)
     if phase == "diffusion":
         # P 16, CONC 64 read, CONC 64 written, GCELL 4 read/written, GNDX 4, GMASK 4, HOME 4,
-        # EPIGEN 64 read (+64 written with genes), GMN 4
-        return n * (16 + 128 + 8 + 4 + 4 + 4 + 64 + (64 if genes else 0) + 4) + 4 * c
+        # EPIGEN 64 read (+64 written with genes), GMN 4; the hit masks read (within the skin)
+        return n * (16 + 128 + 8 + 4 + 4 + 4 + 64 + (64 if genes else 0) + 4 + hm) + 4 * c
     if phase == "scan":
         return 8 * c
     if phase == "place":  # k_place: GCELL, GNDX read, PERM written; k_cellsort: PERM r/w, GNDX w

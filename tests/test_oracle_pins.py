@@ -30,7 +30,7 @@ def expected_cells(pos, grid):
         g[:, k] = (pos[:, k].astype(f) - f(grid.gmin[k])) * f(grid.delta[k])
     ok = np.all(np.isfinite(g), axis=1)
     gi = np.zeros_like(g, dtype=np.int64)
-    gi[ok] = np.trunc(g[ok]).astype(np.int64)
+    gi[ok] = np.trunc(np.clip(g[ok], -2.0, 2.0 ** 40)).astype(np.int64)  # huge finite: rejected
     sm = np.array(list(grid.scan_max))
     ok &= np.all((gi >= 1) & (gi <= sm), axis=1)
     res = list(grid.res)
