@@ -271,7 +271,7 @@ def run_batch(args, count: int = 256):
                    "hipgraph": True,
                    "launch": {"persistent": bool(persistent), "workgroups": workgroups,
                               "ms_per_step_by_form": {k: round(v, 4) for k, v in forms.items()}},
-                   "size_classes": [{"instances": m, "threads": t} for m, t in classes],
+                   "size_classes": bat.class_config(),
                    "autotuned_threads": chosen,
                    "step": ("pressure+force+advance+diffusion+genes+insert+scan+sort, "
                             + ("one persistent launch per step for all instances" if persistent

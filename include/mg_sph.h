@@ -194,10 +194,19 @@ int mg_batch_sync(mg_batch* b);
 int mg_batch_classes(mg_batch* b, int* nclass, int* members, unsigned* threads);
 /* Measures `steps` steps of the dispatch form at each block size (256, 128, 64) and keeps,
  * per size class, the one with the least kernel time (threads_out, one entry per class, may be
- * null, receives it); then times `steps` whole steps of that dispatch form and of the
+ * null, receives it); then, at those block sizes, the pressure sweep's tile shape per class
+ * (candidates tested at a time 8 or 4, staged band of 128 or 64 slots per wavefront) with the
+ * least pressure time; then times `steps` whole steps of that dispatch form and of the
  * persistent form at each workgroup count per CU that stays resident, and keeps the fastest.
  * The steps are real steps of the batch. */
 int mg_batch_autotune(mg_batch* b, int steps, float dt, unsigned flags, unsigned* threads_out);
+/* The dispatch form's configuration of size class cls (0 .. nclass - 1): its instances, block
+ * size, and the pressure sweep's candidate batch and band capacity (any output may be null). */
+int mg_batch_class_config(mg_batch* b, int cls, int* instances, unsigned* threads, int* cand_batch,
+                          int* band_cap);
+/* Sets size class cls's pressure sweep tile shape for the dispatch form: cand_batch 8 or 4,
+ * band_cap 128 or 64 (MG_ERR_ARG otherwise). Every shape gives the same bits. */
+int mg_batch_set_tile_shape(mg_batch* b, int cls, int cand_batch, int band_cap);
 /* The launch form in use: *persistent = 1 for one persistent launch per step (with
  * *workgroups workgroups of 256 threads), 0 for the dispatch form. Either may be null. */
 int mg_batch_launch_config(mg_batch* b, int* persistent, unsigned* workgroups);

@@ -239,6 +239,8 @@ def declare(lib, prefix: str):
         sig["batch_autotune"] = [h, C.c_int, C.c_float, C.c_uint, P(C.c_uint)]
         sig["batch_step_timed"] = [h, C.c_int, C.c_float, C.c_uint, P(C.c_double), P(C.c_int)]
         sig["batch_launch_config"] = [h, P(C.c_int), P(C.c_uint)]
+        sig["batch_class_config"] = [h, C.c_int, P(C.c_int), P(C.c_uint), P(C.c_int), P(C.c_int)]
+        sig["batch_set_tile_shape"] = [h, C.c_int, C.c_int, C.c_int]
         sig["batch_set_persistent"] = [h, C.c_int, C.c_uint]
     else:
         sig["create"] = [P(h), C.c_int, C.c_int, C.c_int]

@@ -46,6 +46,21 @@ class Batch:
         self._call("batch_autotune", self.h, int(steps), float(dt), int(flags), out)
         return list(out)
 
+    def class_config(self):
+        """Per size class: {instances, threads, cand_batch, band_cap} of the dispatch form (block
+        size and the pressure sweep's tile shape)."""
+        out = []
+        for k in range(len(self.classes())):
+            inst, thr, kb, cap = C.c_int(), C.c_uint(), C.c_int(), C.c_int()
+            self._call("batch_class_config", self.h, k, C.byref(inst), C.byref(thr), C.byref(kb), C.byref(cap))
+            out.append({"instances": inst.value, "threads": thr.value, "cand_batch": kb.value,
+                        "band_cap": cap.value})
+        return out
+
+    def set_tile_shape(self, cls: int, cand_batch: int, band_cap: int):
+        """Pins size class cls's pressure sweep shape (cand_batch 8 or 4, band_cap 128 or 64)."""
+        self._call("batch_set_tile_shape", self.h, int(cls), int(cand_batch), int(band_cap))
+
     def launch_config(self):
         """(persistent, workgroups): 1 and the workgroup count for one persistent launch per
         step, 0 for the dispatch form."""

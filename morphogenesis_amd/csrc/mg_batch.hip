@@ -18,6 +18,7 @@ struct mg_batch {
     struct Class {
         std::vector<int> members;     /* handle indices, largest first */
         unsigned threads = 256;       /* block size of the particle / cell launches */
+        int pshape = 0;               /* pressure sweep tile shape (kPShapes) */
         int ng = 1;                   /* diffusion sweep template: 1, 2 or 4 groups */
         BatchInst* I[2] = {};         /* tables for the start parity and the next one */
         uint16_t *pmap = nullptr, *cmap = nullptr, *tmap = nullptr;
@@ -63,6 +64,15 @@ namespace {
 /* size classes by particle count */
 const int kBatchClassBound[] = {8192, 24576};
 const int kBatchClasses = 3;
+
+/* tile shapes of the pressure sweep the autotuner chooses from: candidates tested at a time
+ * and the slots of a wavefront's staged band (the force and diffusion sweeps mostly visit the
+ * pressure sweep's recorded hits, so their shapes stay fixed) */
+struct PShape {
+    int K, cap;
+};
+const PShape kPShapes[] = {{8, 128}, {4, 128}, {8, 64}, {4, 64}};
+const int kNPShapes = 4;
 
 uint64_t handle_sig(const mg_handle* h)
 {
@@ -366,7 +376,24 @@ int batch_launch(mg_batch* b, int q)
         auto& c = b->cls[ci];
         if (c.members.empty()) continue;
         batch_mark(b, (int)ci, MG_PH_PRESSURE, true);
-        hipLaunchKernelGGL(kb_pressure, dim3(c.pblocks), dim3(c.threads), band_lds<0>(c.threads), b->st, c.I[q], c.pmap);
+        switch (c.pshape) {
+        case 1:
+            hipLaunchKernelGGL((kb_pressure<4, 128>), dim3(c.pblocks), dim3(c.threads), (band_lds<0, 128>(c.threads)),
+                               b->st, c.I[q], c.pmap);
+            break;
+        case 2:
+            hipLaunchKernelGGL((kb_pressure<8, 64>), dim3(c.pblocks), dim3(c.threads), (band_lds<0, 64>(c.threads)),
+                               b->st, c.I[q], c.pmap);
+            break;
+        case 3:
+            hipLaunchKernelGGL((kb_pressure<4, 64>), dim3(c.pblocks), dim3(c.threads), (band_lds<0, 64>(c.threads)),
+                               b->st, c.I[q], c.pmap);
+            break;
+        default:
+            hipLaunchKernelGGL((kb_pressure<8, 128>), dim3(c.pblocks), dim3(c.threads), (band_lds<0, 128>(c.threads)),
+                               b->st, c.I[q], c.pmap);
+            break;
+        }
         batch_mark(b, (int)ci, MG_PH_PRESSURE, false);
     }
     for (size_t ci = 0; ci < b->cls.size(); ci++) {
@@ -611,29 +638,41 @@ int mg_batch_classes(mg_batch* b, int* nclass, int* members, unsigned* threads)
     return MG_OK;
 }
 
+/* `steps` dispatch-form steps with HIP events per launch group; ms[class] accumulates the
+ * milliseconds of the launches of phase `ph` (every phase when ph < 0) */
+static int batch_time_classes(mg_batch* b, int steps, float dt, unsigned flags, int ph,
+                              std::vector<double>& ms)
+{
+    b->timing = true;
+    b->marks.clear();
+    int r = batch_steps(b, steps, dt, flags, false);
+    b->timing = false;
+    if (r != MG_OK) return r;
+    HIPCHK(hipStreamSynchronize(b->st));
+    ms.assign(b->cls.size(), 0.0);
+    for (auto& m : b->marks) {
+        if (ph >= 0 && m.ph != ph) continue;
+        float t = 0.f;
+        HIPCHK(hipEventElapsedTime(&t, b->ev[m.ev], b->ev[m.ev + 1]));
+        ms[m.cls] += t;
+    }
+    b->marks.clear();
+    return MG_OK;
+}
+
 int mg_batch_autotune(mg_batch* b, int steps, float dt, unsigned flags, unsigned* threads_out)
 {
     if (!b || steps < 1) { set_err("mg_batch_autotune: bad arguments"); return MG_ERR_ARG; }
     const unsigned cand[] = {256, 128, 64};
     const int nc = (int)b->cls.size();
-    std::vector<double> best(nc, 1e300);
+    std::vector<double> best(nc, 1e300), ms;
     std::vector<unsigned> pick(nc, 256);
+    /* block size per class, at the default tile shape */
+    for (auto& c : b->cls) c.pshape = 0;
     for (unsigned T : cand) {
         for (auto& c : b->cls) c.threads = T;
         b->built = false; /* block geometry changed */
-        b->timing = true;
-        b->marks.clear();
-        int r = batch_steps(b, steps, dt, flags, false);
-        b->timing = false;
-        if (r != MG_OK) return r;
-        HIPCHK(hipStreamSynchronize(b->st));
-        std::vector<double> ms(nc, 0.0);
-        for (auto& m : b->marks) {
-            float t = 0.f;
-            HIPCHK(hipEventElapsedTime(&t, b->ev[m.ev], b->ev[m.ev + 1]));
-            ms[m.cls] += t;
-        }
-        b->marks.clear();
+        CHK(batch_time_classes(b, steps, dt, flags, -1, ms));
         for (int c = 0; c < nc; c++)
             if (!b->cls[c].members.empty() && ms[c] < best[c]) {
                 best[c] = ms[c];
@@ -646,6 +685,19 @@ int mg_batch_autotune(mg_batch* b, int steps, float dt, unsigned flags, unsigned
         if (threads_out) threads_out[c] = pick[c];
     }
     b->built = false;
+    /* the pressure sweep's tile shape per class, at the chosen block sizes */
+    std::vector<double> pbest(nc, 1e300);
+    std::vector<int> pshape(nc, 0);
+    for (int sh = 0; sh < kNPShapes; sh++) {
+        for (auto& c : b->cls) c.pshape = sh;
+        CHK(batch_time_classes(b, steps, dt, flags, MG_PH_PRESSURE, ms));
+        for (int c = 0; c < nc; c++)
+            if (!b->cls[c].members.empty() && ms[c] < pbest[c]) {
+                pbest[c] = ms[c];
+                pshape[c] = sh;
+            }
+    }
+    for (int c = 0; c < nc; c++) b->cls[c].pshape = pshape[c];
     /* the dispatch form with these block sizes against the persistent form at each workgroup
      * count per CU: whole steps between two events */
     hipEvent_t e0, e1;
@@ -686,6 +738,32 @@ int mg_batch_autotune(mg_batch* b, int steps, float dt, unsigned flags, unsigned
     b->persistent = best_persistent;
     b->persist_use_per_cu = best_per_cu;
     return r;
+}
+
+int mg_batch_class_config(mg_batch* b, int cls, int* instances, unsigned* threads, int* cand_batch, int* band_cap)
+{
+    if (!b || cls < 0 || cls >= (int)b->cls.size()) return MG_ERR_ARG;
+    const auto& c = b->cls[cls];
+    if (instances) *instances = (int)c.members.size();
+    if (threads) *threads = c.threads;
+    if (cand_batch) *cand_batch = kPShapes[c.pshape].K;
+    if (band_cap) *band_cap = kPShapes[c.pshape].cap;
+    return MG_OK;
+}
+
+int mg_batch_set_tile_shape(mg_batch* b, int cls, int cand_batch, int band_cap)
+{
+    if (!b || cls < 0 || cls >= (int)b->cls.size()) {
+        set_err("mg_batch_set_tile_shape: bad arguments");
+        return MG_ERR_ARG;
+    }
+    for (int sh = 0; sh < kNPShapes; sh++)
+        if (kPShapes[sh].K == cand_batch && kPShapes[sh].cap == band_cap) {
+            b->cls[cls].pshape = sh;
+            return MG_OK;
+        }
+    set_err("mg_batch_set_tile_shape: no pressure sweep of candidate batch %d and band %d", cand_batch, band_cap);
+    return MG_ERR_ARG;
 }
 
 int mg_batch_launch_config(mg_batch* b, int* persistent, unsigned* workgroups)

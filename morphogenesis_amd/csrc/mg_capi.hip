@@ -253,6 +253,8 @@ struct mg_handle {
     bool force_zero = true;
     /* learned coarse steps: per-slot velocity correction and the fallback counter */
     float4* DV = nullptr;
+    uint4* XB = nullptr;    /* learned step: normalised bf16 inputs, N x 64 */
+    uint32_t* MF = nullptr; /* learned step: per-slot flags (active, inputs in range) */
     uint32_t* fallbacks = nullptr;
     /* home-indexed */
     uint32_t *SLOT = nullptr, *EPIGEN = nullptr, *BPARTNER = nullptr;
@@ -690,8 +692,11 @@ int maybe_rehome(mg_handle* h)
     return MG_OK;
 }
 
-/* dynamic LDS of a sweep kernel: one Band<NX> per wavefront of the block */
-template <int NX> constexpr size_t band_lds(unsigned threads) { return (threads / 64) * sizeof(Band<NX>); }
+/* dynamic LDS of a sweep kernel: one Band<NX, CAP> per wavefront of the block */
+template <int NX, int CAP = BAND_CAP> constexpr size_t band_lds(unsigned threads)
+{
+    return (threads / 64) * sizeof(Band<NX, CAP>);
+}
 
 /* The FCONC float4 groups the diffusion sweep reads and the per-TF coefficients
  * tf_diffusability * diff_scale (the same float operations as the reference's loop). */

@@ -400,12 +400,35 @@ __device__ __forceinline__ void place_body(const DS& d, uint32_t bid)
 
 __global__ LAUNCH_BOUNDS_256 void k_place(DS d) { place_body(d, blockIdx.x); }
 
-/* One lane per cell. Cells of up to CELLSORT_SMALL particles are insertion-sorted by their
- * lane; a larger cell is ranked by the whole wavefront through LDS (each element's rank is
- * the number of smaller slot indices in the run; the values are distinct), so a crowded
- * cell costs O(n^2 / 64) per lane instead of O(n^2) on one lane. Runs longer than
+/* A run of up to M distinct slot indices, ranked in registers by its lane: the run is loaded
+ * with independent loads, each element's rank is the number of smaller values, and the
+ * elements are stored at their ranks (no dependent memory round trips). */
+template <int M>
+__device__ __forceinline__ void cellsort_small(const DS& d, uint32_t* run, uint32_t n)
+{
+    uint32_t v[M];
+#pragma unroll
+    for (int k = 0; k < M; k++) v[k] = ((uint32_t)k < n) ? run[k] : 0xFFFFFFFFu;
+#pragma unroll
+    for (int k = 0; k < M; k++) {
+        if ((uint32_t)k < n) {
+            uint32_t r = 0;
+#pragma unroll
+            for (int q = 0; q < M; q++) r += v[q] < v[k] ? 1u : 0u;
+            run[r] = v[k];
+            d.GNDX[v[k]] = r;
+        }
+    }
+}
+
+/* One lane per cell. Cells of up to CELLSORT_SMALL particles are ranked in registers by their
+ * lane, 8, 16 or 32 wide by the wavefront's largest such cell, so all of a wavefront's cells
+ * sort behind one round of loads; a larger cell is ranked by the whole wavefront (each
+ * element's rank is the number of smaller slot indices in the run; the values are distinct):
+ * up to 64 elements one per lane against lane broadcasts, longer runs through LDS, so a
+ * crowded cell costs O(n^2 / 64) per lane instead of O(n^2) on one lane. Runs longer than
  * CELLSORT_CAP fall back to the owning lane's insertion sort. */
-#define CELLSORT_SMALL 16
+#define CELLSORT_SMALL 32
 #define CELLSORT_CAP 512
 __device__ __forceinline__ void cellsort_body(const DS& d, uint32_t bid)
 {
@@ -414,7 +437,15 @@ __device__ __forceinline__ void cellsort_body(const DS& d, uint32_t bid)
     const uint32_t n = (c < (uint32_t)d.C) ? d.CNT[c] : 0u;
     const uint32_t lane = __lane_id(), w = threadIdx.x >> 6;
     const bool big = n > CELLSORT_SMALL && n <= CELLSORT_CAP;
-    if (n > 1 && !big) {
+    const bool small = n > 1 && n <= CELLSORT_SMALL;
+    if (__ballot(small && n > 8) == 0ull) {
+        if (small) cellsort_small<8>(d, d.PERM + d.OFF[c], n);
+    } else if (__ballot(small && n > 16) == 0ull) {
+        if (small) cellsort_small<16>(d, d.PERM + d.OFF[c], n);
+    } else if (small) {
+        cellsort_small<CELLSORT_SMALL>(d, d.PERM + d.OFF[c], n);
+    }
+    if (n > CELLSORT_CAP) {
         uint32_t* run = d.PERM + d.OFF[c];
         for (uint32_t i = 1; i < n; i++) {
             uint32_t v = run[i];
@@ -430,9 +461,20 @@ __device__ __forceinline__ void cellsort_body(const DS& d, uint32_t bid)
     while (todo) {
         const int src = __ffsll((long long)todo) - 1;
         todo &= todo - 1ull;
-        const uint32_t cb = (uint32_t)__shfl((int)c, src);
-        const uint32_t nb = (uint32_t)__shfl((int)n, src);
+        const uint32_t cb = (uint32_t)__builtin_amdgcn_readlane((int)c, src);
+        const uint32_t nb = (uint32_t)__builtin_amdgcn_readlane((int)n, src);
         uint32_t* run = d.PERM + d.OFF[cb];
+        if (nb <= 64u) {
+            /* one element per lane, ranked against every other through lane broadcasts */
+            const uint32_t v = lane < nb ? run[lane] : 0xFFFFFFFFu;
+            uint32_t rank = 0;
+            for (uint32_t k = 0; k < nb; k++) rank += (uint32_t)__builtin_amdgcn_readlane((int)v, (int)k) < v ? 1u : 0u;
+            if (lane < nb) {
+                run[rank] = v;
+                d.GNDX[v] = rank;
+            }
+            continue;
+        }
         for (uint32_t e = lane; e < nb; e += 64) buf[w][e] = run[e];
         wave_lds_sync();
         for (uint32_t e = lane; e < nb; e += 64) {
@@ -605,10 +647,10 @@ __device__ __forceinline__ float axis_gap(float fr, int o, float inv_delta)
 #define BAND_CAP 128
 #define BAND_PAD 8 /* a batch of up to 8 candidates may read past the band's end */
 
-template <int NX>
+template <int NX, int CAP = BAND_CAP>
 struct Band {
-    float4 p[BAND_CAP + BAND_PAD];
-    float4 x[NX > 0 ? NX : 1][BAND_CAP + BAND_PAD];
+    float4 p[CAP + BAND_PAD];
+    float4 x[NX > 0 ? NX : 1][CAP + BAND_PAD];
 };
 
 
@@ -661,9 +703,10 @@ __device__ __forceinline__ void cell_candidates(uint32_t lo, uint32_t hi, f3 pi,
 }
 
 /* Must be called by all 64 lanes of the wavefront (`valid` selects the lanes that sweep);
- * xload(j, q) is the global load of extra field q of slot j. */
-template <int K, int NX, int HM = 0, class XLoad, class Visit, class EndCell>
-__device__ __forceinline__ void sweep(const DS& d, const MgGrid& g, Band<NX>& wb, bool valid,
+ * xload(j, q) is the global load of extra field q of slot j. K candidates are tested at a
+ * time; bands of up to CAP (<= 128) slots are staged. */
+template <int K, int NX, int HM = 0, int CAP = BAND_CAP, class XLoad, class Visit, class EndCell>
+__device__ __forceinline__ void sweep(const DS& d, const MgGrid& g, Band<NX, CAP>& wb, bool valid,
                                       uint32_t gc0, f3 pi, float lim, XLoad xload, Visit visit,
                                       EndCell end_cell, uint32_t self = 0u,
                                       float rec_lim = 0.0f, bool hm_ok = true)
@@ -697,7 +740,8 @@ __device__ __forceinline__ void sweep(const DS& d, const MgGrid& g, Band<NX>& wb
         const uint32_t a = (uint32_t)g.adj[s * g.srch];
         blo = min(d.OFF[cmin + a], N);
         bhi = min(d.OFF[cmax + a + (uint32_t)g.srch], N);
-        return bhi >= blo && bhi - blo <= BAND_CAP;
+        static_assert(CAP <= 128, "two slots per lane");
+        return bhi >= blo && bhi - blo <= (uint32_t)CAP;
     };
     /* the next band, two slots per lane (lane and lane + 64) */
     float4 rp0 = make_float4(0.f, 0.f, 0.f, 0.f), rp1 = rp0, rx0[NXX], rx1[NXX];
@@ -755,19 +799,20 @@ __device__ __forceinline__ void sweep(const DS& d, const MgGrid& g, Band<NX>& wb
             nxt = row_bounds(d, g, valid ? gc0 : cmin, s + 1);
         }
         if (!valid) continue;
+        /* the first cell boundary of the row above slot j */
+        auto next_bound = [&](uint32_t j) -> uint32_t {
+            uint32_t c = 0xFFFFFFFFu;
+#pragma unroll
+            for (int k = 3; k >= 1; k--)
+                if (k < ncell && j < rb.b[k]) c = rb.b[k];
+            return c;
+        };
         if (hm_r && hm_row != ~0ull) {
             /* the pressure sweep's hits of this row, ascending; a cell's partial sum is
              * flushed where the first hit past its end is visited and after the row (the
              * skipped flushes add +0 to sums that start at +0) */
             uint64_t m = hm_row;
             const uint32_t b0 = rb.b[0];
-            auto next_bound = [&](uint32_t j) -> uint32_t {
-                uint32_t c = 0xFFFFFFFFu;
-#pragma unroll
-                for (int k = 3; k >= 1; k--)
-                    if (k < ncell && j < rb.b[k]) c = rb.b[k];
-                return c;
-            };
             uint32_t cnext = next_bound(b0);
             while (m != 0ull) {
                 const uint32_t j = b0 + (uint32_t)__builtin_ctzll(m);
@@ -807,14 +852,65 @@ __device__ __forceinline__ void sweep(const DS& d, const MgGrid& g, Band<NX>& wb
             const float gz = axis_gap(fr.z, s % g.srch - 1, inv.z);
             gyz = gy * gy + gz * gz;
         }
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
+        auto live_of = [&](int k) -> bool {
             bool live = k < ncell;
             if (cull && live) {
                 const float gx = axis_gap(fr.x, k - 1, inv.x);
                 live = gyz + gx * gx < tlim;
             }
-            if (!live) continue;
+            return live;
+        };
+        /* sweeps that read recorded masks take the candidate path only for unrecorded rows,
+         * and always per cell */
+        bool over = false;
+        if (HM != 2) {
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                if (k < ncell) over |= rb.b[k + 1] - rb.b[k] > 8u;
+        }
+        if (HM != 2 && __ballot(over) != 0ull) {
+            /* a lane of the wavefront has a cell of more than 8 candidates: every lane walks
+             * its live cells as one slot range, K candidates at a time, and flushes a cell's
+             * partial sum at the first candidate past the cell's end and after the range
+             * (fewer partly filled batches once cell occupancies vary; in a lattice of 8 per
+             * cell the per-cell batches below fill exactly). The live cells are contiguous:
+             * a cell's gap grows away from the own column (offset 0, gap 0). */
+            bool any = false;
+            uint32_t flo = 0u, fhi = 0u;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                if (!live_of(k)) continue;
+                if (!any) flo = rb.b[k];
+                any = true;
+                fhi = max(rb.b[k + 1], rb.b[k]);
+            }
+            if (any) {
+                const uint32_t lo = flo, hi = max(fhi, flo);
+                uint32_t cnext = next_bound(lo);
+                auto visit_flat = [&](uint32_t j, float4 pj, f3 dist, float dsq, const float4* x) {
+                    if (j >= cnext) {
+                        end_cell();
+                        cnext = next_bound(j);
+                    }
+                    visit_rec(j, pj, dist, dsq, x);
+                };
+                if (cfits) {
+                    cell_candidates<K, NX>(
+                        lo, hi, pi, tlim, [&](uint32_t j) { return wb.p[j - cblo]; },
+                        [&](uint32_t j, int x) { return wb.x[x][j - cblo]; }, visit_flat);
+                } else {
+                    cell_candidates<K, NX>(
+                        lo, hi, pi, tlim, [&](uint32_t j) { return d.P[j]; },
+                        [&](uint32_t j, int x) { return xload(j, x); }, visit_flat);
+                }
+                end_cell();
+            }
+            if (hm_w) d.HMW[(size_t)s * N + self] = wide ? ~0ull : rec;
+            continue;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if (!live_of(k)) continue;
             const uint32_t lo = rb.b[k], hi = max(rb.b[k + 1], lo);
             if (cfits) {
                 cell_candidates<K, NX>(
@@ -835,8 +931,9 @@ __device__ __forceinline__ void sweep(const DS& d, const MgGrid& g, Band<NX>& wb
  * contributePressure/computePressure, src/fluid_system_cuda.cu:620-712. Per cell partial
  * sums are added to the running total in gridAdj order. Writes FPRESS into P.w and
  * FDENSITY (= 1/density) into V.w. Also clears d.ZERO (the next grid's counters) when set. */
+template <int K = 8, int CAP = BAND_CAP>
 __device__ __forceinline__ void pressure_body(const DS& d, const MgParams& p, const MgGrid& g,
-                                              Band<0>* bands, uint32_t bid, uint32_t nblk)
+                                              Band<0, CAP>* bands, uint32_t bid, uint32_t nblk)
 {
     if (d.ZERO) {
         for (uint32_t z = bid * blockDim.x + threadIdx.x; z < d.zero_n; z += nblk * blockDim.x)
@@ -862,7 +959,7 @@ __device__ __forceinline__ void pressure_body(const DS& d, const MgParams& p, co
     const bool sr_pow2 = (srb & 0x007FFFFFu) == 0u && ((srb >> 23) & 0xFFu) != 0u && ((srb >> 23) & 0xFFu) != 0xFFu;
     if (sr_pow2) {
         const float inv_sr = 1.0f / sr;
-        sweep<8, 0, 1>(
+        sweep<K, 0, 1, CAP>(
             d, g, bands[threadIdx.x >> 6], valid, gc, pi, p.r2, none,
             [&](uint32_t, float4, f3, float dsq, const float4*) {
                 float r = sqrtf(dsq);
@@ -874,7 +971,7 @@ __device__ __forceinline__ void pressure_body(const DS& d, const MgParams& p, co
             },
             flush, i, MG_SKIN_LIM);
     } else {
-        sweep<8, 0, 1>(
+        sweep<K, 0, 1, CAP>(
             d, g, bands[threadIdx.x >> 6], valid, gc, pi, p.r2, none,
             [&](uint32_t, float4, f3, float dsq, const float4*) {
                 float r = sqrtf(dsq);
@@ -2277,12 +2374,14 @@ struct BatchInst {
     uint32_t tb0, tnb; /* scan tiles */
 };
 
+/* the pressure sweep at candidate batch K and band capacity CAP (the autotuner's tile shapes) */
+template <int K, int CAP>
 __global__ LAUNCH_BOUNDS_SWEEP void kb_pressure(const BatchInst* __restrict__ I, const uint16_t* __restrict__ map)
 {
     extern __shared__ float4 mg_lds[];
     const uint32_t b = xcd_block();
     const BatchInst& in = I[map[b]];
-    pressure_body(in.pressure, in.p, in.g, (Band<0>*)mg_lds, b - in.pb0, in.pnb);
+    pressure_body<K, CAP>(in.pressure, in.p, in.g, (Band<0, CAP>*)mg_lds, b - in.pb0, in.pnb);
 }
 
 __global__ LAUNCH_BOUNDS_SWEEP void kb_force_advance(const BatchInst* __restrict__ I,

@@ -55,12 +55,13 @@ unsigned gblk(size_t n) { return (unsigned)((n + 255) / 256); }
 int net_sync(mg_net* n)
 {
     hipLaunchKernelGGL(k_to_bf16, dim3(gblk(kNParams)), dim3(256), 0, n->st, n->params, n->wb, kNParams);
+    /* W1 in input order; W2 and W3 in the order the previous layer's results sit in registers */
     hipLaunchKernelGGL(k_pack_frag, dim3(gblk(MLP_IN * MLP_H)), dim3(256), 0, n->st, n->params + kOffW1, MLP_IN,
-                       MLP_H, n->dev->w1f);
+                       MLP_H, n->dev->w1f, 0);
     hipLaunchKernelGGL(k_pack_frag, dim3(gblk(MLP_H * MLP_H)), dim3(256), 0, n->st, n->params + kOffW2, MLP_H,
-                       MLP_H, n->dev->w2f);
+                       MLP_H, n->dev->w2f, 1);
     hipLaunchKernelGGL(k_pack_frag, dim3(gblk(MLP_H * MLP_OUT)), dim3(256), 0, n->st, n->params + kOffW3, MLP_H,
-                       MLP_OUT, n->dev->w3f);
+                       MLP_OUT, n->dev->w3f, 1);
     HIPCHK(hipMemcpyAsync(n->dev->b1, n->params + kOffB1, MLP_H * 4, hipMemcpyDeviceToDevice, n->st));
     HIPCHK(hipMemcpyAsync(n->dev->b2, n->params + kOffB2, MLP_H * 4, hipMemcpyDeviceToDevice, n->st));
     HIPCHK(hipMemcpyAsync(n->dev->b3, n->params + kOffB3, MLP_OUT * 4, hipMemcpyDeviceToDevice, n->st));
@@ -188,6 +189,12 @@ int ensure_learn_buffers(mg_handle* h)
 {
     if (h->DV) return MG_OK;
     void* q = nullptr;
+    HIPCHK(hipMalloc(&q, (size_t)h->N * (MLP_IN / 8) * sizeof(uint4)));
+    h->allocs.push_back(q);
+    h->XB = (uint4*)q;
+    HIPCHK(hipMalloc(&q, (size_t)h->N * sizeof(uint32_t)));
+    h->allocs.push_back(q);
+    h->MF = (uint32_t*)q;
     HIPCHK(hipMalloc(&q, (size_t)h->N * 2 * sizeof(float4)));
     h->allocs.push_back(q);
     h->DV = (float4*)q;
@@ -208,10 +215,15 @@ int launch_mlp_infer(mg_handle* h, const mg_net* net, float dt)
         set_err("learned step: correction buffer missing");
         return MG_ERR_STATE;
     }
-    const unsigned tiles = (unsigned)((h->N + 63) / 64);
-    const unsigned blocks = std::max(1u, std::min((tiles + MLP_WAVES - 1) / MLP_WAVES, 1024u));
-    hipLaunchKernelGGL(k_mlp_infer, dim3(blocks), dim3(256), sizeof(MlpLds), h->st, make_ds(h), h->p, net->dev,
-                       h->DV, h->fallbacks, dt);
+    hipLaunchKernelGGL(k_mlp_gather, dim3(nblk(h->N)), dim3(256), 0, h->st, make_ds(h), h->p, net->dev, h->XB,
+                       h->MF);
+    HIPCHK(hipGetLastError());
+    /* groups of 32 particles, 4 per block at a time; at most 512 blocks, so the weights are
+     * staged a bounded number of times */
+    const unsigned groups = (unsigned)((h->N + 31) / 32);
+    const unsigned blocks = std::max(1u, std::min((groups + MLP_WAVES - 1) / MLP_WAVES, 512u));
+    hipLaunchKernelGGL(k_mlp_apply, dim3(blocks), dim3(256), sizeof(MlpLds), h->st, net->dev, h->XB, h->MF, h->DV,
+                       h->fallbacks, dt, (uint32_t)h->N);
     HIPCHK(hipGetLastError());
     return MG_OK;
 }
@@ -241,7 +253,7 @@ int mg_net_create(mg_net** out, uint64_t seed)
     if (r == MG_OK) r = nalloc(n, &n->v, kNParams);
     if (r == MG_OK) r = nalloc(n, &n->wb, kNParams);
     if (r != MG_OK) return fail(r);
-    if (hipFuncSetAttribute((const void*)k_mlp_infer, hipFuncAttributeMaxDynamicSharedMemorySize,
+    if (hipFuncSetAttribute((const void*)k_mlp_apply, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)sizeof(MlpLds)) != hipSuccess) {
         set_err("mg_net_create: the inference kernel's LDS (%zu bytes) is not available", sizeof(MlpLds));
         return fail(MG_ERR_HIP);

@@ -79,41 +79,89 @@ struct MlpDev {
 
 /* ------------------------------------------------------------------ features
  * The 64 raw inputs of active particle i (sorted state after the pressure and force passes;
- * FFORCE in d.F). */
+ * FFORCE in d.F). Neighbours: the live own bond slots in slot order, then the incoming bonds
+ * in CSR order, skipping partners outside the active range and broken bonds (rest length
+ * below 1e-9), up to MLP_NBR. Candidates are examined MLP_NBR at a time, each group's
+ * records loaded together (partner, rest length, then slot), and the chosen neighbours'
+ * positions and velocities loaded together at the end. */
 __device__ __forceinline__ void mlp_features(const DS& d, const MgParams& p, uint32_t i, uint32_t na,
                                              float x[MLP_IN])
 {
     const int N = d.N, B = d.B;
+    const float* BPARf = (const float*)d.BPAR;
 #pragma unroll
     for (int k = 0; k < MLP_IN; k++) x[k] = 0.0f;
     const uint32_t h = d.HOME[i];
     const float4 Pi = d.P[i], Vi = d.VEL[i];
+    uint32_t sel[MLP_NBR];
     int m = 0;
-    auto add = [&](uint32_t j) {
-        const float4 Pj = d.P[j], Vj = d.VEL[j];
-        const float f[6] = {Pj.x - Pi.x, Pj.y - Pi.y, Pj.z - Pi.z, Vj.x - Vi.x, Vj.y - Vi.y, Vj.z - Vi.z};
+    auto push = [&](uint32_t j) {
 #pragma unroll
         for (int q = 0; q < MLP_NBR; q++)
-            if (q == m)
-#pragma unroll
-                for (int c = 0; c < 6; c++) x[6 * q + c] = f[c];
+            if (q == m) sel[q] = j;
         m++;
     };
-    for (int a = 0; a < B && m < MLP_NBR; a++) {
-        const size_t bi = (size_t)a * N + h;
-        const uint32_t jh = d.BPARTNER[bi];
-        if (jh == MG_NONE) continue;
-        const uint32_t j = d.SLOT[jh];
-        if (j >= na || (double)d.BPAR[bi].y < 0.000000001) continue;
-        add(j);
+    /* own live slots (LIVE: partner present; B <= 64) */
+    uint64_t live = d.LIVE[h];
+    while (live != 0ull && m < MLP_NBR) {
+        uint32_t a[MLP_NBR], jh[MLP_NBR], jj[MLP_NBR];
+        float rest[MLP_NBR];
+        bool has[MLP_NBR];
+#pragma unroll
+        for (int k = 0; k < MLP_NBR; k++) {
+            has[k] = live != 0ull;
+            a[k] = has[k] ? (uint32_t)__builtin_ctzll(live) : 0u;
+            live &= live - 1ull;
+        }
+#pragma unroll
+        for (int k = 0; k < MLP_NBR; k++) {
+            jh[k] = MG_NONE;
+            rest[k] = 0.0f;
+            if (has[k]) {
+                const size_t bi = (size_t)a[k] * N + h;
+                jh[k] = d.BPARTNER[bi];
+                rest[k] = BPARf[bi * 4 + 1];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < MLP_NBR; k++) jj[k] = (has[k] && jh[k] < (uint32_t)N) ? d.SLOT[jh[k]] : MG_NONE;
+#pragma unroll
+        for (int k = 0; k < MLP_NBR; k++)
+            if (m < MLP_NBR && jj[k] < na && !((double)rest[k] < 0.000000001)) push(jj[k]);
     }
+    /* incoming bonds */
     const uint32_t e0 = d.INC_OFF[h], e1 = d.INC_OFF[h + 1];
-    for (uint32_t e = e0; e < e1 && m < MLP_NBR; e++) {
-        const uint32_t packed = d.INC[e];
-        const uint32_t kh = packed / (uint32_t)B, b = packed - kh * (uint32_t)B;
-        const uint32_t k = d.SLOT[kh];
-        if (k >= na || (double)d.BPAR[(size_t)b * N + kh].y < 0.000000001) continue;
-        add(k);
+    for (uint32_t e = e0; e < e1 && m < MLP_NBR; e += MLP_NBR) {
+        uint32_t kh[MLP_NBR], kk[MLP_NBR];
+        float rest[MLP_NBR];
+#pragma unroll
+        for (int k = 0; k < MLP_NBR; k++) {
+            kh[k] = MG_NONE;
+            rest[k] = 0.0f;
+            if (e + (uint32_t)k < e1) {
+                const uint32_t packed = d.INC[e + (uint32_t)k];
+                kh[k] = packed / (uint32_t)B;
+                const uint32_t b = packed - kh[k] * (uint32_t)B;
+                rest[k] = BPARf[((size_t)b * N + kh[k]) * 4 + 1];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < MLP_NBR; k++) kk[k] = kh[k] < (uint32_t)N ? d.SLOT[kh[k]] : MG_NONE;
+#pragma unroll
+        for (int k = 0; k < MLP_NBR; k++)
+            if (m < MLP_NBR && kk[k] < na && !((double)rest[k] < 0.000000001)) push(kk[k]);
+    }
+#pragma unroll
+    for (int q = 0; q < MLP_NBR; q++) {
+        if (q < m) {
+            const float4 Pj = d.P[sel[q]], Vj = d.VEL[sel[q]];
+            x[6 * q + 0] = Pj.x - Pi.x;
+            x[6 * q + 1] = Pj.y - Pi.y;
+            x[6 * q + 2] = Pj.z - Pi.z;
+            x[6 * q + 3] = Vj.x - Vi.x;
+            x[6 * q + 4] = Vj.y - Vi.y;
+            x[6 * q + 5] = Vj.z - Vi.z;
+        }
     }
     const float4 Vi_eval = d.V[i];
     x[48] = 1.0f / Vi_eval.w; /* density */
@@ -143,28 +191,81 @@ __global__ LAUNCH_BOUNDS_256 void k_mlp_features(DS d, MgParams p, float* __rest
 }
 
 /* --------------------------------------------------------------- inference
- * One wavefront per 64 particles: each lane gathers and normalises its particle's inputs
- * into an LDS tile, then the wave runs the three layers as 16-row MFMA tiles through LDS.
- * DV[2i] = (c_d, 1), DV[2i + 1] = (c_e, 0) for corrected particles, zeros for inactive or
- * fallback ones (a normalised input beyond in_limit, an output beyond out_limit, or a drift
- * correction that would move the particle more than MLP_MAX_SHIFT in the step of length dt);
- * fallbacks are counted in *fallbacks. Weights are staged in LDS once per block. */
+ * Two kernels per learned step.
+ *
+ * k_mlp_gather, one thread per slot: the particle's inputs, normalised and rounded to bf16,
+ * into X[i][0..63] (128 bytes), and flags[i] = active | (every normalised input within
+ * in_limit) << 1.
+ *
+ * k_mlp_apply, one wavefront per group of 32 particles (two 16-particle tiles): the three
+ * layers as transposed products, H^T = W^T X^T, so the weights are the MFMA A operand and the
+ * particles the B operand. The 16 x 16 result of a layer leaves lane l holding neurons
+ * 16 mt + 4 (l / 16) + r of particle l % 16, which is a B operand of the next layer for the
+ * K order nu(32 kc + 8 q + e) = 32 kc + (e < 4 ? 4 q + e : 16 + 4 q + e - 4) (q = l / 16);
+ * W2 and W3 are packed with their input rows in that order (k_pack_frag with perm), so the
+ * activations never leave registers. Weight fragments and biases are staged in LDS once per
+ * block. DV[2i] = (c_d, 1), DV[2i + 1] = (c_e, 0) for corrected particles, zeros for inactive
+ * or fallback ones (a normalised input beyond in_limit, an output beyond out_limit, or a
+ * drift correction that would move the particle more than MLP_MAX_SHIFT in the step of
+ * length dt); fallbacks of active particles are counted in *fallbacks. */
+__global__ LAUNCH_BOUNDS_256 void k_mlp_gather(DS d, MgParams p, const MlpDev* __restrict__ net,
+                                               uint4* __restrict__ X, uint32_t* __restrict__ flags)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (uint32_t)d.N) return;
+    const uint32_t na = d.ctr[MG_C_ACTIVE];
+    const bool act = i < na && d.GCELL[i] != MG_GRID_UNDEF;
+    float x[MLP_IN];
+    if (act) {
+        mlp_features(d, p, i, na, x);
+    } else {
+#pragma unroll
+        for (int k = 0; k < MLP_IN; k++) x[k] = 0.0f;
+    }
+    bool ok = act;
+    uint32_t w[MLP_IN / 2];
+#pragma unroll
+    for (int k = 0; k < MLP_IN; k += 2) {
+        const float v0 = act ? (x[k] - net->mu[k]) * net->inv_sigma[k] : 0.0f;
+        const float v1 = act ? (x[k + 1] - net->mu[k + 1]) * net->inv_sigma[k + 1] : 0.0f;
+        if (!(fabsf(v0) <= net->in_limit) || !(fabsf(v1) <= net->in_limit)) ok = false;
+        const __bf16 b0 = to_bf16(v0), b1 = to_bf16(v1);
+        w[k / 2] = (uint32_t)__builtin_bit_cast(uint16_t, b0) | ((uint32_t)__builtin_bit_cast(uint16_t, b1) << 16);
+    }
+    uint4* o = X + (size_t)i * (MLP_IN / 8);
+#pragma unroll
+    for (int k = 0; k < MLP_IN / 8; k++) o[k] = make_uint4(w[4 * k], w[4 * k + 1], w[4 * k + 2], w[4 * k + 3]);
+    flags[i] = (act ? 1u : 0u) | (ok ? 2u : 0u);
+}
+
 #define MLP_WAVES 4
 struct MlpLds {
     __bf16 w1f[(MLP_IN / 32) * (MLP_H / 16) * 16 * 32];
     __bf16 w2f[(MLP_H / 32) * (MLP_H / 16) * 16 * 32];
     __bf16 w3f[(MLP_H / 32) * (MLP_OUT / 16) * 16 * 32];
-    __bf16 x[MLP_WAVES][64][MLP_IN + 8];   /* +8: row pitch off the bank period */
-    __bf16 h[MLP_WAVES][16][MLP_H + 8];
-    __bf16 h2[MLP_WAVES][16][MLP_H + 8];
-    int ok[MLP_WAVES][64];
+    float b1[MLP_H], b2[MLP_H], b3[MLP_OUT], out_scale[MLP_OUT];
 };
 
 __device__ __forceinline__ bf16x8_t lds8(const __bf16* p) { return *(const bf16x8_t*)p; }
 
-__global__ __launch_bounds__(256) void k_mlp_infer(DS d, MgParams p, const MlpDev* __restrict__ net,
-                                                   float4* __restrict__ DV, uint32_t* __restrict__ fallbacks,
-                                                   float dt)
+/* bias + SiLU of a layer's results (neuron 16 mt + 4 q + r in acc[mt][r]) packed as the next
+ * layer's B operands: op[kc] holds neurons nu(32 kc + 8 q + e), e = 0..7 */
+__device__ __forceinline__ void silu_pack(const f32x4_t acc[MLP_H / 16], const float* bias, int q,
+                                          bf16x8_t op[MLP_H / 32])
+{
+#pragma unroll
+    for (int kc = 0; kc < MLP_H / 32; kc++) {
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const int mt = 2 * kc + (e >> 2), r = e & 3;
+            op[kc][e] = to_bf16(silu(acc[mt][r] + bias[16 * mt + 4 * q + r]));
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_mlp_apply(const MlpDev* __restrict__ net, const uint4* __restrict__ X,
+                                                   const uint32_t* __restrict__ flags, float4* __restrict__ DV,
+                                                   uint32_t* __restrict__ fallbacks, float dt, uint32_t N)
 {
     extern __shared__ float4 mg_lds[];
     MlpLds& L = *(MlpLds*)mg_lds;
@@ -179,108 +280,102 @@ __global__ __launch_bounds__(256) void k_mlp_infer(DS d, MgParams p, const MlpDe
         const uint4* s3 = (const uint4*)net->w3f;
         uint4* d3 = (uint4*)L.w3f;
         for (int k = t; k < (int)(sizeof(L.w3f) / 16); k += 256) d3[k] = s3[k];
+        for (int k = t; k < MLP_H; k += 256) {
+            L.b1[k] = net->b1[k];
+            L.b2[k] = net->b2[k];
+        }
+        if (t < MLP_OUT) {
+            L.b3[t] = net->b3[t];
+            L.out_scale[t] = net->out_scale[t];
+        }
     }
     __syncthreads();
-    const uint32_t na = d.ctr[MG_C_ACTIVE];
-    const uint32_t ntile = ((uint32_t)d.N + 63u) / 64u;
+    const int q = l >> 4, col = l & 15;
+    const uint32_t ngroups = (N + 31u) / 32u;
     uint32_t nfall = 0;
-    for (uint32_t tile = blockIdx.x * MLP_WAVES + w; tile < ntile; tile += gridDim.x * MLP_WAVES) {
-        const uint32_t i = tile * 64u + (uint32_t)l;
-        /* gather + normalise: lane l owns row l */
-        bool act = i < (uint32_t)d.N && i < na && d.GCELL[i] != MG_GRID_UNDEF;
-        bool ok = act;
-        {
-            float x[MLP_IN];
-            if (act) {
-                mlp_features(d, p, i, na, x);
-            } else {
+    for (uint32_t grp = blockIdx.x * MLP_WAVES + (uint32_t)w; grp < ngroups; grp += gridDim.x * MLP_WAVES) {
+        /* layer-1 B operands: particle col of tile tt, inputs 32 kc + 8 q .. + 7 */
+        bf16x8_t xb[2][MLP_IN / 32];
 #pragma unroll
-                for (int k = 0; k < MLP_IN; k++) x[k] = 0.0f;
-            }
+        for (int tt = 0; tt < 2; tt++) {
+            const uint32_t i = grp * 32u + (uint32_t)(16 * tt + col);
 #pragma unroll
-            for (int k = 0; k < MLP_IN; k++) {
-                float v = act ? (x[k] - net->mu[k]) * net->inv_sigma[k] : 0.0f;
-                if (!(fabsf(v) <= net->in_limit)) ok = false;
-                L.x[w][l][k] = to_bf16(act ? v : 0.0f);
+            for (int kc = 0; kc < MLP_IN / 32; kc++) {
+                uint4 v = make_uint4(0u, 0u, 0u, 0u);
+                if (i < N) v = X[(size_t)i * (MLP_IN / 8) + (size_t)(4 * kc + q)];
+                xb[tt][kc] = __builtin_bit_cast(bf16x8_t, v);
             }
         }
-        L.ok[w][l] = ok ? 1 : 0;
-        wave_lds_sync();
-        for (int rt = 0; rt < 4; rt++) {
-            const int row = l & 15, kq = l >> 4;
-            /* layer 1: [16 x 64] x [64 x 128] */
-            for (int nt = 0; nt < MLP_H / 16; nt++) {
-                f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
+        /* layer 1: [128 x 64] x [64 x 16] per tile */
+        f32x4_t acc[2][MLP_H / 16];
 #pragma unroll
-                for (int kc = 0; kc < MLP_IN / 32; kc++) {
-                    const bf16x8_t a = lds8(&L.x[w][rt * 16 + row][kc * 32 + 8 * kq]);
-                    const bf16x8_t b = lds8(&L.w1f[((nt * (MLP_IN / 32) + kc) * 16 + row) * 32 + 8 * kq]);
-                    acc = mfma16(a, b, acc);
-                }
+        for (int mt = 0; mt < MLP_H / 16; mt++) {
+            acc[0][mt] = acc[1][mt] = f32x4_t{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    const int col = nt * 16 + (l & 15);
-                    L.h[w][4 * kq + r][col] = to_bf16(silu(acc[r] + net->b1[col]));
-                }
+            for (int kc = 0; kc < MLP_IN / 32; kc++) {
+                const bf16x8_t a = lds8(&L.w1f[((mt * (MLP_IN / 32) + kc) * 16 + col) * 32 + 8 * q]);
+                acc[0][mt] = mfma16(a, xb[0][kc], acc[0][mt]);
+                acc[1][mt] = mfma16(a, xb[1][kc], acc[1][mt]);
             }
-            wave_lds_sync();
-            /* layer 2: [16 x 128] x [128 x 128] */
-            for (int nt = 0; nt < MLP_H / 16; nt++) {
-                f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
+            __builtin_amdgcn_sched_barrier(0); /* one tile's weight fragments live at a time */
+        }
+        bf16x8_t hb[2][MLP_H / 32];
+        silu_pack(acc[0], L.b1, q, hb[0]);
+        silu_pack(acc[1], L.b1, q, hb[1]);
+        /* layer 2: [128 x 128] x [128 x 16] per tile */
 #pragma unroll
-                for (int kc = 0; kc < MLP_H / 32; kc++) {
-                    const bf16x8_t a = lds8(&L.h[w][row][kc * 32 + 8 * kq]);
-                    const bf16x8_t b = lds8(&L.w2f[((nt * (MLP_H / 32) + kc) * 16 + row) * 32 + 8 * kq]);
-                    acc = mfma16(a, b, acc);
-                }
-#pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    const int col = nt * 16 + (l & 15);
-                    L.h2[w][4 * kq + r][col] = to_bf16(silu(acc[r] + net->b2[col]));
-                }
-            }
-            wave_lds_sync();
-            /* layer 3: [16 x 128] x [128 x 16] */
-            f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
+        for (int mt = 0; mt < MLP_H / 16; mt++) {
+            acc[0][mt] = acc[1][mt] = f32x4_t{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (int kc = 0; kc < MLP_H / 32; kc++) {
-                const bf16x8_t a = lds8(&L.h2[w][row][kc * 32 + 8 * kq]);
-                const bf16x8_t b = lds8(&L.w3f[(kc * 16 + row) * 32 + 8 * kq]);
-                acc = mfma16(a, b, acc);
+                const bf16x8_t a = lds8(&L.w2f[((mt * (MLP_H / 32) + kc) * 16 + col) * 32 + 8 * q]);
+                acc[0][mt] = mfma16(a, hb[0][kc], acc[0][mt]);
+                acc[1][mt] = mfma16(a, hb[1][kc], acc[1][mt]);
             }
-            /* lanes with column < 3 hold outputs of rows 4 kq + r; gather them per row */
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        silu_pack(acc[0], L.b2, q, hb[0]);
+        silu_pack(acc[1], L.b2, q, hb[1]);
+        /* layer 3: [16 x 128] x [128 x 16] per tile */
+        f32x4_t y[2] = {f32x4_t{0.f, 0.f, 0.f, 0.f}, f32x4_t{0.f, 0.f, 0.f, 0.f}};
+#pragma unroll
+        for (int kc = 0; kc < MLP_H / 32; kc++) {
+            const bf16x8_t a = lds8(&L.w3f[(kc * 16 + col) * 32 + 8 * q]);
+            y[0] = mfma16(a, hb[0][kc], y[0]);
+            y[1] = mfma16(a, hb[1][kc], y[1]);
+        }
+        /* lane l holds outputs 4 q + r of particle col; lanes q = 0 take outputs 4 and 5 from
+         * lane l + 16 and write the particle's corrections */
+#pragma unroll
+        for (int tt = 0; tt < 2; tt++) {
+            float o[4];
+            bool lim = true;
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int col = l & 15;
-                const float y = acc[r] + net->b3[col < MLP_OUT ? col : 0];
-                const int prow = rt * 16 + 4 * kq + r; /* particle row within the 64-tile */
-                if (col < MLP_OUT_REAL) {
-                    if (!(fabsf(y) <= net->out_limit)) L.ok[w][prow] = 0;
-                    ((float*)&L.h[w][0][0])[prow * 16 + col] = y * net->out_scale[col]; /* layer-1 tile is free now */
-                }
+                const int c = 4 * q + r;
+                const float v = y[tt][r] + L.b3[c];
+                if (c < MLP_OUT_REAL && !(fabsf(v) <= net->out_limit)) lim = false;
+                o[r] = v * L.out_scale[c];
             }
-            wave_lds_sync();
-            if ((l >> 4) == rt) {
-                /* lanes 16 rt .. 16 rt + 15 write their rows */
-                const int prow = l;
-                const uint32_t ii = tile * 64u + (uint32_t)prow;
-                if (ii < (uint32_t)d.N) {
-                    const float* y = &((float*)&L.h[w][0][0])[prow * 16];
-                    const f3 cd = mk3(y[0], y[1], y[2]), ce = mk3(y[3], y[4], y[5]);
+            const float o4 = __shfl(o[0], l + 16), o5 = __shfl(o[1], l + 16);
+            const bool lim_hi = __shfl((int)lim, l + 16) != 0;
+            if (q == 0) {
+                const uint32_t i = grp * 32u + (uint32_t)(16 * tt + col);
+                if (i < N) {
+                    const uint32_t fl = flags[i];
+                    const f3 cd = mk3(o[0], o[1], o[2]), ce = mk3(o[3], o4, o5);
                     /* the residual the network may apply: a shift of at most MLP_MAX_SHIFT */
-                    const bool use = L.ok[w][prow] != 0 && sqrtf(dot3(cd, cd)) * dt <= MLP_MAX_SHIFT;
+                    const bool use = (fl & 2u) != 0u && lim && lim_hi && sqrtf(dot3(cd, cd)) * dt <= MLP_MAX_SHIFT;
                     if (use) {
-                        DV[2 * (size_t)ii] = make_float4(cd.x, cd.y, cd.z, 1.0f);
-                        DV[2 * (size_t)ii + 1] = make_float4(ce.x, ce.y, ce.z, 0.0f);
+                        DV[2 * (size_t)i] = make_float4(cd.x, cd.y, cd.z, 1.0f);
+                        DV[2 * (size_t)i + 1] = make_float4(ce.x, ce.y, ce.z, 0.0f);
                     } else {
-                        DV[2 * (size_t)ii] = make_float4(0.f, 0.f, 0.f, 0.f);
-                        DV[2 * (size_t)ii + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
+                        DV[2 * (size_t)i] = make_float4(0.f, 0.f, 0.f, 0.f);
+                        DV[2 * (size_t)i + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
                     }
-                    const bool wasact = ii < na && d.GCELL[ii] != MG_GRID_UNDEF;
-                    if (wasact && !use) nfall++;
+                    if ((fl & 1u) && !use) nfall++;
                 }
             }
-            wave_lds_sync();
         }
     }
     for (int o = 32; o > 0; o >>= 1) nfall += (uint32_t)__shfl_xor((int)nfall, o);
@@ -454,11 +549,21 @@ __global__ LAUNCH_BOUNDS_256 void k_adam(float* __restrict__ w, const float* __r
 }
 
 /* W [K x N] fp32 row-major -> fragment layout (ntile, kchunk, col, k) in bf16 */
-__global__ LAUNCH_BOUNDS_256 void k_pack_frag(const float* __restrict__ W, int K, int N, __bf16* __restrict__ F)
+/* Fragment kk of chunk kc, column col of tile nt takes W[k][n] with k = 32 kc + kk, or with
+ * perm the row whose neuron the previous layer's result leaves at that position:
+ * nu(32 kc + 8 q + e) = 32 kc + (e < 4 ? 4 q + e : 16 + 4 q + e - 4). */
+__global__ LAUNCH_BOUNDS_256 void k_pack_frag(const float* __restrict__ W, int K, int N, __bf16* __restrict__ F,
+                                              int perm)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= K * N) return;
     const int k = i / N, n = i % N;
-    const int nt = n / 16, col = n % 16, kc = k / 32, kk = k % 32;
+    const int nt = n / 16, col = n % 16, kc = k / 32;
+    int kk = k % 32;
+    if (perm) {
+        /* the inverse of nu inside the chunk: neuron offset o = kk -> position 8 q + e */
+        const int o = kk, hi = o >= 16 ? 1 : 0, rem = o - 16 * hi;
+        kk = 8 * (rem / 4) + 4 * hi + rem % 4;
+    }
     F[(((size_t)nt * (K / 32) + kc) * 16 + col) * 32 + kk] = to_bf16(W[i]);
 }

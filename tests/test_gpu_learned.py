@@ -133,6 +133,69 @@ def test_fused_inference_matches_reference_and_falls_back(built):
     sim.close()
 
 
+def expected_inputs(sim):
+    """Inputs 0..49 of every active particle restated from the downloaded reference-layout
+    buffers: up to 8 bonded neighbours, first the particle's own bond slots in slot order,
+    then the bonds other particles hold to it ordered by (owner particle ID, slot), skipping
+    partners outside the active range and bonds with rest length below 1e-9; (x_j - x_i,
+    v_j - v_i) each; then density (1 / FDENSITY) and pressure."""
+    nump, na = sim.counts()
+    pos = sim.download(F.FPOS)[:, :3]
+    vel = sim.download(F.FVEL)[:, :3]
+    rec = sim.download(F.FELASTIDX)
+    ids = sim.download(F.FPARTICLE_ID)
+    dens = sim.download(F.FDENSITY)
+    press = sim.download(F.FPRESS)
+    partner = rec[:, :, 0]
+    rest = rec[:, :, 2].view(np.float32).astype(np.float64)
+    B = rec.shape[1]
+    incoming = [[] for _ in range(na)]
+    for k in range(na):
+        for a in range(B):
+            j = int(partner[k, a])
+            if j < na and rest[k, a] >= 1e-9:
+                incoming[j].append((int(ids[k]), a, k))
+    X = np.zeros((na, 50), np.float32)
+    for i in range(na):
+        nb = [int(partner[i, a]) for a in range(B) if partner[i, a] < na and rest[i, a] >= 1e-9]
+        nb += [k for (_, _, k) in sorted(incoming[i])]
+        for q, j in enumerate(nb[:8]):
+            X[i, 6 * q:6 * q + 3] = pos[j] - pos[i]
+            X[i, 6 * q + 3:6 * q + 6] = vel[j] - vel[i]
+        X[i, 48] = np.float32(1.0) / dens[i]
+        X[i, 49] = press[i]
+    return X
+
+
+def test_inputs_match_numpy_restatement(built):
+    """The feature gather (shared by training and inference) against a numpy restatement,
+    bit for bit, on a tissue with shuffled particle IDs and some broken bonds."""
+    from morphogenesis_amd.learned import features
+
+    say("case: learned-step inputs vs numpy restatement")
+    from tests.test_fileio import shuffled_ids
+
+    sc = shuffled_ids(scenes.tissue(n_side=9, b=16, seed=8), 5)
+    rng = np.random.default_rng(8)
+    rec = sc.buffers[F.FELASTIDX]
+    used = rec[:, :, 0] != F.NONE
+    brk = used & (rng.random(used.shape) < 0.15)
+    r = rec[:, :, 2].view(np.float32)
+    r[brk] = 0.0
+    from morphogenesis_amd.fluid_system import FluidSystem
+
+    sim = FluidSystem(sc.n, sc.b)
+    sim.load_scene(sc)
+    sim.Run2PhysicalSort()
+    sim.step(3, sc.dt, sc.flags)
+    X = features(sim)
+    nump, na = sim.counts()
+    want = expected_inputs(sim)
+    np.testing.assert_array_equal(X[:na, :50], want)
+    assert np.count_nonzero(X[:na, :48]) > 0
+    sim.close()
+
+
 def test_zero_network_is_the_plain_step(built):
     from morphogenesis_amd.fluid_system import FluidSystem
     from morphogenesis_amd.learned import Net

@@ -453,3 +453,41 @@ def test_batch_launch_forms_match_single(built, persistent):
         for buf in bufs:
             assert_bits_equal(b.download(buf), a.download(buf), f"instance {k}: {F.BUFFER_NAMES[buf]}")
     bat.close()
+
+
+def test_batch_tile_shapes_match_single(built):
+    """Every pressure sweep tile shape the batch autotuner chooses from (candidate batch 8 or 4,
+    staged band of 128 or 64 slots; a 64-slot band sends most rows of an 8-per-cell tissue to
+    the global-memory path) steps the instances of three size classes bit for bit as alone."""
+    from morphogenesis_amd.batch import Batch
+    from morphogenesis_amd.fluid_system import FluidSystem
+
+    specs = [(9, 6, 61), (21, 8, 62), (30, 8, 63)]
+    scs = [scenes.tissue(n_side=n, b=b, seed=sd) for n, b, sd in specs]
+    say("case: batch tile shapes", [sc.n for sc in scs])
+    solo, inb = [], []
+    for sc in scs:
+        for lst in (solo, inb):
+            s = FluidSystem(sc.n, sc.b)
+            s.load_scene(sc)
+            s.Run2PhysicalSort()
+            lst.append(s)
+    bat = Batch(inb)
+    bat.set_persistent(False)
+    dt, fl = scs[0].dt, scs[0].flags
+    ncls = len(bat.classes())
+    bufs = [F.FPOS, F.FVEL, F.FVEVAL, F.FPRESS, F.FDENSITY, F.FCONC, F.FBIN_COUNT, F.FBIN_OFFSET]
+    for kb, cap in [(8, 128), (4, 128), (8, 64), (4, 64)]:
+        for c in range(ncls):
+            bat.set_tile_shape(c, kb, cap)
+        assert all(cc["cand_batch"] == kb and cc["band_cap"] == cap for cc in bat.class_config())
+        bat.step(4, dt, fl | F.STEP_GRAPH)
+        bat.sync()
+        for s in solo:
+            s.step(4, dt, fl)
+        for k, (a, b) in enumerate(zip(solo, inb)):
+            for buf in bufs:
+                assert_bits_equal(b.download(buf), a.download(buf), f"shape {kb}/{cap}, instance {k}: {F.BUFFER_NAMES[buf]}")
+    with pytest.raises(F.MgError):
+        bat.set_tile_shape(0, 6, 128)
+    bat.close()
