@@ -188,8 +188,6 @@ def cpu_baseline(scene, budget_s: float = 15.0) -> dict:
 def run_batch(args, count: int = 256):
     """The batched workload: `count` bonded tissue blocks of log-uniform size in
     [2048, 65536] particles, each with its own domain, grid and genome, stepped together."""
-    import torch
-
     from morphogenesis_amd import _ffi as F
     from morphogenesis_amd import scenes
     from morphogenesis_amd.batch import Batch
@@ -217,11 +215,9 @@ def run_batch(args, count: int = 256):
     progress(f"autotuned: block sizes {chosen}, persistent launch {persistent} ({workgroups} workgroups)")
     bat.step(args.warmup, dt, flags | F.STEP_GRAPH)
     bat.sync()
-    torch.cuda.synchronize()
     t0 = time.perf_counter()
     bat.step(args.steps, dt, flags | F.STEP_GRAPH)
     bat.sync()
-    torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     # the other launch form over the same number of steps, for comparison (not the value)
     forms = {("persistent" if persistent else "dispatch"): elapsed / args.steps * 1e3}
@@ -579,13 +575,24 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
-    import torch
+    if args.calib_copy_mib > 0:
+        # a runtime device copy of known size, for the traffic counters' scale; torch is
+        # loaded before the library, so each keeps its own HIP runtime
+        import torch
 
+        n = args.calib_copy_mib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+
+    # the library is the only GPU user: creating a system fails loudly without a HIP device
     from morphogenesis_amd import _ffi as F
     from morphogenesis_amd import scenes
     from morphogenesis_amd.fluid_system import FluidSystem
 
-    assert torch.cuda.is_available(), "bench.py needs a GPU"
     if args.workload == "tissue_batch_256":
         run_batch(args)
         return
@@ -604,14 +611,6 @@ def main():
     scene = builders[args.workload]()
     stats = scene_stats(scene)
 
-    if args.calib_copy_mib > 0:
-        n = args.calib_copy_mib * (1 << 20) // 4
-        src = torch.ones(n, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
 
     sim = FluidSystem(scene.n, scene.b)
     sim.load_scene(scene)
@@ -619,11 +618,9 @@ def main():
     flags = scene.flags | (0 if args.no_graph else F.STEP_GRAPH)
     sim.step(args.warmup, scene.dt, flags)
     sim.sync()
-    torch.cuda.synchronize()
     t0 = time.perf_counter()
     sim.step(args.steps, scene.dt, flags)
     sim.sync()
-    torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     nump, na = sim.counts()
     stats["n_active"] = na

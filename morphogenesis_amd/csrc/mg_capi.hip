@@ -438,8 +438,8 @@ int ready(mg_handle* h)
  * (owner home, slot) pairs whose partner is j, ordered by the owner's particle ID, then its
  * home, then the slot. The ID order makes the force's summation order a property of the
  * state itself: homes are renumbered by uploads and re-homing, IDs travel with particles.
- * rebuild_inc builds it on the device; this host build serves particles with more than
- * INC_RUN_MAX incoming bonds. */
+ * rebuild_inc builds it on the device (runs of up to INC_LONG_MAX / INC_PER_WAVE entries);
+ * this host build serves particles with more incoming bonds than that. */
 int rebuild_inc_host(mg_handle* h)
 {
     const int N = h->N, B = h->B;
@@ -506,7 +506,8 @@ int rebuild_inc(mg_handle* h)
     HIPCHK(hipMemcpyAsync(&total, h->INC_OFF + N, 4, hipMemcpyDeviceToHost, h->st));
     HIPCHK(hipMemcpyAsync(&longest, stat, 4, hipMemcpyDeviceToHost, h->st));
     HIPCHK(hipStreamSynchronize(h->st));
-    if (longest > INC_RUN_MAX) return rebuild_inc_host(h);
+    /* a group of INC_PER_WAVE partners holds at most INC_PER_WAVE * longest entries */
+    if ((size_t)longest * INC_PER_WAVE > INC_LONG_MAX) return rebuild_inc_host(h);
     if (std::max<size_t>(total, 1) > h->inc_cap) {
         dfree(h, h->INC);
         h->INC = nullptr;
@@ -519,6 +520,10 @@ int rebuild_inc(mg_handle* h)
                        B);
     const unsigned blocks = (unsigned)(((size_t)N + 4 * INC_PER_WAVE - 1) / (4 * INC_PER_WAVE));
     hipLaunchKernelGGL(k_inc_order, dim3(blocks), dim3(256), 0, h->st, h->INC_OFF, h->INC, h->PID, N, B);
+    if (longest > INC_RUN_MAX) {
+        const unsigned groups = (unsigned)(((size_t)N + INC_PER_WAVE - 1) / INC_PER_WAVE);
+        hipLaunchKernelGGL(k_inc_order_long, dim3(groups), dim3(256), 0, h->st, h->INC_OFF, h->INC, h->PID, N, B);
+    }
     HIPCHK(hipGetLastError());
     return MG_OK;
 }

@@ -366,6 +366,7 @@ __global__ __launch_bounds__(256) void k_inc_order(const uint32_t* __restrict__ 
 #pragma unroll
     for (int k = 0; k <= INC_PER_WAVE; k++) st[k] = OFF[min(j0 + (uint32_t)k, (uint32_t)N)] - o;
     const uint32_t tot = st[INC_PER_WAVE];
+    if (tot > INC_PER_WAVE * INC_RUN_MAX) return; /* k_inc_order_long takes this group */
     for (uint32_t t = (uint32_t)lane; t < tot; t += 64) {
         const uint32_t e = ENT[o + t];
         keys[w][t] = ((unsigned long long)PID[e / (uint32_t)B] << 32) | e;
@@ -382,6 +383,42 @@ __global__ __launch_bounds__(256) void k_inc_order(const uint32_t* __restrict__ 
         const unsigned long long key = keys[w][t];
         uint32_t rank = 0;
         for (uint32_t q = rs; q < re; q++) rank += keys[w][q] < key ? 1u : 0u;
+        ENT[o + rs + rank] = (uint32_t)key;
+    }
+}
+
+/* The groups of INC_PER_WAVE partners whose runs together exceed one wavefront's stage (a hub
+ * particle with hundreds of incoming bonds): one workgroup per group, the group's keys (up to
+ * INC_LONG_MAX) in LDS, each entry written at its rank among its run's keys. */
+#define INC_LONG_MAX 6144
+__global__ __launch_bounds__(256) void k_inc_order_long(const uint32_t* __restrict__ OFF, uint32_t* __restrict__ ENT,
+                                                        const uint32_t* __restrict__ PID, int N, int B)
+{
+    __shared__ unsigned long long keys[INC_LONG_MAX];
+    const uint32_t j0 = (uint32_t)blockIdx.x * INC_PER_WAVE;
+    if (j0 >= (uint32_t)N) return;
+    uint32_t st[INC_PER_WAVE + 1];
+    const uint32_t o = OFF[j0];
+#pragma unroll
+    for (int k = 0; k <= INC_PER_WAVE; k++) st[k] = OFF[min(j0 + (uint32_t)k, (uint32_t)N)] - o;
+    const uint32_t tot = st[INC_PER_WAVE];
+    if (tot <= INC_PER_WAVE * INC_RUN_MAX || tot > INC_LONG_MAX) return; /* k_inc_order's, or the host's */
+    for (uint32_t t = threadIdx.x; t < tot; t += blockDim.x) {
+        const uint32_t e = ENT[o + t];
+        keys[t] = ((unsigned long long)PID[e / (uint32_t)B] << 32) | e;
+    }
+    __syncthreads();
+    for (uint32_t t = threadIdx.x; t < tot; t += blockDim.x) {
+        uint32_t rs = 0, re = st[1];
+#pragma unroll
+        for (int k = 1; k < INC_PER_WAVE; k++)
+            if (t >= st[k]) {
+                rs = st[k];
+                re = st[k + 1];
+            }
+        const unsigned long long key = keys[t];
+        uint32_t rank = 0;
+        for (uint32_t q = rs; q < re; q++) rank += keys[q] < key ? 1u : 0u;
         ENT[o + rs + rank] = (uint32_t)key;
     }
 }

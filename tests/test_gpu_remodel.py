@@ -150,11 +150,13 @@ def test_heal_dense_and_sparse_neighbourhoods(built):
     assert o.download(F.FDENSE_LIST_LENGTHS_CHANGES)[0] > 300
 
 
-@pytest.mark.parametrize("spokes", [200, 300])
+@pytest.mark.parametrize("spokes", [200, 300, 1700])
 def test_hub_particle_incoming_lists(built, spokes):
     """One particle that every other one bonds to: 200 incoming bonds take the device build's
-    multi-pass rank sort, 300 take the host build (runs over 256 entries). Steps bit-equal to
-    the oracle either way, particle IDs shuffled so the incoming order is not the home order."""
+    wavefront rank sort (several lane passes), 300 its workgroup sort for long runs (a group of
+    four partners over 1,024 entries), 1,700 the host build (runs whose group may exceed 6,144
+    entries). Steps bit-equal to the oracle either way, particle IDs shuffled so the incoming
+    order is not the home order."""
     from tests.helpers import mini_scene
 
     rng = np.random.default_rng(spokes)
