@@ -69,6 +69,15 @@ int mg_get_grid(mg_handle* h, MgGrid* out);
 int mg_set_genome(mg_handle* h, const MgGenome* g);
 /* Replaces setFreeze (src/fluid_system.cpp:2020-2035). */
 int mg_set_freeze(mg_handle* h, int freeze);
+/* Selects the SPH kernels of pressure and force for the handle's phase calls and steps
+ * (MG_SPH_REFERENCE, the default: the reference's live Wendland-C2 density and (sr - r)^3
+ * force; MG_SPH_POLY6_SPIKY: Mueller 2003 poly6 density, spiky pressure and viscosity
+ * Laplacian force). No counterpart entry point in the reference: the constants come from
+ * FluidParamCL (src/fluid_system.cpp:550, 568-569, 587) and the forms from the reference's
+ * commented kernels (src/fluid_system_cuda.cu:668-672, 695, 3140-3160, 3213-3220).
+ * MG_ERR_UNSUPPORTED for a handle in a batch (batches step the reference kernels). */
+int mg_set_sph_kernels(mg_handle* h, int mode);
+int mg_get_sph_kernels(mg_handle* h, int* mode);
 
 /* Host -> device copy of one buffer in the reference layout; bytes must equal
  * mg_buffer_bytes(buf_id, N, C, B). Replaces TransferToCL (src/host_CL.cpp:1608-1634). */

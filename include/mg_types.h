@@ -128,6 +128,13 @@ typedef struct MgGenome {
  *   diff_scale    multiplies the (uint) TF diffusability in diffusion;
  *   secrete_scale multiplies the (int) secretion rate in gene action.
  */
+/* SPH kernels (mg_set_sph_kernels): the reference's live Wendland-C2 density and
+ * (sr - r)^3 force (src/fluid_system_cuda.cu:620-712, 3167-3358), or the Mueller (2003)
+ * poly6 density with spiky pressure and viscosity-Laplacian force whose constants FluidParamCL
+ * computes (src/fluid_system.cpp:550, 568-569) and whose forms the reference keeps in
+ * comments (src/fluid_system_cuda.cu:668-672, 695, 3140-3160, 3213-3220). */
+enum { MG_SPH_REFERENCE = 0, MG_SPH_POLY6_SPIKY = 1 };
+
 typedef struct MgParams {
     float psimscale, psmoothradius, pradius, r2, pmass, prest_dens;
     float pextstiff, pintstiff, pvisc, psurface_t, pdamp;

@@ -53,6 +53,9 @@ GRID_UNDEF = 0xFFFFFFFF
 STEP_GENES = 1
 STEP_GRAPH = 2
 STEP_SPECIAL = 4
+# SPH kernels (mg_set_sph_kernels): the reference's live ones, or Mueller 2003 poly6/spiky
+SPH_REFERENCE = 0
+SPH_POLY6_SPIKY = 1
 
 MG_OK = 0
 ERRORS = {-1: "MG_ERR_ARG", -2: "MG_ERR_HIP", -3: "MG_ERR_STATE", -4: "MG_ERR_UNSUPPORTED",
@@ -196,6 +199,7 @@ def declare(lib, prefix: str):
         "set_params": [h, P(MgParams)],
         "set_genome": [h, P(MgGenome)],
         "set_freeze": [h, C.c_int],
+        "set_sph_kernels": [h, C.c_int],
         "upload": [h, C.c_int, C.c_void_p, C.c_size_t],
         "download": [h, C.c_int, C.c_void_p, C.c_size_t],
         "get_counts": [h, P(C.c_uint32), P(C.c_uint32)],

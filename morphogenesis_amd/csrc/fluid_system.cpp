@@ -82,6 +82,8 @@ void FluidSystem::SetParams(const MgParams& p) { check(mg_set_params(h_, &p), "S
 void FluidSystem::UpdateGenome() { check(mg_set_genome(h_, &m_FGenome), "UpdateGenome"); }
 void FluidSystem::setFreeze(bool freeze) { check(mg_set_freeze(h_, freeze ? 1 : 0), "setFreeze"); }
 
+void FluidSystem::SetSPHKernels(int mode) { check(mg_set_sph_kernels(h_, mode), "SetSPHKernels"); }
+
 void FluidSystem::TransferToCL(int id)
 {
     check(mg_upload(h_, id, bufC(id), bufBytes(id)), "TransferToCL");

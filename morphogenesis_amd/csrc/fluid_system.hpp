@@ -48,6 +48,9 @@ public:
     MgGenome GetGenome() const { return m_FGenome; }
     void UpdateGenome();
     void setFreeze(bool freeze);
+    /* SPH kernels of pressure and force: MG_SPH_REFERENCE (default) or MG_SPH_POLY6_SPIKY
+     * (Mueller 2003 poly6 / spiky / viscosity Laplacian; mg_set_sph_kernels) */
+    void SetSPHKernels(int mode);
 
     /* host mirrors in the reference layout (bufC/bufF/bufI of src/fluid.h:244-253) */
     char* bufC(int id);

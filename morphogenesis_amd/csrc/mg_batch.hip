@@ -541,6 +541,10 @@ int mg_batch_create(mg_batch** out, mg_handle* const* handles, int n)
     for (int i = 0; i < n; i++) {
         if (!handles[i]) { set_err("mg_batch_create: null handle %d", i); return MG_ERR_ARG; }
         if (handles[i]->batch) { set_err("mg_batch_create: handle %d is already in a batch", i); return MG_ERR_STATE; }
+        if (handles[i]->sph != MG_SPH_REFERENCE) {
+            set_err("mg_batch_create: handle %d uses the poly6/spiky kernels; batches step the reference kernels", i);
+            return MG_ERR_UNSUPPORTED;
+        }
         for (int j = 0; j < i; j++)
             if (handles[j] == handles[i]) { set_err("mg_batch_create: handle %d repeated", i); return MG_ERR_ARG; }
     }

@@ -236,6 +236,7 @@ __global__ void k_download_fbin(uint32_t* dst, const uint32_t* FBIN, const uint3
 /* ================================================================== handle */
 struct mg_handle {
     int N = 0, B = 0, C = 0;
+    int sph = MG_SPH_REFERENCE; /* SPH kernels of pressure and force (MG_SPH_*) */
     MgParams p;
     MgGrid g;
     MgGenome gen;
@@ -703,6 +704,25 @@ template <int NX, int CAP = BAND_CAP> constexpr size_t band_lds(unsigned threads
     return (threads / 64) * sizeof(Band<NX, CAP>);
 }
 
+/* the pressure and force launches for the handle's SPH kernels */
+void launch_pressure(mg_handle* h, unsigned nb, const DS& d)
+{
+    if (h->sph == MG_SPH_POLY6_SPIKY)
+        hipLaunchKernelGGL(k_pressure<MG_SPH_POLY6_SPIKY>, dim3(nb), dim3(256), band_lds<0>(256), h->st, d, h->p, h->g);
+    else
+        hipLaunchKernelGGL(k_pressure<MG_SPH_REFERENCE>, dim3(nb), dim3(256), band_lds<0>(256), h->st, d, h->p, h->g);
+}
+
+void launch_force(mg_handle* h, unsigned nb, const DS& d, int accumulate)
+{
+    if (h->sph == MG_SPH_POLY6_SPIKY)
+        hipLaunchKernelGGL(k_force<MG_SPH_POLY6_SPIKY>, dim3(nb), dim3(256), band_lds<1>(256), h->st, d, h->p, h->g,
+                           accumulate);
+    else
+        hipLaunchKernelGGL(k_force<MG_SPH_REFERENCE>, dim3(nb), dim3(256), band_lds<1>(256), h->st, d, h->p, h->g,
+                           accumulate);
+}
+
 /* The FCONC float4 groups the diffusion sweep reads and the per-TF coefficients
  * tf_diffusability * diff_scale (the same float operations as the reference's loop). */
 DiffPlan diff_plan(const mg_handle* h)
@@ -915,7 +935,7 @@ int do_fused_step(mg_handle* h, float dt, unsigned flags, const mg_net* net = nu
         d.ZERO = h->CNT[1 - h->gcur];
         d.zero_n = (uint32_t)h->C;
         d.HMW = hm;
-        hipLaunchKernelGGL(k_pressure, dim3(nb), dim3(256), band_lds<0>(256), h->st, d, h->p, h->g);
+        launch_pressure(h, nb, d);
     }
     const int special = (flags & MG_STEP_SPECIAL) ? 1 : 0;
     const float zoff = special ? actuation_offset(h, dt) : 0.0f;
@@ -926,7 +946,7 @@ int do_fused_step(mg_handle* h, float dt, unsigned flags, const mg_net* net = nu
             PhaseScope ps(h, MG_PH_FORCE);
             DS d = make_ds(h);
             d.HMR = hm;
-            hipLaunchKernelGGL(k_force, dim3(nb), dim3(256), band_lds<1>(256), h->st, d, h->p, h->g, 0);
+            launch_force(h, nb, d, 0);
         }
         {
             PhaseScope ps(h, MG_PH_LEARN);
@@ -943,8 +963,12 @@ int do_fused_step(mg_handle* h, float dt, unsigned flags, const mg_net* net = nu
         PhaseScope ps(h, MG_PH_FORCE);
         DS d = make_ds(h);
         d.HMR = hm;
-        hipLaunchKernelGGL(k_force_advance, dim3(nb), dim3(256), band_lds<1>(256), h->st, d, h->p, h->g, dt,
-                           special, zoff);
+        if (h->sph == MG_SPH_POLY6_SPIKY)
+            hipLaunchKernelGGL(k_force_advance<MG_SPH_POLY6_SPIKY>, dim3(nb), dim3(256), band_lds<1>(256), h->st, d,
+                               h->p, h->g, dt, special, zoff);
+        else
+            hipLaunchKernelGGL(k_force_advance<MG_SPH_REFERENCE>, dim3(nb), dim3(256), band_lds<1>(256), h->st, d,
+                               h->p, h->g, dt, special, zoff);
     }
     {
         PhaseScope ps(h, MG_PH_DIFFUSION);
@@ -1061,7 +1085,7 @@ int do_pressure(mg_handle* h, uint64_t* hm = nullptr)
     PhaseScope ps(h, MG_PH_PRESSURE);
     DS d = make_ds(h);
     d.HMW = hm;
-    hipLaunchKernelGGL(k_pressure, dim3(nblk(h->N)), dim3(256), band_lds<0>(256), h->st, d, h->p, h->g);
+    launch_pressure(h, nblk(h->N), d);
     HIPCHK(hipGetLastError());
     return MG_OK;
 }
@@ -1072,7 +1096,7 @@ int do_force(mg_handle* h, const uint64_t* hm = nullptr)
     PhaseScope ps(h, MG_PH_FORCE);
     DS d = make_ds(h);
     d.HMR = hm;
-    hipLaunchKernelGGL(k_force, dim3(nblk(h->N)), dim3(256), band_lds<1>(256), h->st, d, h->p, h->g, 1);
+    launch_force(h, nblk(h->N), d, 1);
     HIPCHK(hipGetLastError());
     return MG_OK;
 }
@@ -1415,6 +1439,29 @@ int mg_set_genome(mg_handle* h, const MgGenome* g)
     if (!h || !g) { set_err("mg_set_genome: null argument"); return MG_ERR_ARG; }
     h->gen = *g;
     h->epoch++;
+    return MG_OK;
+}
+
+int mg_set_sph_kernels(mg_handle* h, int mode)
+{
+    if (!h) { set_err("mg_set_sph_kernels: null handle"); return MG_ERR_ARG; }
+    if (mode != MG_SPH_REFERENCE && mode != MG_SPH_POLY6_SPIKY) {
+        set_err("mg_set_sph_kernels: unknown mode %d", mode);
+        return MG_ERR_ARG;
+    }
+    if (h->batch && mode != MG_SPH_REFERENCE) {
+        set_err("mg_set_sph_kernels: the handle is in a batch (batches step the reference kernels)");
+        return MG_ERR_UNSUPPORTED;
+    }
+    h->sph = mode;
+    h->epoch++; /* captured graphs hold the old kernels */
+    return MG_OK;
+}
+
+int mg_get_sph_kernels(mg_handle* h, int* mode)
+{
+    if (!h || !mode) return MG_ERR_ARG;
+    *mode = h->sph;
     return MG_OK;
 }
 

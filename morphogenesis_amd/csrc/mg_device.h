@@ -143,6 +143,24 @@ __device__ __forceinline__ f3 bond_term(f3 opos, f3 oveval, f3 ppos, f3 pveval, 
 }
 
 /* Fluid pair term for one neighbour within dsq < 1 (src/fluid_system_cuda.cu:3203-3221). */
+/* The selectable Mueller (2003) kernels (MG_SPH_POLY6_SPIKY): the pair force in the arrangement
+ * of Fluids v3, which the reference keeps in comments (src/fluid_system_cuda.cu:3140-3160,
+ * 3213-3220) with the constants FluidParamCL computes (src/fluid_system.cpp:568-569, 587):
+ * sdist = sqrt(dsq d2), c = sr - sdist,
+ * pterm = simscale (-1/2) c spikykern (p_i + p_j) / sdist (spikykern = -45 / (pi sr^6)),
+ * (pterm dist + vterm (v_j - v_i)) c (1/rho_i) (1/rho_j), vterm = lapkern * visc: the
+ * pressure term -(p_i + p_j) / 2 grad W_spiky and the viscosity term mu (v_j - v_i)
+ * lap W_visc, both over rho_i rho_j (times pmass where the per-cell sums are added). */
+__device__ __forceinline__ f3 fluid_term_poly6(f3 dist, float dsq, const MgParams& p, float ipress, float jpress,
+                                               float idens, float jdens, f3 ivev, f3 jvev)
+{
+    const float sdist = sqrtf(dsq * p.d2);
+    const float c = p.psmoothradius - sdist;
+    const float pterm = p.psimscale * -0.5f * c * p.spikykern * (ipress + jpress) / sdist;
+    const float dterm = c * idens * jdens;
+    return (dist * pterm + (jvev - ivev) * p.vterm) * dterm;
+}
+
 __device__ __forceinline__ f3 fluid_term(f3 dist, float r, float sr, float ipress, float jpress,
                                          float idens, float jdens, f3 ivev, f3 jvev,
                                          float surface_t)

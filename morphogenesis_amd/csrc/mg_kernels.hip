@@ -968,7 +968,7 @@ __device__ __forceinline__ void sweep(const DS& d, const MgGrid& g, Band<NX, CAP
  * contributePressure/computePressure, src/fluid_system_cuda.cu:620-712. Per cell partial
  * sums are added to the running total in gridAdj order. Writes FPRESS into P.w and
  * FDENSITY (= 1/density) into V.w. Also clears d.ZERO (the next grid's counters) when set. */
-template <int K = 8, int CAP = BAND_CAP>
+template <int K = 8, int CAP = BAND_CAP, int MODE = MG_SPH_REFERENCE>
 __device__ __forceinline__ void pressure_body(const DS& d, const MgParams& p, const MgGrid& g,
                                               Band<0, CAP>* bands, uint32_t bid, uint32_t nblk)
 {
@@ -994,7 +994,17 @@ __device__ __forceinline__ void pressure_body(const DS& d, const MgParams& p, co
      * skips the correctly rounded division; the sums are the same bits either way */
     const uint32_t srb = __float_as_uint(sr);
     const bool sr_pow2 = (srb & 0x007FFFFFu) == 0u && ((srb >> 23) & 0xFFu) != 0u && ((srb >> 23) & 0xFFu) != 0xFFu;
-    if (sr_pow2) {
+    if (MODE == MG_SPH_POLY6_SPIKY) {
+        /* the poly6 sum the reference keeps in comments (src/fluid_system_cuda.cu:668-672, 695) */
+        const float r2 = p.r2, d2 = p.d2;
+        sweep<K, 0, 1, CAP>(
+            d, g, bands[threadIdx.x >> 6], valid, gc, pi, p.r2, none,
+            [&](uint32_t, float4, f3, float dsq, const float4*) {
+                const float c = (r2 - dsq) * d2;
+                csum += c * c * c;
+            },
+            flush, i, MG_SKIN_LIM);
+    } else if (sr_pow2) {
         const float inv_sr = 1.0f / sr;
         sweep<K, 0, 1, CAP>(
             d, g, bands[threadIdx.x >> 6], valid, gc, pi, p.r2, none,
@@ -1021,17 +1031,19 @@ __device__ __forceinline__ void pressure_body(const DS& d, const MgParams& p, co
             flush, i, MG_SKIN_LIM);
     }
     if (!valid) return;
-    sum = sum * p.pmass * p.wendlandC2kern;
+    sum = sum * p.pmass * (MODE == MG_SPH_POLY6_SPIKY ? p.poly6kern : p.wendlandC2kern);
     if (sum == 0.0f) sum = 1.0f;
     ((float*)&d.P[i])[3] = (sum - p.prest_dens) * p.pintstiff;
     ((float*)&d.V[i])[3] = 1.0f / sum;
 }
 
-/* the sweep kernels take (blockDim.x / 64) Band<NX> of dynamic LDS, one per wavefront */
+/* the sweep kernels take (blockDim.x / 64) Band<NX> of dynamic LDS, one per wavefront;
+ * MODE selects the SPH kernels (MG_SPH_*) */
+template <int MODE>
 __global__ LAUNCH_BOUNDS_SWEEP void k_pressure(DS d, MgParams p, MgGrid g)
 {
     extern __shared__ float4 mg_lds[];
-    pressure_body(d, p, g, (Band<0>*)mg_lds, xcd_block(), gridDim.x);
+    pressure_body<8, BAND_CAP, MODE>(d, p, g, (Band<0>*)mg_lds, xcd_block(), gridDim.x);
 }
 
 /* --------------------------------------------------------------- force (a10)
@@ -1046,6 +1058,7 @@ __global__ LAUNCH_BOUNDS_SWEEP void k_pressure(DS d, MgParams p, MgGrid g)
  * (LIVE, kept with the bond records) are fetched 4 at a time in slot order; incoming bonds
  * are fetched 4 at a time; in both loops the next group's records are fetched while the
  * current group's partner positions load. */
+template <int MODE = MG_SPH_REFERENCE>
 __device__ __forceinline__ f3 force_one(const DS& d, const MgParams& p, const MgGrid& g, Band<1>& wb,
                                         bool valid, uint32_t i, uint32_t na, uint32_t gc, f3 F0)
 {
@@ -1091,12 +1104,19 @@ __device__ __forceinline__ f3 force_one(const DS& d, const MgParams& p, const Mg
     /* fluid terms: per-cell sums added in gridAdj order */
     const float sr = p.psmoothradius, st = p.psurface_t;
     f3 fsum = mk3(0.0f, 0.0f, 0.0f), cf = mk3(0.0f, 0.0f, 0.0f);
+    /* the reference's force tests dsq < 1 (src/fluid_system_cuda.cu:3203); the Mueller kernels
+     * have the smoothing radius as support */
+    const float flim = MODE == MG_SPH_POLY6_SPIKY ? p.r2 : 1.0f;
     sweep<4, 1, 2>(
-        d, g, wb, valid, gc, ipos, 1.0f, [&](uint32_t j, int) { return d.V[j]; },
+        d, g, wb, valid, gc, ipos, flim, [&](uint32_t j, int) { return d.V[j]; },
         [&](uint32_t, float4 Pj, f3 dist, float dsq, const float4* x) {
             const float4 Vj = x[0];
-            float r = sqrtf(dsq);
-            cf = cf + fluid_term(dist, r, sr, ipress, Pj.w, idens, Vj.w, ivev, xyz(Vj), st);
+            if (MODE == MG_SPH_POLY6_SPIKY) {
+                cf = cf + fluid_term_poly6(dist, dsq, p, ipress, Pj.w, idens, Vj.w, ivev, xyz(Vj));
+            } else {
+                float r = sqrtf(dsq);
+                cf = cf + fluid_term(dist, r, sr, ipress, Pj.w, idens, Vj.w, ivev, xyz(Vj), st);
+            }
         },
         [&]() {
             fsum = fsum + cf;
@@ -1205,6 +1225,7 @@ __device__ __forceinline__ f3 force_one(const DS& d, const MgParams& p, const Mg
 }
 
 /* ComputeForceCL: FFORCE[i] += force (accumulate) or = 0 + force. */
+template <int MODE>
 __global__ LAUNCH_BOUNDS_SWEEP void k_force(DS d, MgParams p, MgGrid g, int accumulate)
 {
     extern __shared__ float4 mg_lds[];
@@ -1214,7 +1235,7 @@ __global__ LAUNCH_BOUNDS_SWEEP void k_force(DS d, MgParams p, MgGrid g, int accu
     const uint32_t gc = (i < na) ? d.GCELL[i] : MG_GRID_UNDEF;
     const bool valid = gc != MG_GRID_UNDEF;
     const f3 F0 = (valid && accumulate) ? xyz(d.F[i]) : mk3(0.0f, 0.0f, 0.0f);
-    const f3 F = force_one(d, p, g, bands[threadIdx.x >> 6], valid, i, na, gc, F0);
+    const f3 F = force_one<MODE>(d, p, g, bands[threadIdx.x >> 6], valid, i, na, gc, F0);
     if (valid) d.F[i] = make_float4(F.x, F.y, F.z, 0.0f);
 }
 
@@ -1387,6 +1408,7 @@ __global__ LAUNCH_BOUNDS_256 void k_advance_learned(DS d, MgParams p, float dt, 
 /* Fused ComputeForceCL + AdvanceCL for a step that starts from a sort (FFORCE = 0): the force
  * stays in registers and the advanced state goes to the other half of the ping-pong
  * (P2/V2/VEL2) so neighbours keep reading the pre-step state. Inactive slots copy through. */
+template <int MODE = MG_SPH_REFERENCE>
 __device__ __forceinline__ void force_advance_body(const DS& d, const MgParams& p, const MgGrid& g,
                                                    Band<1>* bands, float dt, uint32_t bid,
                                                    int special = 0, float zoff = 0.0f)
@@ -1395,16 +1417,17 @@ __device__ __forceinline__ void force_advance_body(const DS& d, const MgParams& 
     const uint32_t na = d.ctr[MG_C_ACTIVE];
     const uint32_t gc = (i < na) ? d.GCELL[i] : MG_GRID_UNDEF;
     const bool valid = gc != MG_GRID_UNDEF;
-    const f3 F = force_one(d, p, g, bands[threadIdx.x >> 6], valid, i, na, gc, mk3(0.0f, 0.0f, 0.0f));
+    const f3 F = force_one<MODE>(d, p, g, bands[threadIdx.x >> 6], valid, i, na, gc, mk3(0.0f, 0.0f, 0.0f));
     if (i >= (uint32_t)d.N) return;
     advance_other_half(d, p, dt, i, valid, F, special, zoff);
 }
 
+template <int MODE>
 __global__ LAUNCH_BOUNDS_SWEEP void k_force_advance(DS d, MgParams p, MgGrid g, float dt, int special,
                                                     float zoff)
 {
     extern __shared__ float4 mg_lds[];
-    force_advance_body(d, p, g, (Band<1>*)mg_lds, dt, xcd_block(), special, zoff);
+    force_advance_body<MODE>(d, p, g, (Band<1>*)mg_lds, dt, xcd_block(), special, zoff);
 }
 
 /* ----------------------------------------------------------- diffusion (a12)

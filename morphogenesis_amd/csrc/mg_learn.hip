@@ -405,7 +405,7 @@ int mg_net_features(mg_handle* h, float* X_out)
         DS d = make_ds(h);
         d.HMR = hm;
         d.dry = 1;
-        hipLaunchKernelGGL(k_force, dim3(nblk(h->N)), dim3(256), band_lds<1>(256), h->st, d, h->p, h->g, 0);
+        launch_force(h, nblk(h->N), d, 0);
         HIPCHK(hipGetLastError());
     }
     float* d = nullptr;
@@ -438,7 +438,7 @@ int mg_net_infer(mg_handle* h, mg_net* n, float dt, float* DV_out)
         DS d = make_ds(h);
         d.HMR = hm;
         d.dry = 1;
-        hipLaunchKernelGGL(k_force, dim3(nblk(h->N)), dim3(256), band_lds<1>(256), h->st, d, h->p, h->g, 0);
+        launch_force(h, nblk(h->N), d, 0);
         HIPCHK(hipGetLastError());
     }
     CHK(launch_mlp_infer(h, n, dt));

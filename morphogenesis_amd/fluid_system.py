@@ -70,6 +70,12 @@ class FluidSystem:
     def setFreeze(self, freeze: bool):
         self._call("set_freeze", self.h, int(bool(freeze)))
 
+    def set_sph_kernels(self, mode: int):
+        """F.SPH_REFERENCE (the reference's Wendland-C2 density and (sr - r)^3 force, the
+        default) or F.SPH_POLY6_SPIKY (Mueller 2003 poly6 density, spiky pressure and
+        viscosity-Laplacian force)."""
+        self._call("set_sph_kernels", self.h, int(mode))
+
     def upload(self, buf: int, arr: np.ndarray):
         shape, dtype = F.buffer_shape(buf, self.n, self.c, self.b)
         a = np.ascontiguousarray(arr, dtype=dtype).reshape(shape)

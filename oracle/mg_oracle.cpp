@@ -182,6 +182,7 @@ struct Base {
     uint32_t num_points = 0, active_points = 0;
     int nthreads = 0;
     double time = 0.0;
+    int sph = MG_SPH_REFERENCE; /* SPH kernels of pressure and force */
 };
 
 template <typename R> struct Oracle : Base {
@@ -512,7 +513,8 @@ template <typename R> struct Oracle : Base {
     void pressure() override
     {
         const int na = (int)active_points;
-        const R r2 = (R)p.r2, sr = (R)p.psmoothradius;
+        const R r2 = (R)p.r2, sr = (R)p.psmoothradius, d2 = (R)p.d2;
+        const bool poly6 = sph == MG_SPH_POLY6_SPIKY;
         const int nadj = (1 * g.res[2] + 1) * g.res[0] + 1;
 #pragma omp parallel for schedule(static) num_threads(nth())
         for (int i = 0; i < na; i++) {
@@ -531,18 +533,25 @@ template <typename R> struct Oracle : Base {
                         V3<R> dist = pi - P(pndx);
                         R dsq = dist.x * dist.x + dist.y * dist.y + dist.z * dist.z;
                         if (dsq < r2 && dsq > R(0)) {
-                            R r = std::sqrt(dsq);
-                            R q = r / sr;
-                            R b = R(1) - q;
-                            b *= b;
-                            b *= b;
-                            csum += b * (R(4) * q + R(1));
+                            if (poly6) {
+                                /* Mueller 2003 poly6 (h^2 - r^2)^3, as the reference's commented
+                                 * sum at src/fluid_system_cuda.cu:668-672 */
+                                R c = (r2 - dsq) * d2;
+                                csum += c * c * c;
+                            } else {
+                                R r = std::sqrt(dsq);
+                                R q = r / sr;
+                                R b = R(1) - q;
+                                b *= b;
+                                b *= b;
+                                csum += b * (R(4) * q + R(1));
+                            }
                         }
                     }
                 }
                 sum += csum;
             }
-            sum = sum * (R)p.pmass * (R)p.wendlandC2kern;
+            sum = sum * (R)p.pmass * (R)(poly6 ? p.poly6kern : p.wendlandC2kern); /* :695 */
             if (sum == R(0)) sum = R(1);
             press[i] = (sum - (R)p.prest_dens) * (R)p.pintstiff;
             dens[i] = R(1) / sum;
@@ -557,7 +566,8 @@ template <typename R> struct Oracle : Base {
     {
         const int na = (int)active_points;
         const uint32_t pnum = (uint32_t)na;
-        const R sr = (R)p.psmoothradius;
+        const R sr = (R)p.psmoothradius, r2 = (R)p.r2, d2 = (R)p.d2;
+        const bool poly6 = sph == MG_SPH_POLY6_SPIKY;
         const int nadj = (1 * g.res[2] + 1) * g.res[0] + 1;
         const bool freeze = p.freeze != 0;
         std::vector<R> own((size_t)N * 3, R(0));
@@ -612,6 +622,21 @@ template <typename R> struct Oracle : Base {
                         uint32_t j = bin[cndx];
                         V3<R> dist = ipos - P(j);
                         R dsq = dist.x * dist.x + dist.y * dist.y + dist.z * dist.z;
+                        if (poly6) {
+                            /* Mueller 2003 spiky pressure and viscosity Laplacian terms in the
+                             * arrangement of the reference's comments
+                             * (src/fluid_system_cuda.cu:3140-3160, 3213-3220), over the support
+                             * r < sr */
+                            if (dsq < r2 && dsq > R(0)) {
+                                R sdist = std::sqrt(dsq * d2);
+                                R c = sr - sdist;
+                                R pterm = (R)p.psimscale * R(-0.5) * c * (R)p.spikykern * (ipress + press[j]) / sdist;
+                                R dterm = c * idens * dens[j];
+                                V3<R> delta_v = VE(j) - ivev;
+                                cf = cf + (dist * pterm + delta_v * (R)p.vterm) * dterm;
+                            }
+                            continue;
+                        }
                         R r = std::sqrt(dsq);
                         if (dsq < R(1) && dsq > R(0)) {
                             R t = sr - r;
@@ -1184,6 +1209,13 @@ int mgo_set_freeze(mgo_handle* h, int freeze)
 {
     if (!h) return MG_ERR_ARG;
     h->o->p.freeze = freeze ? 1 : 0;
+    return MG_OK;
+}
+
+int mgo_set_sph_kernels(mgo_handle* h, int mode)
+{
+    if (!h || (mode != MG_SPH_REFERENCE && mode != MG_SPH_POLY6_SPIKY)) return MG_ERR_ARG;
+    h->o->sph = mode;
     return MG_OK;
 }
 

@@ -101,6 +101,9 @@ class Oracle:
     def setFreeze(self, freeze: bool):
         self._call("set_freeze", self.h, int(bool(freeze)))
 
+    def set_sph_kernels(self, mode: int):
+        self._call("set_sph_kernels", self.h, int(mode))
+
     def upload(self, buf, arr):
         shape, dtype = F.buffer_shape(buf, self.n, self.c, self.b)
         a = np.ascontiguousarray(arr, dtype=dtype).reshape(shape)
