@@ -29,6 +29,9 @@ struct mg_net {
     /* training set */
     float *X = nullptr, *T = nullptr, *mask = nullptr;
     size_t rows = 0;
+    std::vector<uint64_t> used_prefix; /* rows with a nonzero mask before each row (rows + 1) */
+    float* loss_dev = nullptr;         /* per-step losses of mg_net_train, downloaded at its end */
+    size_t loss_cap = 0;
     /* work buffers for minibatches of up to cap rows */
     size_t cap = 0;
     __bf16 *Xn = nullptr, *A1b = nullptr, *A2b = nullptr, *dYb = nullptr, *dZ2b = nullptr, *dZ1b = nullptr;
@@ -118,7 +121,19 @@ int gemm(mg_net* n, int M, int N, int K, const __bf16* A, long sam, long sak, co
 }
 
 /* forward (and with grads, backward) over rows [r0, r0 + S) of the training set */
-int net_pass(mg_net* n, size_t r0, size_t S, bool backward, float* loss_host)
+/* one thread: the blocks' loss partials summed in order in double, as a float (the host's
+ * sum of the same partials gives the same bits) */
+__global__ void k_sum_partials(const float* __restrict__ part, unsigned count, float* __restrict__ out)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double s = 0.0;
+    for (unsigned k = 0; k < count; k++) s += part[k];
+    *out = (float)s;
+}
+
+/* forward (and backward) over rows [r0, r0 + S); the mean loss goes to the host (loss_host)
+ * or stays on the device (loss_dev), either may be null */
+int net_pass(mg_net* n, size_t r0, size_t S, bool backward, float* loss_host, float* loss_dev = nullptr)
 {
     CHK(net_reserve(n, S));
     const float* X = n->X + r0 * MLP_IN;
@@ -135,12 +150,8 @@ int net_pass(mg_net* n, size_t r0, size_t S, bool backward, float* loss_host)
     hipLaunchKernelGGL(k_bias_silu, dim3(gblk(S * MLP_OUT)), dim3(256), 0, n->st, n->Y, n->params + kOffB3, MLP_OUT,
                        S, (__bf16*)nullptr, 0);
     if (!backward) return MG_OK;
-    /* count of masked rows in this window, for the mean */
-    std::vector<float> mk(S);
-    HIPCHK(hipMemcpyAsync(mk.data(), n->mask + r0, S * 4, hipMemcpyDeviceToHost, n->st));
-    HIPCHK(hipStreamSynchronize(n->st));
-    double cnt = 0;
-    for (float x : mk) cnt += x != 0.f;
+    /* count of masked rows in this window, for the mean (from the prefix taken at load) */
+    const double cnt = (double)(n->used_prefix[r0 + S] - n->used_prefix[r0]);
     const float inv = cnt > 0 ? (float)(1.0 / cnt) : 0.f;
     const unsigned lb = gblk(S);
     hipLaunchKernelGGL(k_mse, dim3(lb), dim3(256), 0, n->st, n->Y, n->T + r0 * MLP_OUT_REAL, n->mask + r0, S, inv, n->dY,
@@ -161,6 +172,10 @@ int net_pass(mg_net* n, size_t r0, size_t S, bool backward, float* loss_host)
     CHK(gemm(n, MLP_IN, MLP_H, (int)S, n->Xn, 1, MLP_IN, n->dZ1b, MLP_H, 1, g + kOffW1, ks));
     hipLaunchKernelGGL(k_colsum, dim3(MLP_H), dim3(256), 0, n->st, n->dZ1, S, MLP_H, g + kOffB1);
     HIPCHK(hipGetLastError());
+    if (loss_dev) {
+        hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(64), 0, n->st, n->lpart, lb, loss_dev);
+        HIPCHK(hipGetLastError());
+    }
     if (loss_host) {
         std::vector<float> lp(lb);
         HIPCHK(hipMemcpyAsync(lp.data(), n->lpart, lb * 4, hipMemcpyDeviceToHost, n->st));
@@ -331,6 +346,8 @@ int mg_net_load_data(mg_net* n, const float* X, const float* T, const float* mas
     HIPCHK(hipMemcpyAsync(n->T, T, rows * MLP_OUT_REAL * 4, hipMemcpyHostToDevice, n->st));
     HIPCHK(hipMemcpyAsync(n->mask, mask, rows * 4, hipMemcpyHostToDevice, n->st));
     HIPCHK(hipStreamSynchronize(n->st));
+    n->used_prefix.assign(rows + 1, 0);
+    for (size_t r = 0; r < rows; r++) n->used_prefix[r + 1] = n->used_prefix[r] + (mask[r] != 0.f ? 1u : 0u);
     n->rows = rows;
     return MG_OK;
 }
@@ -371,12 +388,15 @@ int mg_net_train(mg_net* n, int steps, size_t batch, float lr, uint64_t seed, fl
     const size_t windows = n->rows / batch;
     size_t w = (size_t)(splitmix(s) % std::max<size_t>(windows, 1));
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    if (loss_out && (size_t)steps > n->loss_cap) {
+        CHK(nalloc(n, &n->loss_dev, (size_t)steps));
+        n->loss_cap = (size_t)steps;
+    }
+    /* no host synchronisation inside the loop: the losses stay on the device until the end */
     for (int k = 0; k < steps; k++) {
         const size_t r0 = (w % std::max<size_t>(windows, 1)) * batch;
         w++;
-        float loss = 0.f;
-        CHK(net_pass(n, r0, batch, true, loss_out ? &loss : nullptr));
-        if (loss_out) loss_out[k] = loss;
+        CHK(net_pass(n, r0, batch, true, nullptr, loss_out ? n->loss_dev + k : nullptr));
         n->adam_t++;
         const float c1 = 1.0f - std::pow(b1, (float)n->adam_t), c2 = 1.0f - std::pow(b2, (float)n->adam_t);
         hipLaunchKernelGGL(k_adam, dim3(gblk(kNParams)), dim3(256), 0, n->st, n->params, n->grads, n->m, n->v,
@@ -384,7 +404,9 @@ int mg_net_train(mg_net* n, int steps, size_t batch, float lr, uint64_t seed, fl
         hipLaunchKernelGGL(k_to_bf16, dim3(gblk(kNParams)), dim3(256), 0, n->st, n->params, n->wb, kNParams);
         HIPCHK(hipGetLastError());
     }
-    return net_sync(n);
+    if (loss_out && steps > 0)
+        HIPCHK(hipMemcpyAsync(loss_out, n->loss_dev, (size_t)steps * 4, hipMemcpyDeviceToHost, n->st));
+    return net_sync(n); /* synchronises the stream */
 }
 
 /* Raw MLP inputs of every slot of h's current sorted state (N x 64 floats; inactive rows

@@ -401,16 +401,70 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(int M, int N, int K, const __
     for (int a = 0; a < 2; a++)
 #pragma unroll
         for (int b = 0; b < 2; b++) acc[a][b] = (f32x4_t){0.f, 0.f, 0.f, 0.f};
+    /* each thread stages one 8-element chunk of A and one of B per K step: 8 consecutive k of
+     * a row when the operand is contiguous in k, else 8 consecutive m (n) of a k when it is
+     * contiguous in m (n), with one 16-byte load when the chunk is in range and aligned */
+    const bool a_k = sak == 1, a_m = !a_k && sam == 1;
+    const bool b_k = sbk == 1, b_n = !b_k && sbn == 1;
+    auto aligned = [](const __bf16* q) { return ((uintptr_t)q & 15u) == 0u; };
     for (int k0 = k_lo; k0 < k_hi; k0 += 32) {
-        /* 64 x 32 elements each of A and B: 8 per thread */
-#pragma unroll
-        for (int e = 0; e < 8; e++) {
-            const int idx = t * 8 + e;
-            const int r = idx >> 5, c = idx & 31;
+        if (a_k || a_m) {
+            const int r = a_k ? t >> 2 : (t & 7) * 8, c = a_k ? (t & 3) * 8 : t >> 3;
             const int m = m0 + r, k = k0 + c;
-            As[r][c] = (m < M && k < k_hi) ? A[(long)m * sam + (long)k * sak] : to_bf16(0.0f);
-            const int n = n0 + r;
-            Bs[r][c] = (n < N && k < k_hi) ? Bm[(long)k * sbk + (long)n * sbn] : to_bf16(0.0f);
+            const __bf16* q = A + (long)m * sam + (long)k * sak;
+            const bool full = a_k ? (m < M && k + 8 <= k_hi) : (k < k_hi && m + 8 <= M);
+            if (full && aligned(q)) {
+                const bf16x8_t v = *(const bf16x8_t*)q;
+#pragma unroll
+                for (int e = 0; e < 8; e++) {
+                    if (a_k) As[r][c + e] = v[e];
+                    else As[r + e][c] = v[e];
+                }
+            } else {
+#pragma unroll
+                for (int e = 0; e < 8; e++) {
+                    const int mm = a_k ? m : m + e, kk = a_k ? k + e : k;
+                    As[a_k ? r : r + e][a_k ? c + e : c] =
+                        (mm < M && kk < k_hi) ? A[(long)mm * sam + (long)kk * sak] : to_bf16(0.0f);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                const int idx = t * 8 + e;
+                const int r = idx >> 5, c = idx & 31;
+                const int m = m0 + r, k = k0 + c;
+                As[r][c] = (m < M && k < k_hi) ? A[(long)m * sam + (long)k * sak] : to_bf16(0.0f);
+            }
+        }
+        if (b_k || b_n) {
+            const int r = b_k ? t >> 2 : (t & 7) * 8, c = b_k ? (t & 3) * 8 : t >> 3;
+            const int n = n0 + r, k = k0 + c;
+            const __bf16* q = Bm + (long)k * sbk + (long)n * sbn;
+            const bool full = b_k ? (n < N && k + 8 <= k_hi) : (k < k_hi && n + 8 <= N);
+            if (full && aligned(q)) {
+                const bf16x8_t v = *(const bf16x8_t*)q;
+#pragma unroll
+                for (int e = 0; e < 8; e++) {
+                    if (b_k) Bs[r][c + e] = v[e];
+                    else Bs[r + e][c] = v[e];
+                }
+            } else {
+#pragma unroll
+                for (int e = 0; e < 8; e++) {
+                    const int nn = b_k ? n : n + e, kk = b_k ? k + e : k;
+                    Bs[b_k ? r : r + e][b_k ? c + e : c] =
+                        (nn < N && kk < k_hi) ? Bm[(long)kk * sbk + (long)nn * sbn] : to_bf16(0.0f);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                const int idx = t * 8 + e;
+                const int r = idx >> 5, c = idx & 31;
+                const int n = n0 + r, k = k0 + c;
+                Bs[r][c] = (n < N && k < k_hi) ? Bm[(long)k * sbk + (long)n * sbn] : to_bf16(0.0f);
+            }
         }
         __syncthreads();
         const int row = l & 15, kq = l >> 4;
