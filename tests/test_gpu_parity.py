@@ -491,3 +491,44 @@ def test_batch_tile_shapes_match_single(built):
     with pytest.raises(F.MgError):
         bat.set_tile_shape(0, 6, 128)
     bat.close()
+
+
+def row_windows(sim, grid):
+    """Slot counts of every particle's 9 neighbour rows (3 consecutive cells each) in the
+    sorted state."""
+    cnt = sim.download(F.FBIN_COUNT).astype(np.int64)
+    off = np.concatenate([sim.download(F.FBIN_OFFSET).astype(np.int64), [0]])
+    off[-1] = off[-2] + cnt[-1]
+    rx, rz = int(grid.res[0]), int(grid.res[2])
+    cell = np.repeat(np.arange(cnt.size), cnt)
+    out = []
+    for dy in (-1, 0, 1):
+        for dz in (-1, 0, 1):
+            c0 = np.clip(cell - 1 + (dy * rz + dz) * rx, 0, cnt.size - 3)
+            out.append(off[c0 + 3] - off[c0])
+    return np.stack(out, 1)
+
+
+@pytest.mark.parametrize("speed", [0.0, 60.0])
+def test_two_word_hit_masks(built, speed):
+    """27 particles per cell (lattice spacing 1/3): most neighbour rows hold 65-128 slots, more
+    than one 64-bit hit mask covers, so the force and diffusion sweeps test those rows'
+    candidates themselves while the other rows use the masks; the cells of more than 8
+    particles take the flat row walk (with a fast particle, the diffusion sweep of its
+    neighbourhood tests every row itself). Bit-equal to the oracle through hipGraph replay."""
+    sc = scenes.tissue(n_side=15, b=6, seed=5, spacing=1.0 / 3.0, vmax=12.0)
+    if speed:
+        sc.buffers[F.FVEL][200, :3] = [speed, 0.0, 0.0]
+        sc.buffers[F.FVEVAL][200, :3] = [speed, 0.0, 0.0]
+    say("case: two-word hit masks, speed", speed, sc.n)
+    g, o = pair(sc)
+    for sim in (g, o):
+        sim.Run2PhysicalSort()
+    w = row_windows(g, sc.grid)
+    frac = ((w > 64) & (w <= 128)).mean()
+    say("  rows of 65-128 slots:", round(float(frac), 3), "max", int(w.max()))
+    assert frac > 0.3
+    g.step(8, sc.dt, sc.flags | F.STEP_GRAPH)
+    o.step(8, sc.dt, sc.flags)
+    compare_all(g, o, f"two-word masks, speed {speed}", [F.FPOS, F.FVEL, F.FVEVAL, F.FCONC, F.FEPIGEN,
+                                                         F.FPRESS, F.FDENSITY, F.FGCELL, F.FBIN_OFFSET])
