@@ -118,6 +118,53 @@ __device__ __forceinline__ f3 divs(f3 a, float s)
     float inv = 1.0f / s;
     return a * inv;
 }
+/* Correctly rounded sqrtf and 1/x in their short forms. With IEEE division and square root
+ * (-fhip-fp32-correctly-rounded-divide-sqrt) the compiler expands sqrtf(x) to: scale x < 2^-96
+ * by 2^32; v_sqrt_f32; step the result one ulp down if fma(-(s - ulp), s, x) <= 0, one ulp up if
+ * fma(-(s + ulp), s, x) > 0; unscale; pass +-0 and +inf through. On [2^-96, +inf] the scaling and
+ * the pass-through do nothing, so the middle steps alone give the same bits. 1/x expands to
+ * v_div_scale, v_rcp_f32, a refinement by FMAs, v_div_fmas and v_div_fixup; for
+ * 2^-64 <= |x| <= 2^64 the scalings are the identity (numerator 1, no denormal reciprocal or
+ * quotient), v_div_fmas is a plain fma and v_div_fixup returns its input, so the FMAs alone give
+ * the same bits. The probe that compared every float of both ranges with sqrtf and 1.0f / x on
+ * the MI355X found no difference. */
+__device__ __forceinline__ float sqrt_rn_short(float x) /* 2^-96 <= x <= +inf */
+{
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float sd = __uint_as_float(__float_as_uint(s) - 1u);
+    const float su = __uint_as_float(__float_as_uint(s) + 1u);
+    const float rd = __builtin_fmaf(-sd, s, x);
+    const float ru = __builtin_fmaf(-su, s, x);
+    float r = rd <= 0.0f ? sd : s;
+    return ru > 0.0f ? su : r;
+}
+__device__ __forceinline__ float rcp_rn_short(float b) /* 2^-64 <= |b| <= 2^64 */
+{
+    float r = __builtin_amdgcn_rcpf(b);
+    r = __builtin_fmaf(__builtin_fmaf(-b, r, 1.0f), r, r);
+    float q = __builtin_fmaf(__builtin_fmaf(-b, r, 1.0f), r, r);
+    return __builtin_fmaf(__builtin_fmaf(-b, q, 1.0f), r, q);
+}
+/* true when every active lane's x lies in [lo, hi] (wave-uniform): the short forms apply */
+__device__ __forceinline__ bool wave_all_in(float x, float lo, float hi)
+{
+    return __ballot(!(x >= lo && x <= hi)) == 0ull;
+}
+/* r = sqrtf(dsq), inv = 1.0f / r, for hits with dsq < 2^96 */
+__device__ __forceinline__ void sqrt_rcp_rn(float dsq, float& r, float& inv)
+{
+    if (wave_all_in(dsq, 0x1p-96f, 0x1p96f)) {
+        r = sqrt_rn_short(dsq);
+        inv = rcp_rn_short(r);
+    } else {
+        r = sqrtf(dsq);
+        inv = 1.0f / r;
+    }
+}
+__device__ __forceinline__ float sqrt_rn(float x)
+{
+    return wave_all_in(x, 0x1p-96f, __builtin_inff()) ? sqrt_rn_short(x) : sqrtf(x);
+}
 __device__ __forceinline__ float dot3(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 __device__ __forceinline__ bool has_nan3(f3 a) { return a.x != a.x || a.y != a.y || a.z != a.z; }
 
@@ -130,11 +177,18 @@ __device__ __forceinline__ f3 bond_term(f3 opos, f3 oveval, f3 ppos, f3 pveval, 
     const float elastic_limit = bp.x, restlength = bp.y, modulus = bp.z, damping_coeff = bp.w;
     f3 dist = opos - ppos;
     float dsq = dist.x * dist.x + dist.y * dist.y + dist.z * dist.z;
-    float abs_dist = sqrtf(dsq) + FLT_MIN;
+    float abs_dist, inv_dist;
+    if (wave_all_in(dsq, 0x1p-96f, 0x1p96f)) {
+        abs_dist = sqrt_rn_short(dsq) + FLT_MIN;
+        inv_dist = rcp_rn_short(abs_dist);
+    } else {
+        abs_dist = sqrtf(dsq) + FLT_MIN;
+        inv_dist = 1.0f / abs_dist;
+    }
     f3 rel_vel = pveval - oveval;
     float spring_strain = fmaxf(0.0f, (abs_dist - restlength) / restlength);
     float fac = (float)(abs_dist < elastic_limit ? 1 : 0);
-    f3 e1 = divs(dist, abs_dist) * spring_strain * modulus;
+    f3 e1 = dist * inv_dist * spring_strain * modulus;
     f3 e2 = damping_coeff * rel_vel;
     f3 eterm = divs(fac * (e1 - e2), pmass);
     *strain_out = spring_strain;
@@ -161,15 +215,15 @@ __device__ __forceinline__ f3 fluid_term_poly6(f3 dist, float dsq, const MgParam
     return (dist * pterm + (jvev - ivev) * p.vterm) * dterm;
 }
 
-__device__ __forceinline__ f3 fluid_term(f3 dist, float r, float sr, float ipress, float jpress,
-                                         float idens, float jdens, f3 ivev, f3 jvev,
+__device__ __forceinline__ f3 fluid_term(f3 dist, float r, float inv_r, float sr, float ipress,
+                                         float jpress, float idens, float jdens, f3 ivev, f3 jvev,
                                          float surface_t)
 {
     float t = sr - r;
     float kern = t * t * t;
     float press = 100.0f * (ipress + jpress);
     float s = idens * jdens * 100.0f;
-    f3 pterm = (s * divs(dist, r)) * (press * kern - surface_t * (t * t));
+    f3 pterm = (s * (dist * inv_r)) * (press * kern - surface_t * (t * t)); /* dist / r: inv_r = 1 / r */
     f3 delta_v = jvev - ivev;
     f3 vterm = (100000.0f * delta_v) * kern;
     return pterm + vterm;

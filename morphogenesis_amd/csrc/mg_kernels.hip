@@ -739,6 +739,29 @@ __device__ __forceinline__ void cell_candidates(uint32_t lo, uint32_t hi, f3 pi,
     }
 }
 
+/* Tests the candidates of slots [lo, hi), K at a time, and sets bit j - wlo of rec for each
+ * with 0 < dsq < lim (no visits); every slot of the range lies in [wlo, wlo + 64). */
+template <int K, class GetP>
+__device__ __forceinline__ void range_record(uint32_t lo, uint32_t hi, uint32_t wlo, f3 pi, float lim,
+                                             GetP getp, uint64_t& rec)
+{
+    for (uint32_t j0 = lo; j0 < hi; j0 += K) {
+        float4 pj[K];
+#pragma unroll
+        for (int q = 0; q < K; q++) pj[q] = getp(j0 + (uint32_t)q);
+        uint32_t hb = 0u;
+#pragma unroll
+        for (int q = 0; q < K; q++) {
+            const f3 dist = pi - xyz(pj[q]);
+            const float dsq = dist.x * dist.x + dist.y * dist.y + dist.z * dist.z;
+            hb |= (dsq < lim && dsq > 0.0f) ? 1u << q : 0u;
+        }
+        /* slots past hi were read (bands and P are padded by K) but are not candidates */
+        if (hi - j0 < (uint32_t)K) hb &= (1u << (hi - j0)) - 1u;
+        rec |= (uint64_t)hb << (j0 - wlo);
+    }
+}
+
 /* Must be called by all 64 lanes of the wavefront (`valid` selects the lanes that sweep);
  * xload(j, q) is the global load of extra field q of slot j. K candidates are tested at a
  * time; bands of up to CAP (<= 128) slots are staged. */
@@ -844,40 +867,55 @@ __device__ __forceinline__ void sweep(const DS& d, const MgGrid& g, Band<NX, CAP
                 if (k < ncell && j < rb.b[k]) c = rb.b[k];
             return c;
         };
+        /* the recorded candidates m of this row (bit k = slot b[0] + k), ascending; a cell's
+         * partial sum is flushed where the first hit past its end is visited and after the row
+         * (the skipped flushes add +0 to sums that start at +0) */
+        /* one loop for the staged band and one for global memory, chosen per row, so that the
+         * band's reads are LDS reads (a select between the two would make them flat loads) */
+#define MG_WALK_MASK(M, GETP, GETX)                                                            \
+    do {                                                                                       \
+        uint64_t m_ = (M);                                                                     \
+        uint32_t cnext_ = next_bound(rb.b[0]);                                                 \
+        while (m_ != 0ull) {                                                                   \
+            const uint32_t j = rb.b[0] + (uint32_t)__builtin_ctzll(m_);                        \
+            m_ &= m_ - 1ull;                                                                   \
+            if (j >= cnext_) {                                                                 \
+                end_cell();                                                                    \
+                cnext_ = next_bound(j);                                                        \
+            }                                                                                  \
+            const float4 pj = GETP(j);                                                         \
+            const f3 dist = pi - xyz(pj);                                                      \
+            const float dsq = dist.x * dist.x + dist.y * dist.y + dist.z * dist.z;             \
+            if (!(dsq < lim && dsq > 0.0f)) continue; /* recorded within the skin only */      \
+            float4 ex[NXX];                                                                    \
+            _Pragma("unroll") for (int x = 0; x < NX; x++) ex[x] = GETX(j, x);                \
+            visit(j, pj, dist, dsq, ex);                                                       \
+        }                                                                                      \
+        end_cell();                                                                            \
+    } while (0)
+#define MG_BAND_P(j) wb.p[(j) - cblo]
+#define MG_BAND_X(j, x) wb.x[x][(j) - cblo]
+#define MG_GLOBAL_P(j) d.P[j]
+#define MG_GLOBAL_X(j, x) xload(j, x)
+        auto walk = [&](uint64_t m) {
+            if (cfits)
+                MG_WALK_MASK(m, MG_BAND_P, MG_BAND_X);
+            else
+                MG_WALK_MASK(m, MG_GLOBAL_P, MG_GLOBAL_X);
+        };
+#undef MG_WALK_MASK
+#undef MG_BAND_P
+#undef MG_BAND_X
+#undef MG_GLOBAL_P
+#undef MG_GLOBAL_X
         if (hm_r && hm_row != ~0ull) {
-            /* the pressure sweep's hits of this row, ascending; a cell's partial sum is
-             * flushed where the first hit past its end is visited and after the row (the
-             * skipped flushes add +0 to sums that start at +0) */
-            uint64_t m = hm_row;
-            const uint32_t b0 = rb.b[0];
-            uint32_t cnext = next_bound(b0);
-            while (m != 0ull) {
-                const uint32_t j = b0 + (uint32_t)__builtin_ctzll(m);
-                m &= m - 1ull;
-                if (j >= cnext) {
-                    end_cell();
-                    cnext = next_bound(j);
-                }
-                const float4 pj = cfits ? wb.p[j - cblo] : d.P[j];
-                const f3 dist = pi - xyz(pj);
-                const float dsq = dist.x * dist.x + dist.y * dist.y + dist.z * dist.z;
-                if (!(dsq < lim && dsq > 0.0f)) continue; /* recorded within the skin only */
-                float4 ex[NXX];
-                if (cfits) {
-#pragma unroll
-                    for (int x = 0; x < NX; x++) ex[x] = wb.x[x][j - cblo];
-                } else {
-#pragma unroll
-                    for (int x = 0; x < NX; x++) ex[x] = xload(j, x);
-                }
-                visit(j, pj, dist, dsq, ex);
-            }
-            end_cell();
+            walk(hm_row); /* the pressure sweep's record */
             continue;
         }
         /* window of the row for the hit record: [b[0], b[ncell]) */
         const uint32_t wlo = rb.b[0];
-        const bool wide = !hm_w || (ncell >= 4 ? rb.b[4] : ncell == 3 ? rb.b[3] : ncell == 2 ? rb.b[2] : rb.b[1]) - wlo > 64u;
+        const bool narrow = (ncell >= 4 ? rb.b[4] : ncell == 3 ? rb.b[3] : ncell == 2 ? rb.b[2] : rb.b[1]) - wlo <= 64u;
+        const bool wide = !hm_w || !narrow;
         uint64_t rec = 0ull;
         auto visit_rec = [&](uint32_t j, float4 pj, f3 dist, float dsq, const float4* x) {
             if (!wide) rec |= 1ull << ((j - wlo) & 63u);
@@ -897,6 +935,32 @@ __device__ __forceinline__ void sweep(const DS& d, const MgGrid& g, Band<NX, CAP
             }
             return live;
         };
+        if (HM == 1 && rows && narrow) {
+            /* a window of at most 64 slots: test the live cells' candidates (one slot range:
+             * the live cells are contiguous, a cell's gap grows away from the own column)
+             * into the row's mask, then visit the recorded ones in order, so the hit path runs
+             * once per recorded candidate of the wavefront's busiest lane rather than at
+             * every candidate slot */
+            bool any = false;
+            uint32_t flo = 0u, fhi = 0u;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                if (!live_of(k)) continue;
+                if (!any) flo = rb.b[k];
+                any = true;
+                fhi = max(rb.b[k + 1], rb.b[k]);
+            }
+            uint64_t m = 0ull;
+            if (any) {
+                if (cfits)
+                    range_record<K>(flo, max(fhi, flo), wlo, pi, tlim, [&](uint32_t j) { return wb.p[j - cblo]; }, m);
+                else
+                    range_record<K>(flo, max(fhi, flo), wlo, pi, tlim, [&](uint32_t j) { return d.P[j]; }, m);
+            }
+            if (hm_w) d.HMW[(size_t)s * N + self] = m;
+            walk(m);
+            continue;
+        }
         /* sweeps that read recorded masks take the candidate path only for unrecorded rows,
          * and always per cell */
         bool over = false;
@@ -1009,7 +1073,7 @@ __device__ __forceinline__ void pressure_body(const DS& d, const MgParams& p, co
         sweep<K, 0, 1, CAP>(
             d, g, bands[threadIdx.x >> 6], valid, gc, pi, p.r2, none,
             [&](uint32_t, float4, f3, float dsq, const float4*) {
-                float r = sqrtf(dsq);
+                float r = sqrt_rn(dsq);
                 float q = r * inv_sr;
                 float b = 1.0f - q;
                 b *= b;
@@ -1021,7 +1085,7 @@ __device__ __forceinline__ void pressure_body(const DS& d, const MgParams& p, co
         sweep<K, 0, 1, CAP>(
             d, g, bands[threadIdx.x >> 6], valid, gc, pi, p.r2, none,
             [&](uint32_t, float4, f3, float dsq, const float4*) {
-                float r = sqrtf(dsq);
+                float r = sqrt_rn(dsq);
                 float q = r / sr;
                 float b = 1.0f - q;
                 b *= b;
@@ -1114,8 +1178,9 @@ __device__ __forceinline__ f3 force_one(const DS& d, const MgParams& p, const Mg
             if (MODE == MG_SPH_POLY6_SPIKY) {
                 cf = cf + fluid_term_poly6(dist, dsq, p, ipress, Pj.w, idens, Vj.w, ivev, xyz(Vj));
             } else {
-                float r = sqrtf(dsq);
-                cf = cf + fluid_term(dist, r, sr, ipress, Pj.w, idens, Vj.w, ivev, xyz(Vj), st);
+                float r, inv_r;
+                sqrt_rcp_rn(dsq, r, inv_r);
+                cf = cf + fluid_term(dist, r, inv_r, sr, ipress, Pj.w, idens, Vj.w, ivev, xyz(Vj), st);
             }
         },
         [&]() {

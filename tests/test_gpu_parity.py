@@ -510,7 +510,7 @@ def row_windows(sim, grid):
 
 
 @pytest.mark.parametrize("speed", [0.0, 60.0])
-def test_two_word_hit_masks(built, speed):
+def test_rows_wider_than_one_hit_mask(built, speed):
     """27 particles per cell (lattice spacing 1/3): most neighbour rows hold 65-128 slots, more
     than one 64-bit hit mask covers, so the force and diffusion sweeps test those rows'
     candidates themselves while the other rows use the masks; the cells of more than 8
