@@ -57,5 +57,19 @@ def build_library(verbose: bool = False) -> str:
     return out
 
 
+def build_rounding_check() -> str:
+    """The GPU test library tests/hip/rounding_check.hip (the short forms of the correctly
+    rounded sqrt and 1/x against the compiler's expansions), with the library's flags."""
+    root = os.path.dirname(HERE)
+    src = os.path.join(root, "tests", "hip", "rounding_check.hip")
+    out_dir = os.path.join(root, "tests", "hip", "lib")
+    os.makedirs(out_dir, exist_ok=True)
+    out = os.path.join(out_dir, "librounding_check.so")
+    tmp = out + ".tmp"
+    run([hipcc()] + HIP_FLAGS + ["-shared", "-o", tmp, src])
+    os.replace(tmp, out)
+    return out
+
+
 if __name__ == "__main__":
     print(build_library(verbose=True))

@@ -126,8 +126,8 @@ __device__ __forceinline__ f3 divs(f3 a, float s)
  * v_div_scale, v_rcp_f32, a refinement by FMAs, v_div_fmas and v_div_fixup; for
  * 2^-64 <= |x| <= 2^64 the scalings are the identity (numerator 1, no denormal reciprocal or
  * quotient), v_div_fmas is a plain fma and v_div_fixup returns its input, so the FMAs alone give
- * the same bits. The probe that compared every float of both ranges with sqrtf and 1.0f / x on
- * the MI355X found no difference. */
+ * the same bits; tests/test_gpu_rounding.py compares every float of both ranges with sqrtf and
+ * 1.0f / x on the GPU. */
 __device__ __forceinline__ float sqrt_rn_short(float x) /* 2^-96 <= x <= +inf */
 {
     const float s = __builtin_amdgcn_sqrtf(x);

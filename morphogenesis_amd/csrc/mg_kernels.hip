@@ -547,17 +547,31 @@ __device__ __forceinline__ void sort_gather_body(const DS& d, float vmax, uint32
         if (nump > d.ctr[MG_C_HWM]) d.ctr[MG_C_HWM] = nump;
         d.ctr[MG_C_LISTS] = 1u;
     }
-    if (dd >= (uint32_t)d.N) return;
-    float4* cdst = (float4*)(d.CONC2 + (size_t)dd * MG_NUM_TF);
+    const uint32_t N = (uint32_t)d.N;
+    const uint32_t s = dd < nump ? d.PERM[dd] : 0u;
+    {
+        /* the 64-byte concentration rows of the wavefront's 64 slots, copied by the whole
+         * wavefront in four passes of 64 consecutive float4 (16 rows each), so every store
+         * writes 1 KB contiguous; row q takes its source slot from lane q's PERM entry, the
+         * reset tail takes zeros */
+        const uint32_t lane = __lane_id(), wbase = dd - lane;
+        const float4* csrc = (const float4*)d.CONC;
+        float4* cdst = (float4*)(d.CONC2 + (size_t)wbase * MG_NUM_TF);
+#pragma unroll
+        for (uint32_t k = 0; k < 4u; k++) {
+            const uint32_t f = k * 64u + lane, q = f >> 2;
+            const uint32_t sq = (uint32_t)__shfl((int)s, (int)q);
+            if (wbase + q < N)
+                cdst[f] = wbase + q < nump ? csrc[(size_t)sq * 4u + (f & 3u)] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+    }
+    if (dd >= N) return;
     if (dd < nump) {
-        const uint32_t s = d.PERM[dd];
         const uint32_t h = d.HOME[s];
         d.SLOT[h] = dd;
         d.P2[dd] = d.P[s];
         d.V2[dd] = d.V[s];
         d.VEL2[dd] = d.VEL[s];
-        const float4* csrc = (const float4*)(d.CONC + (size_t)s * MG_NUM_TF);
-        cdst[0] = csrc[0]; cdst[1] = csrc[1]; cdst[2] = csrc[2]; cdst[3] = csrc[3];
         d.HOME2[dd] = h;
         d.GCELL2[dd] = d.GCELL[s];
         d.GNDX2[dd] = d.GNDX[s];
@@ -574,8 +588,6 @@ __device__ __forceinline__ void sort_gather_body(const DS& d, float vmax, uint32
         d.P2[dd] = make_float4(vmax, vmax, vmax, p.w);
         d.V2[dd] = d.V[dd];
         d.VEL2[dd] = d.VEL[dd];
-        const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
-        cdst[0] = z; cdst[1] = z; cdst[2] = z; cdst[3] = z;
         d.HOME2[dd] = d.HOME[dd];
         d.GCELL2[dd] = d.GCELL[dd];
         d.GNDX2[dd] = d.GNDX[dd];
@@ -739,29 +751,6 @@ __device__ __forceinline__ void cell_candidates(uint32_t lo, uint32_t hi, f3 pi,
     }
 }
 
-/* Tests the candidates of slots [lo, hi), K at a time, and sets bit j - wlo of rec for each
- * with 0 < dsq < lim (no visits); every slot of the range lies in [wlo, wlo + 64). */
-template <int K, class GetP>
-__device__ __forceinline__ void range_record(uint32_t lo, uint32_t hi, uint32_t wlo, f3 pi, float lim,
-                                             GetP getp, uint64_t& rec)
-{
-    for (uint32_t j0 = lo; j0 < hi; j0 += K) {
-        float4 pj[K];
-#pragma unroll
-        for (int q = 0; q < K; q++) pj[q] = getp(j0 + (uint32_t)q);
-        uint32_t hb = 0u;
-#pragma unroll
-        for (int q = 0; q < K; q++) {
-            const f3 dist = pi - xyz(pj[q]);
-            const float dsq = dist.x * dist.x + dist.y * dist.y + dist.z * dist.z;
-            hb |= (dsq < lim && dsq > 0.0f) ? 1u << q : 0u;
-        }
-        /* slots past hi were read (bands and P are padded by K) but are not candidates */
-        if (hi - j0 < (uint32_t)K) hb &= (1u << (hi - j0)) - 1u;
-        rec |= (uint64_t)hb << (j0 - wlo);
-    }
-}
-
 /* Must be called by all 64 lanes of the wavefront (`valid` selects the lanes that sweep);
  * xload(j, q) is the global load of extra field q of slot j. K candidates are tested at a
  * time; bands of up to CAP (<= 128) slots are staged. */
@@ -871,7 +860,11 @@ __device__ __forceinline__ void sweep(const DS& d, const MgGrid& g, Band<NX, CAP
          * partial sum is flushed where the first hit past its end is visited and after the row
          * (the skipped flushes add +0 to sums that start at +0) */
         /* one loop for the staged band and one for global memory, chosen per row, so that the
-         * band's reads are LDS reads (a select between the two would make them flat loads) */
+         * band's reads are LDS reads (a select between the two would make them flat loads).
+         * Rows where some lane holds more than 12 recorded candidates (compressed states)
+         * issue the next candidate's reads before visiting the current one, so their latency
+         * overlaps the visit; shorter rows take the plain loop, where that bookkeeping costs
+         * more than it hides. */
 #define MG_WALK_MASK(M, GETP, GETX)                                                            \
     do {                                                                                       \
         uint64_t m_ = (M);                                                                     \
@@ -893,17 +886,62 @@ __device__ __forceinline__ void sweep(const DS& d, const MgGrid& g, Band<NX, CAP
         }                                                                                      \
         end_cell();                                                                            \
     } while (0)
+#define MG_WALK_MASK_AHEAD(M, GETP, GETX)                                                      \
+    do {                                                                                       \
+        uint64_t m_ = (M);                                                                     \
+        uint32_t cnext_ = next_bound(rb.b[0]);                                                 \
+        bool has_ = m_ != 0ull;                                                                \
+        uint32_t jn_ = 0u;                                                                     \
+        float4 pn_ = make_float4(0.f, 0.f, 0.f, 0.f), xn_[NXX];                                \
+        _Pragma("unroll") for (int x = 0; x < NXX; x++) xn_[x] = pn_;                          \
+        if (has_) {                                                                            \
+            jn_ = rb.b[0] + (uint32_t)__builtin_ctzll(m_);                                     \
+            m_ &= m_ - 1ull;                                                                   \
+            pn_ = GETP(jn_);                                                                   \
+            _Pragma("unroll") for (int x = 0; x < NX; x++) xn_[x] = GETX(jn_, x);             \
+        }                                                                                      \
+        while (has_) {                                                                         \
+            const uint32_t j = jn_;                                                            \
+            const float4 pj = pn_;                                                             \
+            float4 ex[NXX];                                                                    \
+            _Pragma("unroll") for (int x = 0; x < NXX; x++) ex[x] = xn_[x];                    \
+            has_ = m_ != 0ull;                                                                 \
+            if (has_) {                                                                        \
+                jn_ = rb.b[0] + (uint32_t)__builtin_ctzll(m_);                                 \
+                m_ &= m_ - 1ull;                                                               \
+                pn_ = GETP(jn_);                                                               \
+                _Pragma("unroll") for (int x = 0; x < NX; x++) xn_[x] = GETX(jn_, x);         \
+            }                                                                                  \
+            if (j >= cnext_) {                                                                 \
+                end_cell();                                                                    \
+                cnext_ = next_bound(j);                                                        \
+            }                                                                                  \
+            const f3 dist = pi - xyz(pj);                                                      \
+            const float dsq = dist.x * dist.x + dist.y * dist.y + dist.z * dist.z;             \
+            if (dsq < lim && dsq > 0.0f) visit(j, pj, dist, dsq, ex); /* recorded in the skin */ \
+        }                                                                                      \
+        end_cell();                                                                            \
+    } while (0)
 #define MG_BAND_P(j) wb.p[(j) - cblo]
 #define MG_BAND_X(j, x) wb.x[x][(j) - cblo]
 #define MG_GLOBAL_P(j) d.P[j]
 #define MG_GLOBAL_X(j, x) xload(j, x)
         auto walk = [&](uint64_t m) {
-            if (cfits)
-                MG_WALK_MASK(m, MG_BAND_P, MG_BAND_X);
-            else
-                MG_WALK_MASK(m, MG_GLOBAL_P, MG_GLOBAL_X);
+            const bool ahead = __ballot(__builtin_popcountll(m) > 12) != 0ull;
+            if (cfits) {
+                if (ahead)
+                    MG_WALK_MASK_AHEAD(m, MG_BAND_P, MG_BAND_X);
+                else
+                    MG_WALK_MASK(m, MG_BAND_P, MG_BAND_X);
+            } else {
+                if (ahead)
+                    MG_WALK_MASK_AHEAD(m, MG_GLOBAL_P, MG_GLOBAL_X);
+                else
+                    MG_WALK_MASK(m, MG_GLOBAL_P, MG_GLOBAL_X);
+            }
         };
 #undef MG_WALK_MASK
+#undef MG_WALK_MASK_AHEAD
 #undef MG_BAND_P
 #undef MG_BAND_X
 #undef MG_GLOBAL_P
@@ -914,8 +952,7 @@ __device__ __forceinline__ void sweep(const DS& d, const MgGrid& g, Band<NX, CAP
         }
         /* window of the row for the hit record: [b[0], b[ncell]) */
         const uint32_t wlo = rb.b[0];
-        const bool narrow = (ncell >= 4 ? rb.b[4] : ncell == 3 ? rb.b[3] : ncell == 2 ? rb.b[2] : rb.b[1]) - wlo <= 64u;
-        const bool wide = !hm_w || !narrow;
+        const bool wide = !hm_w || (ncell >= 4 ? rb.b[4] : ncell == 3 ? rb.b[3] : ncell == 2 ? rb.b[2] : rb.b[1]) - wlo > 64u;
         uint64_t rec = 0ull;
         auto visit_rec = [&](uint32_t j, float4 pj, f3 dist, float dsq, const float4* x) {
             if (!wide) rec |= 1ull << ((j - wlo) & 63u);
@@ -935,32 +972,6 @@ __device__ __forceinline__ void sweep(const DS& d, const MgGrid& g, Band<NX, CAP
             }
             return live;
         };
-        if (HM == 1 && rows && narrow) {
-            /* a window of at most 64 slots: test the live cells' candidates (one slot range:
-             * the live cells are contiguous, a cell's gap grows away from the own column)
-             * into the row's mask, then visit the recorded ones in order, so the hit path runs
-             * once per recorded candidate of the wavefront's busiest lane rather than at
-             * every candidate slot */
-            bool any = false;
-            uint32_t flo = 0u, fhi = 0u;
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                if (!live_of(k)) continue;
-                if (!any) flo = rb.b[k];
-                any = true;
-                fhi = max(rb.b[k + 1], rb.b[k]);
-            }
-            uint64_t m = 0ull;
-            if (any) {
-                if (cfits)
-                    range_record<K>(flo, max(fhi, flo), wlo, pi, tlim, [&](uint32_t j) { return wb.p[j - cblo]; }, m);
-                else
-                    range_record<K>(flo, max(fhi, flo), wlo, pi, tlim, [&](uint32_t j) { return d.P[j]; }, m);
-            }
-            if (hm_w) d.HMW[(size_t)s * N + self] = m;
-            walk(m);
-            continue;
-        }
         /* sweeps that read recorded masks take the candidate path only for unrecorded rows,
          * and always per cell */
         bool over = false;

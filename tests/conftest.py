@@ -22,6 +22,8 @@ def _ensure_built():
         ob.build_oracle()
     if not os.path.exists(os.path.join(REPO, "morphogenesis_amd", "lib", "libmg_sph.so")):
         mb.build_library()
+    if not os.path.exists(os.path.join(REPO, "tests", "hip", "lib", "librounding_check.so")):
+        mb.build_rounding_check()
 
 
 def pytest_sessionstart(session):
