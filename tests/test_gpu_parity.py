@@ -509,6 +509,59 @@ def row_windows(sim, grid):
     return np.stack(out, 1)
 
 
+def recorded_per_row(sim, grid, r2, sample=3000, seed=0):
+    """For a sample of active particles, the number of candidates of each of its 9 neighbour
+    rows within the hit-mask skin (0 < dsq < 1.0201 r2): what the pressure sweep records."""
+    rng = np.random.default_rng(seed)
+    nump, na = sim.counts()
+    pos = sim.download(F.FPOS)[:na, :3].astype(np.float32)
+    cell = sim.download(F.FGCELL)[:na].astype(np.int64)
+    cnt = sim.download(F.FBIN_COUNT).astype(np.int64)
+    off = sim.download(F.FBIN_OFFSET).astype(np.int64)
+    rx, rz = int(grid.res[0]), int(grid.res[2])
+    out = []
+    for i in rng.choice(na, size=min(sample, na), replace=False):
+        row = []
+        for dy in (-1, 0, 1):
+            for dz in (-1, 0, 1):
+                c0 = cell[i] - 1 + (dy * rz + dz) * rx
+                if c0 < 0 or c0 + 3 > cnt.size:
+                    row.append(0)
+                    continue
+                lo, hi = off[c0], off[c0 + 2] + cnt[c0 + 2]
+                d = pos[lo:hi] - pos[i]
+                dsq = (d * d).sum(1)
+                row.append(int(((dsq > 0) & (dsq < np.float32(1.0201) * np.float32(r2))).sum()))
+        out.append(row)
+    return np.array(out)
+
+
+@pytest.mark.parametrize("speed", [0.0, 60.0])
+def test_rows_of_many_recorded_candidates(built, speed):
+    """About 16 particles per cell (lattice spacing 0.4): neighbour rows hold up to 64 slots, so
+    one hit mask covers them, and central rows record more than 12 candidates, where the
+    force and diffusion sweeps (and the pressure sweep's readers) fetch each recorded
+    candidate one visit ahead. Bit-equal to the oracle through hipGraph replay."""
+    sc = scenes.tissue(n_side=16, b=6, seed=7, spacing=0.4, vmax=12.0)
+    if speed:
+        sc.buffers[F.FVEL][300, :3] = [speed, 0.0, 0.0]
+        sc.buffers[F.FVEVAL][300, :3] = [speed, 0.0, 0.0]
+    say("case: rows of many recorded candidates, speed", speed, sc.n)
+    g, o = pair(sc)
+    for sim in (g, o):
+        sim.Run2PhysicalSort()
+    w = row_windows(g, sc.grid)
+    rec = recorded_per_row(o, sc.grid, float(sc.params.r2))
+    say("  row windows max", int(w.max()), "| recorded per row max", int(rec.max()),
+        "share of sampled particles with a row over 12:", round(float((rec.max(1) > 12).mean()), 3))
+    assert (w <= 64).mean() > 0.9 and (rec.max(1) > 12).mean() > 0.3
+    g.step(8, sc.dt, sc.flags | F.STEP_GRAPH)
+    o.step(8, sc.dt, sc.flags)
+    compare_all(g, o, f"many recorded candidates, speed {speed}",
+                [F.FPOS, F.FVEL, F.FVEVAL, F.FCONC, F.FEPIGEN, F.FPRESS, F.FDENSITY, F.FGCELL,
+                 F.FBIN_OFFSET])
+
+
 @pytest.mark.parametrize("speed", [0.0, 60.0])
 def test_rows_wider_than_one_hit_mask(built, speed):
     """27 particles per cell (lattice spacing 1/3): most neighbour rows hold 65-128 slots, more
