@@ -3,6 +3,7 @@ MI355X, reported as particle-steps/s with the roofline of the dominant kernel an
 oracle timed beside it on the host cores.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload tissue_262k] [--no-cpu]
+                  [--learned]
 
 One step = Run2InnerPhysicalLoop (pressure, force, advance) + Run2GeneAction (diffusion,
 genes) + Run2PhysicalSort (insert, scan, counting sort), with all inputs resident in HBM.
@@ -564,8 +565,12 @@ def main():
                     help="tissue_batch_256: the launch form (auto: the autotuner's choice)")
     ap.add_argument("--no-config-run", action="store_true",
                     help="leave out the timing of the configuration's whole run")
+    ap.add_argument("--learned", action="store_true",
+                    help="add the learned 4 x dt measurement to the tissue_262k line (training "
+                         "and three 1,000-coarse-step rollouts, about 1.5 min; the same as "
+                         "--workload tissue_262k_learned)")
     ap.add_argument("--no-learned", action="store_true",
-                    help="leave out the learned 4 x dt measurement of the default line")
+                    help="leave out the learned measurement (the default)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="also run a plain device copy of this size (traffic counter calibration)")
     args = ap.parse_args()
@@ -677,7 +682,7 @@ def main():
         out["config_run"] = config_run(scene, CONFIG_STEPS[args.workload], flags)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(scene)
-    if args.workload == "tissue_262k" and not args.no_learned:
+    if args.workload == "tissue_262k" and args.learned and not args.no_learned:
         sim.close()
         out["learned"] = learned_measurement(1000, lambda msg: print(f"[bench learned] {msg}",
                                                                       file=sys.stderr, flush=True))
