@@ -120,8 +120,10 @@ void fill_inst(BatchInst& t, mg_handle* h, const MgGenome* dgen, float dt, unsig
     t.pressure.ZERO = h->CNT[1 - h->gcur];
     t.pressure.zero_n = (uint32_t)h->C;
     t.pressure.HMW = hit_masks(h);
+    t.pressure.ETW = bond_records(h, t.pressure.HMW);
     t.force = make_ds(h);
     t.force.HMR = t.pressure.HMW;
+    t.force.ETR = t.pressure.ETW;
     t.diffuse = make_ds(h, 1, 0);
     t.diffuse.GMN = h->GMN;
     t.diffuse.cull = 0;

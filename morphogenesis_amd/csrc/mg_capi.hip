@@ -271,6 +271,7 @@ struct mg_handle {
     uint64_t* LIVE = nullptr; /* per home: bit a set when bond slot a holds a partner */
     uint64_t* HM = nullptr;   /* hit masks, pressure -> force of a fused step: [row][slot] */
     int hm_rows = 0;          /* rows HM holds per slot */
+    float4* ET = nullptr;     /* bond records, pressure -> force beside the masks: [bond slot][home] */
     /* grid */
     uint32_t *CNT[2] = {}, *OFF[2] = {};  /* gcur is current; the fused step bins into 1-gcur */
     uint32_t *GCNT = nullptr, *GOFF = nullptr, *LENS = nullptr;
@@ -419,6 +420,8 @@ DS make_ds(mg_handle* h, int kin = 0, int conc = 0)
     d.DVc = nullptr;
     d.HMW = nullptr;
     d.HMR = nullptr;
+    d.ETW = nullptr;
+    d.ETR = nullptr;
     d.dry = 0;
     d.ctr = h->ctr;
     d.N = h->N; d.B = h->B; d.C = h->C;
@@ -917,12 +920,20 @@ void ensure_hit_masks(mg_handle* h)
         return;
     }
     h->hm_rows = nseg;
+    if (h->ET) dfree(h, h->ET);
+    h->ET = nullptr;
+    if (h->B > 0 && dalloc(h, &h->ET, (size_t)h->B * h->N) != MG_OK)
+        h->ET = nullptr; /* no records: the force evaluates its bonds itself */
 }
 
 uint64_t* hit_masks(const mg_handle* h)
 {
     return (h->grid_scanned && hm_eligible(h) && h->hm_rows >= h->g.srch * h->g.srch) ? h->HM : nullptr;
 }
+
+/* the bond records go with the hit masks: written by a pressure pass that records masks and
+ * read by the force pass that visits them (the same positions) */
+float4* bond_records(const mg_handle* h, const uint64_t* hm) { return hm ? h->ET : nullptr; }
 
 int do_fused_step(mg_handle* h, float dt, unsigned flags, const mg_net* net = nullptr)
 {
@@ -935,6 +946,7 @@ int do_fused_step(mg_handle* h, float dt, unsigned flags, const mg_net* net = nu
         d.ZERO = h->CNT[1 - h->gcur];
         d.zero_n = (uint32_t)h->C;
         d.HMW = hm;
+        d.ETW = bond_records(h, hm);
         launch_pressure(h, nb, d);
     }
     const int special = (flags & MG_STEP_SPECIAL) ? 1 : 0;
@@ -946,6 +958,7 @@ int do_fused_step(mg_handle* h, float dt, unsigned flags, const mg_net* net = nu
             PhaseScope ps(h, MG_PH_FORCE);
             DS d = make_ds(h);
             d.HMR = hm;
+            d.ETR = bond_records(h, hm);
             launch_force(h, nb, d, 0);
         }
         {
@@ -963,6 +976,7 @@ int do_fused_step(mg_handle* h, float dt, unsigned flags, const mg_net* net = nu
         PhaseScope ps(h, MG_PH_FORCE);
         DS d = make_ds(h);
         d.HMR = hm;
+        d.ETR = bond_records(h, hm);
         if (h->sph == MG_SPH_POLY6_SPIKY)
             hipLaunchKernelGGL(k_force_advance<MG_SPH_POLY6_SPIKY>, dim3(nb), dim3(256), band_lds<1>(256), h->st, d,
                                h->p, h->g, dt, special, zoff);
@@ -1085,6 +1099,7 @@ int do_pressure(mg_handle* h, uint64_t* hm = nullptr)
     PhaseScope ps(h, MG_PH_PRESSURE);
     DS d = make_ds(h);
     d.HMW = hm;
+    d.ETW = bond_records(h, hm);
     launch_pressure(h, nblk(h->N), d);
     HIPCHK(hipGetLastError());
     return MG_OK;
@@ -1096,6 +1111,7 @@ int do_force(mg_handle* h, const uint64_t* hm = nullptr)
     PhaseScope ps(h, MG_PH_FORCE);
     DS d = make_ds(h);
     d.HMR = hm;
+    d.ETR = bond_records(h, hm);
     launch_force(h, nblk(h->N), d, 1);
     HIPCHK(hipGetLastError());
     return MG_OK;

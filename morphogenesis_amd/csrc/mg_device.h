@@ -90,6 +90,14 @@ struct DS {
      * cut-off for every recorded candidate. Null when not in use. */
     uint64_t* HMW;
     const uint64_t* HMR;
+    /* Bond records handed from the pressure pass to the force pass of the same positions (the
+     * hit masks' rule): for live bond slot a of home h, at [a * N + h], the owner's bond term
+     * (eterm.xyz) and its strain in w, with the sign bit of w set when the bond breaks this
+     * step (abs_dist >= elastic limit); w is NaN for a slot the force skips (partner inactive
+     * or rest length below 1e-9). The force adds its own slots' records and those of its
+     * incoming bonds instead of evaluating each bond twice. Null when not in use. */
+    float4* ETW;
+    const float4* ETR;
     /* 1: a force pass that changes no bond record (no integrators, no breaking), for the
      * learned step's training inputs */
     int dry;

@@ -1039,6 +1039,54 @@ __device__ __forceinline__ void sweep(const DS& d, const MgGrid& g, Band<NX, CAP
     }
 }
 
+/* The bond terms of active slot i's live bond slots, each as the owner's force pass would
+ * evaluate it (same operands: the step's positions and veval), recorded for that pass
+ * (d.ETW; the record layout is described at DS::ETW). Slots in groups of 4, ascending, the
+ * partners' slots of a group fetched together. */
+__device__ __forceinline__ void record_bond_terms(const DS& d, const MgParams& p, uint32_t i)
+{
+    const uint32_t N = (uint32_t)d.N, na = d.ctr[MG_C_ACTIVE];
+    const uint32_t h = d.HOME[i];
+    const float4 P_i = d.P[i], V_i = d.V[i];
+    const f3 ipos = xyz(P_i), ivev = xyz(V_i);
+    uint64_t live = d.LIVE[h];
+    while (live) {
+        const int q = (__ffsll((long long)live) - 1) & ~3;
+        const uint32_t g4 = (uint32_t)(live >> q) & 0xFu;
+        live &= ~(0xFull << q);
+        uint32_t jc[4];
+        float4 bp[4], pj[4], vj[4];
+        bool use[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const size_t bi = (size_t)(q + k) * N + h;
+            jc[k] = (g4 & (1u << k)) ? d.SLOT[d.BPARTNER[bi]] : MG_NONE;
+            bp[k] = (g4 & (1u << k)) ? d.BPAR[bi] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            use[k] = jc[k] < na && !((double)bp[k].y < 0.000000001);
+            pj[k] = vj[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (use[k]) {
+                pj[k] = d.P[jc[k]];
+                vj[k] = d.V[jc[k]];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if (!(g4 & (1u << k))) continue;
+            float4 r = make_float4(0.f, 0.f, 0.f, __builtin_nanf(""));
+            if (use[k]) {
+                float strain, abs_dist;
+                const f3 eterm = bond_term(ipos, ivev, xyz(pj[k]), xyz(vj[k]), bp[k], p.pmass, &strain, &abs_dist);
+                const uint32_t brk = abs_dist >= bp[k].x ? 0x80000000u : 0u;
+                r = make_float4(eterm.x, eterm.y, eterm.z, __uint_as_float(__float_as_uint(strain) | brk));
+            }
+            d.ETW[(size_t)(q + k) * N + h] = r;
+        }
+    }
+}
+
 /* ------------------------------------------------------------ pressure (a9)
  * contributePressure/computePressure, src/fluid_system_cuda.cu:620-712. Per cell partial
  * sums are added to the running total in gridAdj order. Writes FPRESS into P.w and
@@ -1110,6 +1158,7 @@ __device__ __forceinline__ void pressure_body(const DS& d, const MgParams& p, co
     if (sum == 0.0f) sum = 1.0f;
     ((float*)&d.P[i])[3] = (sum - p.prest_dens) * p.pintstiff;
     ((float*)&d.V[i])[3] = 1.0f / sum;
+    if (d.ETW) record_bond_terms(d, p, i);
 }
 
 /* the sweep kernels take (blockDim.x / 64) Band<NX> of dynamic LDS, one per wavefront;
@@ -1167,6 +1216,7 @@ __device__ __forceinline__ f3 force_one(const DS& d, const MgParams& p, const Mg
             g4 = (uint32_t)(live >> q) & 0xFu;
             live &= ~(0xFull << q);
         }
+        if (d.ETR) return; /* the records replace the partners' positions */
 #pragma unroll
         for (int k = 0; k < 4; k++) j[k] = (g4 & (1u << k)) ? d.SLOT[d.BPARTNER[(size_t)(q + k) * N + h]] : MG_NONE;
     };
@@ -1202,6 +1252,43 @@ __device__ __forceinline__ f3 force_one(const DS& d, const MgParams& p, const Mg
 
     float4 bp[4];
     float2 in[4];
+    if (d.ETR) {
+        /* the pressure pass's records of the own live slots: integrators, force, breaking in
+         * the order below (the same operations on the same values as the evaluation) */
+        while (g4) {
+            const int qc = q;
+            const uint32_t gc4 = g4;
+            float4 r[4];
+            float2 inc[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                r[k] = make_float4(0.f, 0.f, 0.f, __builtin_nanf(""));
+                inc[k] = make_float2(0.f, 0.f);
+                if (gc4 & (1u << k)) {
+                    const size_t bi = (size_t)(qc + k) * N + h;
+                    r[k] = d.ETR[bi];
+                    inc[k] = d.BINT[bi];
+                }
+            }
+            next_group();
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                if (r[k].w != r[k].w) continue; /* not a live slot, or one the force skips */
+                const size_t bi = (size_t)(qc + k) * N + h;
+                const uint32_t wbits = __float_as_uint(r[k].w);
+                const float strain = __uint_as_float(wbits & 0x7FFFFFFFu);
+                if (!d.dry) {
+                    float2 v = inc[k];
+                    v.x = v.x + strain * strain;
+                    v.y = v.y + strain;
+                    d.BINT[bi] = v;
+                }
+                const f3 eterm = mk3(r[k].x, r[k].y, r[k].z);
+                if (!has_nan3(eterm)) force = force - eterm;
+                if ((wbits >> 31) && !freeze && !d.dry) ((float*)&d.BPAR[bi])[1] = 0.0f;
+            }
+        }
+    }
     auto records = [&]() {
 #pragma unroll
         for (int k = 0; k < 4; k++) {
@@ -1270,6 +1357,29 @@ __device__ __forceinline__ f3 force_one(const DS& d, const MgParams& p, const Mg
 #pragma unroll
         for (int k = 0; k < 4; k++) packed[k] = d.INC[min(e + (uint32_t)k, e1 - 1u)];
     };
+    if (d.ETR) {
+        /* the owners' records of the incoming bonds (an inactive owner's record is stale: its
+         * slot decides, as in the evaluation) */
+        uint32_t packed[4] = {first[0], first[1], first[2], first[3]};
+        for (uint32_t e = e0; valid && e < e1; e += 4) {
+            uint32_t ks[4];
+            float4 r[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const uint32_t kh = packed[k] / (uint32_t)B, b = packed[k] - kh * (uint32_t)B;
+                ks[k] = d.SLOT[kh];
+                r[k] = d.ETR[(size_t)b * N + kh];
+            }
+            if (e + 4u < e1) entries(e + 4u, packed);
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                if (!(e + (uint32_t)k < e1 && ks[k] < na && r[k].w == r[k].w)) continue;
+                const f3 eterm = mk3(r[k].x, r[k].y, r[k].z);
+                if (!has_nan3(eterm)) F = F + eterm;
+            }
+        }
+        return F;
+    }
     if (valid && e0 < e1) fetch_in(first);
     for (uint32_t e = e0; valid && e < e1; e += 4) {
         bool use[4];

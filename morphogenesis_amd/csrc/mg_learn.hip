@@ -426,6 +426,7 @@ int mg_net_features(mg_handle* h, float* X_out)
     {
         DS d = make_ds(h);
         d.HMR = hm;
+        d.ETR = bond_records(h, hm);
         d.dry = 1;
         launch_force(h, nblk(h->N), d, 0);
         HIPCHK(hipGetLastError());
@@ -459,6 +460,7 @@ int mg_net_infer(mg_handle* h, mg_net* n, float dt, float* DV_out)
     {
         DS d = make_ds(h);
         d.HMR = hm;
+        d.ETR = bond_records(h, hm);
         d.dry = 1;
         launch_force(h, nblk(h->N), d, 0);
         HIPCHK(hipGetLastError());
