@@ -32,6 +32,12 @@ KERNEL_OF_PHASE = {"pressure": "k_pressure", "force": "k_force_advance",
                    "lists": "k_dense_lists", "fill": "k_fill_u32"}
 
 
+def bond_records_in_use(scene) -> bool:
+    """The fused step hands bond records from the pressure to the force pass together with
+    the hit masks (rows of up to 4 cells, cut-off 1) when the particles have bond slots."""
+    return float(scene.params.r2) == 1.0 and int(scene.grid.srch) <= 4 and scene.b > 0
+
+
 def algorithmic_bytes(phase: str, scene, stats: dict) -> float:
     """Compulsory HBM bytes per launch of the fused step's kernels in this build's layout:
     every array element read or written once, neighbour data counted once (as that
@@ -41,16 +47,23 @@ def algorithmic_bytes(phase: str, scene, stats: dict) -> float:
     bo, bi = stats["bonds_out"], stats["bonds_in"]
     genes = bool(scene.flags & 1)
     srch = int(scene.grid.srch)
-    # hit masks handed from the pressure to the force sweep: 8 bytes per row of cells
+    # hit masks handed from the pressure to the force sweep: 8 bytes per row of cells; the
+    # bond records go with them (a 16-byte record per owned bond)
     hm = 8 * srch * srch if (float(scene.params.r2) == 1.0 and srch <= 4) else 0
+    rec = bond_records_in_use(scene)
     if phase == "pressure":
         # GCELL 4, P 16 read; press, 1/density 8 written; hit masks written; OFF read once,
-        # next grid's CNT cleared
-        return n * (4 + 16 + 8 + hm) + 8 * c
+        # next grid's CNT cleared; with bond records HOME 4, LIVE 8, V 16 and per owned bond
+        # BPARTNER 4, SLOT 4, BPAR 16 read, the record 16 written
+        return n * (4 + 16 + 8 + hm + ((28 + 40 * bo) if rec else 0)) + 8 * c
     if phase == "force":
         # force: GCELL 4, HOME 4, P 16, V 16 (+VEL 16 for the advance); hit masks read;
-        # BPARTNER 4 per slot; per bond BPAR 16 + SLOT 4 + BINT 8 read + 8 written; INC_OFF 8,
+        # without records BPARTNER 4 per slot and per bond BPAR 16 + SLOT 4, with them LIVE 8
+        # and per owned bond its record 16 (an incoming bond's record is its owner's, counted
+        # there) plus SLOT 4 per incoming bond; BINT 8 read + 8 written per bond; INC_OFF 8,
         # INC 4 per incoming bond; advance writes P, V, VEL: 48
+        if rec:
+            return n * (56 + hm + 8 + bo * 32 + 8 + 8 * bi + 48)
         return n * (56 + hm + 4 * b + bo * 36 + 8 + 4 * bi + 48)
     if phase == "diffusion":
         # P 16, CONC 64 read, CONC 64 written, GCELL 4 read/written, GNDX 4, GMASK 4, HOME 4,
@@ -80,14 +93,20 @@ def reference_layout_bytes(phase: str, scene, stats: dict):
       advance    64 position, veval, velocity, force + 4 cell read, 48 written      = 116
       diffusion  64 FCONC read + 64 written                                          = 128
       genes      FEPIGEN and FCONC read and written with the ftemp accumulators      = 256
-    Used bonds are the particle's own live bond slots (bonds_out per particle)."""
+    Used bonds are the particle's own live bond slots (bonds_out per particle). When the
+    pressure pass evaluates the bond terms for the force (the bond records), the force's reads
+    of the bond records (36 per slot) and of veval (16) count as the pressure kernel's."""
     b, bo = scene.b, stats["bonds_out"]
     genes = bool(scene.flags & 1)
+    rec = bond_records_in_use(scene)
     terms = {}
     if phase == "pressure":
         terms = {"pressure": 28.0}
+        if rec:
+            terms["bond terms"] = 16 + 36.0 * b
     elif phase == "force":
-        terms = {"force": 16 + 16 + 4 + 4 + 4 + 36.0 * b + 8.0 * bo + 32, "advance": 116.0}
+        bonds = 0.0 if rec else 16 + 36.0 * b
+        terms = {"force": 16 + 4 + 4 + 4 + bonds + 8.0 * bo + 32, "advance": 116.0}
     elif phase == "diffusion":
         terms = {"diffusion": 128.0, "insert": 88.0}
         if genes:
