@@ -858,9 +858,9 @@ __device__ __forceinline__ void sweep(const DS& d, const MgGrid& g, Band<NX, CAP
         };
         /* the recorded candidates m of this row (bit k = slot b[0] + k), ascending; a cell's
          * partial sum is flushed where the first hit past its end is visited and after the row
-         * (the skipped flushes add +0 to sums that start at +0) */
-        /* one loop for the staged band and one for global memory, chosen per row, so that the
-         * band's reads are LDS reads (a select between the two would make them flat loads).
+         * (the skipped flushes add +0 to sums that start at +0). One loop for the staged band
+         * and one for global memory, chosen per row, so that the band's reads are LDS reads (a
+         * select between the two would make them flat loads).
          * Rows where some lane holds more than 12 recorded candidates (compressed states)
          * issue the next candidate's reads before visiting the current one, so their latency
          * overlaps the visit; shorter rows take the plain loop, where that bookkeeping costs

@@ -573,7 +573,7 @@ def test_rows_wider_than_one_hit_mask(built, speed):
     if speed:
         sc.buffers[F.FVEL][200, :3] = [speed, 0.0, 0.0]
         sc.buffers[F.FVEVAL][200, :3] = [speed, 0.0, 0.0]
-    say("case: two-word hit masks, speed", speed, sc.n)
+    say("case: rows wider than one hit mask, speed", speed, sc.n)
     g, o = pair(sc)
     for sim in (g, o):
         sim.Run2PhysicalSort()
@@ -583,5 +583,5 @@ def test_rows_wider_than_one_hit_mask(built, speed):
     assert frac > 0.3
     g.step(8, sc.dt, sc.flags | F.STEP_GRAPH)
     o.step(8, sc.dt, sc.flags)
-    compare_all(g, o, f"two-word masks, speed {speed}", [F.FPOS, F.FVEL, F.FVEVAL, F.FCONC, F.FEPIGEN,
+    compare_all(g, o, f"wide rows, speed {speed}", [F.FPOS, F.FVEL, F.FVEVAL, F.FCONC, F.FEPIGEN,
                                                          F.FPRESS, F.FDENSITY, F.FGCELL, F.FBIN_OFFSET])
