@@ -594,8 +594,9 @@ def main():
                     help="also run a plain device copy of this size (traffic counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path "
-              "(the step would shard by spatial slabs; see DESIGN.md). Use --gpus 1.",
+        print("bench.py runs on one GPU only: the library has no multi-GPU path (the step "
+              "would shard the domain into spatial slabs, one per GPU, with a halo of one cell "
+              "layer exchanged each step). Use --gpus 1.",
               file=sys.stderr)
         sys.exit(2)
 
