@@ -8,13 +8,13 @@ from morphogenesis_amd import scenes
 
 
 def mini_scene(pos, b=4, vmax=12.0, bonds=(), genome=None, conc=None, epigen=None, vel=None,
-               params_over=None, spacing=0.5, sr=1.0):
+               params_over=None, spacing=0.5, sr=1.0, vmin=0.0):
     """A scene from explicit positions. bonds: (owner, partner, rest, modulus, damping, limit).
     sr is the smoothing radius; the grid cell is 2*sr/PGRID_DENSITY = sr, as the reference's
     set-up derives it, so the 27-cell search covers the density cut-off r < sr."""
     pos = np.asarray(pos, np.float32).reshape(-1, 3)
     n = pos.shape[0]
-    vmin3, vmax3 = (0.0, 0.0, 0.0), (vmax, vmax, vmax)
+    vmin3, vmax3 = (vmin, vmin, vmin), (vmax, vmax, vmax)
     over = dict(params_over or {})
     over.setdefault("sr", sr)
     params = scenes.make_params(vmin3, vmax3, spacing=spacing, **over)

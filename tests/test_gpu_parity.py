@@ -509,6 +509,37 @@ def row_windows(sim, grid):
     return np.stack(out, 1)
 
 
+def test_near_coincident_particles(built):
+    """Two pairs of particles 1e-20 apart near the origin of a domain around it: their squared
+    distance (1e-40) lies below 2^-96, so the wavefronts holding them take the compiler's whole
+    correctly rounded sqrt and 1/x rather than the short forms (one pair is bonded, so the bond
+    term does too). Bit-equal to the oracle through the phase calls (the force evaluating its
+    bonds) and fused steps (the bond records)."""
+    rng = np.random.default_rng(11)
+    pos = scenes.lattice(6, 6, 6, 0.5, (-1.25, -1.25, -1.25), 0.05, rng)
+    extra = np.array([[1e-20, 0.1, 0.1], [2e-20, 0.1, 0.1],
+                      [0.3, -3e-20, 0.2], [0.3, -1e-20, 0.2]], np.float32)
+    pos = np.concatenate([pos.astype(np.float32), extra])
+    n = len(pos)
+    sc = mini_scene(pos, b=4, vmin=-6.0, vmax=6.0, bonds=[(n - 4, n - 3, 0.5, 1000.0, 1.0, 2.0)])
+    d = pos[n - 4] - pos[n - 3]
+    assert 0.0 < float((d * d).sum(dtype=np.float32)) < 2.0 ** -96
+    say("case: near-coincident particles", n)
+    g, o = pair(sc)
+    for sim in (g, o):
+        sim.Run2PhysicalSort()
+    assert g.counts()[1] == n
+    for name, fg, fo in (("pressure", g.ComputePressureCL, o.ComputePressureCL),
+                         ("force", g.ComputeForceCL, o.ComputeForceCL)):
+        fg(); fo()
+        compare_all(g, o, f"near-coincident: {name}")
+    for sim in (g, o):
+        sim.Run2PhysicalSort()
+    g.step(6, sc.dt, sc.flags | F.STEP_GRAPH)
+    o.step(6, sc.dt, sc.flags)
+    compare_all(g, o, "near-coincident: 6 fused steps")
+
+
 def recorded_per_row(sim, grid, r2, sample=3000, seed=0):
     """For a sample of active particles, the number of candidates of each of its 9 neighbour
     rows within the hit-mask skin (0 < dsq < 1.0201 r2): what the pressure sweep records."""
