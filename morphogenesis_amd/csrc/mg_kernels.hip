@@ -199,37 +199,55 @@ __device__ __forceinline__ void scan_lookback_body(const uint32_t* __restrict__ 
         if (k < w) wbase += wsum[k];
         total += wsum[k];
     }
-    if (t == 0) {
-        uint32_t excl = 0;
+    /* Look-back by the first wavefront: lane l reads the status word of tile (top - l), so one
+     * round trip covers 64 predecessors. The nearest inclusive word ends the walk; every word
+     * between it and this tile must be published (aggregate or inclusive), else the window is
+     * read again. A window without an inclusive word adds its 64 aggregates and moves down. */
+    if (w == 0) {
         unsigned long long* st = status + tile;
-        if (tile == 0) {
-            __hip_atomic_store(st, (epoch << 34) | (2ull << 32) | total, __ATOMIC_RELEASE,
-                               __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            __hip_atomic_store(st, (epoch << 34) | (1ull << 32) | total, __ATOMIC_RELEASE,
-                               __HIP_MEMORY_SCOPE_AGENT);
-            int pidx = (int)tile - 1;
-            uint32_t spins = 0;
-            for (;;) {
-                unsigned long long wd = __hip_atomic_load(status + pidx, __ATOMIC_ACQUIRE,
-                                                          __HIP_MEMORY_SCOPE_AGENT);
-                if ((wd >> 34) != epoch || ((wd >> 32) & 3ull) == 0ull) {
-                    /* bounded wait: predecessors always publish, but never spin forever */
-                    if (++spins > (1u << 24)) {
-                        atomicOr(&ctr[MG_C_ERR], 1u);
-                        break;
-                    }
-                    __builtin_amdgcn_s_sleep(1);
-                    continue;
-                }
-                excl += (uint32_t)wd;
-                if (((wd >> 32) & 3ull) == 2ull) break;
-                pidx--;
-            }
-            __hip_atomic_store(st, (epoch << 34) | (2ull << 32) | (uint32_t)(excl + total),
+        if (lane == 0)
+            __hip_atomic_store(st, (epoch << 34) | ((tile == 0 ? 2ull : 1ull) << 32) | total,
                                __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        uint32_t excl = 0;
+        int top = (int)tile - 1;
+        uint32_t spins = 0;
+        while (top >= 0) {
+            const int pidx = top - lane;
+            bool ready = true, incl = true;
+            uint32_t val = 0;
+            if (pidx >= 0) {
+                const unsigned long long wd = __hip_atomic_load(status + pidx, __ATOMIC_ACQUIRE,
+                                                                __HIP_MEMORY_SCOPE_AGENT);
+                const unsigned long long flag = (wd >> 32) & 3ull;
+                ready = (wd >> 34) == epoch && flag != 0ull;
+                incl = ready && flag == 2ull;
+                val = (uint32_t)wd;
+            }
+            const unsigned long long im = __ballot(incl);
+            const int stop = im ? __builtin_ctzll(im) : 64;
+            const unsigned long long need = stop >= 63 ? ~0ull : (2ull << stop) - 1ull;
+            if (__ballot(!ready) & need) {
+                /* bounded wait: predecessors always publish, but never spin forever */
+                if (++spins > (1u << 24)) {
+                    if (lane == 0) atomicOr(&ctr[MG_C_ERR], 1u);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+                continue;
+            }
+            uint32_t part = lane <= stop ? val : 0u;
+#pragma unroll
+            for (int o = 32; o >= 1; o >>= 1) part += __shfl_xor(part, o);
+            excl += part;
+            if (stop < 64) break;
+            top -= 64;
         }
-        s_excl = excl;
+        if (lane == 0) {
+            if (tile != 0)
+                __hip_atomic_store(st, (epoch << 34) | (2ull << 32) | (uint32_t)(excl + total),
+                                   __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            s_excl = excl;
+        }
     }
     __syncthreads();
     uint32_t run = s_excl + wbase + x - local;
