@@ -815,12 +815,11 @@ int do_sort(mg_handle* h, int lists)
     {
         PhaseScope ps(h, MG_PH_PLACE);
         hipLaunchKernelGGL(k_place, dim3(nblk(h->N)), dim3(256), 0, h->st, d);
-        hipLaunchKernelGGL(k_cellsort, dim3(nblk(h->C)), dim3(256), 0, h->st, d);
     }
     {
         PhaseScope ps(h, MG_PH_SORT);
         const float vmax = std::max(std::max(h->g.gmax[0], h->g.gmax[1]), h->g.gmax[2]);
-        hipLaunchKernelGGL(k_sort_gather, dim3(nblk(h->N)), dim3(256), 0, h->st, d, vmax);
+        hipLaunchKernelGGL(k_sort_rank_gather, dim3(nblk(h->N)), dim3(256), 0, h->st, d, vmax);
     }
     HIPCHK(hipGetLastError());
     h->kcur = 1 - h->kcur;
@@ -1011,14 +1010,15 @@ int do_fused_step(mg_handle* h, float dt, unsigned flags, const mg_net* net = nu
         d.CNT = d.CNTN;
         d.OFF = d.OFFN;
         hipLaunchKernelGGL(k_place, dim3(nb), dim3(256), 0, h->st, d);
-        hipLaunchKernelGGL(k_cellsort, dim3(nblk(h->C)), dim3(256), 0, h->st, d);
     }
     {
         PhaseScope ps(h, MG_PH_SORT);
         DS d = make_ds(h, 1, 1);
         d.GMN = h->GMN;
+        d.CNT = d.CNTN; /* the grid k_place binned into */
+        d.OFF = d.OFFN;
         const float vmax = std::max(std::max(h->g.gmax[0], h->g.gmax[1]), h->g.gmax[2]);
-        hipLaunchKernelGGL(k_sort_gather, dim3(nb), dim3(256), 0, h->st, d, vmax);
+        hipLaunchKernelGGL(k_sort_rank_gather, dim3(nb), dim3(256), 0, h->st, d, vmax);
     }
     HIPCHK(hipGetLastError());
     h->icur = 1 - h->icur;

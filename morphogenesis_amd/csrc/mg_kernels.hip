@@ -8,11 +8,11 @@
  *
  * Two launch sequences share the per-particle device functions below, so they produce
  * identical bits:
- *   phase kernels (k_insert, k_scan, k_place, k_cellsort, k_sort_gather, k_pressure,
+ *   phase kernels (k_insert, k_scan, k_place, k_sort_rank_gather, k_pressure,
  *   k_force, k_advance, k_diffusion, k_genes), one per reference phase, used by the
  *   FluidSystem-style entry points;
  *   fused kernels (k_force_advance, k_diffuse_genes_insert), the hot loop of mg_step:
- *   pressure -> force+advance -> diffusion+genes+insert -> scan -> place -> cellsort -> gather.
+ *   pressure -> force+advance -> diffusion+genes+insert -> scan -> place -> rank+gather.
  */
 #include "mg_device.h"
 
@@ -45,8 +45,8 @@ __global__ LAUNCH_BOUNDS_256 void k_fill_u32(uint32_t* p, uint32_t v, size_t n)
  * the wavefront must call it). Lanes of one wavefront that fall in the same cell (the input
  * is cell-sorted from the previous step, so they are contiguous) are counted with one
  * atomicAdd per run: the run head adds the run length and broadcasts the base. FGNDX
- * receives a slot in [0, count) as the reference's atomicAdd does; k_cellsort later
- * replaces it with the stable rank. */
+ * receives a slot in [0, count) as the reference's atomicAdd does; the sort later replaces
+ * it with the stable rank (k_sort_rank_gather; k_cellsort in the batched step). */
 __device__ __forceinline__ void insert_one(const DS& d, const MgGrid& g, uint32_t s, bool valid,
                                            float4 p, uint32_t* CNT, uint32_t* GCNT)
 {
@@ -543,20 +543,26 @@ __device__ __forceinline__ void cellsort_body(const DS& d, uint32_t bid)
     }
 }
 
-__global__ LAUNCH_BOUNDS_256 void k_cellsort(DS d) { cellsort_body(d, blockIdx.x); }
-
 /* ------------------------------------------------------- counting sort (a7)
  * countingSortFull (src/fluid_system_cuda.cu:196-296) with the host resets of
- * src/host_CL.cpp:1156-1206, written as a gather over destination slots: slot dd <
- * mNumPoints takes the particle PERM[dd]; slots past mNumPoints get the reset values (FPOS
- * filled with max(VOLMAX), FCONC zero) and keep the stale ones the reference leaves in
- * place. Bond records and incoming lists do not move: they are addressed by home index,
- * and the partner indices the reference rewrites here are resolved through SLOT.
- * Reads P/V/VEL/CONC/HOME/GCELL/GNDX, writes their "2" halves. The gene membership comes
- * from GMN (precomputed) when given, else from EPIGEN. Block 0 commits the counters. */
+ * src/host_CL.cpp:1156-1206: slot dd < mNumPoints takes the particle of rank dd - OFF[c] in
+ * its cell c (ranked by pre-sort index, the stable order); slots past mNumPoints get the
+ * reset values (FPOS filled with max(VOLMAX), FCONC zero) and keep the stale ones the
+ * reference leaves in place. Bond records and incoming lists do not move: they are addressed
+ * by home index, and the partner indices the reference rewrites here are resolved through
+ * SLOT. Reads P/V/VEL/CONC/HOME/GCELL/GNDX, writes their "2" halves. The gene membership
+ * comes from GMN (precomputed) when given, else from EPIGEN. Block 0 commits the counters.
+ *
+ * RANK = false: PERM holds every cell's run sorted (k_cellsort), thread dd gathers PERM[dd].
+ * RANK = true: PERM holds the runs as k_place left them (insertion order), d.CNT / d.OFF are
+ * the grid k_place used; thread p takes the particle u = PERM[p], counts the smaller indices
+ * of its cell's run (its rank r, as k_cellsort ranks it) and moves u to dd = OFF[c] + r. The
+ * destinations of a wavefront are a permutation of its own slots within their cells' runs,
+ * so the stores stay as contiguous as the gather's. */
+template <bool RANK>
 __device__ __forceinline__ void sort_gather_body(const DS& d, float vmax, uint32_t bid)
 {
-    const uint32_t dd = bid * blockDim.x + threadIdx.x;
+    const uint32_t p = bid * blockDim.x + threadIdx.x;
     const uint32_t nump = d.ctr[MG_C_SCAN_NUM];
     if (bid == 0 && threadIdx.x == 0) {
         d.ctr[MG_C_NUM_POINTS] = nump;
@@ -566,33 +572,59 @@ __device__ __forceinline__ void sort_gather_body(const DS& d, float vmax, uint32
         d.ctr[MG_C_LISTS] = 1u;
     }
     const uint32_t N = (uint32_t)d.N;
-    const uint32_t s = dd < nump ? d.PERM[dd] : 0u;
+    /* the last cell's particles (possible when scan_max reaches the grid's edge) are placed
+     * past mNumPoints = OFF[C - 1]: ranked as well, they move nowhere */
+    const uint32_t lastn = RANK ? d.CNT[d.C - 1] : 0u;
+    const bool placed = p < nump || (RANK && p < N && p - nump < lastn);
+    const uint32_t s = placed ? d.PERM[p] : 0u;
+    uint32_t dd = p, rank = 0, cell = 0;
+    if (RANK && placed) {
+        cell = d.GCELL[s];
+        if (cell < (uint32_t)d.C) {
+            const uint32_t o = min(d.OFF[cell], N);
+            const uint32_t n = min(d.CNT[cell], N - o);
+            const uint32_t* run = d.PERM + o;
+            for (uint32_t k0 = 0; k0 < n; k0 += 8) {
+                uint32_t v[8];
+#pragma unroll
+                for (int j = 0; j < 8; j++) v[j] = run[min(k0 + (uint32_t)j, n - 1u)];
+#pragma unroll
+                for (int j = 0; j < 8; j++) rank += (k0 + (uint32_t)j < n && v[j] < s) ? 1u : 0u;
+            }
+            if (p < nump) dd = min(o + rank, N - 1u); /* s is in its run: rank < n */
+        }
+        /* a reset-tail slot keeps the pre-sort FGNDX of its own index, which for a placed
+         * particle is its rank (what k_cellsort leaves there) */
+        if (s >= nump && s < N) d.GNDX2[s] = rank;
+    }
     {
         /* the 64-byte concentration rows of the wavefront's 64 slots, copied by the whole
          * wavefront in four passes of 64 consecutive float4 (16 rows each), so every store
-         * writes 1 KB contiguous; row q takes its source slot from lane q's PERM entry, the
+         * writes whole rows; row q takes its source slot and destination from lane q, the
          * reset tail takes zeros */
-        const uint32_t lane = __lane_id(), wbase = dd - lane;
+        const uint32_t lane = __lane_id(), wbase = p - lane;
         const float4* csrc = (const float4*)d.CONC;
-        float4* cdst = (float4*)(d.CONC2 + (size_t)wbase * MG_NUM_TF);
+        float4* cdst = (float4*)d.CONC2;
 #pragma unroll
         for (uint32_t k = 0; k < 4u; k++) {
             const uint32_t f = k * 64u + lane, q = f >> 2;
             const uint32_t sq = (uint32_t)__shfl((int)s, (int)q);
+            const uint32_t dq = RANK ? (uint32_t)__shfl((int)dd, (int)q) : wbase + q;
             if (wbase + q < N)
-                cdst[f] = wbase + q < nump ? csrc[(size_t)sq * 4u + (f & 3u)] : make_float4(0.f, 0.f, 0.f, 0.f);
+                cdst[(size_t)dq * 4u + (f & 3u)] =
+                    wbase + q < nump ? csrc[(size_t)sq * 4u + (f & 3u)] : make_float4(0.f, 0.f, 0.f, 0.f);
         }
     }
-    if (dd >= N) return;
-    if (dd < nump) {
+    if (p >= N) return;
+    if (p < nump) {
         const uint32_t h = d.HOME[s];
         d.SLOT[h] = dd;
         d.P2[dd] = d.P[s];
         d.V2[dd] = d.V[s];
         d.VEL2[dd] = d.VEL[s];
         d.HOME2[dd] = h;
-        d.GCELL2[dd] = d.GCELL[s];
-        d.GNDX2[dd] = d.GNDX[s];
+        d.GCELL2[dd] = RANK ? cell : d.GCELL[s];
+        d.GNDX2[dd] = RANK ? rank : d.GNDX[s];
         uint32_t m = 0;
         if (d.GMN) {
             m = d.GMN[s];
@@ -602,18 +634,22 @@ __device__ __forceinline__ void sort_gather_body(const DS& d, float vmax, uint32
         }
         d.GMASK2[dd] = m;
     } else {
-        float4 p = d.P[dd];
-        d.P2[dd] = make_float4(vmax, vmax, vmax, p.w);
-        d.V2[dd] = d.V[dd];
-        d.VEL2[dd] = d.VEL[dd];
-        d.HOME2[dd] = d.HOME[dd];
-        d.GCELL2[dd] = d.GCELL[dd];
-        d.GNDX2[dd] = d.GNDX[dd];
-        d.GMASK2[dd] = 0u;
+        float4 q = d.P[p];
+        d.P2[p] = make_float4(vmax, vmax, vmax, q.w);
+        d.V2[p] = d.V[p];
+        d.VEL2[p] = d.VEL[p];
+        d.HOME2[p] = d.HOME[p];
+        const uint32_t gc = d.GCELL[p];
+        d.GCELL2[p] = gc;
+        if (!RANK || gc >= (uint32_t)d.C) d.GNDX2[p] = d.GNDX[p]; /* placed: its rank, above */
+        d.GMASK2[p] = 0u;
     }
 }
 
-__global__ LAUNCH_BOUNDS_256 void k_sort_gather(DS d, float vmax) { sort_gather_body(d, vmax, xcd_block()); }
+__global__ LAUNCH_BOUNDS_256 void k_sort_gather(DS d, float vmax) { sort_gather_body<false>(d, vmax, xcd_block()); }
+
+/* k_place's runs ranked in the gather (no k_cellsort pass) */
+__global__ LAUNCH_BOUNDS_256 void k_sort_rank_gather(DS d, float vmax) { sort_gather_body<true>(d, vmax, xcd_block()); }
 
 /* countingSortDenseLists (src/fluid_system_cuda.cu:329-404): per bin, particles in order,
  * genes with a nonzero epigenetic state append the particle to the gene's list. */
@@ -2690,7 +2726,7 @@ __global__ LAUNCH_BOUNDS_256 void kb_sort_gather(const BatchInst* __restrict__ I
 {
     const uint32_t b = xcd_block();
     const BatchInst& in = I[map[b]];
-    sort_gather_body(in.sort, in.vmax, b - in.pb0);
+    sort_gather_body<false>(in.sort, in.vmax, b - in.pb0);
 }
 
 /* --------------------------------------------------- persistent batched step
@@ -2776,7 +2812,7 @@ __global__ __launch_bounds__(256) void kb_step(PersistPlan P)
             case 3: scan_lookback_body(in.scan_in, in.scan_out, in.C, in.scan_status, in.scan_ticket, in.ctr, in.gsm, in.tnb); break;
             case 4: place_body(in.place, bid); break;
             case 5: cellsort_body(in.place, bid); break;
-            default: sort_gather_body(in.sort, in.vmax, bid); break;
+            default: sort_gather_body<false>(in.sort, in.vmax, bid); break;
             }
             __syncthreads(); /* the bodies' LDS is reused by the next item */
         }
