@@ -28,7 +28,7 @@ PHASES = ["pressure", "force", "advance", "diffusion", "genes", "insert", "scan"
 # kernels of the fused step (mg_step) behind each timed phase
 KERNEL_OF_PHASE = {"pressure": "k_pressure", "force": "k_force_advance",
                    "diffusion": "k_diffuse_genes_insert", "scan": "k_scan_lookback",
-                   "place": "k_place", "sort": "k_sort_rank_gather",
+                   "place": "k_place", "sort": "k_sort_gather",
                    "lists": "k_dense_lists", "fill": "k_fill_u32"}
 
 
@@ -38,12 +38,11 @@ def bond_records_in_use(scene) -> bool:
     return float(scene.params.r2) == 1.0 and int(scene.grid.srch) <= 4 and scene.b > 0
 
 
-def algorithmic_bytes(phase: str, scene, stats: dict, ranked: bool = True) -> float:
+def algorithmic_bytes(phase: str, scene, stats: dict) -> float:
     """Compulsory HBM bytes per launch of the fused step's kernels in this build's layout:
     every array element read or written once, neighbour data counted once (as that
     particle's own read). n = active particles, C = cells, B = bond slots, bo / bi = mean
-    outgoing / incoming bonds per particle. ranked: the gather ranks the cell runs itself
-    (single instances); False for the batched step, whose place phase adds k_cellsort."""
+    outgoing / incoming bonds per particle."""
     n, c, b = stats["n_active"], scene.c, scene.b
     bo, bi = stats["bonds_out"], stats["bonds_in"]
     genes = bool(scene.flags & 1)
@@ -73,13 +72,12 @@ def algorithmic_bytes(phase: str, scene, stats: dict, ranked: bool = True) -> fl
     if phase == "scan":
         return 8 * c
     if phase == "place":  # k_place: GCELL, GNDX read, PERM written; OFF read per cell
-        # (+ k_cellsort: PERM read and written, GNDX written; CNT, OFF per cell)
-        return n * (8 + 4) + 4 * c + (0 if ranked else n * 12 + 8 * c)
+        return n * (8 + 4) + 4 * c
     if phase == "sort":
         # PERM 4 (the cell runs the ranks count re-read it); P, V, VEL 48 and CONC 64 read and
-        # written; HOME/GCELL 8 read (+ GNDX 4 unranked), HOME/GCELL/GNDX 12 written; GMN 4
-        # read, GMASK 4 and SLOT 4 written; OFF/CNT 8 per cell when ranked
-        return n * (4 + 2 * (48 + 64) + 8 + 12 + 4 + 4 + 4) + (8 * c if ranked else 4 * n)
+        # written; HOME/GCELL 8 read, HOME/GCELL/GNDX 12 written; GMN 4 read, GMASK 4 and
+        # SLOT 4 written; OFF/CNT 8 per cell
+        return n * (4 + 2 * (48 + 64) + 8 + 12 + 4 + 4 + 4) + 8 * c
     return 0.0
 
 
@@ -260,7 +258,7 @@ def run_batch(args, count: int = 256):
     bat._call("batch_step_timed", bat.h, args.timed_steps, dt, flags, ms, calls)
     per_step = {ph: ms[i] / args.timed_steps for i, ph in enumerate(PHASES) if calls[i]}
     kern = {"pressure": "kb_pressure", "force": "kb_force_advance",
-            "diffusion": "kb_diffuse_genes_insert", "scan": "kb_scan", "place": "kb_place+kb_cellsort",
+            "diffusion": "kb_diffuse_genes_insert", "scan": "kb_scan", "place": "kb_place",
             "sort": "kb_sort_gather"}
     # algorithmic bytes of a launch group = sum over the instances
     stats = []
@@ -270,13 +268,13 @@ def run_batch(args, count: int = 256):
         stats.append(st)
     kernels = {}
     for p, v in per_step.items():
-        bts = sum(algorithmic_bytes(p, sc, st, ranked=False) for sc, st in zip(scs, stats))
+        bts = sum(algorithmic_bytes(p, sc, st) for sc, st in zip(scs, stats))
         if p in kern and bts > 0:
             kernels[kern[p]] = {"ms": round(v, 5), "alg_bytes": int(bts), "GB/s": round(bts / (v * 1e-3) / 1e9, 1)}
     dom = max((p for p in per_step if p in kern and p != "place"), key=lambda p: per_step[p])
     ref = sum(reference_layout_bytes(dom, sc, st)[0] for sc, st in zip(scs, stats))
     terms = reference_layout_bytes(dom, scs[0], stats[0])[1]
-    lay = sum(algorithmic_bytes(dom, sc, st, ranked=False) for sc, st in zip(scs, stats))
+    lay = sum(algorithmic_bytes(dom, sc, st) for sc, st in zip(scs, stats))
     achieved = ref / (per_step[dom] * 1e-3) / 1e9
     lay_ach = lay / (per_step[dom] * 1e-3) / 1e9
     classes = bat.classes()

@@ -2,11 +2,11 @@
  * mg_batch.hip — many independent handles stepped together (included by mg_capi.hip).
  *
  * A step of every instance is one persistent launch (kb_step): resident workgroups walk the
- * seven phases of the fused step (pressure, force + advance, diffusion + genes + insert,
- * scan, place, cellsort, gather) over per-phase queues of (instance, block) items, with grid
+ * six phases of the fused step (pressure, force + advance, diffusion + genes + insert,
+ * scan, place, rank + gather) over per-phase queues of (instance, block) items, with grid
  * barriers between phases. The alternative is the dispatch form: instances grouped into
- * size classes by particle count, each class running the seven phases as launches that cover
- * all of its instances (7 x classes dispatches per step). Both look up their instance in a
+ * size classes by particle count, each class running the six phases as launches that cover
+ * all of its instances (6 x classes dispatches per step). Both look up their instance in a
  * device table (BatchInst) built from each handle exactly as its own fused step builds its
  * kernel arguments, so every instance evolves bit for bit as it would alone. Tables cover the
  * two buffer parities of consecutive steps; two consecutive steps are captured into one
@@ -21,8 +21,8 @@ struct mg_batch {
         int pshape = 0;               /* pressure sweep tile shape (kPShapes) */
         int ng = 1;                   /* diffusion sweep template: 1, 2 or 4 groups */
         BatchInst* I[2] = {};         /* tables for the start parity and the next one */
-        uint16_t *pmap = nullptr, *cmap = nullptr, *tmap = nullptr;
-        uint32_t pblocks = 0, cblocks = 0, tblocks = 0;
+        uint16_t *pmap = nullptr, *tmap = nullptr;
+        uint32_t pblocks = 0, tblocks = 0;
         double ms = 0.0;              /* accumulated kernel time (autotuning) */
     };
     std::vector<mg_handle*> hs;
@@ -108,7 +108,7 @@ void batch_free_tables(mg_batch* b)
     b->persist_ok = false;
     for (auto& c : b->cls) {
         c.I[0] = c.I[1] = nullptr;
-        c.pmap = c.cmap = c.tmap = nullptr;
+        c.pmap = c.tmap = nullptr;
     }
     b->built = false;
 }
@@ -133,6 +133,8 @@ void fill_inst(BatchInst& t, mg_handle* h, const MgGenome* dgen, float dt, unsig
     t.place.OFF = t.place.OFFN;
     t.sort = make_ds(h, 1, 1);
     t.sort.GMN = h->GMN;
+    t.sort.CNT = t.sort.CNTN; /* the grid k_place bins into */
+    t.sort.OFF = t.sort.OFFN;
     t.p = h->p;
     t.g = h->g;
     t.dp = diff_plan(h);
@@ -173,9 +175,9 @@ int persist_build(mg_batch* b, float dt, unsigned flags)
     bool fits = n <= 65535;
     for (int k = 0; k < n && fits; k++) {
         mg_handle* h = b->hs[order[k]];
-        const uint32_t pnb = nblk(h->N, 256), cnb = nblk(h->C, 256);
+        const uint32_t pnb = nblk(h->N, 256);
         const uint32_t tnb = (uint32_t)((h->C + SCAN_TILE - 1) / SCAN_TILE);
-        if (pnb > 65536 || cnb > 65536 || tnb > 65536) { fits = false; break; }
+        if (pnb > 65536 || tnb > 65536) { fits = false; break; }
         for (int q = 0; q < 2; q++) {
             const int ic = h->icur, gc = h->gcur;
             const bool gs = h->grid_scanned, pb = h->pos_binned, cf = h->cells_fresh;
@@ -187,7 +189,6 @@ int persist_build(mg_batch* b, float dt, unsigned flags)
             BatchInst t;
             fill_inst(t, h, b->dgen[order[k]], dt, flags);
             t.pb0 = 0; t.pnb = pnb;
-            t.cb0 = 0; t.cnb = cnb;
             t.tb0 = 0; t.tnb = tnb;
             tab[q].push_back(t);
             h->icur = ic;
@@ -200,9 +201,8 @@ int persist_build(mg_batch* b, float dt, unsigned flags)
         ng = std::max(ng, g <= 1 ? 1 : (g == 2 ? 2 : 4));
         const uint32_t base = (uint32_t)k << 16;
         for (uint32_t x = 0; x < pnb; x++)
-            for (int ph : {0, 1, 2, 4, 6}) items[ph].push_back(base | x);
+            for (int ph : {0, 1, 2, 4, 5}) items[ph].push_back(base | x);
         for (uint32_t x = 0; x < tnb; x++) items[3].push_back(base | x);
-        for (uint32_t x = 0; x < cnb; x++) items[5].push_back(base | x);
     }
     b->persist_ok = false;
     if (!fits) return MG_OK; /* the dispatch form serves this batch */
@@ -272,17 +272,16 @@ int batch_build(mg_batch* b, float dt, unsigned flags)
     for (auto& c : b->cls) {
         if (c.members.empty()) continue;
         const unsigned T = c.threads;
-        std::vector<uint16_t> pm, cm, tm;
+        std::vector<uint16_t> pm, tm;
         std::vector<BatchInst> tab[2];
         c.ng = 1;
         for (int q = 0; q < 2; q++) tab[q].resize(c.members.size());
         for (size_t k = 0; k < c.members.size(); k++) {
             mg_handle* h = b->hs[c.members[k]];
-            const uint32_t pnb = nblk(h->N, T), cnb = nblk(h->C, T);
+            const uint32_t pnb = nblk(h->N, T);
             const uint32_t tnb = (uint32_t)((h->C + SCAN_TILE - 1) / SCAN_TILE);
-            const uint32_t pb0 = (uint32_t)pm.size(), cb0 = (uint32_t)cm.size(), tb0 = (uint32_t)tm.size();
+            const uint32_t pb0 = (uint32_t)pm.size(), tb0 = (uint32_t)tm.size();
             pm.insert(pm.end(), pnb, (uint16_t)k);
-            cm.insert(cm.end(), cnb, (uint16_t)k);
             tm.insert(tm.end(), tnb, (uint16_t)k);
             for (int q = 0; q < 2; q++) {
                 /* q = 1: the state one fused step later */
@@ -296,7 +295,6 @@ int batch_build(mg_batch* b, float dt, unsigned flags)
                 BatchInst& t = tab[q][k];
                 fill_inst(t, h, b->dgen[c.members[k]], dt, flags);
                 t.pb0 = pb0; t.pnb = pnb;
-                t.cb0 = cb0; t.cnb = cnb;
                 t.tb0 = tb0; t.tnb = tnb;
                 h->icur = ic;
                 h->gcur = gc;
@@ -308,7 +306,6 @@ int batch_build(mg_batch* b, float dt, unsigned flags)
             c.ng = std::max(c.ng, ng <= 1 ? 1 : (ng == 2 ? 2 : 4));
         }
         c.pblocks = (uint32_t)pm.size();
-        c.cblocks = (uint32_t)cm.size();
         c.tblocks = (uint32_t)tm.size();
         for (int q = 0; q < 2; q++) {
             CHK(balloc(b, &c.I[q], tab[q].size()));
@@ -316,10 +313,8 @@ int batch_build(mg_batch* b, float dt, unsigned flags)
                                   hipMemcpyHostToDevice, b->st));
         }
         CHK(balloc(b, &c.pmap, pm.size()));
-        CHK(balloc(b, &c.cmap, cm.size()));
         CHK(balloc(b, &c.tmap, tm.size()));
         HIPCHK(hipMemcpyAsync(c.pmap, pm.data(), pm.size() * 2, hipMemcpyHostToDevice, b->st));
-        HIPCHK(hipMemcpyAsync(c.cmap, cm.data(), cm.size() * 2, hipMemcpyHostToDevice, b->st));
         HIPCHK(hipMemcpyAsync(c.tmap, tm.data(), tm.size() * 2, hipMemcpyHostToDevice, b->st));
         /* the host copies go away with this scope: wait for the uploads */
         HIPCHK(hipStreamSynchronize(b->st));
@@ -433,7 +428,6 @@ int batch_launch(mg_batch* b, int q)
         if (c.members.empty()) continue;
         batch_mark(b, (int)ci, MG_PH_PLACE, true);
         hipLaunchKernelGGL(kb_place, dim3(c.pblocks), dim3(c.threads), 0, b->st, c.I[q], c.pmap);
-        hipLaunchKernelGGL(kb_cellsort, dim3(c.cblocks), dim3(c.threads), 0, b->st, c.I[q], c.cmap);
         batch_mark(b, (int)ci, MG_PH_PLACE, false);
     }
     for (size_t ci = 0; ci < b->cls.size(); ci++) {

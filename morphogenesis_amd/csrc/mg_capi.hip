@@ -819,7 +819,7 @@ int do_sort(mg_handle* h, int lists)
     {
         PhaseScope ps(h, MG_PH_SORT);
         const float vmax = std::max(std::max(h->g.gmax[0], h->g.gmax[1]), h->g.gmax[2]);
-        hipLaunchKernelGGL(k_sort_rank_gather, dim3(nblk(h->N)), dim3(256), 0, h->st, d, vmax);
+        hipLaunchKernelGGL(k_sort_gather, dim3(nblk(h->N)), dim3(256), 0, h->st, d, vmax);
     }
     HIPCHK(hipGetLastError());
     h->kcur = 1 - h->kcur;
@@ -1018,7 +1018,7 @@ int do_fused_step(mg_handle* h, float dt, unsigned flags, const mg_net* net = nu
         d.CNT = d.CNTN; /* the grid k_place binned into */
         d.OFF = d.OFFN;
         const float vmax = std::max(std::max(h->g.gmax[0], h->g.gmax[1]), h->g.gmax[2]);
-        hipLaunchKernelGGL(k_sort_rank_gather, dim3(nb), dim3(256), 0, h->st, d, vmax);
+        hipLaunchKernelGGL(k_sort_gather, dim3(nb), dim3(256), 0, h->st, d, vmax);
     }
     HIPCHK(hipGetLastError());
     h->icur = 1 - h->icur;

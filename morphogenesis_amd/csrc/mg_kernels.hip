@@ -8,7 +8,7 @@
  *
  * Two launch sequences share the per-particle device functions below, so they produce
  * identical bits:
- *   phase kernels (k_insert, k_scan, k_place, k_sort_rank_gather, k_pressure,
+ *   phase kernels (k_insert, k_scan, k_place, k_sort_gather, k_pressure,
  *   k_force, k_advance, k_diffusion, k_genes), one per reference phase, used by the
  *   FluidSystem-style entry points;
  *   fused kernels (k_force_advance, k_diffuse_genes_insert), the hot loop of mg_step:
@@ -46,7 +46,7 @@ __global__ LAUNCH_BOUNDS_256 void k_fill_u32(uint32_t* p, uint32_t v, size_t n)
  * is cell-sorted from the previous step, so they are contiguous) are counted with one
  * atomicAdd per run: the run head adds the run length and broadcasts the base. FGNDX
  * receives a slot in [0, count) as the reference's atomicAdd does; the sort later replaces
- * it with the stable rank (k_sort_rank_gather; k_cellsort in the batched step). */
+ * it with the stable rank (k_sort_gather). */
 __device__ __forceinline__ void insert_one(const DS& d, const MgGrid& g, uint32_t s, bool valid,
                                            float4 p, uint32_t* CNT, uint32_t* GCNT)
 {
@@ -441,9 +441,9 @@ __global__ __launch_bounds__(256) void k_inc_order_long(const uint32_t* __restri
     }
 }
 
-/* ------------------------------------------------------ stable placement
- * PERM[OFF[cell] + slot] = s, then each cell's run of PERM is sorted ascending so the
- * intra-cell order is the pre-sort index order; FGNDX becomes that rank. */
+/* ------------------------------------------------------ placement
+ * PERM[OFF[cell] + slot] = s: each cell's run of PERM holds its particles in insertion order;
+ * the gather ranks them by pre-sort index (the stable order) and FGNDX becomes that rank. */
 __device__ __forceinline__ void place_body(const DS& d, uint32_t bid)
 {
     const uint32_t s = bid * blockDim.x + threadIdx.x;
@@ -455,94 +455,6 @@ __device__ __forceinline__ void place_body(const DS& d, uint32_t bid)
 
 __global__ LAUNCH_BOUNDS_256 void k_place(DS d) { place_body(d, blockIdx.x); }
 
-/* A run of up to M distinct slot indices, ranked in registers by its lane: the run is loaded
- * with independent loads, each element's rank is the number of smaller values, and the
- * elements are stored at their ranks (no dependent memory round trips). */
-template <int M>
-__device__ __forceinline__ void cellsort_small(const DS& d, uint32_t* run, uint32_t n)
-{
-    uint32_t v[M];
-#pragma unroll
-    for (int k = 0; k < M; k++) v[k] = ((uint32_t)k < n) ? run[k] : 0xFFFFFFFFu;
-#pragma unroll
-    for (int k = 0; k < M; k++) {
-        if ((uint32_t)k < n) {
-            uint32_t r = 0;
-#pragma unroll
-            for (int q = 0; q < M; q++) r += v[q] < v[k] ? 1u : 0u;
-            run[r] = v[k];
-            d.GNDX[v[k]] = r;
-        }
-    }
-}
-
-/* One lane per cell. Cells of up to CELLSORT_SMALL particles are ranked in registers by their
- * lane, 8, 16 or 32 wide by the wavefront's largest such cell, so all of a wavefront's cells
- * sort behind one round of loads; a larger cell is ranked by the whole wavefront (each
- * element's rank is the number of smaller slot indices in the run; the values are distinct):
- * up to 64 elements one per lane against lane broadcasts, longer runs through LDS, so a
- * crowded cell costs O(n^2 / 64) per lane instead of O(n^2) on one lane. Runs longer than
- * CELLSORT_CAP fall back to the owning lane's insertion sort. */
-#define CELLSORT_SMALL 32
-#define CELLSORT_CAP 512
-__device__ __forceinline__ void cellsort_body(const DS& d, uint32_t bid)
-{
-    __shared__ uint32_t buf[4][CELLSORT_CAP];
-    const uint32_t c = bid * blockDim.x + threadIdx.x;
-    const uint32_t n = (c < (uint32_t)d.C) ? d.CNT[c] : 0u;
-    const uint32_t lane = __lane_id(), w = threadIdx.x >> 6;
-    const bool big = n > CELLSORT_SMALL && n <= CELLSORT_CAP;
-    const bool small = n > 1 && n <= CELLSORT_SMALL;
-    if (__ballot(small && n > 8) == 0ull) {
-        if (small) cellsort_small<8>(d, d.PERM + d.OFF[c], n);
-    } else if (__ballot(small && n > 16) == 0ull) {
-        if (small) cellsort_small<16>(d, d.PERM + d.OFF[c], n);
-    } else if (small) {
-        cellsort_small<CELLSORT_SMALL>(d, d.PERM + d.OFF[c], n);
-    }
-    if (n > CELLSORT_CAP) {
-        uint32_t* run = d.PERM + d.OFF[c];
-        for (uint32_t i = 1; i < n; i++) {
-            uint32_t v = run[i];
-            uint32_t j = i;
-            while (j > 0 && run[j - 1] > v) { run[j] = run[j - 1]; j--; }
-            run[j] = v;
-        }
-        for (uint32_t r = 0; r < n; r++) d.GNDX[run[r]] = r;
-    } else if (n == 1) {
-        d.GNDX[d.PERM[d.OFF[c]]] = 0u;
-    }
-    uint64_t todo = __ballot(big);
-    while (todo) {
-        const int src = __ffsll((long long)todo) - 1;
-        todo &= todo - 1ull;
-        const uint32_t cb = (uint32_t)__builtin_amdgcn_readlane((int)c, src);
-        const uint32_t nb = (uint32_t)__builtin_amdgcn_readlane((int)n, src);
-        uint32_t* run = d.PERM + d.OFF[cb];
-        if (nb <= 64u) {
-            /* one element per lane, ranked against every other through lane broadcasts */
-            const uint32_t v = lane < nb ? run[lane] : 0xFFFFFFFFu;
-            uint32_t rank = 0;
-            for (uint32_t k = 0; k < nb; k++) rank += (uint32_t)__builtin_amdgcn_readlane((int)v, (int)k) < v ? 1u : 0u;
-            if (lane < nb) {
-                run[rank] = v;
-                d.GNDX[v] = rank;
-            }
-            continue;
-        }
-        for (uint32_t e = lane; e < nb; e += 64) buf[w][e] = run[e];
-        wave_lds_sync();
-        for (uint32_t e = lane; e < nb; e += 64) {
-            const uint32_t v = buf[w][e];
-            uint32_t rank = 0;
-            for (uint32_t k = 0; k < nb; k++) rank += buf[w][k] < v ? 1u : 0u;
-            run[rank] = v;
-            d.GNDX[v] = rank;
-        }
-        wave_lds_sync();
-    }
-}
-
 /* ------------------------------------------------------- counting sort (a7)
  * countingSortFull (src/fluid_system_cuda.cu:196-296) with the host resets of
  * src/host_CL.cpp:1156-1206: slot dd < mNumPoints takes the particle of rank dd - OFF[c] in
@@ -553,13 +465,11 @@ __device__ __forceinline__ void cellsort_body(const DS& d, uint32_t bid)
  * SLOT. Reads P/V/VEL/CONC/HOME/GCELL/GNDX, writes their "2" halves. The gene membership
  * comes from GMN (precomputed) when given, else from EPIGEN. Block 0 commits the counters.
  *
- * RANK = false: PERM holds every cell's run sorted (k_cellsort), thread dd gathers PERM[dd].
- * RANK = true: PERM holds the runs as k_place left them (insertion order), d.CNT / d.OFF are
- * the grid k_place used; thread p takes the particle u = PERM[p], counts the smaller indices
- * of its cell's run (its rank r, as k_cellsort ranks it) and moves u to dd = OFF[c] + r. The
- * destinations of a wavefront are a permutation of its own slots within their cells' runs,
- * so the stores stay as contiguous as the gather's. */
-template <bool RANK>
+ * PERM holds each cell's run in insertion order (k_place), d.CNT / d.OFF are the grid k_place
+ * used. Thread p takes the particle s = PERM[p], counts the smaller pre-sort indices in its
+ * cell's run (its rank r: the stable order) and moves s to dd = OFF[c] + r. The destinations
+ * of a wavefront are a permutation of its own slots within their cells' runs, so the stores
+ * stay as contiguous as a gather's; the runs it re-reads are a few cache lines per cell. */
 __device__ __forceinline__ void sort_gather_body(const DS& d, float vmax, uint32_t bid)
 {
     const uint32_t p = bid * blockDim.x + threadIdx.x;
@@ -574,11 +484,11 @@ __device__ __forceinline__ void sort_gather_body(const DS& d, float vmax, uint32
     const uint32_t N = (uint32_t)d.N;
     /* the last cell's particles (possible when scan_max reaches the grid's edge) are placed
      * past mNumPoints = OFF[C - 1]: ranked as well, they move nowhere */
-    const uint32_t lastn = RANK ? d.CNT[d.C - 1] : 0u;
-    const bool placed = p < nump || (RANK && p < N && p - nump < lastn);
+    const uint32_t lastn = d.CNT[d.C - 1];
+    const bool placed = p < nump || (p < N && p - nump < lastn);
     const uint32_t s = placed ? d.PERM[p] : 0u;
     uint32_t dd = p, rank = 0, cell = 0;
-    if (RANK && placed) {
+    if (placed) {
         cell = d.GCELL[s];
         if (cell < (uint32_t)d.C) {
             const uint32_t o = min(d.OFF[cell], N);
@@ -594,7 +504,7 @@ __device__ __forceinline__ void sort_gather_body(const DS& d, float vmax, uint32
             if (p < nump) dd = min(o + rank, N - 1u); /* s is in its run: rank < n */
         }
         /* a reset-tail slot keeps the pre-sort FGNDX of its own index, which for a placed
-         * particle is its rank (what k_cellsort leaves there) */
+         * particle is its rank (the reference's insert slot, taken in index order) */
         if (s >= nump && s < N) d.GNDX2[s] = rank;
     }
     {
@@ -609,7 +519,7 @@ __device__ __forceinline__ void sort_gather_body(const DS& d, float vmax, uint32
         for (uint32_t k = 0; k < 4u; k++) {
             const uint32_t f = k * 64u + lane, q = f >> 2;
             const uint32_t sq = (uint32_t)__shfl((int)s, (int)q);
-            const uint32_t dq = RANK ? (uint32_t)__shfl((int)dd, (int)q) : wbase + q;
+            const uint32_t dq = (uint32_t)__shfl((int)dd, (int)q);
             if (wbase + q < N)
                 cdst[(size_t)dq * 4u + (f & 3u)] =
                     wbase + q < nump ? csrc[(size_t)sq * 4u + (f & 3u)] : make_float4(0.f, 0.f, 0.f, 0.f);
@@ -623,8 +533,8 @@ __device__ __forceinline__ void sort_gather_body(const DS& d, float vmax, uint32
         d.V2[dd] = d.V[s];
         d.VEL2[dd] = d.VEL[s];
         d.HOME2[dd] = h;
-        d.GCELL2[dd] = RANK ? cell : d.GCELL[s];
-        d.GNDX2[dd] = RANK ? rank : d.GNDX[s];
+        d.GCELL2[dd] = cell;
+        d.GNDX2[dd] = rank;
         uint32_t m = 0;
         if (d.GMN) {
             m = d.GMN[s];
@@ -641,15 +551,12 @@ __device__ __forceinline__ void sort_gather_body(const DS& d, float vmax, uint32
         d.HOME2[p] = d.HOME[p];
         const uint32_t gc = d.GCELL[p];
         d.GCELL2[p] = gc;
-        if (!RANK || gc >= (uint32_t)d.C) d.GNDX2[p] = d.GNDX[p]; /* placed: its rank, above */
+        if (gc >= (uint32_t)d.C) d.GNDX2[p] = d.GNDX[p]; /* placed: its rank, above */
         d.GMASK2[p] = 0u;
     }
 }
 
-__global__ LAUNCH_BOUNDS_256 void k_sort_gather(DS d, float vmax) { sort_gather_body<false>(d, vmax, xcd_block()); }
-
-/* k_place's runs ranked in the gather (no k_cellsort pass) */
-__global__ LAUNCH_BOUNDS_256 void k_sort_rank_gather(DS d, float vmax) { sort_gather_body<true>(d, vmax, xcd_block()); }
+__global__ LAUNCH_BOUNDS_256 void k_sort_gather(DS d, float vmax) { sort_gather_body(d, vmax, xcd_block()); }
 
 /* countingSortDenseLists (src/fluid_system_cuda.cu:329-404): per bin, particles in order,
  * genes with a nonzero epigenetic state append the particle to the gene's list. */
@@ -2670,7 +2577,6 @@ struct BatchInst {
     int C, gsm, genes;
     float vmax, dt;
     uint32_t pb0, pnb; /* particle launches */
-    uint32_t cb0, cnb; /* per-cell launch */
     uint32_t tb0, tnb; /* scan tiles */
 };
 
@@ -2716,22 +2622,16 @@ __global__ LAUNCH_BOUNDS_256 void kb_place(const BatchInst* __restrict__ I, cons
     place_body(in.place, blockIdx.x - in.pb0);
 }
 
-__global__ LAUNCH_BOUNDS_256 void kb_cellsort(const BatchInst* __restrict__ I, const uint16_t* __restrict__ map)
-{
-    const BatchInst& in = I[map[blockIdx.x]];
-    cellsort_body(in.place, blockIdx.x - in.cb0);
-}
-
 __global__ LAUNCH_BOUNDS_256 void kb_sort_gather(const BatchInst* __restrict__ I, const uint16_t* __restrict__ map)
 {
     const uint32_t b = xcd_block();
     const BatchInst& in = I[map[b]];
-    sort_gather_body<false>(in.sort, in.vmax, b - in.pb0);
+    sort_gather_body(in.sort, in.vmax, b - in.pb0);
 }
 
 /* --------------------------------------------------- persistent batched step
  * One launch per step for every instance of a batch: a grid of workgroups that stay resident
- * (sized from the occupancy of this kernel) walks the seven phases of the fused step, each
+ * (sized from the occupancy of this kernel) walks the six phases of the fused step, each
  * phase a queue of (instance, block) items taken with one atomic per item (largest instances
  * first), with a grid barrier between phases. Each item runs the same body with the same
  * block-local index as the instance's own launch, so every instance evolves bit for bit as
@@ -2740,12 +2640,12 @@ __global__ LAUNCH_BOUNDS_256 void kb_sort_gather(const BatchInst* __restrict__ I
  * Barrier: sense reversal. The last workgroup to arrive resets the arrival count and the
  * queue of the phase that just ended (every workgroup has left it), then flips `sense`; the
  * others wait for the flip with an agent-scope acquire, so the phase's writes, released by
- * every arriving workgroup, are visible to all XCDs after it. Queue 6 (the last phase) is
+ * every arriving workgroup, are visible to all XCDs after it. Queue 5 (the last phase) is
  * reset at the first barrier of the next launch. Nothing needs clearing between launches,
  * so the launch replays from a graph as it is. A wait that exceeds its bound raises `err`
  * and gives up (the step's results are then invalid) instead of spinning forever, e.g.
  * when the GPU cannot keep the whole grid resident. */
-#define PERSIST_PHASES 7
+#define PERSIST_PHASES 6
 struct PersistCtl {
     unsigned count, sense, err, pad;
     unsigned q[PERSIST_PHASES + 1];
@@ -2811,8 +2711,7 @@ __global__ __launch_bounds__(256) void kb_step(PersistPlan P)
                 break;
             case 3: scan_lookback_body(in.scan_in, in.scan_out, in.C, in.scan_status, in.scan_ticket, in.ctr, in.gsm, in.tnb); break;
             case 4: place_body(in.place, bid); break;
-            case 5: cellsort_body(in.place, bid); break;
-            default: sort_gather_body<false>(in.sort, in.vmax, bid); break;
+            default: sort_gather_body(in.sort, in.vmax, bid); break;
             }
             __syncthreads(); /* the bodies' LDS is reused by the next item */
         }

@@ -23,7 +23,7 @@ import sys
 from collections import defaultdict
 
 STEP_KERNELS = ("k_pressure", "k_force_advance", "k_diffuse_genes_insert", "k_scan_lookback",
-                "k_place", "k_cellsort", "k_sort_gather", "k_sort_rank_gather")
+                "k_place", "k_sort_gather")
 
 
 def short_name(name: str) -> str:
