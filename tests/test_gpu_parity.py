@@ -540,6 +540,34 @@ def test_near_coincident_particles(built):
     compare_all(g, o, "near-coincident: 6 fused steps")
 
 
+def test_crowded_cell_and_unbinned_low_indices(built):
+    """The sort's ranks within a cell: one cell holding 700 particles (runs far longer than a
+    wavefront) beside a jittered lattice, and the first 40 particles in the grid's outer layer
+    of cells, which the insert leaves unbinned. The binned particles with the highest indices
+    then lie past mNumPoints, so the reset tail keeps their ranks as FGNDX. Bit-equal to the
+    oracle after the phase sort and after fused steps."""
+    rng = np.random.default_rng(5)
+    outer = np.column_stack([rng.uniform(0.05, 0.95, 40), rng.uniform(3.0, 8.0, 40),
+                             rng.uniform(3.0, 8.0, 40)])
+    crowd = rng.uniform(5.05, 5.95, (700, 3))
+    lat = scenes.lattice(8, 8, 8, 0.5, (2.2, 2.2, 6.2), 0.05, rng)
+    pos = np.concatenate([outer, crowd, lat]).astype(np.float32)
+    n = len(pos)
+    sc = mini_scene(pos, b=4)
+    say("case: crowded cell and unbinned low indices", n)
+    g, o = pair(sc)
+    for sim in (g, o):
+        sim.Run2PhysicalSort()
+    nump, _ = g.counts()
+    assert g.counts() == o.counts() and nump == n - 40
+    cnt = o.download(F.FBIN_COUNT)
+    assert cnt.max() >= 700
+    compare_all(g, o, "crowded cell: phase sort")
+    g.step(3, sc.dt, sc.flags | F.STEP_GRAPH)
+    o.step(3, sc.dt, sc.flags)
+    compare_all(g, o, "crowded cell: 3 fused steps")
+
+
 def recorded_per_row(sim, grid, r2, sample=3000, seed=0):
     """For a sample of active particles, the number of candidates of each of its 9 neighbour
     rows within the hit-mask skin (0 < dsq < 1.0201 r2): what the pressure sweep records."""
