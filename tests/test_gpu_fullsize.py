@@ -2,7 +2,7 @@
 (or, for the batch, against each instance stepped alone):
 
   tissue_262k       262,144 particles, B = 32, genes on: first sort, then 100 steps (the
-                    north star's 100-step bar), compared every 25 steps;
+                    FP32 agreement is required after 100 steps), compared every 25 steps;
   mixed_4m          4,194,304 particles (bonded slab + fluid column with dense droplets):
                     first sort, then 2 steps;
   tissue_batch_256  256 bonded tissues of log-uniform size (4.58M particles): 4 batched
