@@ -1661,9 +1661,9 @@ __device__ __forceinline__ void diffuse_sweep(const DS& d, const MgParams& p, co
  * of particle i (cell gc, not GRID_UNDEF) from the concentrations CONC (read-only):
  *   out[t] = tf_breakdown_rate[t] * (cur[t] + nw[t]),  nw[t] = +0 for TFs that do not diffuse.
  * Called by all lanes of the wavefront; out is written for valid lanes only. */
-template <int NG>
+template <int NG, class Gen>
 __device__ __forceinline__ void diffuse_one(const DS& d, const MgParams& p, const MgGrid& g,
-                                            const MgGenome& gen, const DiffPlan& dp, Band<NG>& wb,
+                                            const Gen& gen, const DiffPlan& dp, Band<NG>& wb,
                                             bool valid, uint32_t i, uint32_t gc, float out[MG_NUM_TF])
 {
     float nwf[MG_NUM_TF];
@@ -1707,7 +1707,8 @@ __global__ LAUNCH_BOUNDS_SWEEP void k_diffusion(DS d, MgParams p, MgGrid g, MgGe
  * in registers, then the tally when the particle is a member of the living list (gene 2)
  * below mActivePoints. Fixed semantics are those of the oracle's genes() (float FCONC in
  * AoS, one tally, bounded countdown). Updates epi[] and conc[] in place. */
-__device__ __forceinline__ void genes_one(const MgParams& p, const MgGenome& gen, uint32_t m,
+template <class Gen>
+__device__ __forceinline__ void genes_one(const MgParams& p, const Gen& gen, uint32_t m,
                                           bool tally, uint32_t epi[MG_NUM_GENES],
                                           float conc[MG_NUM_TF])
 {
@@ -1777,9 +1778,9 @@ __global__ LAUNCH_BOUNDS_256 void k_genes(DS d, MgParams p, MgGenome gen)
  * after the advance (d.P), the pre-step concentrations (d.CONC) and the pre-step grid
  * (d.CNT / d.OFF); writes CONC2, updates EPIGEN, records the post-gene membership in GMN for
  * the coming sort and bins the particle into the next grid (d.CNTN). */
-template <int NG>
+template <int NG, class Gen>
 __device__ __forceinline__ void diffuse_genes_insert_body(const DS& d, const MgParams& p, const MgGrid& g,
-                                                          const MgGenome& gen, const DiffPlan& dp,
+                                                          const Gen& gen, const DiffPlan& dp,
                                                           int genes, Band<NG>* bands, uint32_t bid)
 {
     const uint32_t i = bid * blockDim.x + threadIdx.x;
@@ -2599,6 +2600,13 @@ __global__ LAUNCH_BOUNDS_SWEEP void kb_force_advance(const BatchInst* __restrict
     force_advance_body(in.force, in.p, in.g, (Band<1>*)mg_lds, in.dt, b - in.pb0);
 }
 
+/* An instance's genome read through the constant address space: nothing writes it while a
+ * step runs, so its wavefront-uniform reads become scalar loads (through the table's generic
+ * pointer they are flat loads, and every gene waits on memory). The gene and diffusion bodies
+ * take the genome's type as a template parameter so the reads keep this address space. */
+typedef const __attribute__((address_space(4))) MgGenome ConstGenome;
+__device__ __forceinline__ ConstGenome& const_genome(const MgGenome* g) { return *(ConstGenome*)g; }
+
 template <int NG>
 __global__ LAUNCH_BOUNDS_SWEEP void kb_diffuse_genes_insert(const BatchInst* __restrict__ I,
                                                             const uint16_t* __restrict__ map)
@@ -2606,7 +2614,7 @@ __global__ LAUNCH_BOUNDS_SWEEP void kb_diffuse_genes_insert(const BatchInst* __r
     extern __shared__ float4 mg_lds[];
     const uint32_t b = xcd_block();
     const BatchInst& in = I[map[b]];
-    diffuse_genes_insert_body<NG>(in.diffuse, in.p, in.g, *in.gen, in.dp, in.genes, (Band<NG>*)mg_lds,
+    diffuse_genes_insert_body<NG>(in.diffuse, in.p, in.g, const_genome(in.gen), in.dp, in.genes, (Band<NG>*)mg_lds,
                                   b - in.pb0);
 }
 
@@ -2707,7 +2715,7 @@ __global__ __launch_bounds__(256) void kb_step(PersistPlan P)
             case 0: pressure_body(in.pressure, in.p, in.g, (Band<0>*)mg_lds, bid, in.pnb); break;
             case 1: force_advance_body(in.force, in.p, in.g, (Band<1>*)mg_lds, in.dt, bid); break;
             case 2:
-                diffuse_genes_insert_body<NG>(in.diffuse, in.p, in.g, *in.gen, in.dp, in.genes, (Band<NG>*)mg_lds, bid);
+                diffuse_genes_insert_body<NG>(in.diffuse, in.p, in.g, const_genome(in.gen), in.dp, in.genes, (Band<NG>*)mg_lds, bid);
                 break;
             case 3: scan_lookback_body(in.scan_in, in.scan_out, in.C, in.scan_status, in.scan_ticket, in.ctr, in.gsm, in.tnb); break;
             case 4: place_body(in.place, bid); break;
