@@ -2581,6 +2581,33 @@ struct BatchInst {
     uint32_t tb0, tnb; /* scan tiles */
 };
 
+/* The instance table's buffers are device allocations. The batched kernels load their
+ * pointers from the table, so the compiler cannot tell their address space and issues flat
+ * accesses, which also count against the LDS counter the sweeps wait on. A local copy of the
+ * view whose pointers pass through the global address space (by value, which the compiler
+ * cannot fold back into the generic pointer) lets it address them as global memory. */
+template <class T>
+__device__ __forceinline__ T* as_global(T* q)
+{
+    return (T*)(__attribute__((address_space(1))) T*)(uintptr_t)q;
+}
+
+__device__ __forceinline__ DS global_view(const DS& in)
+{
+    DS d = in;
+#define MG_GLOBAL(f) d.f = as_global(d.f)
+    MG_GLOBAL(P); MG_GLOBAL(V); MG_GLOBAL(VEL); MG_GLOBAL(F); MG_GLOBAL(CONC); MG_GLOBAL(HOME);
+    MG_GLOBAL(GCELL); MG_GLOBAL(GNDX); MG_GLOBAL(GMASK); MG_GLOBAL(P2); MG_GLOBAL(V2);
+    MG_GLOBAL(VEL2); MG_GLOBAL(CONC2); MG_GLOBAL(HOME2); MG_GLOBAL(GCELL2); MG_GLOBAL(GNDX2);
+    MG_GLOBAL(GMASK2); MG_GLOBAL(SLOT); MG_GLOBAL(EPIGEN); MG_GLOBAL(BPARTNER); MG_GLOBAL(BPAR);
+    MG_GLOBAL(BINT); MG_GLOBAL(INC_OFF); MG_GLOBAL(INC); MG_GLOBAL(LIVE); MG_GLOBAL(CNT);
+    MG_GLOBAL(OFF); MG_GLOBAL(GCNT); MG_GLOBAL(GOFF); MG_GLOBAL(PERM); MG_GLOBAL(LISTS);
+    MG_GLOBAL(LENS); MG_GLOBAL(CNTN); MG_GLOBAL(OFFN); MG_GLOBAL(GMN); MG_GLOBAL(ZERO);
+    MG_GLOBAL(DVc); MG_GLOBAL(HMW); MG_GLOBAL(HMR); MG_GLOBAL(ETW); MG_GLOBAL(ETR); MG_GLOBAL(ctr);
+#undef MG_GLOBAL
+    return d;
+}
+
 /* the pressure sweep at candidate batch K and band capacity CAP (the autotuner's tile shapes) */
 template <int K, int CAP>
 __global__ LAUNCH_BOUNDS_SWEEP void kb_pressure(const BatchInst* __restrict__ I, const uint16_t* __restrict__ map)
@@ -2588,7 +2615,7 @@ __global__ LAUNCH_BOUNDS_SWEEP void kb_pressure(const BatchInst* __restrict__ I,
     extern __shared__ float4 mg_lds[];
     const uint32_t b = xcd_block();
     const BatchInst& in = I[map[b]];
-    pressure_body<K, CAP>(in.pressure, in.p, in.g, (Band<0, CAP>*)mg_lds, b - in.pb0, in.pnb);
+    pressure_body<K, CAP>(global_view(in.pressure), in.p, in.g, (Band<0, CAP>*)mg_lds, b - in.pb0, in.pnb);
 }
 
 __global__ LAUNCH_BOUNDS_SWEEP void kb_force_advance(const BatchInst* __restrict__ I,
@@ -2597,7 +2624,7 @@ __global__ LAUNCH_BOUNDS_SWEEP void kb_force_advance(const BatchInst* __restrict
     extern __shared__ float4 mg_lds[];
     const uint32_t b = xcd_block();
     const BatchInst& in = I[map[b]];
-    force_advance_body(in.force, in.p, in.g, (Band<1>*)mg_lds, in.dt, b - in.pb0);
+    force_advance_body(global_view(in.force), in.p, in.g, (Band<1>*)mg_lds, in.dt, b - in.pb0);
 }
 
 /* An instance's genome read through the constant address space: nothing writes it while a
@@ -2614,27 +2641,28 @@ __global__ LAUNCH_BOUNDS_SWEEP void kb_diffuse_genes_insert(const BatchInst* __r
     extern __shared__ float4 mg_lds[];
     const uint32_t b = xcd_block();
     const BatchInst& in = I[map[b]];
-    diffuse_genes_insert_body<NG>(in.diffuse, in.p, in.g, const_genome(in.gen), in.dp, in.genes, (Band<NG>*)mg_lds,
-                                  b - in.pb0);
+    diffuse_genes_insert_body<NG>(global_view(in.diffuse), in.p, in.g, const_genome(in.gen), in.dp, in.genes,
+                                  (Band<NG>*)mg_lds, b - in.pb0);
 }
 
 __global__ __launch_bounds__(256) void kb_scan(const BatchInst* __restrict__ I, const uint16_t* __restrict__ map)
 {
     const BatchInst& in = I[map[blockIdx.x]];
-    scan_lookback_body(in.scan_in, in.scan_out, in.C, in.scan_status, in.scan_ticket, in.ctr, in.gsm, in.tnb);
+    scan_lookback_body(as_global(in.scan_in), as_global(in.scan_out), in.C, as_global(in.scan_status),
+                       as_global(in.scan_ticket), as_global(in.ctr), in.gsm, in.tnb);
 }
 
 __global__ LAUNCH_BOUNDS_256 void kb_place(const BatchInst* __restrict__ I, const uint16_t* __restrict__ map)
 {
     const BatchInst& in = I[map[blockIdx.x]];
-    place_body(in.place, blockIdx.x - in.pb0);
+    place_body(global_view(in.place), blockIdx.x - in.pb0);
 }
 
 __global__ LAUNCH_BOUNDS_256 void kb_sort_gather(const BatchInst* __restrict__ I, const uint16_t* __restrict__ map)
 {
     const uint32_t b = xcd_block();
     const BatchInst& in = I[map[b]];
-    sort_gather_body(in.sort, in.vmax, b - in.pb0);
+    sort_gather_body(global_view(in.sort), in.vmax, b - in.pb0);
 }
 
 /* --------------------------------------------------- persistent batched step
@@ -2712,14 +2740,18 @@ __global__ __launch_bounds__(256) void kb_step(PersistPlan P)
             const BatchInst& in = P.I[it >> 16];
             const uint32_t bid = it & 0xFFFFu;
             switch (ph) {
-            case 0: pressure_body(in.pressure, in.p, in.g, (Band<0>*)mg_lds, bid, in.pnb); break;
-            case 1: force_advance_body(in.force, in.p, in.g, (Band<1>*)mg_lds, in.dt, bid); break;
+            case 0: pressure_body(global_view(in.pressure), in.p, in.g, (Band<0>*)mg_lds, bid, in.pnb); break;
+            case 1: force_advance_body(global_view(in.force), in.p, in.g, (Band<1>*)mg_lds, in.dt, bid); break;
             case 2:
-                diffuse_genes_insert_body<NG>(in.diffuse, in.p, in.g, const_genome(in.gen), in.dp, in.genes, (Band<NG>*)mg_lds, bid);
+                diffuse_genes_insert_body<NG>(global_view(in.diffuse), in.p, in.g, const_genome(in.gen), in.dp, in.genes,
+                                              (Band<NG>*)mg_lds, bid);
                 break;
-            case 3: scan_lookback_body(in.scan_in, in.scan_out, in.C, in.scan_status, in.scan_ticket, in.ctr, in.gsm, in.tnb); break;
-            case 4: place_body(in.place, bid); break;
-            default: sort_gather_body(in.sort, in.vmax, bid); break;
+            case 3:
+                scan_lookback_body(as_global(in.scan_in), as_global(in.scan_out), in.C, as_global(in.scan_status),
+                                   as_global(in.scan_ticket), as_global(in.ctr), in.gsm, in.tnb);
+                break;
+            case 4: place_body(global_view(in.place), bid); break;
+            default: sort_gather_body(global_view(in.sort), in.vmax, bid); break;
             }
             __syncthreads(); /* the bodies' LDS is reused by the next item */
         }
